@@ -1,0 +1,175 @@
+/*
+ * s2hip.h -- C ABI of the MI355X (gfx950) S2 two-point-correlation engine.
+ *
+ * Drop-in boundary for RustMSPT's S2 path.  Each entry point names the reference
+ * interface it replaces (file:line in the RustMSPT source tree).  All functions
+ * are extern "C", take plain pointers and sizes, never throw, and report failure
+ * through a status code plus a thread-local message (s2h_last_error), which the
+ * caller maps to its own Result<_, String> and fallback policy exactly as the
+ * reference callers do for the wgpu path (src/pipeline/measure.rs:225-246).
+ *
+ * Ownership: the context owns the device, streams, its device-memory accounting
+ * and the per-(grid, r_max, pitch) shell plans.  A volume handle is owned by the
+ * context until s2h_vol_free.  Host pointers are borrowed for the call only.
+ * Output arrays are caller-allocated with explicit lengths; a length mismatch is
+ * an error (S2H_ERR_SIZE), never a truncation.
+ *
+ * Threading: a context is not thread-safe (the reference serializes its GPU
+ * objects behind a Mutex, src/pipeline/optimize_execution.rs:147); use one context
+ * per host thread or lock around it.  The serve_* calls are meant for one owner
+ * thread.
+ *
+ * Packed volume layout (shared with the reference's OccupancyBits,
+ * src/geometry/s2.rs:1035-1054): row (x, y) is (ceil(nz/64)+1) little-endian
+ * uint64 words at word index (x*ny + y)*row_words; voxel z is bit z%64 of word
+ * z/64; the trailing pad word and every bit at z >= nz are zero.
+ *
+ * Offset order: "shell order" is the reference's curve summation order -- radii
+ * 1..r_max, each radius' offsets in shell_offset_iter order (dx, then dy, then dz
+ * ascending; src/geometry/s2.rs:364-385) -- keeping only in-domain offsets
+ * (|d_i| < n_i, src/geometry/s2.rs:1028-1030).
+ */
+#ifndef S2HIP_H
+#define S2HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct s2h_ctx s2h_ctx;
+typedef struct s2h_vol s2h_vol;
+typedef int32_t s2h_status;
+
+#define S2H_OK 0
+#define S2H_ERR_ARG 1         /* invalid argument (message says which) */
+#define S2H_ERR_HIP 2         /* HIP runtime error */
+#define S2H_ERR_BUDGET 3      /* working set exceeds the context's device budget */
+#define S2H_ERR_SIZE 4        /* caller buffer length does not match */
+#define S2H_ERR_UNSUPPORTED 5 /* shape outside what the kernels handle */
+#define S2H_ERR_IO 6          /* file read failure */
+
+/* ------------------------------------------------------------------ context */
+
+/* Replaces the shared wgpu device (src/gpu/context.rs:115-119, shared_device) and
+ * the budget of AccelerationConfig.gpu_memory_limit_mb
+ * (src/config/acceleration.rs:15).  hbm_budget_bytes = 0 means no budget. */
+s2h_status s2h_create(int device, uint64_t hbm_budget_bytes, s2h_ctx **out);
+void s2h_destroy(s2h_ctx *ctx);
+/* Thread-local message for the last failing call on this thread. */
+const char *s2h_last_error(void);
+/* Device name for "Compute Backend: exact=hip/<name>" (src/compute/backend.rs:89-93). */
+s2h_status s2h_device_name(s2h_ctx *ctx, char *buf, uint64_t len);
+/* Device bytes currently held by the context (volumes, plans, scratch). */
+s2h_status s2h_bytes_in_use(s2h_ctx *ctx, uint64_t *bytes);
+
+/* Kernel timing on the context's stream: when enabled, each exact / FFT / MC call
+ * records HIP events around its phases; s2h_profile_read returns, in ms, for the
+ * last call: [0] count kernels, [1] gather + fold, [2] whole device work, and
+ * [3] the number of kernel launches in [0].  No counterpart in the reference
+ * (it has no GPU kernel timestamps, AGENTS.md:638); used by the benchmark. */
+s2h_status s2h_profile(s2h_ctx *ctx, int enable);
+s2h_status s2h_profile_read(s2h_ctx *ctx, double *ms, uint64_t n);
+
+/* ------------------------------------------------------------------ volumes */
+
+/* Upload a packed volume (layout above); the pad word and bits z >= nz must be 0. */
+s2h_status s2h_vol_from_packed(s2h_ctx *ctx, const uint64_t *words, uint64_t nwords,
+                               const uint64_t dims[3], s2h_vol **out);
+/* Upload a bool grid (x-major, z fastest, one byte per voxel) and pack it on the
+ * device (replaces OccupancyBits::new, src/geometry/s2.rs:1042-1054). */
+s2h_status s2h_vol_from_bytes(s2h_ctx *ctx, const uint8_t *occ, const uint64_t dims[3],
+                              s2h_vol **out);
+s2h_status s2h_vol_read_packed(s2h_ctx *ctx, const s2h_vol *vol, uint64_t *words, uint64_t nwords);
+s2h_status s2h_vol_dims(const s2h_vol *vol, uint64_t dims[3]);
+/* Occupied voxel count (the numerator of vf, src/geometry/s2.rs:1486-1491). */
+s2h_status s2h_vol_occupied(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);
+void s2h_vol_free(s2h_ctx *ctx, s2h_vol *vol);
+
+/* --------------------------------------------------------------- voxelization */
+
+/* Replaces build_bbox_occupancy / VoxelS2::new (src/geometry/s2.rs:160-201, 1475)
+ * and GpuVoxelPipeline::voxelize (src/gpu/voxel.rs:285).  The host passes the
+ * welded mesh (vertices as f64 triples, src/io/stl.rs:55-80) and the granule id of
+ * every triangle (split_mesh_into_granules, src/geometry/mesh_ops.rs:48); the
+ * device evaluates the reference's f64 parity predicate per granule at every
+ * voxel centre of the granule's range and ORs granules, bit-exactly.  pitch must
+ * be > 0 (the caller applies the reference's max(1e-9) / fallback-to-1.0 rules). */
+s2h_status s2h_voxelize(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri,
+                        uint64_t ntris, const uint32_t *granule_of_tri, uint32_t ngranules,
+                        const double bbox_min[3], double pitch, const uint64_t dims[3],
+                        s2h_vol **out, uint64_t *occupied);
+
+/* ------------------------------------------------------------------ shells */
+
+/* Number of in-domain offsets of radii 1..r_max in shell order (sizes the outputs
+ * of s2h_exact_counts / s2h_fft32_counts; exact_shell_work's K,
+ * src/geometry/s2.rs:623-692). */
+s2h_status s2h_shell_count(const uint64_t dims[3], uint32_t r_max, double pitch, uint64_t *n_offsets);
+/* The offsets themselves (3 int64 each) and radius_start[r_max+2] (offsets of
+ * radius r are [radius_start[r], radius_start[r+1])). */
+s2h_status s2h_shell_offsets(const uint64_t dims[3], uint32_t r_max, double pitch, int64_t *offs,
+                             uint64_t n_offsets, uint64_t *radius_start, uint64_t n_radius_start);
+
+/* ------------------------------------------------------------------ exact */
+
+/* Exact pair counts ("hits") of every in-domain offset in shell order (replaces
+ * direct_pair_counts_bits over the shells, src/geometry/s2.rs:1092-1120,
+ * 1187-1201, and GpuShellS2Pipeline::compute_shell_input, src/gpu/s2_shell.rs:459).
+ * valid is analytic (prod n_i-|d_i|) and is not returned. */
+s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch,
+                            uint64_t *hits, uint64_t n_offsets);
+/* Finished exact S2 curve r = 0..r_max (replaces VoxelS2::calculate_exact_with,
+ * src/geometry/s2.rs:1482-1495, and try_calculate_s2_gpu_exact's result,
+ * src/geometry/s2.rs:1616-1678): per-radius ordered f64 fold of hits/valid in
+ * shell order, smooth fill of unsupported radii, S2(0)=vf; all-empty -> zeros.
+ * Bit-identical to the reference CPU exact path. */
+s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, double *s2,
+                        uint64_t n);
+
+/* ------------------------------------------------------------------ FFT (FP32) */
+
+/* FP32 real-FFT autocorrelation counts of every in-domain offset in shell order
+ * (the FFT exact kernel's correlation, src/geometry/s2.rs:965-1012, 1219-1261, in
+ * FP32 and NOT rounded: values carry FP32 round-off). */
+s2h_status s2h_fft32_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch,
+                            float *counts, uint64_t n_offsets);
+/* S2 curve from the FP32 correlation (same fold and fill as s2h_exact_s2). */
+s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, double *s2,
+                        uint64_t n);
+
+/* ------------------------------------------------------------------ Monte Carlo */
+
+/* Voxel Monte Carlo counts (estimator of voxel_mc_radii, src/geometry/s2.rs:1412-1464)
+ * with a Philox4x32-10 counter-based stream keyed by seed: hits_valid has
+ * 2*(r_max+1) entries, (hits, valid) per radius, radius 0 zero. */
+s2h_status s2h_mc_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch,
+                         uint64_t samples, uint64_t seed, uint64_t *hits_valid, uint64_t n);
+/* Finished MC curve (VoxelS2::calculate_seeded "monte_carlo", src/geometry/s2.rs:1521-1537):
+ * hits/valid per radius, unsupported radii filled, S2(0)=vf. samples is per radius
+ * and is raised to 200 as in the reference. */
+s2h_status s2h_mc_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, uint64_t samples,
+                     uint64_t seed, double *s2, uint64_t n);
+
+/* --------------------------------------------------------------- host helpers */
+
+/* Fill unsupported radii (fill_missing_s2_with_smooth_interpolation,
+ * src/geometry/s2.rs:393-501). */
+s2h_status s2h_fill_missing(double *values, const uint8_t *has_support, uint64_t n, double vf);
+/* Binary/ASCII STL load with the reference's f32->f64 conversion and 1e-6 vertex
+ * weld (src/io/stl.rs:55-80, 157-189).  Arrays are allocated by the library and
+ * released with s2h_free. */
+s2h_status s2h_stl_load(const char *path, double **xyz, uint64_t *nverts, uint32_t **tri,
+                        uint64_t *ntris);
+void s2h_free(void *p);
+/* Connected components over shared vertices (split_mesh_into_granules,
+ * src/geometry/mesh_ops.rs:48-~120): granule id per triangle, in the reference's
+ * component order. */
+s2h_status s2h_mesh_granules(const uint32_t *tri, uint64_t ntris, uint64_t nverts,
+                             uint32_t *granule_of_tri, uint32_t *ngranules);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* S2HIP_H */

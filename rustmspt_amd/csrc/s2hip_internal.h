@@ -1,0 +1,157 @@
+// Internal declarations shared by the translation units of libs2hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/s2hip.h"
+
+namespace s2h {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+s2h_status fail(s2h_status code, const std::string &msg);
+
+#define S2H_HIP(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ::s2h::fail(S2H_ERR_HIP, std::string(#expr " failed: ") +          \
+                                                hipGetErrorString(e_));                 \
+    } while (0)
+
+#define S2H_TRY(expr)                   \
+    do {                                \
+        s2h_status st_ = (expr);        \
+        if (st_ != S2H_OK) return st_;  \
+    } while (0)
+
+// ---------------------------------------------------------------- shells (host)
+struct Offset {
+    int32_t dx, dy, dz;
+};
+
+// In-domain offsets of radii 1..r_max in the reference's shell order, with
+// radius_start[r_max+2].
+void shell_order_offsets(const uint64_t dims[3], uint32_t r_max, double pitch,
+                         std::vector<Offset> &offs, std::vector<uint64_t> &radius_start);
+// Every offset of shell r (not domain-filtered), in shell order.
+void shell_offsets_full(double distance_vox, double half_width_vox, std::vector<Offset> &out);
+
+void fill_missing(double *values, const uint8_t *has_support, uint64_t n, double vf);
+
+// ---------------------------------------------------------------- device memory
+struct DevBuf {
+    void *ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
+// Plan for exact counting on one (dims, r_max, pitch): shell-order offsets, the
+// canonical half-ball column table and per-wave work items.
+struct ColInfo {
+    int32_t dx, dy;   // column (dx, dy) of the canonical half ball
+    int32_t m;        // largest dz of the column (-1 = idle lane)
+    uint32_t base;    // index of (dx, dy, 0) in the dense count table
+};
+
+struct WorkItem {
+    uint32_t group;   // 64-column group
+    uint16_t dz0;     // first dz (multiple of 32)
+    uint16_t dzb;     // 8, 16 or 32 dz values
+    uint16_t x0, x1;  // b-row x range
+    uint16_t y0, y1;  // b-row y range
+    uint16_t nw;      // dwords of the shifted partner that can be non-zero
+    uint16_t pad;
+};
+
+struct ExactPlan {
+    uint64_t dims[3];
+    uint32_t r_max;
+    double pitch;
+    std::vector<Offset> offs;             // shell order
+    std::vector<uint64_t> radius_start;   // r_max + 2
+    uint64_t table_size = 0;              // dense count-table entries
+    uint32_t ngroups = 0;
+    std::vector<WorkItem> items[3];       // per dz-block size 8, 16, 32
+    // device copies
+    DevBuf d_offs;     // Offset[K]
+    DevBuf d_gather;   // int32[K] table index (-1 = zero offset)
+    DevBuf d_rstart;   // uint64[r_max+2]
+    DevBuf d_cols;     // ColInfo[ngroups*64]
+    DevBuf d_items[3]; // WorkItem
+    DevBuf d_table;    // uint64[table_size]
+    DevBuf d_terms;    // double[K]
+    DevBuf d_hits;     // uint64[K]
+    DevBuf d_curve;    // double[r_max+1] values, then uint8 support
+};
+
+struct McPlan {
+    uint32_t r_max;
+    double pitch;
+    std::vector<uint32_t> start;   // r_max + 2 (full shells, not domain-filtered)
+    DevBuf d_shell;                // int16x4 per offset
+    DevBuf d_start;                // uint32[r_max+2]
+    DevBuf d_hv;                   // uint64[2*(r_max+1)]
+};
+
+struct FftPlan;  // defined in s2hip_fft.hip
+void fft_plan_release(s2h_ctx *ctx, FftPlan *p);
+
+}  // namespace s2h
+
+struct s2h_vol {
+    uint64_t dims[3];
+    uint64_t row_words;   // u64 words per row
+    uint64_t nwords;      // total u64 words
+    uint64_t *d_words;
+    uint64_t alloc_bytes; // nwords*8 plus the zeroed tail
+    uint64_t occupied;    // cached popcount, UINT64_MAX = unknown
+};
+
+// Zeroed bytes after the last row of every volume buffer: kernels may read up to
+// 64 dwords past a row start without a bounds check.
+constexpr uint64_t kVolTailBytes = 512;
+
+struct s2h_ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    uint64_t in_use = 0;
+    hipStream_t stream = nullptr;
+    std::string name;
+    std::map<std::tuple<uint64_t, uint64_t, uint64_t, uint32_t, uint64_t>,
+             std::unique_ptr<s2h::ExactPlan>>
+        exact_plans;
+    std::map<std::tuple<uint32_t, uint64_t>, std::unique_ptr<s2h::McPlan>> mc_plans;
+    std::map<std::tuple<uint64_t, uint64_t, uint64_t>, s2h::FftPlan *> fft_plans;
+    s2h::DevBuf d_scalar;   // small scratch (popcount result etc.)
+    bool profiling = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double prof_ms[4] = {0, 0, 0, 0};
+    int launches = 0;
+};
+
+namespace s2h {
+s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes);
+void dev_free(s2h_ctx *ctx, DevBuf &b);
+s2h_status upload(s2h_ctx *ctx, DevBuf &b, const void *src, uint64_t bytes);
+s2h_status vol_alloc(s2h_ctx *ctx, const uint64_t dims[3], s2h_vol **out);
+s2h_status vol_popcount(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);
+
+s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
+                          ExactPlan **out);
+// Launch the count kernels for vol into plan->d_table (zeroed first) on ctx->stream.
+s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan);
+// Gather table -> per-offset hits and f64 terms, fold per radius into d_curve.
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied,
+                             const float *d_fcounts /* nullptr: integer table */);
+// Copy folded curve values and supported flags back and finish the curve.
+s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *plan, double vf, double *s2);
+// Profiling marks: k = 0 (before counts), 1 (after counts), 2 (after fold).
+void prof_mark(s2h_ctx *ctx, int k);
+void prof_collect(s2h_ctx *ctx);
+}  // namespace s2h
