@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--particles", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (counter calibration for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only "
@@ -79,6 +81,16 @@ def main():
     import numpy as np
 
     from rustmspt_amd import s2, synth
+
+    if args.copy_probe_mb:
+        import torch
+        n = args.copy_probe_mb * 1024 * 1024 // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     grid, r_max = args.grid, args.r_max
     mesh, bbox, parts = synth.config_large(seed=2, n=args.particles, grid=grid)

@@ -1,23 +1,613 @@
-// FP32 real-FFT autocorrelation path (placeholder until the hand-written
-// transform lands: every entry point reports S2H_ERR_UNSUPPORTED).
+// FP32 real-FFT autocorrelation S2 (gfx950).
+//
+// Restates the FFT exact kernel's correlation (src/geometry/s2.rs:965-1012,
+// 1219-1261) in FP32: the occupancy is zero-padded to the reference's
+// 2,3,5-smooth lengths (padded_fft_dims, src/geometry/s2.rs:554-562), transformed,
+// squared in magnitude and transformed back; the correlation at each shell
+// offset d (wrapped, d mod f) is the pair count C(d) plus FP32 round-off.  The
+// values are NOT rounded: this path reports its error against an FP64 FFT and
+// the exact integer counts instead of claiming exactness.
+//
+// Layout and passes (H = fz/2 + 1, R_a = largest |d_a| of any shell offset):
+//   z-fwd : two real z rows per complex FFT (a + ib) read straight from the
+//           packed bits; DIF in LDS; split into the two half spectra
+//                                                  -> G1[nx][fy][H]  (y < ny)
+//   y-fwd : per (x, tile of 16 kz) DIF along y, rows y >= ny are zero -> G1
+//   x     : per (ky, tile of kz) DIF along x (x >= nx zero), |F|^2, DIT back,
+//           keep the 2R_x+1 needed x           -> G2[2Rx+1][fy][H]
+//   y-inv : per (ix, tile of kz) DIT along y, keep 2R_y+1 y -> G3[2Rx+1][2Ry+1][H]
+//   z-inv : two spectra per complex inverse (Hermitian extension), keep 2R_z+1
+//           z, scale by 1/(fx fy fz)            -> cube[2Rx+1][2Ry+1][2Rz+1]
+//   gather: cube value at every shell offset, in shell order.
+// Forward transforms are in-place decimation-in-frequency (output in
+// digit-reversed order) and inverses the matching decimation-in-time (input in
+// that order, output natural), so no reordering pass is needed; the pointwise
+// |F|^2 is order-independent.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "s2hip_internal.h"
 
 namespace s2h {
-struct FftPlan {
-    int unused;
+
+static constexpr int kMaxFactors = 24;
+static constexpr int kTileMax = 16;   // kz columns per workgroup in the y and x passes (max)
+static constexpr int kFftThreads = 256;
+
+struct AxisPlan {
+    int L = 1;
+    int nf = 0;
+    int fac[kMaxFactors];
 };
-void fft_plan_release(s2h_ctx *, FftPlan *p) { delete p; }
+
+struct FftPlan {
+    uint64_t dims[3];
+    uint64_t f[3];
+    int R[3];
+    AxisPlan ax[3];
+    DevBuf tw[3];    // float2[L] = exp(-2 pi i k / L), per axis
+    DevBuf rev_z;    // int32[fz]: natural frequency held at each DIF output position
+    DevBuf pos_z;    // int32[fz]: DIF output position of each natural frequency
+    DevBuf g1, g2, g3, cube, fcounts;
+};
+
+void fft_plan_release(s2h_ctx *ctx, FftPlan *p) {
+    if (!p) return;
+    for (auto &b : p->tw) dev_free(ctx, b);
+    DevBuf *bufs[] = {&p->rev_z, &p->pos_z, &p->g1, &p->g2, &p->g3, &p->cube, &p->fcounts};
+    for (DevBuf *b : bufs) dev_free(ctx, *b);
+    delete p;
+}
+
+namespace {
+
+struct Fac {
+    int n;
+    int f[kMaxFactors];
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// R-point DFT in registers; sign = -1 forward, +1 inverse (unnormalized).
+template <int SIGN>
+__device__ __forceinline__ void small_dft(float2 *a, int R) {
+    if (R == 2) {
+        float2 t = a[0];
+        a[0] = cadd(t, a[1]);
+        a[1] = csub(t, a[1]);
+    } else if (R == 4) {
+        float2 s0 = cadd(a[0], a[2]), d0 = csub(a[0], a[2]);
+        float2 s1 = cadd(a[1], a[3]), d1 = csub(a[1], a[3]);
+        // multiply d1 by -i (forward) or +i (inverse)
+        float2 jd1 = SIGN < 0 ? make_float2(d1.y, -d1.x) : make_float2(-d1.y, d1.x);
+        a[0] = cadd(s0, s1);
+        a[2] = csub(s0, s1);
+        a[1] = cadd(d0, jd1);
+        a[3] = csub(d0, jd1);
+    } else if (R == 3) {
+        const float c = -0.5f, s = SIGN * 0.86602540378443864676f;
+        float2 t1 = cadd(a[1], a[2]), t2 = csub(a[1], a[2]);
+        float2 m = make_float2(a[0].x + c * t1.x, a[0].y + c * t1.y);
+        float2 js = make_float2(-s * t2.y, s * t2.x);  // i*s*t2
+        a[0] = cadd(a[0], t1);
+        a[1] = cadd(m, js);
+        a[2] = csub(m, js);
+    } else {  // R == 5
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = SIGN * 0.95105651629515357212f, s2 = SIGN * 0.58778525229247312917f;
+        float2 b1 = cadd(a[1], a[4]), b4 = csub(a[1], a[4]);
+        float2 b2 = cadd(a[2], a[3]), b3 = csub(a[2], a[3]);
+        float2 x0 = a[0];
+        float2 m1 = make_float2(x0.x + c1 * b1.x + c2 * b2.x, x0.y + c1 * b1.y + c2 * b2.y);
+        float2 m2 = make_float2(x0.x + c2 * b1.x + c1 * b2.x, x0.y + c2 * b1.y + c1 * b2.y);
+        // i * (s1 b4 + s2 b3) and i * (s2 b4 - s1 b3)
+        float2 n1 = make_float2(-(s1 * b4.y + s2 * b3.y), s1 * b4.x + s2 * b3.x);
+        float2 n2 = make_float2(-(s2 * b4.y - s1 * b3.y), s2 * b4.x - s1 * b3.x);
+        a[0] = cadd(x0, cadd(b1, b2));
+        a[1] = cadd(m1, n1);
+        a[4] = csub(m1, n1);
+        a[2] = cadd(m2, n2);
+        a[3] = csub(m2, n2);
+    }
+}
+
+// One radix-R stage over all lines: in-place butterflies on sub-length m.
+template <int R, bool INV>
+__device__ __forceinline__ void fft_stage(float2 *d, int L, int S, int lines, int m, const float2 *tw) {
+    const int mp = m / R, nb = L / R, step = L / m;
+    for (int t = threadIdx.x; t < nb * lines; t += blockDim.x) {
+        const int line = t / nb, b = t - line * nb;
+        const int grp = b / mp, k = b - grp * mp;
+        float2 *base = d + line * S + grp * m + k;
+        float2 a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = base[r * mp];
+        if (!INV) {
+            small_dft<-1>(a, R);
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[k * q * step]);
+        } else {
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[k * q * step]);
+            small_dft<1>(a, R);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) base[r * mp] = a[r];
+    }
+    __syncthreads();
+}
+
+template <bool INV>
+__device__ __forceinline__ void fft_stage_any(float2 *d, int L, int S, int lines, int R, int m, const float2 *tw) {
+    switch (R) {
+        case 4: fft_stage<4, INV>(d, L, S, lines, m, tw); break;
+        case 2: fft_stage<2, INV>(d, L, S, lines, m, tw); break;
+        case 3: fft_stage<3, INV>(d, L, S, lines, m, tw); break;
+        default: fft_stage<5, INV>(d, L, S, lines, m, tw); break;
+    }
+}
+
+// In-place mixed-radix transform of `lines` lines of length L held in LDS at
+// d[line*S + pos].  Forward: decimation in frequency, natural in, digit-reversed
+// out.  Inverse: decimation in time with conjugate twiddles, digit-reversed in,
+// natural out (unnormalized).  tw = exp(-2 pi i k / L); k*q*step < L always.
+template <bool INV>
+__device__ void lds_fft(float2 *d, int L, int S, int lines, const Fac &fc, const float2 *tw) {
+    if (!INV) {
+        int m = L;
+        for (int s = 0; s < fc.n; ++s) {
+            fft_stage_any<false>(d, L, S, lines, fc.f[s], m, tw);
+            m /= fc.f[s];
+        }
+    } else {
+        int m = 1;  // stage s works on sub-length m_s = f_s * m_{s+1}, m_n = 1
+        for (int s = fc.n - 1; s >= 0; --s) {
+            m *= fc.f[s];
+            fft_stage_any<true>(d, L, S, lines, fc.f[s], m, tw);
+        }
+    }
+}
+
+__device__ __forceinline__ void load_tw(float2 *s_tw, const float2 *tw, int L) {
+    for (int i = threadIdx.x; i < L; i += blockDim.x) s_tw[i] = tw[i];
+}
+
+// ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
+__global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restrict__ words, int nx, int ny, int nz,
+                                                         int row_words, int fy, int H, Fac fc, int L,
+                                                         const float2 *__restrict__ tw, const int *__restrict__ pos,
+                                                         int pairs_per_block, float2 *__restrict__ g1) {
+    extern __shared__ float2 sm[];
+    float2 *s_tw = sm;
+    float2 *d = sm + L;
+    load_tw(s_tw, tw, L);
+    const int npairs = nx * ((ny + 1) / 2);
+    const int p0 = blockIdx.x * pairs_per_block;
+    const int np = min(pairs_per_block, npairs - p0);
+    const int ypairs = (ny + 1) / 2;
+    for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
+        const int pl = t / L, z = t - pl * L;
+        const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
+        float re = 0.f, im = 0.f;
+        if (z < nz) {
+            const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
+            re = (float)((ra[z >> 6] >> (z & 63)) & 1ull);
+            if (y + 1 < ny) im = (float)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
+        }
+        d[pl * L + z] = make_float2(re, im);
+    }
+    __syncthreads();
+    lds_fft<false>(d, L, L, np, fc, s_tw);
+    for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
+        const int pl = t / H, k = t - pl * H;
+        const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
+        const float2 zk = d[pl * L + pos[k]];
+        const float2 zn = d[pl * L + pos[(L - k) % L]];
+        // A = (Z[k] + conj Z[-k]) / 2,  B = (Z[k] - conj Z[-k]) / (2i)
+        const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
+        const float2 B = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+        g1[((uint64_t)x * fy + y) * H + k] = A;
+        if (y + 1 < ny) g1[((uint64_t)x * fy + y + 1) * H + k] = B;
+    }
+}
+
+// ---- y forward: per (x, kz tile), rows y >= ny are zero; in place in G1
+__global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, int tiles_kz, int tile, Fac fc,
+                                                         const float2 *__restrict__ tw, float2 *__restrict__ g1) {
+    extern __shared__ float2 sm[];
+    const int L = fy, S = L + 1;
+    float2 *s_tw = sm;
+    float2 *d = sm + L;
+    load_tw(s_tw, tw, L);
+    const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
+    const int nk = min(tile, H - kz0);
+    float2 *base = g1 + (uint64_t)x * fy * H + kz0;
+    for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
+        const int y = t / tile, c = t - y * tile;
+        float2 v = make_float2(0.f, 0.f);
+        if (y < ny && c < nk) v = base[(uint64_t)y * H + c];
+        d[c * S + y] = v;
+    }
+    __syncthreads();
+    lds_fft<false>(d, L, S, nk, fc, s_tw);
+    for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
+        const int y = t / tile, c = t - y * tile;
+        if (c < nk) base[(uint64_t)y * H + c] = d[c * S + y];
+    }
+}
+
+// ---- x: forward, |F|^2, inverse; keep the needed x.  One block per (ky, kz tile).
+__global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H, int tiles_kz, int tile, Fac fc,
+                                                           const float2 *__restrict__ tw, int Rx,
+                                                           const float2 *__restrict__ g1, float2 *__restrict__ g2) {
+    extern __shared__ float2 sm[];
+    const int L = (int)fc.f[kMaxFactors - 1];  // unused slot carries L
+    const int S = L + 1;
+    float2 *s_tw = sm;
+    float2 *d = sm + L;
+    load_tw(s_tw, tw, L);
+    const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
+    const int nk = min(tile, H - kz0);
+    for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
+        const int x = t / tile, c = t - x * tile;
+        float2 v = make_float2(0.f, 0.f);
+        if (x < nx && c < nk) v = g1[((uint64_t)x * fy + ky) * H + kz0 + c];
+        d[c * S + x] = v;
+    }
+    __syncthreads();
+    lds_fft<false>(d, L, S, nk, fc, s_tw);
+    for (int t = threadIdx.x; t < L * nk; t += blockDim.x) {
+        const int c = t / L, x = t - c * L;
+        const float2 v = d[c * S + x];
+        d[c * S + x] = make_float2(v.x * v.x + v.y * v.y, 0.f);
+    }
+    __syncthreads();
+    lds_fft<true>(d, L, S, nk, fc, s_tw);
+    const int nxo = 2 * Rx + 1;
+    for (int t = threadIdx.x; t < nxo * tile; t += blockDim.x) {
+        const int ix = t / tile, c = t - ix * tile;
+        if (c >= nk) continue;
+        const int x = ix <= Rx ? ix : L - nxo + ix;
+        g2[((uint64_t)ix * fy + ky) * H + kz0 + c] = d[c * S + x];
+    }
+}
+
+// ---- y inverse: per (ix, kz tile), keep the needed y
+__global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tiles_kz, int tile, Fac fc,
+                                                         const float2 *__restrict__ tw, int Ry,
+                                                         const float2 *__restrict__ g2, float2 *__restrict__ g3) {
+    extern __shared__ float2 sm[];
+    const int L = fy, S = L + 1;
+    float2 *s_tw = sm;
+    float2 *d = sm + L;
+    load_tw(s_tw, tw, L);
+    const int ix = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
+    const int nk = min(tile, H - kz0);
+    const float2 *base = g2 + (uint64_t)ix * fy * H + kz0;
+    for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
+        const int y = t / tile, c = t - y * tile;
+        d[c * S + y] = c < nk ? base[(uint64_t)y * H + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    lds_fft<true>(d, L, S, nk, fc, s_tw);
+    const int nyo = 2 * Ry + 1;
+    for (int t = threadIdx.x; t < nyo * tile; t += blockDim.x) {
+        const int iy = t / tile, c = t - iy * tile;
+        if (c >= nk) continue;
+        const int y = iy <= Ry ? iy : L - nyo + iy;
+        g3[((uint64_t)ix * nyo + iy) * H + kz0 + c] = d[c * S + y];
+    }
+}
+
+// ---- z inverse: two Hermitian half spectra per complex inverse; keep needed z
+__global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac fc, int L,
+                                                         const float2 *__restrict__ tw, const int *__restrict__ rev,
+                                                         int pairs_per_block, int Rz, float scale,
+                                                         const float2 *__restrict__ g3, float *__restrict__ cube) {
+    extern __shared__ float2 sm[];
+    float2 *s_tw = sm;
+    float2 *d = sm + L;
+    load_tw(s_tw, tw, L);
+    const int p0 = blockIdx.x * pairs_per_block;
+    const int npairs = (nlines + 1) / 2;
+    const int np = min(pairs_per_block, npairs - p0);
+    for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
+        const int pl = t / L, pos = t - pl * L;
+        const int la = 2 * (p0 + pl), lb = la + 1;
+        const int k = rev[pos];
+        const bool upper = k >= H;
+        const int kk = upper ? L - k : k;
+        float2 A = g3[(uint64_t)la * H + kk];
+        float2 B = lb < nlines ? g3[(uint64_t)lb * H + kk] : make_float2(0.f, 0.f);
+        if (upper) {
+            A.y = -A.y;
+            B.y = -B.y;
+        }
+        d[pl * L + pos] = make_float2(A.x - B.y, A.y + B.x);  // A + iB
+    }
+    __syncthreads();
+    lds_fft<true>(d, L, L, np, fc, s_tw);
+    const int nzo = 2 * Rz + 1;
+    for (int t = threadIdx.x; t < np * nzo; t += blockDim.x) {
+        const int pl = t / nzo, iz = t - pl * nzo;
+        const int z = iz <= Rz ? iz : L - nzo + iz;
+        const int la = 2 * (p0 + pl), lb = la + 1;
+        const float2 v = d[pl * L + z];
+        cube[(uint64_t)la * nzo + iz] = v.x * scale;
+        if (lb < nlines) cube[(uint64_t)lb * nzo + iz] = v.y * scale;
+    }
+}
+
+__global__ void fft_gather(const Offset *__restrict__ offs, uint64_t n, const float *__restrict__ cube, int Rx,
+                           int Ry, int Rz, float *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Offset o = offs[i];
+    const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
+    const int ix = o.dx >= 0 ? o.dx : 2 * Rx + 1 + o.dx;
+    const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
+    const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
+    out[i] = cube[((uint64_t)ix * nyo + iy) * nzo + iz];
+}
+
+// ---------------------------------------------------------------- host plan
+
+AxisPlan axis_plan(int L) {
+    AxisPlan a;
+    a.L = L;
+    a.nf = 0;
+    int n = L;
+    while (n % 4 == 0 && a.nf < kMaxFactors - 1) a.fac[a.nf++] = 4, n /= 4;
+    while (n % 2 == 0 && a.nf < kMaxFactors - 1) a.fac[a.nf++] = 2, n /= 2;
+    while (n % 3 == 0 && a.nf < kMaxFactors - 1) a.fac[a.nf++] = 3, n /= 3;
+    while (n % 5 == 0 && a.nf < kMaxFactors - 1) a.fac[a.nf++] = 5, n /= 5;
+    if (n != 1) a.nf = -1;
+    return a;
+}
+
+Fac to_fac(const AxisPlan &a) {
+    Fac f;
+    f.n = a.nf;
+    for (int i = 0; i < kMaxFactors; ++i) f.f[i] = i < a.nf ? a.fac[i] : 0;
+    f.f[kMaxFactors - 1] = a.L;
+    return f;
+}
+
+// src/geometry/s2.rs:519-551 smooth_fft_length
+uint64_t smooth_length(uint64_t min_len) {
+    uint64_t target = std::max<uint64_t>(min_len, 1), best = 0;
+    for (uint64_t p5 = 1;; p5 *= 5) {
+        for (uint64_t p35 = p5;; p35 *= 3) {
+            uint64_t v = p35;
+            while (v < target) v *= 2;
+            if (best == 0 || v < best) best = v;
+            if (p35 >= target) break;
+        }
+        if (p5 >= target) break;
+    }
+    return best;
+}
+
+s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], FftPlan **out) {
+    auto key = std::make_tuple(dims[0] | ((uint64_t)R[0] << 32), dims[1] | ((uint64_t)R[1] << 32),
+                               dims[2] | ((uint64_t)R[2] << 32));
+    auto it = ctx->fft_plans.find(key);
+    if (it != ctx->fft_plans.end()) {
+        *out = it->second;
+        return S2H_OK;
+    }
+    auto *p = new FftPlan();
+    for (int a = 0; a < 3; ++a) {
+        p->dims[a] = dims[a];
+        p->f[a] = smooth_length(2 * dims[a] - 1);
+        p->R[a] = R[a];
+        p->ax[a] = axis_plan((int)p->f[a]);
+        if (p->ax[a].nf < 0 || p->f[a] > 8192) {
+            delete p;
+            return fail(S2H_ERR_UNSUPPORTED, "FFT length " + std::to_string(p->f[a]) + " unsupported");
+        }
+    }
+    auto cleanup = [&](s2h_status st) {
+        fft_plan_release(ctx, p);
+        return st;
+    };
+    for (int a = 0; a < 3; ++a) {
+        const int L = (int)p->f[a];
+        std::vector<float2> tw(L);
+        for (int k = 0; k < L; ++k) {
+            const double ang = -2.0 * M_PI * (double)k / (double)L;
+            tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        s2h_status st = upload(ctx, p->tw[a], tw.data(), (size_t)L * sizeof(float2));
+        if (st != S2H_OK) return cleanup(st);
+    }
+    {   // natural frequency held at each DIF output position (z axis)
+        const AxisPlan &az = p->ax[2];
+        std::vector<int> rev(az.L);
+        for (int pos = 0; pos < az.L; ++pos) {
+            int k = 0, mult = 1, m = az.L, q = pos;
+            for (int s = 0; s < az.nf; ++s) {
+                const int mp = m / az.fac[s];
+                k += (q / mp) * mult;
+                q %= mp;
+                mult *= az.fac[s];
+                m = mp;
+            }
+            rev[pos] = k;
+        }
+        std::vector<int> posv(az.L);
+        for (int pos = 0; pos < az.L; ++pos) posv[rev[pos]] = pos;
+        s2h_status st = upload(ctx, p->rev_z, rev.data(), rev.size() * sizeof(int));
+        if (st == S2H_OK) st = upload(ctx, p->pos_z, posv.data(), posv.size() * sizeof(int));
+        if (st != S2H_OK) return cleanup(st);
+    }
+    const uint64_t H = p->f[2] / 2 + 1;
+    const uint64_t nxo = 2 * (uint64_t)R[0] + 1, nyo = 2 * (uint64_t)R[1] + 1, nzo = 2 * (uint64_t)R[2] + 1;
+    s2h_status st;
+    if ((st = dev_alloc(ctx, p->g1, dims[0] * p->f[1] * H * 8)) != S2H_OK ||
+        (st = dev_alloc(ctx, p->g2, nxo * p->f[1] * H * 8)) != S2H_OK ||
+        (st = dev_alloc(ctx, p->g3, nxo * nyo * H * 8)) != S2H_OK ||
+        (st = dev_alloc(ctx, p->cube, nxo * nyo * nzo * 4)) != S2H_OK)
+        return cleanup(st);
+    ctx->fft_plans[key] = p;
+    *out = p;
+    return S2H_OK;
+}
+
+size_t lds_bytes(int L, int lines, int stride) { return ((size_t)L + (size_t)lines * stride) * sizeof(float2); }
+
+s2h_status set_lds(const void *fn, size_t bytes) {
+    if (bytes > 160 * 1024) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
+    S2H_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return S2H_OK;
+}
+
+// Run every pass; leaves the correlation cube in p->cube.
+s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
+    const int nx = (int)vol->dims[0], ny = (int)vol->dims[1], nz = (int)vol->dims[2];
+    const int fx = (int)p->f[0], fy = (int)p->f[1], fz = (int)p->f[2];
+    const int H = fz / 2 + 1;
+    const int Rx = p->R[0], Ry = p->R[1], Rz = p->R[2];
+    const int nxo = 2 * Rx + 1, nyo = 2 * Ry + 1;
+    const Fac fcx = to_fac(p->ax[0]), fcy = to_fac(p->ax[1]), fcz = to_fac(p->ax[2]);
+    hipStream_t s = ctx->stream;
+    const int lmax = std::max(fx, fy);
+    int tile = kTileMax;
+    while (tile > 1 && lds_bytes(lmax, tile, lmax + 1) > 150 * 1024) tile /= 2;
+    const int tiles = (H + tile - 1) / tile;
+    // z forward
+    {
+        const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * 8))));
+        const size_t lds = lds_bytes(fz, ppb, fz);
+        S2H_TRY(set_lds((const void *)fft_z_fwd, lds));
+        const int npairs = nx * ((ny + 1) / 2);
+        fft_z_fwd<<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
+            vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, fcz, fz, (const float2 *)p->tw[2].ptr,
+            (const int *)p->pos_z.ptr, ppb, (float2 *)p->g1.ptr);
+        S2H_HIP(hipGetLastError());
+        ++ctx->launches;
+    }
+    {
+        const size_t lds = lds_bytes(fy, tile, fy + 1);
+        S2H_TRY(set_lds((const void *)fft_y_fwd, lds));
+        fft_y_fwd<<<nx * tiles, kFftThreads, lds, s>>>(ny, fy, H, tiles, tile, fcy, (const float2 *)p->tw[1].ptr,
+                                                       (float2 *)p->g1.ptr);
+        S2H_HIP(hipGetLastError());
+        ++ctx->launches;
+    }
+    {
+        const size_t lds = lds_bytes(fx, tile, fx + 1);
+        S2H_TRY(set_lds((const void *)fft_x_power, lds));
+        fft_x_power<<<fy * tiles, kFftThreads, lds, s>>>(nx, fy, H, tiles, tile, fcx, (const float2 *)p->tw[0].ptr, Rx,
+                                                         (const float2 *)p->g1.ptr, (float2 *)p->g2.ptr);
+        S2H_HIP(hipGetLastError());
+        ++ctx->launches;
+    }
+    {
+        const size_t lds = lds_bytes(fy, tile, fy + 1);
+        S2H_TRY(set_lds((const void *)fft_y_inv, lds));
+        fft_y_inv<<<nxo * tiles, kFftThreads, lds, s>>>(fy, H, tiles, tile, fcy, (const float2 *)p->tw[1].ptr, Ry,
+                                                        (const float2 *)p->g2.ptr, (float2 *)p->g3.ptr);
+        S2H_HIP(hipGetLastError());
+        ++ctx->launches;
+    }
+    {
+        const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * 8))));
+        const size_t lds = lds_bytes(fz, ppb, fz);
+        S2H_TRY(set_lds((const void *)fft_z_inv, lds));
+        const int nlines = nxo * nyo;
+        const int npairs = (nlines + 1) / 2;
+        const float scale = (float)(1.0 / ((double)fx * (double)fy * (double)fz));
+        fft_z_inv<<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
+            nlines, H, fcz, fz, (const float2 *)p->tw[2].ptr, (const int *)p->rev_z.ptr, ppb, Rz, scale,
+            (const float2 *)p->g3.ptr, (float *)p->cube.ptr);
+        S2H_HIP(hipGetLastError());
+        ++ctx->launches;
+    }
+    return S2H_OK;
+}
+
+// FP32 counts of every shell offset into p->fcounts (device) using the exact plan's offset list.
+s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, ExactPlan **ep_out,
+                             FftPlan **fp_out) {
+    ExactPlan *ep = nullptr;
+    S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pitch, &ep));
+    int R[3] = {0, 0, 0};
+    for (const Offset &o : ep->offs) {
+        R[0] = std::max(R[0], std::abs(o.dx));
+        R[1] = std::max(R[1], std::abs(o.dy));
+        R[2] = std::max(R[2], std::abs(o.dz));
+    }
+    FftPlan *fp = nullptr;
+    S2H_TRY(fft_plan_get(ctx, vol->dims, R, &fp));
+    const uint64_t K = ep->offs.size();
+    S2H_TRY(dev_alloc(ctx, fp->fcounts, std::max<uint64_t>(K, 1) * 4));
+    prof_mark(ctx, 0);
+    S2H_TRY(fft_run(ctx, vol, fp));
+    if (K) {
+        fft_gather<<<(unsigned)((K + 255) / 256), 256, 0, ctx->stream>>>(
+            (const Offset *)ep->d_offs.ptr, K, (const float *)fp->cube.ptr, R[0], R[1], R[2],
+            (float *)fp->fcounts.ptr);
+        S2H_HIP(hipGetLastError());
+    }
+    *ep_out = ep;
+    *fp_out = fp;
+    return S2H_OK;
+}
+
+}  // namespace
 }  // namespace s2h
 
 using namespace s2h;
 
 extern "C" {
-s2h_status s2h_fft32_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t, double, float *, uint64_t) {
-    if (!ctx || !vol) return fail(S2H_ERR_ARG, "null argument");
-    return fail(S2H_ERR_UNSUPPORTED, "FP32 FFT path not built yet");
+
+s2h_status s2h_fft32_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, float *counts,
+                            uint64_t n_offsets) {
+    if (!ctx || !vol || (!counts && n_offsets)) return fail(S2H_ERR_ARG, "null argument");
+    if (!(pitch > 0.0) || !std::isfinite(pitch)) return fail(S2H_ERR_ARG, "pitch must be finite and > 0");
+    S2H_HIP(hipSetDevice(ctx->device));
+    ExactPlan *ep = nullptr;
+    S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pitch, &ep));
+    if (ep->offs.size() != n_offsets)
+        return fail(S2H_ERR_SIZE, "counts buffer holds " + std::to_string(n_offsets) + " entries, plan has " +
+                                      std::to_string(ep->offs.size()));
+    FftPlan *fp = nullptr;
+    S2H_TRY(fft_counts_device(ctx, vol, r_max, pitch, &ep, &fp));
+    prof_mark(ctx, 1);
+    prof_mark(ctx, 2);
+    if (n_offsets)
+        S2H_HIP(hipMemcpyAsync(counts, fp->fcounts.ptr, n_offsets * 4, hipMemcpyDeviceToHost, ctx->stream));
+    S2H_HIP(hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    return S2H_OK;
 }
-s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t, double, double *, uint64_t) {
-    if (!ctx || !vol) return fail(S2H_ERR_ARG, "null argument");
-    return fail(S2H_ERR_UNSUPPORTED, "FP32 FFT path not built yet");
+
+s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, double *s2, uint64_t n) {
+    if (!ctx || !vol || !s2) return fail(S2H_ERR_ARG, "null argument");
+    if (n != (uint64_t)r_max + 1) return fail(S2H_ERR_SIZE, "s2 buffer must hold r_max+1 values");
+    S2H_HIP(hipSetDevice(ctx->device));
+    uint64_t occupied = 0;
+    S2H_TRY(vol_popcount(ctx, vol, &occupied));
+    if (occupied == 0) {
+        std::fill(s2, s2 + n, 0.0);
+        return S2H_OK;
+    }
+    const double pm = pitch < 1e-9 ? 1e-9 : pitch;
+    ExactPlan *ep = nullptr;
+    FftPlan *fp = nullptr;
+    S2H_TRY(fft_counts_device(ctx, vol, r_max, pm, &ep, &fp));
+    S2H_TRY(exact_gather_fold(ctx, ep, occupied, (const float *)fp->fcounts.ptr));
+    const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
+    return finish_curve_from_device(ctx, ep, vf, s2);
 }
-}
+
+}  // extern "C"

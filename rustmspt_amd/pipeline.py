@@ -1,0 +1,204 @@
+"""`measure` pipeline on the HIP engine (src/pipeline/measure.rs:49-291).
+
+Same inputs (YAML), bbox precedence, grid normalisation, working-set gate,
+per-method backend resolution with the reference's fallback policy, report file
+format and [Timing] lines.  The exact / Monte Carlo curves come from
+libs2hip.so.  A method that resolves to the CPU backend belongs to the host
+crate's own CPU implementation; this engine does not contain one, so such a
+resolution raises CpuPathUnavailable instead of computing anything.
+"""
+from __future__ import annotations
+
+import math
+import os
+import resource
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+from . import config as cfg
+from . import s2
+from .mesh import BoundingBox, granules, load_stl_or_merge_folder, mesh_bbox
+
+DEFAULT_CPU_EXACT_BUDGET_BYTES = 768 * 1024 * 1024
+
+
+class CpuPathUnavailable(RuntimeError):
+    """The policy chose the CPU backend, which lives in the host crate."""
+
+
+class StageTimer:
+    """src/pipeline/timing.rs:3-81: `[Timing] <pipeline> stage=<name> seconds=<s>`."""
+
+    def __init__(self, pipeline: str, out=sys.stdout):
+        self.pipeline = pipeline
+        self.out = out
+        self.t0 = self.last = time.perf_counter()
+
+    def restart(self):
+        self.last = time.perf_counter()
+
+    def stage(self, name: str):
+        now = time.perf_counter()
+        print(f"[Timing] {self.pipeline} stage={name} seconds={now - self.last:.6f}", file=self.out)
+        self.last = now
+
+    def total(self, name: str):
+        print(f"[Timing] {self.pipeline} stage={name} seconds={time.perf_counter() - self.t0:.6f}", file=self.out)
+
+    def report_resources(self):
+        rss = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss * 1024
+        print(f"[Timing] {self.pipeline} peak_rss_bytes={rss}", file=self.out)
+
+
+def l2_error(a, b) -> float:
+    """src/pipeline/measure.rs:27-39."""
+    n = min(len(a), len(b))
+    if n == 0:
+        return 0.0
+    return math.sqrt(sum((float(a[i]) - float(b[i])) ** 2 for i in range(n)))
+
+
+def volume_fraction_in_bbox(mesh, bbox: BoundingBox, occupancy_vf: float):
+    """src/geometry/volume.rs:437-469 for granules wholly inside the box
+    (sum of |signed volume| per granule); a granule crossing the box would need
+    the reference's clipping, which is outside this engine -- the occupancy VF
+    is reported instead, with a warning."""
+    v = mesh.vertices
+    inside = np.all((v >= np.asarray(bbox.min)) & (v <= np.asarray(bbox.max)), axis=1)
+    box_vol = max(np.prod(np.maximum(np.asarray(bbox.max) - np.asarray(bbox.min), 0.0)), 1e-12)
+    if len(mesh.faces) == 0:
+        return 0.0
+    if not np.all(inside[mesh.faces.reshape(-1)]):
+        print("[Warning] particles cross the measurement box; Volume Fraction reports the occupancy fraction")
+        return occupancy_vf
+    gid, n = granules(mesh)
+    a, b, c = (v[mesh.faces[:, k]] for k in range(3))
+    cross = np.stack([b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1], b[:, 2] * c[:, 0] - b[:, 0] * c[:, 2],
+                      b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0]], 1)
+    tet = (a[:, 0] * cross[:, 0] + a[:, 1] * cross[:, 1] + a[:, 2] * cross[:, 2]) / 6.0
+    total = 0.0
+    for g in range(n):
+        total += abs(float(np.sum(tet[gid == g])))
+    return min(max(total / box_vol, 0.0), 1.0)
+
+
+class MeasurePipeline:
+    def __init__(self, params: cfg.MeasurementParams, env=None, out=sys.stdout):
+        self.p = params
+        self.env = os.environ if env is None else env
+        self.out = out
+
+    def log(self, msg):
+        print(msg, file=self.out)
+
+    def run(self):
+        p = self.p
+        if not math.isfinite(p.voxel_pitch):
+            raise cfg.ConfigError("measurement requires finite pitch and representable radius count")
+        requested = cfg.configured_mode(p.acceleration, self.env)
+        timer = StageTimer("measure", self.out)
+        mesh = load_stl_or_merge_folder(p.stl_path)
+        timer.stage("load")
+        bbox = None
+        for box in (p.bounding_box, p.stl_bounding_box):
+            if box:
+                bbox = cfg.parse_box_dimensions(box)
+                break
+        if bbox is None:
+            bbox = mesh_bbox(mesh) or BoundingBox.from_size((1.0, 1.0, 1.0))
+        self.log(f"[Info] STL file(s) loaded from: {p.stl_path}")
+        self.log("[Info] Measurement bbox: min=({:.4f},{:.4f},{:.4f}), max=({:.4f},{:.4f},{:.4f})".format(
+            *bbox.min, *bbox.max))
+        _, particle_count = granules(mesh)
+        raw = p.mc_method.strip().lower()
+        method = "exact" if raw == "exact" else ("both" if raw == "both" else ("fft32" if raw == "fft32" else
+                                                                              "monte_carlo"))
+        samples = p.mc_samples if p.mc_samples is not None else 10_000
+        self.log(f"[Info] S2 config: method={method}, r_max={p.r_max}, mc_samples={samples}, "
+                 f"voxel_pitch={p.voxel_pitch:.6f}")
+        pitch = 1.0 if p.voxel_pitch <= 0.0 else p.voxel_pitch
+        dims, voxel_count = [], 1
+        for a in range(3):
+            extent = bbox.max[a] - bbox.min[a]
+            n = max(math.ceil(extent / pitch), 1.0) if math.isfinite(extent / pitch) else float("inf")
+            if not math.isfinite(extent) or extent <= 0.0 or not math.isfinite(n):
+                raise cfg.ConfigError("measurement grid dimensions are unsupported")
+            dims.append(int(n))
+            voxel_count *= int(n)
+        methods = ["exact", "monte_carlo"] if method == "both" else [method]
+        curves, backends, grid = [], [], None
+        ctx = None
+        timer.restart()
+        try:
+            for m in methods:
+                exactish = m in ("exact", "fft32")
+                est = cfg.hip_exact_bytes(dims, p.r_max, pitch, len(mesh.faces), fft=(m == "fft32")) if exactish \
+                    else dims[0] * dims[1] * s2.row_words(dims[2]) * 8 + len(mesh.faces) * 136 + 64 * (p.r_max + 2)
+                sel = cfg.resolve_execution(p.acceleration, requested, voxel_count, p.acceleration.gpu_min_voxels,
+                                            p.voxel_pitch > 0.0 or exactish, est)
+                if sel.fallback_reason:
+                    print(f"[Info] {m}: {sel.fallback_reason}", file=sys.stderr)
+                if sel.backend != "hip":
+                    raise CpuPathUnavailable(
+                        f"measure {m}: the policy selected the CPU backend, which is the host crate's own "
+                        f"implementation and not part of this engine")
+                if ctx is None:
+                    budget = (p.acceleration.gpu_memory_limit_mb or 0) * 1024 * 1024
+                    ctx = s2.Context(0, budget)
+                if grid is None:
+                    self.log("[Info] Preparing shared HIP S2 occupancy grid")
+                    grid = s2.VoxelS2Hip(ctx, mesh, bbox, pitch)
+                seed = int(self.env.get("RUSTMSPT_S2_SEED", "0"))
+                curve = grid.calculate_seeded(p.r_max, m, samples, seed)
+                backends.append(f"{m}=hip/{ctx.device_name}")
+                self.log(f"[Info] Measure execution: method={m}, backend=hip, device={ctx.device_name}")
+                curves.append(curve)
+                timer.stage(f"s2_{m}")
+            occ_vf = grid.volume.occupied / float(np.prod(grid.dims)) if grid else 0.0
+            vf = volume_fraction_in_bbox(mesh, bbox, occ_vf)
+        finally:
+            if grid is not None:
+                grid.volume.free()
+            if ctx is not None:
+                ctx.close()
+        lines = [f"Volume Fraction: {vf:.6f}", f"Compute Backend: {', '.join(backends)}", f"Method: {method}"]
+        for m, curve in zip(methods, curves):
+            lines.append(f"S2(0)-VF diff [{m}]: {abs(curve[0] - vf):.6f}")
+            lines.append("S2 Values:" if len(methods) == 1 else f"S2 Values [{m}]:")
+            lines += [f"{r}: {v:.6f}" for r, v in enumerate(curve)]
+        if len(curves) == 2:
+            lines.append(f"L2 error [exact vs monte_carlo]: {l2_error(curves[0], curves[1]):.6f}")
+        out = Path(p.output_path)
+        out.parent.mkdir(parents=True, exist_ok=True)
+        out.write_text("\n".join(lines) + "\n")
+        timer.stage("write")
+        self.log(f"[Info] Measurement completed: {particle_count} particles, VF {vf:.6f}; output {p.output_path}")
+        timer.total("total_in_pool")
+        timer.report_resources()
+        return curves
+
+
+def main(argv=None):
+    """`python -m rustmspt_amd.pipeline measure --config cfg.yaml [--input X] [--output Y]`
+    (src/main.rs:224-253)."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="rustmspt-hip")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    m = sub.add_parser("measure")
+    m.add_argument("--config", required=True)
+    m.add_argument("--input")
+    m.add_argument("--output")
+    args = ap.parse_args(argv)
+    params = cfg.load_measurement_config(args.config)
+    if args.input:
+        params.stl_path = args.input
+    if args.output:
+        params.output_path = args.output
+    MeasurePipeline(params).run()
+
+
+if __name__ == "__main__":
+    main()

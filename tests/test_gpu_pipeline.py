@@ -1,0 +1,62 @@
+"""measure pipeline end to end on the GPU: the reference's integration contracts
+(tests/measure_execution_tests.rs) with acceleration.mode hip."""
+import json
+
+import numpy as np
+import pytest
+
+from oracle import oracle as o
+from rustmspt_amd import config as cfg
+from rustmspt_amd import mesh as M
+from rustmspt_amd import pipeline, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def run(tmp_path, m, env=None, **over):
+    doc = {"measurement": {"stl_path": str(tmp_path / "in.stl"), "output_path": str(tmp_path / "out.txt"),
+                           "r_max": 2, "voxel_pitch": 0.0, "mc_method": "exact", "mc_samples": 200,
+                           "bounding_box": [4, 4, 4], "acceleration": {"mode": "hip"}}}
+    doc["measurement"].update(over)
+    M.save_stl(tmp_path / "in.stl", m)
+    (tmp_path / "c.json").write_text(json.dumps(doc))
+    params = cfg.load_measurement_config(tmp_path / "c.json")
+    pipeline.MeasurePipeline(params, env=env or {}).run()
+    return (tmp_path / "out.txt").read_text()
+
+
+def test_single_voxel_cube_report(tmp_path):
+    text = run(tmp_path, M.box_mesh([0.2, 0.2, 0.2], [1.4, 1.4, 1.4]))
+    assert "exact=hip/" in text
+    for r, v in enumerate([1 / 64, 0.0, 0.0]):
+        assert f"{r}: {v:.6f}" in text
+    assert "Method: exact" in text and text.startswith("Volume Fraction: ")
+
+
+def test_policy_paths(tmp_path):
+    cube = M.box_mesh([0.2, 0.2, 0.2], [1.4, 1.4, 1.4])
+    with pytest.raises(pipeline.CpuPathUnavailable):
+        run(tmp_path, cube, env={"RUSTMSPT_ACCELERATION": "cpu"})
+    with pytest.raises(cfg.ConfigError):
+        run(tmp_path, cube, env={"RUSTMSPT_ACCELERATION": "invalid"})
+    with pytest.raises(cfg.GpuError):
+        run(tmp_path, cube, acceleration={"mode": "hip", "cpu_fallback": False, "gpu_memory_limit_mb": 0})
+
+
+def test_solid_box_all_ones(tmp_path):
+    text = run(tmp_path, M.box_mesh([0, 0, 0], [32, 32, 32]), r_max=16, voxel_pitch=1.0, bounding_box=[32, 32, 32],
+               acceleration={"mode": "hip", "cpu_fallback": False, "gpu_memory_limit_mb": 64})
+    for r in range(17):
+        assert f"{r}: 1.000000" in text
+
+
+def test_both_methods_match_oracle(tmp_path):
+    mesh, bbox, _ = synth.config_plain(seed=5, n=8, grid=40)
+    text = run(tmp_path, mesh, r_max=8, voxel_pitch=1.0, bounding_box=[40, 40, 40], mc_method="both",
+               mc_samples=100000)
+    occ = o.voxelize(mesh.vertices, mesh.faces, [0, 0, 0], 1.0, [40, 40, 40])
+    want = o.exact_s2(occ, 8)
+    block = text.split("S2 Values [exact]:\n")[1].split("S2(0)-VF")[0]
+    for r, v in enumerate(want):
+        assert f"{r}: {v:.6f}" in block
+    assert "L2 error [exact vs monte_carlo]:" in text and "monte_carlo=hip/" in text
