@@ -60,6 +60,59 @@ def cpu_baseline(occ_words, dims, r_max, offs, pair_work_total, target_s):
     }
 
 
+def batched_leg(ctx, ncand: int, seed: int, plain_sample: int):
+    """256 one-move candidates over 4 resident seed packings (grids 96, 128, 192,
+    256; 150-400 ellipsoids each; r_max = grid/4), evaluated in one batch, vs the
+    plain per-call path (s2h_exact_s2 on each candidate's resident volume; the
+    candidate volumes are voxelized outside the timed region)."""
+    import numpy as np
+
+    from rustmspt_amd import batch, s2, synth
+
+    rng = synth.rng_for(seed, 77)
+    seeds, parts = [], []
+    for g in (96, 128, 192, 256):
+        n = int(rng.integers(150, 401))
+        r_eq = (0.2 * g ** 3 / n * 3 / (4 * np.pi)) ** (1 / 3)  # VF ~ 0.2
+        ps = synth.place_particles(n, g, rng, synth.ellipsoid_maker(0.8 * r_eq, 1.2 * r_eq), overlap=True)
+        seeds.append(batch.Seed(ctx, [p.mesh() for p in ps], s2.BoundingBox.from_size((g, g, g)), 1.0, g // 4))
+        parts.append(ps)
+    moves = []
+    for _ in range(ncand):
+        si = int(rng.integers(0, 4))
+        pi = int(rng.integers(0, len(parts[si])))
+        moves.append((si, pi, parts[si][pi].moved(rng.uniform(-2, 2, 3), synth.random_rotation(rng)).mesh().vertices))
+    batch.batch_eval(ctx, seeds, moves[:8])  # warm-up (plans, code objects)
+    t0 = time.perf_counter()
+    curves = batch.batch_eval(ctx, seeds, moves)
+    bt = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    # plain path on a sample of the same candidates
+    idx = list(range(0, ncand, max(1, ncand // plain_sample)))[:plain_sample]
+    vols = [s2.voxelize(ctx, seeds[moves[i][0]].moved_mesh(moves[i][1], moves[i][2]), seeds[moves[i][0]].bbox, 1.0)
+            for i in idx]
+    vols[0].exact_s2(seeds[moves[idx[0]][0]].r_max)
+    plain_t = 0.0
+    mism = 0
+    for k, i in enumerate(idx):
+        sd = seeds[moves[i][0]]
+        t0 = time.perf_counter()
+        c = vols[k].exact_s2(sd.r_max)
+        plain_t += time.perf_counter() - t0
+        mism += int(not np.array_equal(c, curves[i]))
+        vols[k].free()
+    plain_rate = len(idx) / plain_t
+    # plain time per grid class scales with the candidate mix; weight by the sample
+    out = {"candidates": ncand, "batched_s": bt, "batched_evals_per_s": ncand / bt,
+           "plain_evals_per_s": plain_rate, "plain_sample": len(idx),
+           "batched_over_plain": (ncand / bt) / plain_rate, "bit_exact_mismatches": mism,
+           "batched_device_ms": prof["device_ms"], "grids": [96, 128, 192, 256],
+           "particles": [len(p) for p in parts]}
+    for sd in seeds:
+        sd.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +123,8 @@ def main():
     ap.add_argument("--particles", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-candidates", type=int, default=256)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration for profiling)")
     args = ap.parse_args()
@@ -147,6 +202,8 @@ def main():
                      "algorithmic_ops_per_eval": alg_ops},
         "s2_head": [round(float(v), 6) for v in curve[:5]],
     }
+    if not args.no_batch:
+        out["batched"] = batched_leg(ctx, args.batch_candidates, seed=3, plain_sample=32)
     if not args.no_cpu:
         words = vol.read_packed()
         out["cpu_baseline"] = cpu_baseline(words, [grid] * 3, r_max, offs, pair_work, args.cpu_seconds)

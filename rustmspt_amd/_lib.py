@@ -49,10 +49,21 @@ SIGNATURES = {
     "s2h_mc_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _u64p, C.c_uint64]),
     "s2h_mc_s2": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _f64p, C.c_uint64]),
     "s2h_fill_missing": (C.c_int32, [_f64p, _u8p, C.c_uint64, C.c_double]),
+    "s2h_seed_create": (C.c_int32, [_vp, _f64p, C.c_uint64, _u32p, C.c_uint64, _u32p, C.c_uint32, _f64p,
+                                    C.c_double, _u64p, C.c_uint32, C.POINTER(_vp)]),
+    "s2h_seed_free": (None, [_vp, _vp]),
+    "s2h_seed_volume": (C.c_int32, [_vp, C.POINTER(_vp)]),
+    "s2h_batch_eval": (C.c_int32, [_vp, C.POINTER(_vp), C.c_uint32, _vp, C.c_uint32, _f64p, C.c_uint64]),
     "s2h_stl_load": (C.c_int32, [C.c_char_p, C.POINTER(_f64p), _u64p, C.POINTER(_u32p), _u64p]),
     "s2h_free": (None, [_vp]),
     "s2h_mesh_granules": (C.c_int32, [_u32p, C.c_uint64, C.c_uint64, _u32p, _u32p]),
 }
+
+
+class Candidate(C.Structure):
+    """s2h_candidate (include/s2hip.h)."""
+    _fields_ = [("seed", C.c_uint32), ("ntris", C.c_uint32), ("ngranules", C.c_uint32), ("reserved", C.c_uint32),
+                ("old_tri", _f64p), ("new_tri", _f64p), ("granule", _u32p)]
 
 
 class S2HipError(RuntimeError):

@@ -30,6 +30,7 @@ UNITS = {
     "s2hip_voxel.hip": ["-ffp-contract=off"],
     "s2hip_mc.hip": ["-ffp-contract=off"],
     "s2hip_fft.hip": ["-ffp-contract=fast"],
+    "s2hip_batch.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = {"s2hip_host.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]}
 

@@ -1,0 +1,238 @@
+// Device-side pieces of the bit-exact voxel predicate shared by the full
+// voxelizer and the batched per-particle re-voxelization (see s2hip_voxel.hip
+// for the algorithm and its reference citations).
+#pragma once
+#include "s2hip_internal.h"
+
+#include <vector>
+
+namespace s2h {
+namespace vox {
+
+constexpr double kRayX = 0.9428090415820634, kRayY = 0.2705980500730985, kRayZ = 0.19611613513818402;
+constexpr int kTileX = 4, kTileY = 8, kTileZ = 8;  // 256 voxel centres per workgroup
+constexpr int kMaxHits = 16;
+constexpr int kListCap = 2048;  // culled triangle indices per pass
+
+struct TriGeo {
+    double ax, ay, az;     // vertex a
+    double e1x, e1y, e1z;  // b - a
+    double e2x, e2y, e2z;  // c - a
+    double hx, hy, hz;     // dir x e2
+    double inv;            // 1 / (e1 . h)
+    double umin, umax, vmin, vmax;  // projected bound incl. margin (infinite: never culled)
+};
+
+struct GranuleInfo {
+    double mn[3], mx[3];   // granule bbox (over referenced vertices)
+    uint32_t tri_start, tri_end;
+    uint32_t owner;        // batch path: candidate index (unused by the full voxelizer)
+    int32_t sign;          // batch path: +1 new position, -1 old position
+};
+
+struct Tile {
+    uint32_t granule;
+    uint16_t x0, y0, z0;   // tile origin (voxel index)
+    uint16_t x1, y1, z1;   // exclusive end, clipped to the granule range
+};
+
+// Projection basis normal to the ray direction: u = normalize(dir x e_z),
+// v = dir x u (|dir . u|, |dir . v| < 1e-16; the cull margin covers it).
+constexpr double kUx = 0.27587465517742293, kUy = -0.9611936197409646, kUz = 0.0;
+constexpr double kVx = 0.18850557782307928, kVy = 0.05410347115597539, kVz = -0.9808731791583722;
+
+// Moller-Trumbore in the reference's operation order on precomputed
+// e1, e2, h = dir x e2 and inv = 1/det (identical values to computing them per
+// test).  Returns t > 0 on a hit, -1 on a miss.
+__device__ __forceinline__ double mt_hit(const TriGeo &g, double px, double py, double pz) {
+    const double eps = 1e-10;
+    const double sx = px - g.ax, sy = py - g.ay, sz = pz - g.az;
+    const double u = g.inv * (sx * g.hx + sy * g.hy + sz * g.hz);
+    if (!(u >= 0.0 - eps && u <= 1.0 + eps)) return -1.0;
+    const double qx = sy * g.e1z - sz * g.e1y;
+    const double qy = sz * g.e1x - sx * g.e1z;
+    const double qz = sx * g.e1y - sy * g.e1x;
+    const double v = g.inv * (kRayX * qx + kRayY * qy + kRayZ * qz);
+    if (v < -eps || (u + v) > 1.0 + eps) return -1.0;
+    const double t = g.inv * (g.e2x * qx + g.e2y * qy + g.e2z * qz);
+    return t > eps ? t : -1.0;
+}
+
+__device__ __forceinline__ bool outside_bbox(const GranuleInfo &gi, double px, double py, double pz) {
+    const double eps = 1e-9;
+    return px < gi.mn[0] - eps || px > gi.mx[0] + eps || py < gi.mn[1] - eps || py > gi.mx[1] + eps ||
+           pz < gi.mn[2] - eps || pz > gi.mx[2] + eps;
+}
+
+__device__ __forceinline__ void set_voxel(unsigned long long *words, uint64_t ny, uint64_t row_words, uint32_t x,
+                                          uint32_t y, uint32_t z) {
+    atomicOr(&words[((uint64_t)x * ny + y) * row_words + (z >> 6)], 1ull << (z & 63));
+}
+
+
+// One workgroup (256 threads) evaluates the parity predicate at the centres of
+// one tile; out(gi, x, y, z) receives every centre found inside.  Rays with more
+// than kMaxHits hits are appended (tile_id, thread) to `overflow`.
+template <class Out>
+__device__ void voxelize_tile(const TriGeo *__restrict__ tris, const GranuleInfo *__restrict__ gran, const Tile tl,
+                              uint32_t tile_id, double bx, double by, double bz, double pitch,
+                              uint32_t *__restrict__ overflow, uint32_t overflow_cap, Out out) {
+    __shared__ uint32_t list[kListCap];
+    __shared__ uint32_t list_n;
+    const GranuleInfo gi = gran[tl.granule];
+    const int tx = threadIdx.x / (kTileY * kTileZ), ty = (threadIdx.x / kTileZ) % kTileY, tz = threadIdx.x % kTileZ;
+    const uint32_t x = tl.x0 + tx, y = tl.y0 + ty, z = tl.z0 + tz;
+    const bool active = x < tl.x1 && y < tl.y1 && z < tl.z1;
+    // voxel centre, src/geometry/s2.rs:150-152
+    const double px = bx + ((double)x + 0.5) * pitch;
+    const double py = by + ((double)y + 0.5) * pitch;
+    const double pz = bz + ((double)z + 0.5) * pitch;
+    const bool live = active && !outside_bbox(gi, px, py, pz);
+    const double pu = kUx * px + kUy * py + kUz * pz;
+    const double pv = kVx * px + kVy * py + kVz * pz;
+    // projected bound of the tile's centres (exact up to rounding; the per-
+    // triangle margin covers it)
+    double tu0 = live ? pu : INFINITY, tu1 = live ? pu : -INFINITY;
+    double tv0 = live ? pv : INFINITY, tv1 = live ? pv : -INFINITY;
+    for (int o = 32; o > 0; o >>= 1) {
+        tu0 = fmin(tu0, __shfl_xor(tu0, o, 64));
+        tu1 = fmax(tu1, __shfl_xor(tu1, o, 64));
+        tv0 = fmin(tv0, __shfl_xor(tv0, o, 64));
+        tv1 = fmax(tv1, __shfl_xor(tv1, o, 64));
+    }
+    __shared__ double wb[4][4];
+    const int wid = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) {
+        wb[wid][0] = tu0;
+        wb[wid][1] = tu1;
+        wb[wid][2] = tv0;
+        wb[wid][3] = tv1;
+    }
+    __syncthreads();
+    tu0 = fmin(fmin(wb[0][0], wb[1][0]), fmin(wb[2][0], wb[3][0]));
+    tu1 = fmax(fmax(wb[0][1], wb[1][1]), fmax(wb[2][1], wb[3][1]));
+    tv0 = fmin(fmin(wb[0][2], wb[1][2]), fmin(wb[2][2], wb[3][2]));
+    tv1 = fmax(fmax(wb[0][3], wb[1][3]), fmax(wb[2][3], wb[3][3]));
+    if (!(tu0 <= tu1)) return;  // no live centre in this tile (uniform)
+
+    double hits[kMaxHits];
+    int nh = 0;
+    bool over = false;
+    for (uint32_t c0 = gi.tri_start; c0 < gi.tri_end; c0 += kListCap) {
+        const uint32_t c1 = min(gi.tri_end, c0 + (uint32_t)kListCap);
+        if (threadIdx.x == 0) list_n = 0;
+        __syncthreads();
+        for (uint32_t i = c0 + threadIdx.x; i < c1; i += blockDim.x) {
+            const TriGeo &g = tris[i];
+            if (!(g.umax < tu0 || g.umin > tu1 || g.vmax < tv0 || g.vmin > tv1)) {
+                const uint32_t k = atomicAdd(&list_n, 1u);
+                list[k] = i;
+            }
+        }
+        __syncthreads();
+        const uint32_t n = list_n;
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t i = __builtin_amdgcn_readfirstlane(list[k]);
+            const TriGeo g = tris[i];
+            if (!live || pu < g.umin || pu > g.umax || pv < g.vmin || pv > g.vmax) continue;
+            const double t = mt_hit(g, px, py, pz);
+            if (t < 0.0) continue;
+            if (nh >= kMaxHits) {
+                over = true;
+                continue;
+            }
+            // sorted insert with static register indices
+            double cur = t;
+#pragma unroll
+            for (int s = 0; s < kMaxHits; ++s) {
+                if (s < nh) {
+                    const double lo = hits[s] < cur ? hits[s] : cur;
+                    const double hi = hits[s] < cur ? cur : hits[s];
+                    hits[s] = lo;
+                    cur = hi;
+                } else if (s == nh) {
+                    hits[s] = cur;
+                }
+            }
+            ++nh;
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+    if (over) {
+        const uint32_t k = atomicAdd(&overflow[0], 1u);
+        if (k < overflow_cap) {
+            overflow[1 + 2 * k] = tile_id;
+            overflow[2 + 2 * k] = threadIdx.x;
+        }
+        return;
+    }
+    // anchored dedup, src/geometry/mesh_query.rs:105-114
+    int unique = 0;
+    double last = -INFINITY;
+#pragma unroll
+    for (int s = 0; s < kMaxHits; ++s) {
+        if (s < nh && fabs(hits[s] - last) > 1e-8) {
+            ++unique;
+            last = hits[s];
+        }
+    }
+    if (unique & 1) out(gi, x, y, z);
+}
+
+// Exact parity for rays with more than kMaxHits hits.  In the sorted hit list
+// the anchored rule keeps t_0 = min, then repeatedly the smallest t with
+// t - last > 1e-8; that is a repeated minimum over all triangles, needing no
+// storage.  One thread per overflowing ray.
+template <class Out>
+__device__ void voxel_overflow_ray(const TriGeo *__restrict__ tris, const GranuleInfo *__restrict__ gran,
+                                   const Tile *__restrict__ tiles, double bx, double by, double bz, double pitch,
+                                   const uint32_t *__restrict__ overflow, uint32_t k, Out out) {
+    const Tile tl = tiles[overflow[1 + 2 * k]];
+    const uint32_t lt = overflow[2 + 2 * k];
+    const GranuleInfo gi = gran[tl.granule];
+    const int tx = lt / (kTileY * kTileZ), ty = (lt / kTileZ) % kTileY, tz = lt % kTileZ;
+    const uint32_t x = tl.x0 + tx, y = tl.y0 + ty, z = tl.z0 + tz;
+    const double px = bx + ((double)x + 0.5) * pitch;
+    const double py = by + ((double)y + 0.5) * pitch;
+    const double pz = bz + ((double)z + 0.5) * pitch;
+    double last = -INFINITY;
+    int unique = 0;
+    while (true) {
+        double best = INFINITY;
+        bool found = false;
+        for (uint32_t i = gi.tri_start; i < gi.tri_end; ++i) {
+            const double t = mt_hit(tris[i], px, py, pz);
+            if (t < 0.0) continue;
+            if (t - last > 1e-8 && (!found || t < best)) {
+                best = t;
+                found = true;
+            }
+        }
+        if (!found) break;
+        ++unique;
+        last = best;
+    }
+    if (unique & 1) out(gi, x, y, z);
+}
+
+
+struct SetBit {
+    unsigned long long *words;
+    uint64_t ny, row_words;
+    __device__ void operator()(const GranuleInfo &, uint32_t x, uint32_t y, uint32_t z) const {
+        set_voxel(words, ny, row_words, x, y, z);
+    }
+};
+
+// Host-side preparation of one granule's triangles (precomputed edges, 1/det and
+// the conservative projected bound) and its voxel range.  Returns false when the
+// granule's range is empty.
+bool prepare_granule(const double *tri_xyz /* 9 per triangle */, uint32_t ntri, const double bbox_min[3],
+                     double pitch, const uint64_t dims[3], TriGeo *geo, GranuleInfo &gi, int64_t lo[3],
+                     int64_t hi[3]);
+// Tiles of kTileX x kTileY x kTileZ centres covering [lo, hi) for granule g.
+void push_tiles(uint32_t g, const int64_t lo[3], const int64_t hi[3], std::vector<Tile> &tiles);
+
+}  // namespace vox
+}  // namespace s2h
