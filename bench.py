@@ -113,6 +113,25 @@ def batched_leg(ctx, ncand: int, seed: int, plain_sample: int):
     return out
 
 
+def served_leg(rate: float, duration: float, pool_size: int, slots: int, budget_gb: float, seed: int):
+    """Poisson stream of mixed requests (grid 64^3-384^3 log-uniform, VF
+    U[0.1, 0.5], r_max U{16..96}, 70 % exact / 30 % FP32 FFT) through the serving
+    loop; p50/p99 latency from submission to completion, evals/s, admission."""
+    from rustmspt_amd import serve
+
+    pool = serve.volume_pool(pool_size, seed)
+    sched = serve.poisson_schedule(rate, duration, pool, seed)
+    with serve.Server(slots=slots, budget_bytes=int(budget_gb * (1 << 30))) as sv:
+        # warm-up: one small request per method
+        for m in (serve.EXACT, serve.FFT32):
+            t = sv.submit(pool[0].words, pool[0].dims, 16, m)
+            sv.poll(t, wait=True)
+        rep = serve.run_stream(sv, pool, sched, timeout=900)
+    rep.update({"rate": rate, "duration_s": duration, "pool_volumes": pool_size, "slots": slots,
+                "budget_gb": budget_gb, "grids": sorted({v.dims[0] for v in pool})})
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +144,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-candidates", type=int, default=256)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration for profiling)")
     args = ap.parse_args()
@@ -204,6 +227,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched_leg(ctx, args.batch_candidates, seed=3, plain_sample=32)
+    if not args.no_serve:
+        out["served"] = served_leg(args.serve_rate, args.serve_seconds, 24, args.serve_slots, 64.0, seed=4)
     if not args.no_cpu:
         words = vol.read_packed()
         out["cpu_baseline"] = cpu_baseline(words, [grid] * 3, r_max, offs, pair_work, args.cpu_seconds)

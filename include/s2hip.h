@@ -189,6 +189,44 @@ s2h_status s2h_batch_eval(s2h_ctx *ctx, s2h_seed *const *seeds, uint32_t nseeds,
                           const s2h_candidate *cands, uint32_t ncand, double *curves,
                           uint64_t ncurve);
 
+/* ------------------------------------------------------------------ serving */
+
+/* A request loop with admission by device-memory footprint (no counterpart in the
+ * reference, whose GPU objects serve one call at a time behind a Mutex,
+ * src/pipeline/optimize_execution.rs:274-288).  nslots worker contexts, each on
+ * its own stream, run admitted requests concurrently; a waiting request is
+ * admitted as soon as a slot is free and its footprint fits the budget, in
+ * arrival order (a request that does not fit is passed over and counted as
+ * deferred).  Volumes stay resident in HBM while their request runs. */
+typedef struct s2h_server s2h_server;
+typedef struct {
+    const uint64_t *words;  /* packed volume (copied at submit) */
+    uint64_t nwords;
+    uint64_t dims[3];
+    uint32_t r_max;
+    int32_t method;         /* 0 exact, 1 FP32 FFT */
+    double pitch;
+    uint64_t tag;           /* caller's id, echoed in the result */
+} s2h_request;
+typedef struct {
+    int32_t ready;          /* 0 while the request is waiting or running */
+    int32_t status;
+    uint32_t deferrals;     /* admission passes that skipped this request */
+    uint32_t reserved;
+    uint64_t tag;
+    uint64_t footprint;     /* device bytes reserved while admitted */
+    double t_submit, t_start, t_done;  /* seconds since the server started */
+} s2h_result;
+s2h_status s2h_serve_create(int device, uint64_t hbm_budget_bytes, uint32_t nslots, s2h_server **out);
+void s2h_serve_destroy(s2h_server *server);
+s2h_status s2h_serve_submit(s2h_server *server, const s2h_request *req, uint64_t *ticket);
+/* Result of a ticket (wait != 0 blocks until it is done).  Once a finished
+ * result has been returned its ticket is released. */
+s2h_status s2h_serve_poll(s2h_server *server, uint64_t ticket, int32_t wait, s2h_result *res,
+                          double *s2, uint64_t n);
+/* [admitted, deferred, completed, failed, waiting, bytes_in_use, peak_bytes, budget] */
+s2h_status s2h_serve_stats(s2h_server *server, uint64_t *stats, uint64_t n);
+
 /* --------------------------------------------------------------- host helpers */
 
 /* Fill unsupported radii (fill_missing_s2_with_smooth_interpolation,

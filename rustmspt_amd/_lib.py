@@ -49,6 +49,11 @@ SIGNATURES = {
     "s2h_mc_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _u64p, C.c_uint64]),
     "s2h_mc_s2": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _f64p, C.c_uint64]),
     "s2h_fill_missing": (C.c_int32, [_f64p, _u8p, C.c_uint64, C.c_double]),
+    "s2h_serve_create": (C.c_int32, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(_vp)]),
+    "s2h_serve_destroy": (None, [_vp]),
+    "s2h_serve_submit": (C.c_int32, [_vp, _vp, _u64p]),
+    "s2h_serve_poll": (C.c_int32, [_vp, C.c_uint64, C.c_int32, _vp, _f64p, C.c_uint64]),
+    "s2h_serve_stats": (C.c_int32, [_vp, _u64p, C.c_uint64]),
     "s2h_seed_create": (C.c_int32, [_vp, _f64p, C.c_uint64, _u32p, C.c_uint64, _u32p, C.c_uint32, _f64p,
                                     C.c_double, _u64p, C.c_uint32, C.POINTER(_vp)]),
     "s2h_seed_free": (None, [_vp, _vp]),
@@ -64,6 +69,19 @@ class Candidate(C.Structure):
     """s2h_candidate (include/s2hip.h)."""
     _fields_ = [("seed", C.c_uint32), ("ntris", C.c_uint32), ("ngranules", C.c_uint32), ("reserved", C.c_uint32),
                 ("old_tri", _f64p), ("new_tri", _f64p), ("granule", _u32p)]
+
+
+class Request(C.Structure):
+    """s2h_request (include/s2hip.h)."""
+    _fields_ = [("words", _u64p), ("nwords", C.c_uint64), ("dims", C.c_uint64 * 3), ("r_max", C.c_uint32),
+                ("method", C.c_int32), ("pitch", C.c_double), ("tag", C.c_uint64)]
+
+
+class Result(C.Structure):
+    """s2h_result (include/s2hip.h)."""
+    _fields_ = [("ready", C.c_int32), ("status", C.c_int32), ("deferrals", C.c_uint32), ("reserved", C.c_uint32),
+                ("tag", C.c_uint64), ("footprint", C.c_uint64), ("t_submit", C.c_double), ("t_start", C.c_double),
+                ("t_done", C.c_double)]
 
 
 class S2HipError(RuntimeError):

@@ -31,6 +31,7 @@ UNITS = {
     "s2hip_mc.hip": ["-ffp-contract=off"],
     "s2hip_fft.hip": ["-ffp-contract=fast"],
     "s2hip_batch.hip": ["-ffp-contract=off"],
+    "s2hip_serve.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = {"s2hip_host.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]}
 
@@ -58,7 +59,7 @@ def build(verbose: bool = False) -> Path:
                 print(r.stdout, r.stderr, file=sys.stderr)
     objs = [str(OBJ / (s + ".o")) for s in list(UNITS) + list(HOST_UNITS)]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", str(tmp)])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-lpthread", "-o", str(tmp)])
     os.replace(tmp, LIB)
     return LIB
 

@@ -2,6 +2,7 @@
 // every path of the S2 engine.
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 #include "s2hip_internal.h"
 
@@ -184,14 +185,66 @@ void shell_offsets_full(double distance_vox, double half_width_vox, std::vector<
     }
 }
 
-void shell_order_offsets(const uint64_t dims[3], uint32_t r_max, double pitch,
-                         std::vector<Offset> &offs, std::vector<uint64_t> &radius_start) {
-    offs.clear();
-    radius_start.assign(r_max + 2, 0);
+// Full (not domain-filtered) shells of radii 1..r_max, cached per (r_max, pitch):
+// the enumeration is the expensive part and many grids share an r_max.
+struct ShellSet {
+    std::vector<Offset> offs;
+    std::vector<uint64_t> start;  // r_max + 2
+    int32_t dmax[3] = {0, 0, 0};  // largest |d| per axis
+};
+
+static std::shared_ptr<const ShellSet> full_shells(uint32_t r_max, double pitch) {
+    static std::mutex mu;
+    static std::map<std::pair<uint32_t, uint64_t>, std::shared_ptr<const ShellSet>> cache;
+    static std::vector<std::pair<uint32_t, uint64_t>> order;
+    uint64_t pb;
+    std::memcpy(&pb, &pitch, 8);
+    const auto key = std::make_pair(r_max, pb);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    auto set = std::make_shared<ShellSet>();
+    set->start.assign(r_max + 2, 0);
     std::vector<Offset> shell;
     for (uint32_t r = 1; r <= r_max; ++r) {
         shell_offsets_full((double)r / pitch, 0.5 / pitch, shell);
         for (const Offset &o : shell) {
+            set->offs.push_back(o);
+            set->dmax[0] = std::max(set->dmax[0], std::abs(o.dx));
+            set->dmax[1] = std::max(set->dmax[1], std::abs(o.dy));
+            set->dmax[2] = std::max(set->dmax[2], std::abs(o.dz));
+        }
+        set->start[r + 1] = set->offs.size();
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    cache[key] = set;
+    order.push_back(key);
+    if (order.size() > 24) {  // keep the most recent entries
+        cache.erase(order.front());
+        order.erase(order.begin());
+    }
+    return set;
+}
+
+void shell_order_offsets(const uint64_t dims[3], uint32_t r_max, double pitch,
+                         std::vector<Offset> &offs, std::vector<uint64_t> &radius_start) {
+    auto set = full_shells(r_max, pitch);
+    const bool all_in = (uint64_t)set->dmax[0] < dims[0] && (uint64_t)set->dmax[1] < dims[1] &&
+                        (uint64_t)set->dmax[2] < dims[2];
+    if (all_in) {
+        offs = set->offs;
+        radius_start = set->start;
+        return;
+    }
+    offs.clear();
+    radius_start.assign(r_max + 2, 0);
+    for (uint32_t r = 1; r <= r_max; ++r) {
+        for (uint64_t i = set->start[r]; i < set->start[r + 1]; ++i) {
+            const Offset &o = set->offs[i];
             if ((uint64_t)std::abs((int64_t)o.dx) < dims[0] && (uint64_t)std::abs((int64_t)o.dy) < dims[1] &&
                 (uint64_t)std::abs((int64_t)o.dz) < dims[2])
                 offs.push_back(o);
