@@ -23,6 +23,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 VALU_PEAK_OPS = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
+# HBM bytes of one evaluation's count phase at 256^3 r_max=64 (500-particle
+# packing), from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) summed
+# over the three count dispatches; factors from a 512 MiB device copy.  Command
+# lines and per-dispatch figures: profiles/exact_count_kernel_256cube_r64.md.
+COUNT_PHASE_TRAFFIC = (11369 + 10615 + 10500) * 1024 * 2.0 + (389510 + 24791 + 6885) * 1024 * 1.0
 
 
 def cpu_baseline(occ_words, dims, r_max, offs, pair_work_total, target_s):
@@ -113,6 +118,72 @@ def batched_leg(ctx, ncand: int, seed: int, plain_sample: int):
     return out
 
 
+def per_radius_sums(values, rs, r_max):
+    import numpy as np
+
+    out = np.zeros(r_max + 1)
+    for r in range(1, r_max + 1):
+        out[r] = float(np.sum(values[rs[r]:rs[r + 1]], dtype=np.float64))
+    return out
+
+
+def fft_leg(ctx, vol, r_max, hits, offs, rs, curve_exact, steps):
+    """FP32 FFT autocorrelation on the same resident volume: evals/s, and its
+    error against the exact integer counts per radius bin (max |count error|
+    over the bin's offsets, relative to the largest count of the bin) and in
+    the finished curve."""
+    import numpy as np
+
+    vol.fft32_s2(r_max, 1.0)  # plan + warm-up
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        curve = vol.fft32_s2(r_max, 1.0)
+        dev.append(ctx.profile_read()["device_ms"])
+    el = time.perf_counter() - t0
+    fc = vol.fft32_counts(r_max, 1.0).astype(np.float64)
+    err = np.abs(fc - hits.astype(np.float64))
+    per_bin_abs = [0.0] + [float(err[rs[r]:rs[r + 1]].max()) for r in range(1, r_max + 1)]
+    per_bin_rel = [0.0] + [float(err[rs[r]:rs[r + 1]].max() / max(1.0, hits[rs[r]:rs[r + 1]].max()))
+                           for r in range(1, r_max + 1)]
+    d_s2 = np.abs(curve - curve_exact)
+    return {"evals_per_s": steps / el, "device_ms_per_eval": float(np.mean(dev)),
+            "max_abs_count_error": float(err.max()), "max_rel_count_error": float(max(per_bin_rel)),
+            "rounds_to_exact": bool(np.all(np.rint(fc) == hits)),
+            "max_abs_s2_error": float(d_s2.max()),
+            "per_bin_max_abs_count_error": [float(f"{v:.3g}") for v in per_bin_abs],
+            "per_bin_abs_s2_error": [float(f"{v:.3g}") for v in d_s2]}
+
+
+def mc_leg(ctx, vol, r_max, hits, offs, rs, total_samples, seed):
+    """Philox voxel Monte Carlo with total_samples pair samples spread evenly
+    over the r_max radii; each radius' estimate hits/valid is compared with the
+    exact estimand sum(hits)/sum(valid) of the same radius."""
+    import numpy as np
+
+    per_r = max(200, total_samples // r_max)
+    vol.mc_counts(r_max, 1.0, per_r, seed)  # warm-up
+    t0 = time.perf_counter()
+    hv = vol.mc_counts(r_max, 1.0, per_r, seed).astype(np.float64)
+    el = time.perf_counter() - t0
+    dims = np.array(vol.dims, dtype=np.float64)
+    valid = np.prod(dims[None, :] - np.abs(offs), axis=1)
+    # the estimand samples the shell offsets uniformly then a uniform voxel, so the
+    # exact value is sum over the shell of hits / sum of valid
+    ex_h = per_radius_sums(hits.astype(np.float64), rs, r_max)
+    ex_v = per_radius_sums(valid, rs, r_max)
+    z = []
+    for r in range(1, r_max + 1):
+        p = ex_h[r] / ex_v[r]
+        n = float(hv[r, 1])
+        est = hv[r, 0] / n
+        z.append((est - p) / max(np.sqrt(p * (1 - p) / n), 1e-300))
+    z = np.abs(np.array(z))
+    return {"samples_total": per_r * r_max, "samples_per_radius": per_r, "seconds": el,
+            "samples_per_s": per_r * r_max / el, "max_abs_z": float(z.max()),
+            "radii_beyond_3sigma": int((z > 3).sum()), "fraction_beyond_2sigma": float((z > 2).mean())}
+
+
 def served_leg(rate: float, duration: float, pool_size: int, slots: int, budget_gb: float, seed: int):
     """Poisson stream of mixed requests (grid 64^3-384^3 log-uniform, VF
     U[0.1, 0.5], r_max U{16..96}, 70 % exact / 30 % FP32 FFT) through the serving
@@ -144,6 +215,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-candidates", type=int, default=256)
+    ap.add_argument("--no-fft", action="store_true")
+    ap.add_argument("--no-mc", action="store_true")
+    ap.add_argument("--mc-samples", type=int, default=10_000_000, help="total pair samples over all radii")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -219,12 +293,19 @@ def main():
                    "grid": grid, "r_max": r_max, "offsets": int(len(offs)), "pair_work": pair_work,
                    "volume_fraction": occupied / grid ** 3, "voxelize_s": vox_s},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12,
-                     "unit": "TOP/s (int32 lane-ops)", "frac": achieved / VALU_PEAK_OPS, "traffic": None,
+                     "unit": "TOP/s (int32 lane-ops)", "frac": achieved / VALU_PEAK_OPS,
+                     "traffic": COUNT_PHASE_TRAFFIC if (grid, r_max, args.particles) == (256, 64, 500) else None,
                      "kernel": "exact_count_kernel", "kernel_ms": count_avg, "launches_per_eval": launches,
                      "device_ms_per_eval": float(np.mean(dev_ms)),
                      "algorithmic_ops_per_eval": alg_ops},
         "s2_head": [round(float(v), 6) for v in curve[:5]],
     }
+    if not (args.no_fft and args.no_mc):
+        hits = vol.exact_counts(r_max, 1.0)
+        if not args.no_fft:
+            out["fft32"] = fft_leg(ctx, vol, r_max, hits, offs, rs, curve, steps=5)
+        if not args.no_mc:
+            out["monte_carlo"] = mc_leg(ctx, vol, r_max, hits, offs, rs, args.mc_samples, seed=7)
     if not args.no_batch:
         out["batched"] = batched_leg(ctx, args.batch_candidates, seed=3, plain_sample=32)
     if not args.no_serve:

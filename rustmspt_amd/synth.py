@@ -137,14 +137,21 @@ def grf_volume(n: int, corr_len: float, vf: float, seed: int) -> np.ndarray:
     """Gaussian random field (white noise filtered by a Gaussian kernel of the
     given correlation length, periodic FFT filter) thresholded at the (1-vf)
     quantile; returns a uint8 grid, x-major, z fastest."""
+    import os
+
+    import scipy.fft as sfft
+
+    workers = max(1, min(16, os.cpu_count() or 1))
     rng = rng_for(seed, 5)
     noise = rng.standard_normal((n, n, n), dtype=np.float32)
     k = [np.fft.fftfreq(n).astype(np.float32)] * 2 + [np.fft.rfftfreq(n).astype(np.float32)]
-    f = np.fft.rfftn(noise)
+    f = sfft.rfftn(noise, workers=workers)
+    del noise
     s2 = (2 * math.pi * corr_len) ** 2 / 2.0
     f *= np.exp(-s2 * k[0][:, None, None] ** 2)
     f *= np.exp(-s2 * k[1][None, :, None] ** 2)
     f *= np.exp(-s2 * k[2][None, None, :] ** 2)
-    field = np.fft.irfftn(f, s=(n, n, n), axes=(0, 1, 2)).astype(np.float32)
+    field = sfft.irfftn(f, s=(n, n, n), axes=(0, 1, 2), workers=workers).astype(np.float32, copy=False)
+    del f
     thr = np.quantile(field, 1.0 - vf)
     return (field > thr).astype(np.uint8)
