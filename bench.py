@@ -222,6 +222,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--tune", type=str, default="",
+                    help="band,waves: force the exact launch shape instead of the autotuner's pick")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration for profiling)")
     args = ap.parse_args()
@@ -257,6 +259,9 @@ def main():
     pair_work = float(valid.sum())          # W: ordered pair-tests of all shell offsets
     half_work = pair_work / 2.0             # C(d) = C(-d): unordered pair-tests
 
+    if args.tune:
+        band, waves = (int(v) for v in args.tune.split(","))
+        ctx.exact_tune_set([grid] * 3, r_max, 1.0, band, waves)
     curve = None
     for _ in range(args.warmup):
         curve = vol.exact_s2(r_max, 1.0)
@@ -274,6 +279,7 @@ def main():
     # algorithmic VALU work: one v_and_b32 + one v_bcnt_u32_b32 per 32 unordered pair-tests
     alg_ops = half_work / 32.0 * 2.0
     achieved = alg_ops / (count_avg * 1e-3)
+    valu_measured = ctx.valu_probe()
 
     out = {
         "metric": "exact-S2 evals/s at 256^3 r_max=64, batched vs plain; p50/p99 served latency",
@@ -297,7 +303,9 @@ def main():
                      "traffic": COUNT_PHASE_TRAFFIC if (grid, r_max, args.particles) == (256, 64, 500) else None,
                      "kernel": "exact_count_kernel", "kernel_ms": count_avg, "launches_per_eval": launches,
                      "device_ms_per_eval": float(np.mean(dev_ms)),
-                     "algorithmic_ops_per_eval": alg_ops},
+                     "algorithmic_ops_per_eval": alg_ops,
+                     "peak_measured": valu_measured / 1e12, "frac_of_measured_peak": achieved / valu_measured,
+                     "launch_shape": ctx.exact_tune_info([grid] * 3, r_max, 1.0)},
         "s2_head": [round(float(v), 6) for v in curve[:5]],
     }
     if not (args.no_fft and args.no_mc):

@@ -71,6 +71,11 @@ s2h_status s2h_bytes_in_use(s2h_ctx *ctx, uint64_t *bytes);
  * (it has no GPU kernel timestamps, AGENTS.md:638); used by the benchmark. */
 s2h_status s2h_profile(s2h_ctx *ctx, int enable);
 s2h_status s2h_profile_read(s2h_ctx *ctx, double *ms, uint64_t n);
+/* Measured integer-VALU issue rate of the device, in 32-bit lane-operations per
+ * second, for the exact counting loop's instruction pair (v_and_b32 with a
+ * wave-uniform operand + v_bcnt_u32_b32 accumulate) with every SIMD occupied:
+ * the roofline peak of the exact count kernel.  No counterpart in the reference. */
+s2h_status s2h_valu_probe(s2h_ctx *ctx, double *lane_ops_per_s);
 
 /* ------------------------------------------------------------------ volumes */
 
@@ -127,6 +132,28 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
  * Bit-identical to the reference CPU exact path. */
 s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, double *s2,
                         uint64_t n);
+
+/* Shape of the exact count launch for (dims, r_max, pitch) -- no counterpart in
+ * the reference, whose direct kernel has no launch shape.  The half-ball columns
+ * are grouped 64 to a wave in bands of `band` consecutive dz extents, and the
+ * launch is split into about `waves` waves.  The first exact call on a new shape
+ * ranks every candidate with a cost model and, when a run is short, times the
+ * best few on the device and keeps the fastest (S2HIP_AUTOTUNE=0: model only).
+ * Counts never depend on the shape.
+ * info (8 values): [0] band, [1] target waves, [2] 1 picked by device timing,
+ * -1 by the cost model, 0 not run yet, [3] candidates timed, [4] work items,
+ * [5] column groups, [6] modelled VALU wave-instructions per run, [7] 0. */
+s2h_status s2h_exact_tune_info(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
+                               int64_t *info, uint64_t n);
+/* Force a shape (band >= 1, waves >= 1) for this context; the autotuner is
+ * skipped for it. */
+s2h_status s2h_exact_tune_set(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
+                              int32_t band, int32_t waves);
+/* enable != 0: a new exact plan's first run times the best-modelled shapes
+ * (default); 0: the cost model's pick is used as is (for callers that see many
+ * distinct shapes once each, such as the serving loop).  A shape timed once is
+ * reused process-wide by every context. */
+s2h_status s2h_set_autotune(s2h_ctx *ctx, int32_t enable);
 
 /* ------------------------------------------------------------------ FFT (FP32) */
 

@@ -32,6 +32,7 @@ SIGNATURES = {
     "s2h_bytes_in_use": (C.c_int32, [_vp, _u64p]),
     "s2h_profile": (C.c_int32, [_vp, C.c_int]),
     "s2h_profile_read": (C.c_int32, [_vp, _f64p, C.c_uint64]),
+    "s2h_valu_probe": (C.c_int32, [_vp, _f64p]),
     "s2h_vol_from_packed": (C.c_int32, [_vp, _u64p, C.c_uint64, _u64p, C.POINTER(_vp)]),
     "s2h_vol_from_bytes": (C.c_int32, [_vp, _u8p, _u64p, C.POINTER(_vp)]),
     "s2h_vol_read_packed": (C.c_int32, [_vp, _vp, _u64p, C.c_uint64]),
@@ -44,6 +45,9 @@ SIGNATURES = {
     "s2h_shell_offsets": (C.c_int32, [_u64p, C.c_uint32, C.c_double, _i64p, C.c_uint64, _u64p, C.c_uint64]),
     "s2h_exact_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, _u64p, C.c_uint64]),
     "s2h_exact_s2": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, _f64p, C.c_uint64]),
+    "s2h_exact_tune_info": (C.c_int32, [_vp, _u64p, C.c_uint32, C.c_double, _i64p, C.c_uint64]),
+    "s2h_exact_tune_set": (C.c_int32, [_vp, _u64p, C.c_uint32, C.c_double, C.c_int32, C.c_int32]),
+    "s2h_set_autotune": (C.c_int32, [_vp, C.c_int32]),
     "s2h_fft32_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, _f32p, C.c_uint64]),
     "s2h_fft32_s2": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, _f64p, C.c_uint64]),
     "s2h_mc_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _u64p, C.c_uint64]),

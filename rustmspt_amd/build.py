@@ -32,6 +32,7 @@ UNITS = {
     "s2hip_fft.hip": ["-ffp-contract=fast"],
     "s2hip_batch.hip": ["-ffp-contract=off"],
     "s2hip_serve.hip": ["-ffp-contract=off"],
+    "s2hip_probe.hip": [],
 }
 HOST_UNITS = {"s2hip_host.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]}
 

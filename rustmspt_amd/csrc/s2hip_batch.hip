@@ -208,8 +208,8 @@ __device__ void delta_item(const SeedDev &s, const CandDev &c, uint32_t cand, co
                 const uint64_t pn = ((uint64_t)bn1 << 32) | bn0, po = ((uint64_t)bo1 << 32) | bo0;
 #pragma unroll
                 for (int j = 0; j < DZB; ++j) {
-                    cn[j] += __popc(an & (uint32_t)(pn >> j));
-                    co[j] += __popc(ao & (uint32_t)(po >> j));
+                    cn[j] = popc_acc(an & (uint32_t)(pn >> j), cn[j]);
+                    co[j] = popc_acc(ao & (uint32_t)(po >> j), co[j]);
                 }
             }
         }
@@ -226,8 +226,8 @@ __device__ void delta_item(const SeedDev &s, const CandDev &c, uint32_t cand, co
                 const uint64_t pn = ((uint64_t)bn1 << 32) | bn0, po = ((uint64_t)bo1 << 32) | bo0;
 #pragma unroll
                 for (int j = 0; j < DZB; ++j) {
-                    cn[j] += __popc(a & (uint32_t)((pn << j) >> 32));
-                    co[j] += __popc(a & (uint32_t)((po << j) >> 32));
+                    cn[j] = popc_acc(a & (uint32_t)((pn << j) >> 32), cn[j]);
+                    co[j] = popc_acc(a & (uint32_t)((po << j) >> 32), co[j]);
                 }
             }
         }
@@ -265,9 +265,12 @@ __global__ __launch_bounds__(64 * kDeltaWaves) void delta_count_kernel(const See
         const uint64_t local = item - c.item_start;
         const GB gb = s.gbs[local / c.nchunks];
         const int chunk = (int)(local % c.nchunks);
-        if (gb.dzb == 8) delta_item<8>(s, c, lo, gb, chunk, rows, delta);
-        else if (gb.dzb == 16) delta_item<16>(s, c, lo, gb, chunk, rows, delta);
-        else delta_item<32>(s, c, lo, gb, chunk, rows, delta);
+        switch (gb.dzb) {
+            case 8: delta_item<8>(s, c, lo, gb, chunk, rows, delta); break;
+            case 16: delta_item<16>(s, c, lo, gb, chunk, rows, delta); break;
+            case 24: delta_item<24>(s, c, lo, gb, chunk, rows, delta); break;
+            default: delta_item<32>(s, c, lo, gb, chunk, rows, delta); break;
+        }
     }
 }
 
@@ -492,12 +495,7 @@ s2h_status s2h_seed_create(s2h_ctx *ctx, const double *xyz, uint64_t nverts, con
         return cleanup(st);
     S2H_HIP(hipMemcpyAsync(sd->table.ptr, sd->plan->d_table.ptr, sd->plan->table_size * 8,
                            hipMemcpyDeviceToDevice, ctx->stream));
-    for (int s = 0; s < 3; ++s)
-        for (const WorkItem &w : sd->plan->items[s]) {
-            batch::GB g{w.group, w.dz0, w.dzb};
-            bool dup = !sd->gb_host.empty() && sd->gb_host.back().group == g.group && sd->gb_host.back().dz0 == g.dz0;
-            if (!dup) sd->gb_host.push_back(g);
-        }
+    for (const DzBlock &b : sd->plan->blocks) sd->gb_host.push_back(batch::GB{b.group, b.dz0, b.dzb});
     if ((st = upload(ctx, sd->gbs, sd->gb_host.data(), sd->gb_host.size() * sizeof(batch::GB))) != S2H_OK)
         return cleanup(st);
     S2H_HIP(hipStreamSynchronize(ctx->stream));

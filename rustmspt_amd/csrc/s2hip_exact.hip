@@ -22,7 +22,10 @@
 // ordered f64 sum (src/geometry/s2.rs:1202-1207) -- so the curve is bit-identical.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <numeric>
 
 #include "s2hip_internal.h"
@@ -37,20 +40,11 @@ static constexpr int kWavesPerBlock = 4;
 // unguarded read of up to 64 dwords past any row start stays inside the
 // allocation; bits that do not belong to the row are masked to zero.
 template <int NW, int DZB>
-__global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
-    const uint32_t *__restrict__ vol, uint32_t vol_bytes, int nx, int ny, int row_dw,
-    const WorkItem *__restrict__ items, int nitems, const ColInfo *__restrict__ cols,
-    unsigned long long *__restrict__ table) {
-    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + threadIdx.x / 64));
-    if (wave >= nitems) return;
-    const int lane = threadIdx.x & 63;
-    const WorkItem it = items[wave];
-    const ColInfo c = cols[it.group * 64 + lane];
+__device__ __forceinline__ void count_item(const uint32_t *__restrict__ vol, const __amdgpu_buffer_rsrc_t rsrc, int nx,
+                                           int ny, int row_dw, const WorkItem &it, const ColInfo &c,
+                                           unsigned long long *__restrict__ table) {
     const int q0 = it.dz0 >> 5;
     const int nw = it.nw;  // partner dwords q0 .. q0+nw-1 hold voxels; later ones read as 0
-    // per-lane rows through a buffer resource: an out-of-range offset reads 0
-    const __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc((void *)vol, 0, (int)vol_bytes, 0x00020000);
     const int oob = 0x7ffffff0;
 
     uint32_t cnt[DZB];
@@ -92,7 +86,7 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
             for (int w = 0; w < NW; ++w) {
                 const uint64_t pair = ((uint64_t)b[w + 1] << 32) | b[w];
 #pragma unroll
-                for (int j = 0; j < DZB; ++j) cnt[j] += __popc(a[w] & (uint32_t)(pair >> j));
+                for (int j = 0; j < DZB; ++j) cnt[j] = popc_acc(a[w] & (uint32_t)(pair >> j), cnt[j]);
             }
         }
     }
@@ -100,6 +94,29 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
     for (int j = 0; j < DZB; ++j) {
         const int dz = it.dz0 + j;
         if (c.m >= 0 && dz <= c.m && cnt[j]) atomicAdd(&table[c.base + dz], (unsigned long long)cnt[j]);
+    }
+}
+
+// One wave per work item; every dz-block width runs in the same launch (the
+// branch is wave-uniform), items are ordered heaviest first.
+template <int NW>
+__global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
+    const uint32_t *__restrict__ vol, uint32_t vol_bytes, int nx, int ny, int row_dw,
+    const WorkItem *__restrict__ items, int nitems, const ColInfo *__restrict__ cols,
+    unsigned long long *__restrict__ table) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + threadIdx.x / 64));
+    if (wave >= nitems) return;
+    const int lane = threadIdx.x & 63;
+    const WorkItem it = items[wave];
+    const ColInfo c = cols[it.group * 64 + lane];
+    // per-lane rows through a buffer resource: an out-of-range offset reads 0
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)vol, 0, (int)vol_bytes, 0x00020000);
+    switch (it.dzb) {
+        case 8: count_item<NW, 8>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 16: count_item<NW, 16>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 24: count_item<NW, 24>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        default: count_item<NW, 32>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
     }
 }
 
@@ -168,44 +185,37 @@ static int supported_nw(int need) {
     return -1;
 }
 
-static s2h_status build_plan(s2h_ctx *ctx, ExactPlan *p) {
+static Offset canonical(Offset o) {
+    if (o.dz < 0 || (o.dz == 0 && (o.dx < 0 || (o.dx == 0 && o.dy < 0)))) return Offset{-o.dx, -o.dy, -o.dz};
+    return o;
+}
+
+// Shape-independent part: shell offsets, the half-ball columns (dense table
+// layout in (dx, dy) order), the gather map and the fixed device buffers.
+static s2h_status build_base(s2h_ctx *ctx, ExactPlan *p) {
     shell_order_offsets(p->dims, p->r_max, p->pitch, p->offs, p->radius_start);
     const uint64_t K = p->offs.size();
-    const int nx = (int)p->dims[0], ny = (int)p->dims[1], nz = (int)p->dims[2];
-    // canonical half ball: column (dx, dy) -> largest dz
     int dmax = 0;
     for (const Offset &o : p->offs) dmax = std::max({dmax, std::abs(o.dx), std::abs(o.dy)});
     const int span = 2 * dmax + 1;
     std::vector<int32_t> colm((size_t)span * span, -1);
-    auto canon = [](Offset o) {
-        if (o.dz < 0 || (o.dz == 0 && (o.dx < 0 || (o.dx == 0 && o.dy < 0)))) return Offset{-o.dx, -o.dy, -o.dz};
-        return o;
-    };
     for (const Offset &o0 : p->offs) {
         if (o0.dx == 0 && o0.dy == 0 && o0.dz == 0) continue;
-        Offset o = canon(o0);
+        const Offset o = canonical(o0);
         int32_t &m = colm[(size_t)(o.dx + dmax) * span + (o.dy + dmax)];
         m = std::max(m, o.dz);
     }
-    struct Col {
-        int dx, dy, m;
-        double ang;
-    };
-    std::vector<Col> cols;
-    for (int dx = -dmax; dx <= dmax; ++dx)
-        for (int dy = -dmax; dy <= dmax; ++dy) {
-            int m = colm[(size_t)(dx + dmax) * span + (dy + dmax)];
-            if (m >= 0) cols.push_back({dx, dy, m, std::atan2((double)dy, (double)dx)});
-        }
-    // dense table: column c holds dz = 0..m
-    std::vector<uint32_t> base(cols.size());
     std::vector<int32_t> colidx((size_t)span * span, -1);
     uint64_t tsize = 0;
-    for (size_t i = 0; i < cols.size(); ++i) {
-        base[i] = (uint32_t)tsize;
-        colidx[(size_t)(cols[i].dx + dmax) * span + (cols[i].dy + dmax)] = (int32_t)i;
-        tsize += (uint64_t)cols[i].m + 1;
-    }
+    p->colset.clear();
+    for (int dx = -dmax; dx <= dmax; ++dx)
+        for (int dy = -dmax; dy <= dmax; ++dy) {
+            const int m = colm[(size_t)(dx + dmax) * span + (dy + dmax)];
+            if (m < 0) continue;
+            colidx[(size_t)(dx + dmax) * span + (dy + dmax)] = (int32_t)p->colset.size();
+            p->colset.push_back(ColInfo{dx, dy, m, (uint32_t)tsize});
+            tsize += (uint64_t)m + 1;
+        }
     if (tsize >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "offset table too large");
     p->table_size = tsize;
     std::vector<int32_t> gather(K);
@@ -215,36 +225,76 @@ static s2h_status build_plan(s2h_ctx *ctx, ExactPlan *p) {
             gather[i] = -1;
             continue;
         }
-        Offset o = canon(o0);
-        int32_t ci = colidx[(size_t)(o.dx + dmax) * span + (o.dy + dmax)];
-        gather[i] = (int32_t)(base[ci] + (uint32_t)o.dz);
+        const Offset o = canonical(o0);
+        const int32_t ci = colidx[(size_t)(o.dx + dmax) * span + (o.dy + dmax)];
+        gather[i] = (int32_t)(p->colset[ci].base + (uint32_t)o.dz);
     }
-    // groups of 64 columns with nearly equal m (then by angle, keeping each
-    // group's (dx, dy) spread, and so its idle boundary rows, small)
-    std::vector<size_t> order(cols.size());
+    std::vector<uint64_t> rs(p->radius_start.begin(), p->radius_start.end());
+    S2H_TRY(upload(ctx, p->d_offs, p->offs.data(), K * sizeof(Offset)));
+    S2H_TRY(upload(ctx, p->d_gather, gather.data(), K * sizeof(int32_t)));
+    S2H_TRY(upload(ctx, p->d_rstart, rs.data(), rs.size() * 8));
+    S2H_TRY(dev_alloc(ctx, p->d_table, std::max<uint64_t>(tsize, 1) * 8));
+    S2H_TRY(dev_alloc(ctx, p->d_terms, std::max<uint64_t>(K, 1) * 8));
+    S2H_TRY(dev_alloc(ctx, p->d_hits, std::max<uint64_t>(K, 1) * 8));
+    S2H_TRY(dev_alloc(ctx, p->d_curve, (p->r_max + 1) * 9 + 16));
+    return S2H_OK;
+}
+
+// Per-row and per-item VALU overheads of the count kernel beyond the
+// and/popcount pairs (address math, prefetch, counter flush), in
+// wave-instructions; from the instruction counts of the kernel.
+static constexpr double kRowOverhead = 12.0, kItemOverhead = 400.0;
+
+// Columns grouped 64 to a wave for one band width, with the (group, dz block)
+// list and each block's modelled cost in VALU wave-instructions.
+struct Grouping {
+    int band = 0;
+    uint32_t ngroups = 0;
+    std::vector<ColInfo> lanes;   // ngroups * 64
+    struct Blk {
+        DzBlock b;
+        int x0, x1, y0, y1, nw;
+        double cost;
+    };
+    std::vector<Blk> blks;
+    double total = 0;
+};
+
+static void group_columns(const ExactPlan *p, int band, Grouping &g) {
+    const int nx = (int)p->dims[0], ny = (int)p->dims[1], nz = (int)p->dims[2];
+    g.band = band;
+    const std::vector<ColInfo> &cs = p->colset;
+    // bands of `band` consecutive m values, counted from the largest m down
+    std::vector<int32_t> ms;
+    for (const ColInfo &c : cs) ms.push_back(c.m);
+    std::sort(ms.begin(), ms.end(), std::greater<int32_t>());
+    ms.erase(std::unique(ms.begin(), ms.end()), ms.end());
+    std::vector<int> bnd(cs.size());
+    std::vector<double> ang(cs.size());
+    for (size_t i = 0; i < cs.size(); ++i) {
+        const size_t rank =
+            (size_t)(std::lower_bound(ms.begin(), ms.end(), cs[i].m, std::greater<int32_t>()) - ms.begin());
+        bnd[i] = (int)(rank / (size_t)std::max(1, band));
+        ang[i] = std::atan2((double)cs[i].dy, (double)cs[i].dx);
+    }
+    std::vector<size_t> order(cs.size());
     std::iota(order.begin(), order.end(), 0);
     std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-        if (cols[a].m != cols[b].m) return cols[a].m > cols[b].m;
-        return cols[a].ang < cols[b].ang;
+        if (bnd[a] != bnd[b]) return bnd[a] < bnd[b];
+        if (ang[a] != ang[b]) return ang[a] < ang[b];
+        return cs[a].m > cs[b].m;
     });
-    const size_t ngroups = (cols.size() + 63) / 64;
-    p->ngroups = (uint32_t)ngroups;
-    std::vector<ColInfo> ci(ngroups * 64, ColInfo{0, 0, -1, 0});
-    for (size_t k = 0; k < order.size(); ++k) {
-        const Col &c = cols[order[k]];
-        ci[k] = ColInfo{c.dx, c.dy, c.m, base[order[k]]};
-    }
-    // work items
+    g.ngroups = (uint32_t)((cs.size() + 63) / 64);
+    g.lanes.assign((size_t)g.ngroups * 64, ColInfo{0, 0, -1, 0});
+    for (size_t k = 0; k < order.size(); ++k) g.lanes[k] = cs[order[k]];
     const int need_dw = (nz + 31) / 32;
-    for (auto &v : p->items) v.clear();
-    // x chunking: aim for enough waves to fill 256 CUs several times over
-    double total_rows = 0;
-    for (size_t g = 0; g < ngroups; ++g) total_rows += (double)nx * ny;
-    const int chunk = std::max(1, std::min(nx, (int)std::ceil((double)nx * ngroups * 3 / 16384.0)));
-    for (size_t g = 0; g < ngroups; ++g) {
+    const int NW = std::max(1, supported_nw(need_dw));
+    g.blks.clear();
+    g.total = 0;
+    for (uint32_t gi = 0; gi < g.ngroups; ++gi) {
         int M = -1, dxmin = 1 << 30, dxmax = -(1 << 30), dymin = 1 << 30, dymax = -(1 << 30);
         for (int l = 0; l < 64; ++l) {
-            const ColInfo &c = ci[g * 64 + l];
+            const ColInfo &c = g.lanes[(size_t)gi * 64 + l];
             if (c.m < 0) continue;
             M = std::max(M, c.m);
             dxmin = std::min(dxmin, c.dx);
@@ -253,46 +303,150 @@ static s2h_status build_plan(s2h_ctx *ctx, ExactPlan *p) {
             dymax = std::max(dymax, c.dy);
         }
         if (M < 0) continue;
-        // b row p valid for some lane: xb - dx in [0, nx) for some dx in the group
+        // b row p is useful to some lane: p - (dx, dy) inside the grid
         const int x0 = std::max(0, dxmin), x1 = std::min(nx, nx + dxmax);
         const int y0 = std::max(0, dymin), y1 = std::min(ny, ny + dymax);
         if (x0 >= x1 || y0 >= y1) continue;
         for (int dz0 = 0; dz0 <= M; dz0 += 32) {
-            const int rem = M + 1 - dz0;
-            const int dzb = rem > 16 ? 32 : (rem > 8 ? 16 : 8);
-            const int slot = dzb == 8 ? 0 : (dzb == 16 ? 1 : 2);
-            const int nwi = std::max(0, need_dw - dz0 / 32);
-            if (nwi == 0) continue;
-            for (int xs = x0; xs < x1; xs += chunk) {
-                WorkItem w;
-                w.group = (uint32_t)g;
-                w.dz0 = (uint16_t)dz0;
-                w.dzb = (uint16_t)dzb;
-                w.x0 = (uint16_t)xs;
-                w.x1 = (uint16_t)std::min(x1, xs + chunk);
-                w.y0 = (uint16_t)y0;
-                w.y1 = (uint16_t)y1;
-                w.nw = (uint16_t)nwi;
-                w.pad = 0;
-                p->items[slot].push_back(w);
-            }
+            const int rem = std::min(32, M + 1 - dz0);
+            const int dzb = (rem + 7) / 8 * 8;
+            const int nwi = need_dw - dz0 / 32;
+            if (nwi <= 0) continue;
+            const double rows = (double)(x1 - x0) * (y1 - y0);
+            const double cost = rows * (NW * dzb * 2.0 + kRowOverhead);
+            g.blks.push_back(Grouping::Blk{DzBlock{gi, (uint16_t)dz0, (uint16_t)dzb}, x0, x1, y0, y1, nwi, cost});
+            g.total += cost;
         }
     }
-    (void)total_rows;
-    // device copies
-    std::vector<uint64_t> rs(p->radius_start.begin(), p->radius_start.end());
-    S2H_TRY(upload(ctx, p->d_offs, p->offs.data(), K * sizeof(Offset)));
-    S2H_TRY(upload(ctx, p->d_gather, gather.data(), K * sizeof(int32_t)));
-    S2H_TRY(upload(ctx, p->d_rstart, rs.data(), rs.size() * 8));
-    S2H_TRY(upload(ctx, p->d_cols, ci.data(), ci.size() * sizeof(ColInfo)));
-    for (int s = 0; s < 3; ++s)
-        S2H_TRY(upload(ctx, p->d_items[s], p->items[s].data(), p->items[s].size() * sizeof(WorkItem)));
-    S2H_TRY(dev_alloc(ctx, p->d_table, std::max<uint64_t>(tsize, 1) * 8));
-    S2H_TRY(dev_alloc(ctx, p->d_terms, std::max<uint64_t>(K, 1) * 8));
-    S2H_TRY(dev_alloc(ctx, p->d_hits, std::max<uint64_t>(K, 1) * 8));
-    S2H_TRY(dev_alloc(ctx, p->d_curve, (p->r_max + 1) * 9 + 16));
-    S2H_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// x-chunks of a block for a wave target: proportional to its cost.
+static int block_chunks(const Grouping &g, const Grouping::Blk &b, int waves) {
+    const int n = (int)std::lround(b.cost / std::max(g.total, 1.0) * waves);
+    return std::max(1, std::min(b.x1 - b.x0, n));
+}
+
+// Cost model: executed VALU wave-instructions at the issue rate of 1024 SIMDs
+// (4 cycles each), scaled down when there are too few waves to hide latency,
+// and bounded below by the longest item.
+static double model_ms(const Grouping &g, int waves, double *ops_out) {
+    double items = 0, maxitem = 0;
+    for (const Grouping::Blk &b : g.blks) {
+        const int n = block_chunks(g, b, waves);
+        items += n;
+        maxitem = std::max(maxitem, b.cost * std::ceil((double)(b.x1 - b.x0) / n) / (b.x1 - b.x0) + kItemOverhead);
+    }
+    const double ops = g.total + kItemOverhead * items;
+    if (ops_out) *ops_out = ops;
+    const double simds = 1024.0, clk = 2.4e9;
+    const double fill = std::min(1.0, items / (simds * 6.0));
+    return 1e3 * std::max(ops * 4.0 / (simds * clk * std::max(fill, 0.05)), maxitem * 4.0 * 2.0 / clk);
+}
+
+static s2h_status apply_shape(s2h_ctx *ctx, ExactPlan *p, const Grouping &g, int waves) {
+    p->tune = ExactTune{g.band, waves};
+    p->ngroups = g.ngroups;
+    p->blocks.clear();
+    p->items.clear();
+    std::vector<double> icost;
+    for (const Grouping::Blk &b : g.blks) {
+        p->blocks.push_back(b.b);
+        const int span = b.x1 - b.x0, n = block_chunks(g, b, waves);
+        for (int k = 0; k < n; ++k) {
+            const int xa = b.x0 + (int)((int64_t)span * k / n), xb = b.x0 + (int)((int64_t)span * (k + 1) / n);
+            if (xa >= xb) continue;
+            WorkItem w;
+            w.group = b.b.group;
+            w.dz0 = b.b.dz0;
+            w.dzb = b.b.dzb;
+            w.x0 = (uint16_t)xa;
+            w.x1 = (uint16_t)xb;
+            w.y0 = (uint16_t)b.y0;
+            w.y1 = (uint16_t)b.y1;
+            w.nw = (uint16_t)b.nw;
+            w.pad = 0;
+            p->items.push_back(w);
+            icost.push_back(b.cost * (double)(xb - xa) / span);
+        }
+    }
+    // heaviest first: the last waves to start are the short ones
+    std::vector<size_t> io(p->items.size());
+    std::iota(io.begin(), io.end(), 0);
+    std::stable_sort(io.begin(), io.end(), [&](size_t a, size_t b) { return icost[a] > icost[b]; });
+    std::vector<WorkItem> sorted(p->items.size());
+    for (size_t k = 0; k < io.size(); ++k) sorted[k] = p->items[io[k]];
+    p->items.swap(sorted);
+    model_ms(g, waves, &p->model_ops);
+    dev_free(ctx, p->d_cols);
+    dev_free(ctx, p->d_items);
+    S2H_TRY(upload(ctx, p->d_cols, g.lanes.data(), g.lanes.size() * sizeof(ColInfo)));
+    S2H_TRY(upload(ctx, p->d_items, p->items.data(), std::max<size_t>(p->items.size(), 1) * sizeof(WorkItem)));
     return S2H_OK;
+}
+
+static s2h_status apply_tune(s2h_ctx *ctx, ExactPlan *p, ExactTune t) {
+    Grouping g;
+    group_columns(p, t.band, g);
+    return apply_shape(ctx, p, g, t.waves);
+}
+
+struct Ranked {
+    ExactTune tune;
+    double ms;
+};
+
+// Every (band, waves) candidate, best modelled first.
+static std::vector<Ranked> rank_tunes(const ExactPlan *p) {
+    std::vector<Ranked> out;
+    for (int band : {1, 2, 3, 4, 6}) {
+        Grouping g;
+        group_columns(p, band, g);
+        for (int waves : {8192, 16384, 32768}) out.push_back(Ranked{ExactTune{band, waves}, model_ms(g, waves, nullptr)});
+    }
+    std::stable_sort(out.begin(), out.end(), [](const Ranked &a, const Ranked &b) { return a.ms < b.ms; });
+    return out;
+}
+
+// Process-wide table of chosen shapes per (dims, r_max, pitch): a shape timed
+// once on the device serves every later plan of that key in any context.
+struct TuneEntry {
+    ExactTune tune;
+    int tuned;  // 1 timed on the device, -1 model pick
+    int timed;
+};
+using TuneKey = std::tuple<uint64_t, uint64_t, uint64_t, uint32_t, uint64_t>;
+static std::mutex g_tune_mu;
+static std::map<TuneKey, TuneEntry> g_tunes;
+
+static TuneKey tune_key(const ExactPlan *p) {
+    uint64_t pbits;
+    std::memcpy(&pbits, &p->pitch, 8);
+    return TuneKey{p->dims[0], p->dims[1], p->dims[2], p->r_max, pbits};
+}
+
+static void tune_store(const ExactPlan *p, const TuneEntry &e) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneEntry &slot = g_tunes[tune_key(p)];
+    if (slot.tuned != 1 || e.tuned == 1) slot = e;  // never replace a timed pick by a model pick
+}
+
+static bool tune_lookup(const ExactPlan *p, TuneEntry &e) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tunes.find(tune_key(p));
+    if (it == g_tunes.end()) return false;
+    e = it->second;
+    return true;
+}
+
+static bool autotune_enabled(const s2h_ctx *ctx) {
+    const char *env = std::getenv("S2HIP_AUTOTUNE");
+    return ctx->autotune && !(env && env[0] == '0');
+}
+
+void exact_plan_release(s2h_ctx *ctx, ExactPlan *p) {
+    DevBuf *bufs[] = {&p->d_offs, &p->d_gather, &p->d_rstart, &p->d_cols, &p->d_items,
+                      &p->d_table, &p->d_terms, &p->d_hits, &p->d_curve};
+    for (DevBuf *b : bufs) dev_free(ctx, *b);
 }
 
 s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
@@ -310,12 +464,24 @@ s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, 
     for (int a = 0; a < 3; ++a) p->dims[a] = dims[a];
     p->r_max = r_max;
     p->pitch = pitch;
-    s2h_status st = build_plan(ctx, p.get());
+    s2h_status st = build_base(ctx, p.get());
+    if (st == S2H_OK) {
+        TuneEntry e;
+        if (tune_lookup(p.get(), e)) {
+            st = apply_tune(ctx, p.get(), e.tune);
+            // a model pick may still be timed by a context that tunes
+            p->tuned = (e.tuned == 1 || !autotune_enabled(ctx)) ? e.tuned : 0;
+            p->timed_candidates = e.timed;
+        } else {
+            const std::vector<Ranked> ranked = rank_tunes(p.get());
+            st = apply_tune(ctx, p.get(), ranked.front().tune);
+            p->tuned = autotune_enabled(ctx) ? 0 : -1;
+            if (st == S2H_OK) tune_store(p.get(), TuneEntry{ranked.front().tune, -1, 0});
+        }
+    }
+    if (st == S2H_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = fail(S2H_ERR_HIP, "plan upload failed");
     if (st != S2H_OK) {
-        DevBuf *bufs[] = {&p->d_offs, &p->d_gather, &p->d_rstart, &p->d_cols, &p->d_items[0],
-                          &p->d_items[1], &p->d_items[2], &p->d_table, &p->d_terms, &p->d_hits,
-                          &p->d_curve};
-        for (DevBuf *b : bufs) dev_free(ctx, *b);
+        exact_plan_release(ctx, p.get());
         return st;
     }
     *out = p.get();
@@ -331,29 +497,18 @@ static s2h_status launch_counts_nw(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
     const ColInfo *cols = (const ColInfo *)p->d_cols.ptr;
     const int nx = (int)vol->dims[0], ny = (int)vol->dims[1];
     const uint32_t vbytes = (uint32_t)(vol->nwords * 8);
-    for (int s = 0; s < 3; ++s) {
-        const int n = (int)p->items[s].size();
-        if (n == 0) continue;
-        const WorkItem *items = (const WorkItem *)p->d_items[s].ptr;
-        const unsigned blocks = (unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock);
-        const dim3 block(64 * kWavesPerBlock);
-        if (s == 0)
-            exact_count_kernel<NW, 8><<<blocks, block, 0, ctx->stream>>>(v, vbytes, nx, ny, row_dw, items, n, cols, table);
-        else if (s == 1)
-            exact_count_kernel<NW, 16><<<blocks, block, 0, ctx->stream>>>(v, vbytes, nx, ny, row_dw, items, n, cols, table);
-        else
-            exact_count_kernel<NW, 32><<<blocks, block, 0, ctx->stream>>>(v, vbytes, nx, ny, row_dw, items, n, cols, table);
-        S2H_HIP(hipGetLastError());
-        ++ctx->launches;
-    }
+    const int n = (int)p->items.size();
+    if (n == 0) return S2H_OK;
+    const WorkItem *items = (const WorkItem *)p->d_items.ptr;
+    const unsigned blocks = (unsigned)((n + kWavesPerBlock - 1) / kWavesPerBlock);
+    exact_count_kernel<NW><<<blocks, dim3(64 * kWavesPerBlock), 0, ctx->stream>>>(v, vbytes, nx, ny, row_dw, items,
+                                                                                   n, cols, table);
+    S2H_HIP(hipGetLastError());
+    ++ctx->launches;
     return S2H_OK;
 }
 
-s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
-    if (vol->nwords * 8 + kVolTailBytes >= (1ull << 31))
-        return fail(S2H_ERR_UNSUPPORTED, "packed volume above 2 GiB");
-    S2H_HIP(hipMemsetAsync(p->d_table.ptr, 0, std::max<uint64_t>(p->table_size, 1) * 8, ctx->stream));
-    prof_mark(ctx, 0);
+static s2h_status launch_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
     const int need = (int)((vol->dims[2] + 31) / 32);
     switch (supported_nw(need)) {
         case 1: return launch_counts_nw<1>(ctx, vol, p);
@@ -371,6 +526,69 @@ s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
         default:
             return fail(S2H_ERR_UNSUPPORTED, "nz=" + std::to_string(vol->dims[2]) + " exceeds 1024");
     }
+}
+
+// Autotuner: the cost model ranks every (band, waves) candidate; when one run
+// is short enough (modelled below kTuneMaxMs), the best kTuneCandidates are
+// timed on the device with HIP events on the context's stream and the fastest
+// is kept for the plan and recorded process-wide.  Contexts with autotuning
+// off (s2h_set_autotune, or S2HIP_AUTOTUNE=0) keep the model's choice.  Counts
+// are integers, so the choice never changes a result.
+static constexpr double kTuneMaxMs = 80.0;
+static constexpr int kTuneCandidates = 4;
+
+static s2h_status autotune(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
+    const std::vector<Ranked> ranked = rank_tunes(p);
+    if (ranked.empty() || ranked.front().ms > kTuneMaxMs) return S2H_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    S2H_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(S2H_ERR_HIP, "hipEventCreate failed");
+    }
+    s2h_status st = S2H_OK;
+    double best_ms = 1e30;
+    int best = -1, timed = 0;
+    const int ncand = std::min<int>(kTuneCandidates, (int)ranked.size());
+    for (int k = 0; k < ncand && st == S2H_OK; ++k) {
+        st = apply_tune(ctx, p, ranked[k].tune);
+        if (st == S2H_OK) st = launch_counts(ctx, vol, p);  // warm-up
+        if (st != S2H_OK) break;
+        hipError_t e = hipEventRecord(e0, ctx->stream);
+        if (e == hipSuccess) st = launch_counts(ctx, vol, p);
+        if (e == hipSuccess && st == S2H_OK) e = hipEventRecord(e1, ctx->stream);
+        if (e == hipSuccess && st == S2H_OK) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess && st == S2H_OK) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
+        if (st == S2H_OK && ms < best_ms) {
+            best_ms = ms;
+            best = k;
+        }
+        ++timed;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != S2H_OK) return st;
+    if (best >= 0) S2H_TRY(apply_tune(ctx, p, ranked[best].tune));
+    p->tuned = 1;
+    p->timed_candidates = timed;
+    tune_store(p, TuneEntry{p->tune, 1, timed});
+    return S2H_OK;
+}
+
+s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
+    if (vol->nwords * 8 + kVolTailBytes >= (1ull << 31))
+        return fail(S2H_ERR_UNSUPPORTED, "packed volume above 2 GiB");
+    if (supported_nw((int)((vol->dims[2] + 31) / 32)) < 0)
+        return fail(S2H_ERR_UNSUPPORTED, "nz=" + std::to_string(vol->dims[2]) + " exceeds 1024");
+    if (p->tuned == 0) {
+        if (autotune_enabled(ctx)) S2H_TRY(autotune(ctx, vol, p));
+        if (p->tuned == 0) p->tuned = -1;  // the model's choice is kept; not retried
+    }
+    S2H_HIP(hipMemsetAsync(p->d_table.ptr, 0, std::max<uint64_t>(p->table_size, 1) * 8, ctx->stream));
+    prof_mark(ctx, 0);
+    return launch_counts(ctx, vol, p);
 }
 
 s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const float *d_fcounts) {
@@ -434,6 +652,44 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
         S2H_HIP(hipMemcpyAsync(hits, p->d_hits.ptr, n_offsets * 8, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
+    return S2H_OK;
+}
+
+s2h_status s2h_exact_tune_info(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch, int64_t *info,
+                               uint64_t n) {
+    if (!ctx || !dims || !info) return fail(S2H_ERR_ARG, "null argument");
+    if (n != 8) return fail(S2H_ERR_SIZE, "info buffer must hold 8 values");
+    S2H_HIP(hipSetDevice(ctx->device));
+    ExactPlan *p = nullptr;
+    S2H_TRY(exact_plan_get(ctx, dims, r_max, pitch < 1e-9 ? 1e-9 : pitch, &p));
+    info[0] = p->tune.band;
+    info[1] = p->tune.waves;
+    info[2] = p->tuned;
+    info[3] = p->timed_candidates;
+    info[4] = (int64_t)p->items.size();
+    info[5] = p->ngroups;
+    info[6] = (int64_t)p->model_ops;
+    info[7] = 0;
+    return S2H_OK;
+}
+
+s2h_status s2h_exact_tune_set(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch, int32_t band,
+                              int32_t waves) {
+    if (!ctx || !dims) return fail(S2H_ERR_ARG, "null argument");
+    if (band < 1 || waves < 1) return fail(S2H_ERR_ARG, "band and waves must be >= 1");
+    S2H_HIP(hipSetDevice(ctx->device));
+    ExactPlan *p = nullptr;
+    S2H_TRY(exact_plan_get(ctx, dims, r_max, pitch < 1e-9 ? 1e-9 : pitch, &p));
+    S2H_TRY(apply_tune(ctx, p, ExactTune{band, waves}));
+    S2H_HIP(hipStreamSynchronize(ctx->stream));
+    p->tuned = -1;
+    p->timed_candidates = 0;
+    return S2H_OK;
+}
+
+s2h_status s2h_set_autotune(s2h_ctx *ctx, int32_t enable) {
+    if (!ctx) return fail(S2H_ERR_ARG, "null argument");
+    ctx->autotune = enable != 0;
     return S2H_OK;
 }
 

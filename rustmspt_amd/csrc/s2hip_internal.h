@@ -31,6 +31,16 @@ s2h_status fail(s2h_status code, const std::string &msg);
         if (st_ != S2H_OK) return st_;  \
     } while (0)
 
+// popcount(x) + acc as one v_bcnt_u32_b32.  Written out because the compiler
+// otherwise reassociates a chain of `acc += popcount(..)` into pairs of
+// popcounts joined by v_add3_u32: one extra VALU instruction per two words in
+// the counting loops.  Register operands only; no memory access.
+__device__ __forceinline__ uint32_t popc_acc(uint32_t x, uint32_t acc) {
+    uint32_t r;
+    asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+    return r;
+}
+
 // ---------------------------------------------------------------- shells (host)
 struct Offset {
     int32_t dx, dy, dz;
@@ -62,11 +72,28 @@ struct ColInfo {
 struct WorkItem {
     uint32_t group;   // 64-column group
     uint16_t dz0;     // first dz (multiple of 32)
-    uint16_t dzb;     // 8, 16 or 32 dz values
+    uint16_t dzb;     // 8, 16, 24 or 32 dz values
     uint16_t x0, x1;  // b-row x range
     uint16_t y0, y1;  // b-row y range
     uint16_t nw;      // dwords of the shifted partner that can be non-zero
     uint16_t pad;
+};
+
+// One (column group, dz block) of a plan: the unit the batched delta counts
+// iterate over.
+struct DzBlock {
+    uint32_t group;
+    uint16_t dz0, dzb;
+};
+
+// Tunable shape of an exact plan.  Results do not depend on it (integer counts).
+//   band  : columns are grouped in bands of `band` consecutive m values (ordered
+//           by angle inside a band): 1 keeps every group's m equal (no dz waste,
+//           long arcs -> idle boundary rows), larger bands shorten the arcs.
+//   waves : target number of waves of the count launch (x-chunking of blocks).
+struct ExactTune {
+    int32_t band = 3;
+    int32_t waves = 16384;
 };
 
 struct ExactPlan {
@@ -76,14 +103,21 @@ struct ExactPlan {
     std::vector<Offset> offs;             // shell order
     std::vector<uint64_t> radius_start;   // r_max + 2
     uint64_t table_size = 0;              // dense count-table entries
+    std::vector<ColInfo> colset;          // every half-ball column, table order
+    // tuned part: groups, blocks and items for `tune`
+    ExactTune tune;
+    int tuned = 0;                        // 0 not yet, 1 timed on the device, -1 model pick
+    int timed_candidates = 0;
+    double model_ops = 0;                 // modelled executed VALU lane-ops per run
     uint32_t ngroups = 0;
-    std::vector<WorkItem> items[3];       // per dz-block size 8, 16, 32
+    std::vector<DzBlock> blocks;
+    std::vector<WorkItem> items;          // heaviest first
     // device copies
     DevBuf d_offs;     // Offset[K]
     DevBuf d_gather;   // int32[K] table index (-1 = zero offset)
     DevBuf d_rstart;   // uint64[r_max+2]
     DevBuf d_cols;     // ColInfo[ngroups*64]
-    DevBuf d_items[3]; // WorkItem
+    DevBuf d_items;    // WorkItem[items]
     DevBuf d_table;    // uint64[table_size]
     DevBuf d_terms;    // double[K]
     DevBuf d_hits;     // uint64[K]
@@ -130,6 +164,7 @@ struct s2h_ctx {
     std::map<std::tuple<uint64_t, uint64_t, uint64_t>, s2h::FftPlan *> fft_plans;
     s2h::DevBuf d_scalar;   // small scratch (popcount result etc.)
     bool profiling = false;
+    bool autotune = true;   // time launch-shape candidates on a new exact plan
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double prof_ms[4] = {0, 0, 0, 0};
     int launches = 0;
@@ -144,6 +179,7 @@ s2h_status vol_popcount(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);
 
 s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
                           ExactPlan **out);
+void exact_plan_release(s2h_ctx *ctx, ExactPlan *plan);
 // Launch the count kernels for vol into plan->d_table (zeroed first) on ctx->stream.
 s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan);
 // Gather table -> per-offset hits and f64 terms, fold per radius into d_curve.

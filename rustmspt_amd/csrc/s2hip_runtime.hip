@@ -352,13 +352,7 @@ void s2h_destroy(s2h_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto &kv : ctx->exact_plans) {
-        ExactPlan *p = kv.second.get();
-        DevBuf *bufs[] = {&p->d_offs, &p->d_gather, &p->d_rstart, &p->d_cols, &p->d_items[0],
-                          &p->d_items[1], &p->d_items[2], &p->d_table, &p->d_terms, &p->d_hits,
-                          &p->d_curve};
-        for (DevBuf *b : bufs) dev_free(ctx, *b);
-    }
+    for (auto &kv : ctx->exact_plans) exact_plan_release(ctx, kv.second.get());
     for (auto &kv : ctx->mc_plans) {
         dev_free(ctx, kv.second->d_shell);
         dev_free(ctx, kv.second->d_start);

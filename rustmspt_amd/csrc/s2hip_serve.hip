@@ -210,12 +210,7 @@ void dispatch_loop(s2h_server *sv) {
 
 void exact_plans_clear(s2h_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto &kv : ctx->exact_plans) {
-        ExactPlan *p = kv.second.get();
-        DevBuf *bufs[] = {&p->d_offs, &p->d_gather, &p->d_rstart, &p->d_cols, &p->d_items[0], &p->d_items[1],
-                          &p->d_items[2], &p->d_table, &p->d_terms, &p->d_hits, &p->d_curve};
-        for (DevBuf *b : bufs) dev_free(ctx, *b);
-    }
+    for (auto &kv : ctx->exact_plans) exact_plan_release(ctx, kv.second.get());
     ctx->exact_plans.clear();
     for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
     ctx->fft_plans.clear();
@@ -241,7 +236,11 @@ s2h_status s2h_serve_create(int device, uint64_t hbm_budget_bytes, uint32_t nslo
     for (uint32_t i = 0; i < nslots; ++i) {
         auto *slot = new s2h_server::Slot();
         s2h_status st = s2h_create(device, 0, &slot->ctx);
+        // a stream of mixed shapes sees most (grid, r_max) pairs once: take the
+        // cost model's launch shape (or one timed earlier in the process)
+        if (st == S2H_OK) st = s2h_set_autotune(slot->ctx, 0);
         if (st != S2H_OK) {
+            if (slot->ctx) s2h_destroy(slot->ctx);
             delete slot;
             s2h_serve_destroy(sv);
             return st;

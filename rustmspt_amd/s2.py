@@ -79,6 +79,29 @@ class Context:
         L.check(L.lib().s2h_profile_read(self.handle, L.ptr(out, L._f64p), 4))
         return {"count_ms": out[0], "fold_ms": out[1], "device_ms": out[2], "launches": int(out[3])}
 
+    def valu_probe(self) -> float:
+        """Measured int32 VALU lane-operations per second (s2h_valu_probe)."""
+        v = C.c_double()
+        L.check(L.lib().s2h_valu_probe(self.handle, C.byref(v)))
+        return float(v.value)
+
+    def exact_tune_info(self, dims, r_max: int, pitch: float = 1.0) -> dict:
+        """Launch shape of the exact count kernel for this plan (see s2h_exact_tune_info)."""
+        out = np.zeros(8, np.int64)
+        L.check(L.lib().s2h_exact_tune_info(self.handle, L.ptr(_dims(dims), L._u64p), int(r_max), float(pitch),
+                                            L.ptr(out, L._i64p), 8))
+        return {"band": int(out[0]), "waves": int(out[1]),
+                "picked_by": {1: "device timing", -1: "cost model", 0: "not run"}[int(out[2])],
+                "timed_candidates": int(out[3]), "items": int(out[4]), "groups": int(out[5]),
+                "model_wave_instructions": int(out[6])}
+
+    def set_autotune(self, enable: bool) -> None:
+        L.check(L.lib().s2h_set_autotune(self.handle, 1 if enable else 0))
+
+    def exact_tune_set(self, dims, r_max: int, pitch: float, band: int, waves: int) -> None:
+        L.check(L.lib().s2h_exact_tune_set(self.handle, L.ptr(_dims(dims), L._u64p), int(r_max), float(pitch),
+                                           int(band), int(waves)))
+
     @property
     def bytes_in_use(self) -> int:
         v = C.c_uint64()

@@ -112,3 +112,38 @@ def test_256_cube_r64_sampled_offsets_and_properties(ctx):
     curve = vol.exact_s2(64)
     vf = occ.sum() / occ.size
     assert np.array_equal(curve, o.curve_from_hits(occ.shape, 64, offs, rs, got, vf))
+
+
+@pytest.mark.parametrize("band,waves", [(1, 64), (2, 1000), (3, 16384), (6, 100000), (40, 7)])
+def test_counts_independent_of_launch_shape(ctx, band, waves):
+    rng = np.random.default_rng(band * 1000 + waves)
+    occ = (rng.random((40, 37, 70)) < 0.3).astype(np.uint8)
+    vol = s2.Volume.from_bytes(ctx, occ)
+    offs, rs, want = hits_oracle(occ, 14)
+    ctx.exact_tune_set(occ.shape, 14, 1.0, band, waves)
+    assert np.array_equal(vol.exact_counts(14), want)
+    info = ctx.exact_tune_info(occ.shape, 14, 1.0)
+    assert (info["band"], info["waves"], info["picked_by"]) == (band, waves, "cost model")
+
+
+def test_autotuner_times_candidates_on_first_call(ctx):
+    from rustmspt_amd import synth
+    occ = synth.grf_volume(96, 4.0, 0.3, seed=3)[:95, :93, :91].copy()  # a shape no other test uses
+    vol = s2.Volume.from_bytes(ctx, occ)
+    before = ctx.exact_tune_info(occ.shape, 23, 1.0)
+    assert before["picked_by"] == "not run"
+    got = vol.exact_counts(23)
+    info = ctx.exact_tune_info(occ.shape, 23, 1.0)
+    assert info["picked_by"] == "device timing" and info["timed_candidates"] >= 2
+    # the timed shape is reused by a second context without timing again
+    with s2.Context(0) as other:
+        assert other.exact_tune_info(occ.shape, 23, 1.0)["picked_by"] == "device timing"
+    offs, rs, want = hits_oracle(occ, 23)
+    assert np.array_equal(got, want)
+    assert np.array_equal(vol.exact_counts(23), want)  # second call reuses the tuned plan
+
+
+def test_valu_probe_is_plausible(ctx):
+    # 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s at one 64-lane VALU op per 4 cycles per SIMD
+    rate = ctx.valu_probe()
+    assert 5e12 < rate < 200e12, rate
