@@ -24,10 +24,10 @@ sys.path.insert(0, str(ROOT))
 
 VALU_PEAK_OPS = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
 # HBM bytes of one evaluation's count phase at 256^3 r_max=64 (500-particle
-# packing), from rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) summed
-# over the three count dispatches; factors from a 512 MiB device copy.  Command
-# lines and per-dispatch figures: profiles/exact_count_kernel_256cube_r64.md.
-COUNT_PHASE_TRAFFIC = (11369 + 10615 + 10500) * 1024 * 2.0 + (389510 + 24791 + 6885) * 1024 * 1.0
+# packing, launch shape band 3 / 32768 waves), from rocprofv3 --pmc FETCH_SIZE
+# (x2.0) and WRITE_SIZE (x1.0) of the count dispatch; factors from a 512 MiB
+# device copy.  Command lines and figures: profiles/exact_count_kernel_256cube_r64.md.
+COUNT_PHASE_TRAFFIC = 12549 * 1024 * 2.0 + 1696499 * 1024 * 1.0
 
 
 def cpu_baseline(occ_words, dims, r_max, offs, pair_work_total, target_s):
@@ -280,6 +280,8 @@ def main():
     alg_ops = half_work / 32.0 * 2.0
     achieved = alg_ops / (count_avg * 1e-3)
     valu_measured = ctx.valu_probe()
+    shape = ctx.exact_tune_info([grid] * 3, r_max, 1.0)
+    profiled = (grid, r_max, args.particles, shape["band"], shape["waves"]) == (256, 64, 500, 3, 32768)
 
     out = {
         "metric": "exact-S2 evals/s at 256^3 r_max=64, batched vs plain; p50/p99 served latency",
@@ -300,12 +302,12 @@ def main():
                    "volume_fraction": occupied / grid ** 3, "voxelize_s": vox_s},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12,
                      "unit": "TOP/s (int32 lane-ops)", "frac": achieved / VALU_PEAK_OPS,
-                     "traffic": COUNT_PHASE_TRAFFIC if (grid, r_max, args.particles) == (256, 64, 500) else None,
+                     "traffic": COUNT_PHASE_TRAFFIC if profiled else None,
                      "kernel": "exact_count_kernel", "kernel_ms": count_avg, "launches_per_eval": launches,
                      "device_ms_per_eval": float(np.mean(dev_ms)),
                      "algorithmic_ops_per_eval": alg_ops,
                      "peak_measured": valu_measured / 1e12, "frac_of_measured_peak": achieved / valu_measured,
-                     "launch_shape": ctx.exact_tune_info([grid] * 3, r_max, 1.0)},
+                     "launch_shape": shape},
         "s2_head": [round(float(v), 6) for v in curve[:5]],
     }
     if not (args.no_fft and args.no_mc):

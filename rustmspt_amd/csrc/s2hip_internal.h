@@ -165,6 +165,8 @@ struct s2h_ctx {
     s2h::DevBuf d_scalar;   // small scratch (popcount result etc.)
     bool profiling = false;
     bool autotune = true;   // time launch-shape candidates on a new exact plan
+    std::multimap<uint64_t, void *> block_cache;  // released device blocks by size
+    uint64_t cached_bytes = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double prof_ms[4] = {0, 0, 0, 0};
     int launches = 0;
@@ -173,6 +175,7 @@ struct s2h_ctx {
 namespace s2h {
 s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes);
 void dev_free(s2h_ctx *ctx, DevBuf &b);
+void dev_cache_release(s2h_ctx *ctx);  // hipFree every cached block
 s2h_status upload(s2h_ctx *ctx, DevBuf &b, const void *src, uint64_t bytes);
 s2h_status vol_alloc(s2h_ctx *ctx, const uint64_t dims[3], s2h_vol **out);
 s2h_status vol_popcount(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);

@@ -17,32 +17,79 @@ s2h_status fail(s2h_status code, const std::string &msg) {
     return code;
 }
 
+// Device memory: blocks released by a context are kept in its cache and
+// reused for later requests of a similar size.  hipFree synchronizes the whole
+// device, which would drain every other stream (the serving slots) each time a
+// request releases its volume; a context has one stream, so a cached block is
+// reused in stream order without any synchronization.  The cache is trimmed
+// (hipFree) only past kCacheMaxBytes or when the budget needs the room.
+static constexpr uint64_t kCacheMaxBytes = 2ull << 30;
+
+static uint64_t round_block(uint64_t bytes) {
+    const uint64_t g = bytes <= (1ull << 20) ? 4096 : (64ull << 10);
+    return (bytes + g - 1) / g * g;
+}
+
+static void cache_trim(s2h_ctx *ctx, uint64_t keep) {
+    while (ctx->cached_bytes > keep && !ctx->block_cache.empty()) {
+        auto it = std::prev(ctx->block_cache.end());  // largest first
+        (void)hipFree(it->second);
+        ctx->cached_bytes -= it->first;
+        ctx->block_cache.erase(it);
+    }
+}
+
 s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes) {
     if (b.ptr && b.bytes >= bytes) return S2H_OK;
     dev_free(ctx, b);
     if (bytes == 0) bytes = 8;
-    if (ctx->budget && ctx->in_use + bytes > ctx->budget)
-        return fail(S2H_ERR_BUDGET, "device working set of " + std::to_string(ctx->in_use + bytes) +
+    const uint64_t want = round_block(bytes);
+    // a cached block no more than a quarter (+64 KiB) larger than asked for
+    auto it = ctx->block_cache.lower_bound(want);
+    if (it != ctx->block_cache.end() && it->first <= want + want / 4 + (64ull << 10)) {
+        if (ctx->budget && ctx->in_use + it->first > ctx->budget)
+            return fail(S2H_ERR_BUDGET, "device working set of " + std::to_string(ctx->in_use + it->first) +
+                                            " bytes exceeds budget " + std::to_string(ctx->budget));
+        b.ptr = it->second;
+        b.bytes = it->first;
+        ctx->cached_bytes -= it->first;
+        ctx->block_cache.erase(it);
+        ctx->in_use += b.bytes;
+        return S2H_OK;
+    }
+    if (ctx->budget && ctx->in_use + want > ctx->budget)
+        return fail(S2H_ERR_BUDGET, "device working set of " + std::to_string(ctx->in_use + want) +
                                         " bytes exceeds budget " + std::to_string(ctx->budget));
-    hipError_t e = hipMalloc(&b.ptr, bytes);
+    if (ctx->budget && ctx->in_use + ctx->cached_bytes + want > ctx->budget)
+        cache_trim(ctx, ctx->budget - ctx->in_use - want);
+    hipError_t e = hipMalloc(&b.ptr, want);
+    if (e != hipSuccess && ctx->cached_bytes) {
+        (void)hipGetLastError();
+        cache_trim(ctx, 0);
+        e = hipMalloc(&b.ptr, want);
+    }
     if (e != hipSuccess) {
         b.ptr = nullptr;
-        return fail(S2H_ERR_HIP, std::string("hipMalloc(") + std::to_string(bytes) +
+        return fail(S2H_ERR_HIP, std::string("hipMalloc(") + std::to_string(want) +
                                      ") failed: " + hipGetErrorString(e));
     }
-    b.bytes = bytes;
-    ctx->in_use += bytes;
+    b.bytes = want;
+    ctx->in_use += want;
     return S2H_OK;
 }
 
 void dev_free(s2h_ctx *ctx, DevBuf &b) {
     if (b.ptr) {
-        (void)hipFree(b.ptr);
         ctx->in_use -= b.bytes;
+        ctx->block_cache.emplace(b.bytes, b.ptr);
+        ctx->cached_bytes += b.bytes;
+        if (ctx->cached_bytes > kCacheMaxBytes) cache_trim(ctx, kCacheMaxBytes / 2);
     }
     b.ptr = nullptr;
     b.bytes = 0;
 }
+
+void dev_cache_release(s2h_ctx *ctx) { cache_trim(ctx, 0); }
 
 s2h_status upload(s2h_ctx *ctx, DevBuf &b, const void *src, uint64_t bytes) {
     S2H_TRY(dev_alloc(ctx, b, bytes));
@@ -95,6 +142,7 @@ s2h_status vol_alloc(s2h_ctx *ctx, const uint64_t dims[3], s2h_vol **out) {
         return st;
     }
     v->d_words = (uint64_t *)b.ptr;
+    v->alloc_bytes = b.bytes;  // the block as allocated (rounded up)
     hipError_t e = hipMemsetAsync((char *)b.ptr + v->nwords * 8, 0, kVolTailBytes, ctx->stream);
     if (e != hipSuccess) {
         dev_free(ctx, b);
@@ -360,6 +408,7 @@ void s2h_destroy(s2h_ctx *ctx) {
     }
     for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
     dev_free(ctx, ctx->d_scalar);
+    dev_cache_release(ctx);
     for (hipEvent_t &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);

@@ -188,6 +188,7 @@ def run_stream(server: Server, pool: List[Volume], schedule: List[Arrival], time
         time.sleep(0.002)
     th.join()
     lat = np.array([info["t_done"] - info["t_submit"] for _, info in done])
+    busy = float(sum(info["t_done"] - info["t_start"] for _, info in done))
     t_first = min(info["t_submit"] for _, info in done)
     t_last = max(info["t_done"] for _, info in done)
     st = server.stats()
@@ -196,5 +197,6 @@ def run_stream(server: Server, pool: List[Volume], schedule: List[Arrival], time
             "offered_rate": len(schedule) / (schedule[-1].t if schedule else 1.0),
             "throughput_evals_per_s": len(done) / max(t_last - t_first, 1e-9),
             "latency_p50_ms": float(np.percentile(lat, 50) * 1e3), "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "latency_max_ms": float(lat.max() * 1e3), "admitted": st["admitted"], "deferred": st["deferred"],
+            "latency_max_ms": float(lat.max() * 1e3), "slot_busy_s": busy, "makespan_s": t_last - t_first,
+            "admitted": st["admitted"], "deferred": st["deferred"],
             "failed": st["failed"], "peak_bytes": st["peak_bytes"], "budget": st["budget"]}
