@@ -87,7 +87,10 @@ def batched_leg(ctx, ncand: int, seed: int, plain_sample: int):
         si = int(rng.integers(0, 4))
         pi = int(rng.integers(0, len(parts[si])))
         moves.append((si, pi, parts[si][pi].moved(rng.uniform(-2, 2, 3), synth.random_rotation(rng)).mesh().vertices))
-    batch.batch_eval(ctx, seeds, moves[:8])  # warm-up (plans, code objects)
+    # first call: code objects, graph capture and the one-time row-chunk tuning
+    t0 = time.perf_counter()
+    batch.batch_eval(ctx, seeds, moves)
+    first = time.perf_counter() - t0
     t0 = time.perf_counter()
     curves = batch.batch_eval(ctx, seeds, moves)
     bt = time.perf_counter() - t0
@@ -111,7 +114,7 @@ def batched_leg(ctx, ncand: int, seed: int, plain_sample: int):
     out = {"candidates": ncand, "batched_s": bt, "batched_evals_per_s": ncand / bt,
            "plain_evals_per_s": plain_rate, "plain_sample": len(idx),
            "batched_over_plain": (ncand / bt) / plain_rate, "bit_exact_mismatches": mism,
-           "batched_device_ms": prof["device_ms"], "grids": [96, 128, 192, 256],
+           "batched_device_ms": prof["device_ms"], "first_call_s": first, "grids": [96, 128, 192, 256],
            "particles": [len(p) for p in parts]}
     for sd in seeds:
         sd.free()

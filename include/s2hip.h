@@ -183,38 +183,45 @@ s2h_status s2h_mc_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, u
 
 /* A resident seed packing for batched simulated-annealing evaluation: packed
  * occupancy, per-voxel granule coverage counts (VoxelCoverage,
- * src/geometry/s2.rs:266-296) and the seed's exact counts.  Arguments as for
- * s2h_voxelize, plus the r_max every candidate of this seed is evaluated at. */
+ * src/geometry/s2.rs:266-296), the seed's exact counts, and every particle's
+ * geometry.  Arguments as for s2h_voxelize, plus the particles -- particle p owns
+ * vertices [particle_vert_start[p], particle_vert_start[p+1]) and triangles
+ * [particle_tri_start[p], particle_tri_start[p+1]), and its triangles use only
+ * its own vertices (the per-particle meshes of the packing, concatenated) --
+ * and the r_max every candidate of this seed is evaluated at. */
 typedef struct s2h_seed s2h_seed;
 s2h_status s2h_seed_create(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri,
                            uint64_t ntris, const uint32_t *granule_of_tri, uint32_t ngranules,
-                           const double bbox_min[3], double pitch, const uint64_t dims[3],
-                           uint32_t r_max, s2h_seed **out);
+                           const uint32_t *particle_vert_start, const uint32_t *particle_tri_start,
+                           uint32_t nparticles, const double bbox_min[3], double pitch,
+                           const uint64_t dims[3], uint32_t r_max, s2h_seed **out);
 void s2h_seed_free(s2h_ctx *ctx, s2h_seed *seed);
 /* The seed's packed volume (owned by the seed; valid until s2h_seed_free). */
 s2h_status s2h_seed_volume(s2h_seed *seed, s2h_vol **vol);
 
-/* One candidate: one particle of a seed moved (VoxelCoverage::replace,
- * src/geometry/s2.rs:314-317).  old_tri / new_tri hold 9 doubles per triangle
- * (vertices a, b, c) -- old_tri exactly as the particle appears in the seed --
- * and granule[] the particle's component id per triangle (0..ngranules-1). */
+/* One candidate: particle `particle` of seed `seed` moved to new vertex
+ * positions (VoxelCoverage::replace, src/geometry/s2.rs:314-317).  new_xyz holds
+ * 3 doubles per vertex of that particle, in the seed's vertex order. */
 typedef struct {
     uint32_t seed;
-    uint32_t ntris;
-    uint32_t ngranules;
-    uint32_t reserved;
-    const double *old_tri;
-    const double *new_tri;
-    const uint32_t *granule;
+    uint32_t particle;
+    const double *new_xyz;
 } s2h_candidate;
 
 /* Exact S2 curves of every candidate packing in one pass (replaces the per-
  * candidate island_s2 -> evaluate_voxel_grid of run_sa_island,
  * src/pipeline/optimize.rs:719-739): curves receives, per candidate in order,
- * r_max(seed)+1 values, bit-identical to a full evaluation of that packing. */
+ * r_max(seed)+1 values, bit-identical to a full evaluation of that packing.
+ * The device work is one replay of a HIP graph captured once per context. */
 s2h_status s2h_batch_eval(s2h_ctx *ctx, s2h_seed *const *seeds, uint32_t nseeds,
                           const s2h_candidate *cands, uint32_t ncand, double *curves,
                           uint64_t ncurve);
+/* Commit a move into the seed (run_sa_island's accept path,
+ * src/pipeline/optimize.rs:740-771, with VoxelCoverage::replace): coverage,
+ * occupancy rows, exact counts and the particle's vertices are updated on the
+ * device; curve (r_max+1 values, may be null) receives the new seed's curve. */
+s2h_status s2h_seed_accept(s2h_ctx *ctx, s2h_seed *seed, uint32_t particle, const double *new_xyz,
+                           double *curve, uint64_t n);
 
 /* ------------------------------------------------------------------ serving */
 

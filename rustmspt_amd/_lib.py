@@ -58,8 +58,9 @@ SIGNATURES = {
     "s2h_serve_submit": (C.c_int32, [_vp, _vp, _u64p]),
     "s2h_serve_poll": (C.c_int32, [_vp, C.c_uint64, C.c_int32, _vp, _f64p, C.c_uint64]),
     "s2h_serve_stats": (C.c_int32, [_vp, _u64p, C.c_uint64]),
-    "s2h_seed_create": (C.c_int32, [_vp, _f64p, C.c_uint64, _u32p, C.c_uint64, _u32p, C.c_uint32, _f64p,
-                                    C.c_double, _u64p, C.c_uint32, C.POINTER(_vp)]),
+    "s2h_seed_create": (C.c_int32, [_vp, _f64p, C.c_uint64, _u32p, C.c_uint64, _u32p, C.c_uint32, _u32p, _u32p,
+                                    C.c_uint32, _f64p, C.c_double, _u64p, C.c_uint32, C.POINTER(_vp)]),
+    "s2h_seed_accept": (C.c_int32, [_vp, _vp, C.c_uint32, _f64p, _f64p, C.c_uint64]),
     "s2h_seed_free": (None, [_vp, _vp]),
     "s2h_seed_volume": (C.c_int32, [_vp, C.POINTER(_vp)]),
     "s2h_batch_eval": (C.c_int32, [_vp, C.POINTER(_vp), C.c_uint32, _vp, C.c_uint32, _f64p, C.c_uint64]),
@@ -71,8 +72,7 @@ SIGNATURES = {
 
 class Candidate(C.Structure):
     """s2h_candidate (include/s2hip.h)."""
-    _fields_ = [("seed", C.c_uint32), ("ntris", C.c_uint32), ("ngranules", C.c_uint32), ("reserved", C.c_uint32),
-                ("old_tri", _f64p), ("new_tri", _f64p), ("granule", _u32p)]
+    _fields_ = [("seed", C.c_uint32), ("particle", C.c_uint32), ("new_xyz", _f64p)]
 
 
 class Request(C.Structure):

@@ -1,10 +1,13 @@
 """Batched evaluation of simulated-annealing candidates (one particle move each)
-against resident seed packings: s2h_seed_create / s2h_batch_eval.
+against resident seed packings: s2h_seed_create / s2h_batch_eval /
+s2h_seed_accept.
 
-Mirrors the per-candidate work of run_sa_island (src/pipeline/optimize.rs:719-739:
+Mirrors the per-candidate work of run_sa_island (src/pipeline/optimize.rs:719-771:
 coverage.replace, evaluate_voxel_grid, then accept or restore) for many
 candidates at once; each returned curve is bit-identical to a full exact
-evaluation of the candidate packing.
+evaluation of the candidate packing.  A candidate is (seed, particle, new
+vertices): the seed keeps every particle's geometry on the device, so a batch
+uploads only the moved particles' new vertex positions.
 """
 from __future__ import annotations
 
@@ -19,7 +22,8 @@ from .s2 import Context, Volume, occupancy_dims
 
 
 class Seed:
-    """A resident seed packing built from per-particle meshes."""
+    """A resident seed packing built from per-particle meshes (particle i owns
+    the i-th mesh's vertices and triangles)."""
 
     def __init__(self, ctx: Context, particle_meshes: Sequence[Mesh], bbox: BoundingBox, pitch: float, r_max: int):
         self.ctx = ctx
@@ -27,40 +31,28 @@ class Seed:
         self.r_max = int(r_max)
         self.bbox = bbox
         self.dims = occupancy_dims(bbox, self.pitch)
-        self.meshes = [Mesh(m.vertices.copy(), m.faces.copy()) for m in particle_meshes]
+        self.meshes = [Mesh(np.ascontiguousarray(m.vertices, np.float64), m.faces.copy()) for m in particle_meshes]
         mesh = merge_meshes(self.meshes)
         gid, ng = granules(mesh)
+        vstart = np.zeros(len(self.meshes) + 1, np.uint32)
+        tstart = np.zeros(len(self.meshes) + 1, np.uint32)
+        vstart[1:] = np.cumsum([len(m.vertices) for m in self.meshes])
+        tstart[1:] = np.cumsum([len(m.faces) for m in self.meshes])
         h = C.c_void_p()
         L.check(L.lib().s2h_seed_create(ctx.handle, L.ptr(mesh.vertices.reshape(-1), L._f64p), len(mesh.vertices),
                                         L.ptr(mesh.faces.reshape(-1), L._u32p), len(mesh.faces), L.ptr(gid, L._u32p),
-                                        ng, L.ptr(np.asarray(bbox.min, np.float64), L._f64p), self.pitch,
+                                        ng, L.ptr(vstart, L._u32p), L.ptr(tstart, L._u32p), len(self.meshes),
+                                        L.ptr(np.asarray(bbox.min, np.float64), L._f64p), self.pitch,
                                         L.ptr(np.asarray(self.dims, np.uint64), L._u64p), self.r_max, C.byref(h)))
         self.handle = h
-        # granule ids of each particle, local to the particle
-        self._local_gran = []
-        for m in self.meshes:
-            g, _ = granules(m)
-            self._local_gran.append(np.ascontiguousarray(g, np.uint32))
 
     def volume(self) -> Volume:
         v = C.c_void_p()
         L.check(L.lib().s2h_seed_volume(self.handle, C.byref(v)))
-        vol = Volume(self.ctx, v)
-        vol.handle = None  # owned by the seed: never freed through this view
-        vol._view = v
-        return vol
+        return Volume(self.ctx, v, owned=False)
 
     def exact_s2(self) -> np.ndarray:
-        view = self.volume()
-        out = np.zeros(self.r_max + 1, np.float64)
-        L.check(L.lib().s2h_exact_s2(self.ctx.handle, view._view, self.r_max, self.pitch, L.ptr(out, L._f64p),
-                                     len(out)))
-        return out
-
-    def triangles(self, index: int, vertices=None) -> np.ndarray:
-        m = self.meshes[index]
-        v = m.vertices if vertices is None else np.asarray(vertices, np.float64)
-        return np.ascontiguousarray(v[m.faces].reshape(-1))
+        return self.volume().exact_s2(self.r_max, self.pitch)
 
     def moved_mesh(self, index: int, new_vertices) -> Mesh:
         """The whole packing with particle `index` at new_vertices."""
@@ -68,9 +60,17 @@ class Seed:
         parts[index] = Mesh(np.asarray(new_vertices, np.float64), self.meshes[index].faces)
         return merge_meshes(parts)
 
-    def accept(self, index: int, new_vertices) -> None:
-        """Record a move on the host copy (the device seed is rebuilt by the caller)."""
-        self.meshes[index] = Mesh(np.asarray(new_vertices, np.float64), self.meshes[index].faces)
+    def accept(self, index: int, new_vertices) -> np.ndarray:
+        """Commit a move on the device (coverage, rows, counts, geometry) and
+        return the new seed's exact curve."""
+        newv = np.ascontiguousarray(np.asarray(new_vertices, np.float64).reshape(-1, 3))
+        if newv.shape != self.meshes[index].vertices.shape:
+            raise ValueError("new vertices must match the particle's vertex count")
+        out = np.zeros(self.r_max + 1, np.float64)
+        L.check(L.lib().s2h_seed_accept(self.ctx.handle, self.handle, int(index), L.ptr(newv.reshape(-1), L._f64p),
+                                        L.ptr(out, L._f64p), len(out)))
+        self.meshes[index] = Mesh(newv.copy(), self.meshes[index].faces)
+        return out
 
     def free(self):
         if self.handle:
@@ -93,16 +93,13 @@ def batch_eval(ctx: Context, seeds: List[Seed], moves) -> List[np.ndarray]:
     total = 0
     for i, (si, pi, newv) in enumerate(moves):
         sd = seeds[si]
-        old = sd.triangles(pi)
-        new = sd.triangles(pi, newv)
-        g = sd._local_gran[pi]
-        keep += [old, new, g]
+        v = np.ascontiguousarray(newv, np.float64)
+        if v.shape != sd.meshes[pi].vertices.shape:
+            raise ValueError("new vertices must match the particle's vertex count")
+        keep.append(v)
         cands[i].seed = si
-        cands[i].ntris = len(sd.meshes[pi].faces)
-        cands[i].ngranules = int(g.max()) + 1 if len(g) else 0
-        cands[i].old_tri = L.ptr(old, L._f64p)
-        cands[i].new_tri = L.ptr(new, L._f64p)
-        cands[i].granule = L.ptr(g, L._u32p)
+        cands[i].particle = pi
+        cands[i].new_xyz = L.ptr(v.reshape(-1), L._f64p)
         total += sd.r_max + 1
     out = np.zeros(max(total, 1), np.float64)
     handles = (C.c_void_p * max(len(seeds), 1))(*[s.handle for s in seeds])

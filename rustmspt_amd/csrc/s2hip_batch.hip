@@ -9,23 +9,30 @@
 //
 //  1. the moved particle's granules are re-voxelized at the old and the new
 //     position with the bit-exact predicate, as -1 / +1 coverage deltas inside
-//     the candidate's changed box B;
-//  2. the rows crossing B are rebuilt: new bit = (coverage + delta) > 0;
-//  3. exact delta counts.  C'(d) - C(d) is a sum over row pairs (u, u+(dx,dy))
-//     with at least one changed row.  T1 handles pairs whose partner row
-//     q = u+(dx,dy) changed: q is the wave-uniform row (shifted right by dz),
-//     lanes hold u = q-(dx,dy), new and old versions.  T2 handles pairs whose
-//     first row q changed and whose partner v = q+(dx,dy) lies outside B: q is
-//     uniform and shifted LEFT by dz against the unchanged lane row v.  Every
-//     pair with a changed row is counted exactly once; only the words whose bits
-//     can differ are visited;
+//     the candidate's changed box B (triangle data prepared on the device from
+//     the seed's resident geometry and the uploaded new vertices);
+//  2. for every row crossing B, the added bits A = new & ~old and the removed
+//     bits D = old & ~new (new bit = coverage + delta > 0), and each row's
+//     z-span of changes;
+//  3. exact delta counts.  With n = o + A - D,
+//        C'(d) - C(d) = X_A(d) + X_A(-d) - X_D(d) - X_D(-d) + Y(d),
+//        X_M(d) = sum_u o(u) M(u + d),   Y(d) = sum_u (A - D)(u) (A - D)(u + d).
+//     A wave takes one changed row q (wave-uniform: its A and D words shifted
+//     by dz on the scalar unit) and 64 half-ball columns (dx, dy) as lanes,
+//     which read the unchanged seed rows q -/+ (dx, dy) -- so only the words
+//     inside q's own change span are visited, and unchanged rows are skipped;
 //  4. C' = C + dC, then the reference's ordered f64 fold per radius -> curves.
 // All integer arithmetic, so every curve is bit-identical to a full evaluation
-// of the candidate packing.  The kernels are grid-stride over device-resident
-// work descriptors, so one launch shape serves any batch and the sequence can
-// be captured in a HIP graph and replayed.
+// of the candidate packing.
+//
+// Every kernel is grid-stride over work described in a device header, so the
+// launch sequence does not depend on the batch: it is captured once into a HIP
+// graph over a persistent workspace and replayed for every later batch (the
+// inputs are copied in before each replay).  A committed move (s2h_seed_accept)
+// applies one candidate's deltas to the seed's coverage, rows and counts.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "s2hip_internal.h"
@@ -56,23 +63,66 @@ struct SeedDev {
 struct CandDev {
     uint32_t seed;
     int32_t bx0, by0, bz0, bx1, by1, bz1;   // changed box, voxel index ranges
-    uint32_t nchunks;                        // row chunks of the footprint
-    uint64_t rows_off;                       // u32 offset of the rebuilt rows
+    uint32_t nchunks;                        // row chunks of the box
+    uint64_t ad_off;                         // uint2 offset of the (A, D) words (box rows x nwd)
+    uint64_t info_off;                       // uint2 offset of the row change spans (box rows)
     uint64_t box_off;                        // int32 offset of the coverage delta box
     uint64_t delta_off;                      // int32 offset of the delta table
     uint64_t item_start;                     // first delta item of this candidate
     uint64_t curve_off;                      // double offset of the curve values
 };
 
-static constexpr int kRowChunk = 64;
-
 struct BatchHeader {
-    uint64_t nitems;      // delta-count items
+    uint64_t nitems;       // delta-count items
+    uint64_t ad_total;     // uint2 words of (A, D) rows
+    uint64_t info_total;   // rows of all boxes
+    uint64_t box_words;    // int32 entries of all coverage-delta boxes
+    uint64_t delta_words;  // int32 entries of all delta tables
     uint32_t ncand;
-    uint32_t nfold;       // (candidate, radius) fold tasks
+    uint32_t nfold;        // (candidate, radius) fold tasks
+    uint32_t ngran;        // prepared granules (both positions of every candidate)
+    uint32_t ntiles;
+    uint32_t row_chunk;    // box rows per delta item
+    uint32_t pad;
 };
 
-// ---------------------------------------------------------------- 1. coverage deltas
+constexpr uint32_t kOverflowCap = 1u << 16;
+
+// ---------------------------------------------------------------- 0. clear
+
+__global__ void clear_kernel(const BatchHeader *__restrict__ hdr, int32_t *__restrict__ delta,
+                             int32_t *__restrict__ box, uint2 *__restrict__ info, int *__restrict__ occ_delta,
+                             uint32_t *__restrict__ overflow) {
+    const uint64_t nd = hdr->delta_words, nb = hdr->box_words, ni = hdr->info_total, nc = hdr->ncand;
+    const uint64_t total = nd + nb + ni + nc + 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < nd) delta[i] = 0;
+        else if (i < nd + nb) box[i - nd] = 0;
+        else if (i < nd + nb + ni) info[i - nd - nb] = make_uint2(0u, 0u);
+        else if (i < nd + nb + ni + nc) occ_delta[i - nd - nb - ni] = 0;
+        else overflow[0] = 0;
+    }
+}
+
+// ---------------------------------------------------------------- 1. triangle data and coverage deltas
+
+// One block per granule: the predicate's precomputed triangle data from the
+// granule's vertices (resident seed geometry or the uploaded new vertices).
+__global__ __launch_bounds__(256) void prep_kernel(const vox::GranuleInfo *__restrict__ gran,
+                                                   const BatchHeader *__restrict__ hdr,
+                                                   vox::TriGeo *__restrict__ geo) {
+    const uint32_t ng = hdr->ngran;
+    for (uint32_t g = blockIdx.x; g < ng; g += gridDim.x) {
+        const vox::GranuleInfo gi = gran[g];
+        const uint32_t n = gi.tri_end - gi.tri_start;
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+            const uint32_t *t = gi.tri + 3 * (uint64_t)k;
+            vox::tri_prepare(gi.xyz + 3 * (uint64_t)t[0], gi.xyz + 3 * (uint64_t)t[1], gi.xyz + 3 * (uint64_t)t[2],
+                             gi.cmax, gi.smax, gi.amax, geo[gi.tri_start + k]);
+        }
+    }
+}
 
 struct AddDelta {
     const CandDev *cands;
@@ -89,17 +139,16 @@ struct AddDelta {
 __global__ __launch_bounds__(256) void batch_voxel_kernel(const vox::TriGeo *__restrict__ tris,
                                                           const vox::GranuleInfo *__restrict__ gran,
                                                           const vox::Tile *__restrict__ tiles,
-                                                          const uint32_t *__restrict__ ntiles,
+                                                          const BatchHeader *__restrict__ hdr,
                                                           const double *__restrict__ origin,  // per candidate: x,y,z,pitch
                                                           const CandDev *__restrict__ cands,
-                                                          int32_t *__restrict__ box, uint32_t *__restrict__ overflow,
-                                                          uint32_t overflow_cap) {
-    const uint32_t n = *ntiles;
+                                                          int32_t *__restrict__ box, uint32_t *__restrict__ overflow) {
+    const uint32_t n = hdr->ntiles;
     for (uint32_t t = blockIdx.x; t < n; t += gridDim.x) {
         const vox::Tile tl = tiles[t];
         const uint32_t c = gran[tl.granule].owner;
         const double *o = origin + 4 * (uint64_t)c;
-        vox::voxelize_tile(tris, gran, tl, t, o[0], o[1], o[2], o[3], overflow, overflow_cap, AddDelta{cands, box});
+        vox::voxelize_tile(tris, gran, tl, t, o[0], o[1], o[2], o[3], overflow, kOverflowCap, AddDelta{cands, box});
         __syncthreads();
     }
 }
@@ -108,8 +157,8 @@ __global__ void batch_voxel_overflow_kernel(const vox::TriGeo *__restrict__ tris
                                             const vox::GranuleInfo *__restrict__ gran,
                                             const vox::Tile *__restrict__ tiles, const double *__restrict__ origin,
                                             const CandDev *__restrict__ cands, int32_t *__restrict__ box,
-                                            const uint32_t *__restrict__ overflow, uint32_t cap) {
-    const uint32_t count = min(overflow[0], cap);
+                                            const uint32_t *__restrict__ overflow) {
+    const uint32_t count = min(overflow[0], kOverflowCap);
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
         const vox::Tile tl = tiles[overflow[1 + 2 * k]];
         const uint32_t c = gran[tl.granule].owner;
@@ -118,28 +167,32 @@ __global__ void batch_voxel_overflow_kernel(const vox::TriGeo *__restrict__ tris
     }
 }
 
-// ---------------------------------------------------------------- 2. rebuilt rows
+// ---------------------------------------------------------------- 2. added / removed bits
 
-// One thread per (candidate, footprint row, dword).  Bits outside [bz0, bz1)
-// are copied from the seed; inside, bit = coverage + delta > 0.
-__global__ void build_rows_kernel(const SeedDev *__restrict__ seeds, const CandDev *__restrict__ cands,
-                                  uint32_t ncand, const uint64_t *__restrict__ row_prefix,
-                                  const int32_t *__restrict__ box, uint32_t *__restrict__ rows,
-                                  int *__restrict__ occ_delta) {
-    const uint64_t total = row_prefix[ncand];
+// One thread per (candidate, box row, dword < nwd).  Inside [bz0, bz1) the new
+// bit is coverage + delta > 0; outside, nothing changes.  Row change spans are
+// kept as (nz - lowest changed z, highest changed z + 1) via atomic max, so a
+// zeroed entry means "row unchanged".
+__global__ void build_ad_kernel(const SeedDev *__restrict__ seeds, const CandDev *__restrict__ cands,
+                                const BatchHeader *__restrict__ hdr, const uint64_t *__restrict__ ad_prefix,
+                                const int32_t *__restrict__ box, uint2 *__restrict__ ad, uint2 *__restrict__ info,
+                                int *__restrict__ occ_delta) {
+    const uint32_t ncand = hdr->ncand;
+    const uint64_t total = ad_prefix[ncand];
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t lo = 0, hi = ncand;  // candidate of dword i
+        uint32_t lo = 0, hi = ncand;  // candidate of word i
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) / 2;
-            if (row_prefix[mid] <= i) lo = mid;
+            if (ad_prefix[mid] <= i) lo = mid;
             else hi = mid;
         }
         const CandDev &c = cands[lo];
         const SeedDev &s = seeds[c.seed];
-        const uint64_t local = i - row_prefix[lo];
-        const uint64_t r = local / s.row_dw;
-        const int w = (int)(local % s.row_dw);
+        const int nwd = (s.nz + 31) / 32;
+        const uint64_t local = i - ad_prefix[lo];
+        const uint64_t r = local / nwd;
+        const int w = (int)(local % nwd);
         const int sy = c.by1 - c.by0, sz = c.bz1 - c.bz0;
         const int x = c.bx0 + (int)(r / sy), y = c.by0 + (int)(r % sy);
         const uint32_t old = s.vol[((uint64_t)x * s.ny + y) * s.row_dw + w];
@@ -152,82 +205,94 @@ __global__ void build_rows_kernel(const SeedDev *__restrict__ seeds, const CandD
             const uint32_t bit = 1u << (z - 32 * w);
             v = cnt > 0 ? (v | bit) : (v & ~bit);
         }
-        rows[c.rows_off + local] = v;
-        const int d = __popc(v) - __popc(old);
-        if (d) atomicAdd(&occ_delta[lo], d);
+        const uint32_t add = v & ~old, rem = old & ~v, chg = add | rem;
+        ad[c.ad_off + local] = make_uint2(add, rem);
+        if (chg) {
+            const int zlo = 32 * w + __builtin_ctz(chg), zhi = 32 * w + 32 - __builtin_clz(chg);
+            atomicMax(&info[c.info_off + r].x, (uint32_t)(s.nz - zlo));
+            atomicMax(&info[c.info_off + r].y, (uint32_t)zhi);
+            atomicAdd(&occ_delta[lo], __popc(add) - __popc(rem));
+        }
     }
 }
 
 // ---------------------------------------------------------------- 3. delta counts
 
-// uniform dword k of a row (zero beyond the row's voxels)
-__device__ __forceinline__ uint32_t urow(const uint32_t *row, int k, int nwd) {
+// (A, D) words k of a changed row (zero outside the row's words)
+__device__ __forceinline__ uint2 adw(const uint2 *row, int k, int nwd) {
     const int kk = k < 0 ? 0 : (k < nwd ? k : nwd - 1);
-    const uint32_t v = row[kk];
-    return (k >= 0 && k < nwd) ? v : 0u;
+    const uint2 v = row[kk];
+    return (k >= 0 && k < nwd) ? v : make_uint2(0u, 0u);
 }
 
 template <int DZB>
-__device__ void delta_item(const SeedDev &s, const CandDev &c, uint32_t cand, const GB gb, int chunk,
-                           const uint32_t *__restrict__ rows, int32_t *__restrict__ delta) {
+__device__ void delta_item(const SeedDev &s, const CandDev &c, const GB gb, int chunk, int row_chunk,
+                           const uint2 *__restrict__ ad, const __amdgpu_buffer_rsrc_t ad_rs,
+                           const uint2 *__restrict__ info, int32_t *__restrict__ delta) {
     const int lane = threadIdx.x & 63;
     const ColInfo col = s.cols[gb.group * 64 + lane];
     const int nwd = (s.nz + 31) / 32;
-    const int q0 = gb.dz0 >> 5;
-    const int sy = c.by1 - c.by0;
-    const int nrows = (c.bx1 - c.bx0) * sy;
-    const int r0 = chunk * kRowChunk, r1 = min(nrows, r0 + kRowChunk);
+    const int q0 = gb.dz0 >> 5, dz0 = gb.dz0;
+    const int sx = c.bx1 - c.bx0, sy = c.by1 - c.by0;
+    const int nrows = sx * sy;
+    const int r0 = chunk * row_chunk, r1 = min(nrows, r0 + row_chunk);
     const __amdgpu_buffer_rsrc_t seed_rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)s.vol, 0, (int)s.vol_bytes, 0x00020000);
-    const int oob = 0x7ffffff0;
-    // T1 words: a changed in [bz0, bz1); (b >> dz) changed at z in [bz0 - dz, bz1 - dz)
-    const int t1_lo = max(0, c.bz0 - gb.dz0 - DZB + 1) >> 5, t1_hi = min(nwd, (c.bz1 + 31) >> 5);
-    // T2 words: (b << dz) changed at z in [bz0 + dz, bz1 + dz)
-    const int t2_lo = min(nwd, (c.bz0 + gb.dz0) >> 5), t2_hi = min(nwd, (c.bz1 + gb.dz0 + DZB - 1 + 31) >> 5);
-    uint32_t cn[DZB], co[DZB];
+    const int oob = 0x7fffff00;  // + up to 255 bytes: past any buffer, no int overflow
+    uint32_t cp[DZB], cm[DZB];  // added and subtracted pair counts per dz
 #pragma unroll
-    for (int j = 0; j < DZB; ++j) cn[j] = co[j] = 0;
+    for (int j = 0; j < DZB; ++j) cp[j] = cm[j] = 0;
     const bool active = col.m >= 0;
     for (int r = r0; r < r1; ++r) {
+        const uint2 span = info[c.info_off + r];
+        if (span.y == 0) continue;  // row unchanged (uniform)
+        const int zlo = s.nz - (int)span.x, zhi = (int)span.y;
         const int xq = c.bx0 + r / sy, yq = c.by0 + r % sy;
-        const uint32_t *qn = rows + c.rows_off + (uint64_t)r * s.row_dw;               // new q row
-        const uint32_t *qo = s.vol + ((uint64_t)xq * s.ny + yq) * s.row_dw;         // old q row
-        // ---- T1: lanes u = q - (dx, dy)
+        const uint2 *m = ad + c.ad_off + (uint64_t)r * nwd;  // uniform (A, D) row of q
+        // ---- X(+d) and Y: lanes u = q - (dx, dy); M(q) >> dz meets u at z in [zlo - dz, zhi - dz)
         {
             const int xu = xq - col.dx, yu = yq - col.dy;
             const bool in = active && xu >= 0 && xu < s.nx && yu >= 0 && yu < s.ny;
-            const bool fb = in && xu >= c.bx0 && xu < c.bx1 && yu >= c.by0 && yu < c.by1;
             const int aoff = in ? (int)(((uint64_t)xu * s.ny + yu) * s.row_dw * 4) : oob;
-            const uint32_t *anew = fb ? rows + c.rows_off + ((uint64_t)(xu - c.bx0) * sy + (yu - c.by0)) * s.row_dw
-                                      : nullptr;
-            for (int w = t1_lo; w < t1_hi; ++w) {
-                const uint32_t ao = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, aoff + 4 * w, 0, 0);
-                const uint32_t an = fb ? anew[w] : ao;
-                const uint32_t bn0 = urow(qn, w + q0, nwd), bn1 = urow(qn, w + q0 + 1, nwd);
-                const uint32_t bo0 = urow(qo, w + q0, nwd), bo1 = urow(qo, w + q0 + 1, nwd);
-                const uint64_t pn = ((uint64_t)bn1 << 32) | bn0, po = ((uint64_t)bo1 << 32) | bo0;
+            const bool inbox = in && xu >= c.bx0 && xu < c.bx1 && yu >= c.by0 && yu < c.by1;
+            const int uoff = inbox ? (int)((c.ad_off + ((uint64_t)(xu - c.bx0) * sy + (yu - c.by0)) * nwd) * 8) : oob;
+            const bool any_y = __any(inbox);
+            const int w_lo = max(0, zlo - dz0 - DZB + 1) >> 5, w_hi = min(nwd, (zhi - dz0 + 31) >> 5);
+            for (int w = w_lo; w < w_hi; ++w) {
+                const uint32_t a = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, aoff + 4 * w, 0, 0);
+                const uint2 m0 = adw(m, w + q0, nwd), m1 = adw(m, w + q0 + 1, nwd);
+                const uint64_t pa = ((uint64_t)m1.x << 32) | m0.x, pd = ((uint64_t)m1.y << 32) | m0.y;
 #pragma unroll
                 for (int j = 0; j < DZB; ++j) {
-                    cn[j] = popc_acc(an & (uint32_t)(pn >> j), cn[j]);
-                    co[j] = popc_acc(ao & (uint32_t)(po >> j), co[j]);
+                    cp[j] = popc_acc(a & (uint32_t)(pa >> j), cp[j]);
+                    cm[j] = popc_acc(a & (uint32_t)(pd >> j), cm[j]);
+                }
+                if (any_y) {
+                    const uint32_t ua = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w, 0, 0);
+                    const uint32_t ud = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w + 4, 0, 0);
+#pragma unroll
+                    for (int j = 0; j < DZB; ++j) {
+                        const uint32_t sa = (uint32_t)(pa >> j), sd = (uint32_t)(pd >> j);
+                        cp[j] = popc_acc(ua & sa, popc_acc(ud & sd, cp[j]));
+                        cm[j] = popc_acc(ua & sd, popc_acc(ud & sa, cm[j]));
+                    }
                 }
             }
         }
-        // ---- T2: lanes v = q + (dx, dy) outside the changed box
+        // ---- X(-d): lanes v = q + (dx, dy); M(q) << dz meets v at z in [zlo + dz, zhi + dz)
         {
             const int xv = xq + col.dx, yv = yq + col.dy;
-            const bool in = active && xv >= 0 && xv < s.nx && yv >= 0 && yv < s.ny &&
-                            !(xv >= c.bx0 && xv < c.bx1 && yv >= c.by0 && yv < c.by1);
-            const int aoff = in ? (int)(((uint64_t)xv * s.ny + yv) * s.row_dw * 4) : oob;
-            for (int w = t2_lo; w < t2_hi; ++w) {
-                const uint32_t a = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, aoff + 4 * w, 0, 0);
-                const uint32_t bn1 = urow(qn, w - q0, nwd), bn0 = urow(qn, w - q0 - 1, nwd);
-                const uint32_t bo1 = urow(qo, w - q0, nwd), bo0 = urow(qo, w - q0 - 1, nwd);
-                const uint64_t pn = ((uint64_t)bn1 << 32) | bn0, po = ((uint64_t)bo1 << 32) | bo0;
+            const bool in = active && xv >= 0 && xv < s.nx && yv >= 0 && yv < s.ny;
+            const int voff = in ? (int)(((uint64_t)xv * s.ny + yv) * s.row_dw * 4) : oob;
+            const int w_lo = min(nwd, (zlo + dz0) >> 5), w_hi = min(nwd, (zhi + dz0 + DZB - 1 + 31) >> 5);
+            for (int w = w_lo; w < w_hi; ++w) {
+                const uint32_t a = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, voff + 4 * w, 0, 0);
+                const uint2 m1 = adw(m, w - q0, nwd), m0 = adw(m, w - q0 - 1, nwd);
+                const uint64_t pa = ((uint64_t)m1.x << 32) | m0.x, pd = ((uint64_t)m1.y << 32) | m0.y;
 #pragma unroll
                 for (int j = 0; j < DZB; ++j) {
-                    cn[j] = popc_acc(a & (uint32_t)((pn << j) >> 32), cn[j]);
-                    co[j] = popc_acc(a & (uint32_t)((po << j) >> 32), co[j]);
+                    cp[j] = popc_acc(a & (uint32_t)((pa << j) >> 32), cp[j]);
+                    cm[j] = popc_acc(a & (uint32_t)((pd << j) >> 32), cm[j]);
                 }
             }
         }
@@ -235,11 +300,10 @@ __device__ void delta_item(const SeedDev &s, const CandDev &c, uint32_t cand, co
     int32_t *dt = delta + c.delta_off;
 #pragma unroll
     for (int j = 0; j < DZB; ++j) {
-        const int dz = gb.dz0 + j;
-        const int d = (int)(cn[j] - co[j]);
+        const int dz = dz0 + j;
+        const int d = (int)(cp[j] - cm[j]);
         if (active && dz <= col.m && d) atomicAdd(&dt[col.base + dz], d);
     }
-    (void)cand;
 }
 
 static constexpr int kDeltaWaves = 4;
@@ -247,10 +311,14 @@ static constexpr int kDeltaWaves = 4;
 __global__ __launch_bounds__(64 * kDeltaWaves) void delta_count_kernel(const SeedDev *__restrict__ seeds,
                                                                        const CandDev *__restrict__ cands,
                                                                        const BatchHeader *__restrict__ hdr,
-                                                                       const uint32_t *__restrict__ rows,
+                                                                       const uint2 *__restrict__ ad,
+                                                                       uint32_t ad_bytes,
+                                                                       const uint2 *__restrict__ info,
                                                                        int32_t *__restrict__ delta) {
     const uint64_t nitems = hdr->nitems;
     const uint32_t ncand = hdr->ncand;
+    const int row_chunk = (int)hdr->row_chunk;
+    const __amdgpu_buffer_rsrc_t ad_rs = __builtin_amdgcn_make_buffer_rsrc((void *)ad, 0, (int)ad_bytes, 0x00020000);
     const uint64_t nwaves = (uint64_t)gridDim.x * kDeltaWaves;
     const uint64_t wave0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kDeltaWaves + threadIdx.x / 64));
     for (uint64_t item = wave0; item < nitems; item += nwaves) {
@@ -266,10 +334,10 @@ __global__ __launch_bounds__(64 * kDeltaWaves) void delta_count_kernel(const See
         const GB gb = s.gbs[local / c.nchunks];
         const int chunk = (int)(local % c.nchunks);
         switch (gb.dzb) {
-            case 8: delta_item<8>(s, c, lo, gb, chunk, rows, delta); break;
-            case 16: delta_item<16>(s, c, lo, gb, chunk, rows, delta); break;
-            case 24: delta_item<24>(s, c, lo, gb, chunk, rows, delta); break;
-            default: delta_item<32>(s, c, lo, gb, chunk, rows, delta); break;
+            case 8: delta_item<8>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
+            case 16: delta_item<16>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
+            case 24: delta_item<24>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
+            default: delta_item<32>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
         }
     }
 }
@@ -328,29 +396,47 @@ __global__ __launch_bounds__(256) void batch_fold_kernel(const SeedDev *__restri
     }
 }
 
-__global__ void zero_kernel(uint32_t *__restrict__ p, uint64_t n) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = 0;
+// ---------------------------------------------------------------- 5. committing a move
+
+// coverage += box delta of candidate 0
+__global__ void apply_cov_kernel(const CandDev *__restrict__ cands, const int32_t *__restrict__ box,
+                                 uint32_t *__restrict__ cov, int ny, int nz) {
+    const CandDev c = cands[0];
+    const int sx = c.bx1 - c.bx0, sy = c.by1 - c.by0, sz = c.bz1 - c.bz0;
+    const uint64_t n = (uint64_t)sx * sy * sz;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const int lz = (int)(i % sz), ly = (int)((i / sz) % sy), lx = (int)(i / ((uint64_t)sz * sy));
+        const int d = box[c.box_off + i];
+        if (d) {
+            const uint64_t v = ((uint64_t)(c.bx0 + lx) * ny + (c.by0 + ly)) * nz + (c.bz0 + lz);
+            cov[v] = (uint32_t)((int)cov[v] + d);
+        }
+    }
 }
 
-}  // namespace batch
-}  // namespace s2h
+// seed rows of the box: new = (old & ~D) | A
+__global__ void apply_rows_kernel(const CandDev *__restrict__ cands, const uint2 *__restrict__ ad,
+                                  uint32_t *__restrict__ vol, int ny, int nwd, int row_dw) {
+    const CandDev c = cands[0];
+    const int sy = c.by1 - c.by0;
+    const uint64_t n = (uint64_t)(c.bx1 - c.bx0) * sy * nwd;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = i / nwd;
+        const int w = (int)(i % nwd);
+        const uint2 m = ad[c.ad_off + i];
+        const int x = c.bx0 + (int)(r / sy), y = c.by0 + (int)(r % sy);
+        uint32_t &v = vol[((uint64_t)x * ny + y) * row_dw + w];
+        v = (v & ~m.y) | m.x;
+    }
+}
 
-// ================================================================ host side
-
-struct s2h_seed {
-    s2h_vol *vol = nullptr;
-    s2h::DevBuf cov, table, gbs;
-    s2h::ExactPlan *plan = nullptr;
-    double bmin[3];
-    double pitch;
-    uint32_t r_max;
-    uint64_t occupied;
-    std::vector<s2h::batch::GB> gb_host;
-};
-
-namespace s2h {
-namespace batch {
+// seed table += delta table of candidate 0
+__global__ void apply_table_kernel(const CandDev *__restrict__ cands, const int32_t *__restrict__ delta,
+                                   unsigned long long *__restrict__ table, uint64_t n) {
+    const uint64_t off = cands[0].delta_off;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        table[i] = (unsigned long long)((long long)table[i] + delta[off + i]);
+}
 
 struct CovAdd {
     uint32_t *cov;
@@ -378,157 +464,173 @@ __global__ void cov_overflow_kernel(const vox::TriGeo *__restrict__ tris, const 
     vox::voxel_overflow_ray(tris, gran, tiles, bx, by, bz, pitch, overflow, k, CovAdd{cov, ny, nz});
 }
 
-// Group the triangles of one mesh by granule and prepare tiles (owner/sign set).
-static void prepare_mesh(const double *tri_xyz, uint32_t ntri, const uint32_t *granule, uint32_t ngran,
-                         const double bmin[3], double pitch, const uint64_t dims[3], uint32_t owner, int sign,
-                         std::vector<vox::TriGeo> &geo, std::vector<vox::GranuleInfo> &gran,
-                         std::vector<vox::Tile> &tiles, int64_t box_lo[3], int64_t box_hi[3]) {
-    std::vector<uint32_t> start(ngran + 1, 0);
-    for (uint32_t f = 0; f < ntri; ++f) start[granule[f] + 1]++;
-    for (uint32_t g = 0; g < ngran; ++g) start[g + 1] += start[g];
-    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
-    std::vector<double> txyz(9 * (size_t)ntri);
-    for (uint32_t f = 0; f < ntri; ++f) std::memcpy(&txyz[9 * (size_t)fill[granule[f]]++], tri_xyz + 9 * (size_t)f, 72);
-    const uint32_t base = (uint32_t)geo.size();
-    geo.resize(geo.size() + ntri);
-    for (uint32_t g = 0; g < ngran; ++g) {
-        vox::GranuleInfo gi;
-        gi.tri_start = base + start[g];
-        gi.tri_end = base + start[g + 1];
-        gi.owner = owner;
-        gi.sign = sign;
-        int64_t lo[3], hi[3];
-        const bool ok = vox::prepare_granule(txyz.data() + 9 * (size_t)start[g], start[g + 1] - start[g], bmin, pitch,
-                                             dims, geo.data() + base + start[g], gi, lo, hi);
-        const uint32_t gidx = (uint32_t)gran.size();
-        gran.push_back(gi);
-        if (!ok) continue;
-        vox::push_tiles(gidx, lo, hi, tiles);
-        for (int a = 0; a < 3; ++a) {
-            box_lo[a] = std::min(box_lo[a], lo[a]);
-            box_hi[a] = std::max(box_hi[a], hi[a]);
-        }
-    }
+__global__ void zero_u32_kernel(uint32_t *__restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
 }
+
+// Fixed launch shapes of the batch sequence (grid-stride kernels).
+constexpr unsigned kClearBlocks = 2048, kPrepBlocks = 1024, kVoxelBlocks = 8192, kOverflowBlocks = 64,
+                   kAdBlocks = 4096, kDeltaBlocks = 2048, kFoldBlocks = 8192;
 
 }  // namespace batch
+
+// Persistent batch workspace of a context: grow-only device buffers, pinned
+// staging for the per-batch inputs and outputs, and the captured graph.
+struct BatchWS {
+    DevBuf seeds, cands, hdr, geo, gran, tiles, origin, adpre, foldpre, delta, box, info, occ, ad, curves, over,
+        newxyz;
+    void *pin = nullptr;
+    uint64_t pin_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> captured;  // buffer addresses the graph was captured with
+    std::map<std::vector<uint64_t>, uint32_t> row_chunk;  // tuned rows per delta item, by seed shapes
+};
+
+void batch_ws_release(s2h_ctx *ctx) {
+    BatchWS *w = ctx->batch_ws;
+    if (!w) return;
+    if (w->exec) (void)hipGraphExecDestroy(w->exec);
+    if (w->graph) (void)hipGraphDestroy(w->graph);
+    DevBuf *bufs[] = {&w->seeds, &w->cands, &w->hdr, &w->geo, &w->gran, &w->tiles, &w->origin, &w->adpre, &w->foldpre,
+                      &w->delta, &w->box, &w->info, &w->occ, &w->ad, &w->curves, &w->over, &w->newxyz};
+    for (DevBuf *b : bufs) dev_free(ctx, *b);
+    if (w->pin) (void)hipHostFree(w->pin);
+    delete w;
+    ctx->batch_ws = nullptr;
+}
+
 }  // namespace s2h
 
-using namespace s2h;
+// ================================================================ host side
 
-extern "C" {
+struct s2h_seed {
+    s2h_vol *vol = nullptr;
+    s2h::DevBuf cov, table, gbs, d_xyz, d_tri;
+    s2h::ExactPlan *plan = nullptr;
+    double bmin[3];
+    double pitch;
+    uint32_t r_max;
+    uint64_t occupied;
+    std::vector<s2h::batch::GB> gb_host;
+    // particles: vertices [vstart[p], vstart[p+1]), triangles [tstart[p], tstart[p+1])
+    uint32_t nparticles = 0;
+    std::vector<uint32_t> vstart, tstart;
+    std::vector<double> xyz;           // current vertex positions (host copy of d_xyz)
+    std::vector<int32_t> vgran;        // local granule of each vertex (-1: not referenced)
+    std::vector<uint32_t> pgran;       // per particle: index of its first granule start in gtri
+    std::vector<uint32_t> gtri;        // per particle: its granules' first triangles in d_tri order, then the end
+    std::vector<uint32_t> ngran;       // per particle: local granule count
+};
 
-s2h_status s2h_seed_create(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,
-                           const uint32_t *granule_of_tri, uint32_t ngranules, const double bbox_min[3], double pitch,
-                           const uint64_t dims[3], uint32_t r_max, s2h_seed **out) {
-    if (!ctx || !out || !dims || !bbox_min) return fail(S2H_ERR_ARG, "null argument");
-    S2H_HIP(hipSetDevice(ctx->device));
-    auto *sd = new s2h_seed();
-    auto cleanup = [&](s2h_status st) {
-        s2h_seed_free(ctx, sd);
-        return st;
-    };
-    s2h_status st = s2h_voxelize(ctx, xyz, nverts, tri, ntris, granule_of_tri, ngranules, bbox_min, pitch, dims,
-                                 &sd->vol, &sd->occupied);
-    if (st != S2H_OK) return cleanup(st);
-    for (int a = 0; a < 3; ++a) sd->bmin[a] = bbox_min[a];
-    sd->pitch = pitch;
-    sd->r_max = r_max;
-    // coverage counts: the same predicate, counting granules per centre
+namespace s2h {
+namespace batch {
+
+// Whole-mesh coverage counts of a seed (one count per granule covering a centre).
+static s2h_status seed_coverage(s2h_ctx *ctx, s2h_seed *sd, const double *xyz, const uint32_t *tri, uint64_t ntris,
+                                const uint32_t *granule_of_tri, uint32_t ngranules, const uint64_t dims[3]) {
     const uint64_t cells = dims[0] * dims[1] * dims[2];
-    if ((st = dev_alloc(ctx, sd->cov, cells * 4)) != S2H_OK) return cleanup(st);
-    batch::zero_kernel<<<4096, 256, 0, ctx->stream>>>((uint32_t *)sd->cov.ptr, cells);
-    {
-        std::vector<double> txyz(9 * ntris);
-        for (uint64_t f = 0; f < ntris; ++f)
-            for (int c = 0; c < 3; ++c)
-                for (int a = 0; a < 3; ++a) txyz[9 * f + 3 * c + a] = xyz[3 * (uint64_t)tri[3 * f + c] + a];
-        std::vector<vox::TriGeo> geo;
-        std::vector<vox::GranuleInfo> gran;
-        std::vector<vox::Tile> tiles;
-        int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
-        batch::prepare_mesh(txyz.data(), (uint32_t)ntris, granule_of_tri, ngranules, bbox_min, pitch, dims, 0, 1, geo,
-                            gran, tiles, lo, hi);
-        if (!tiles.empty()) {
-            DevBuf dg, dgr, dt, dov;
-            const uint32_t cap = 1u << 16;
-            st = upload(ctx, dg, geo.data(), geo.size() * sizeof(vox::TriGeo));
-            if (st == S2H_OK) st = upload(ctx, dgr, gran.data(), gran.size() * sizeof(vox::GranuleInfo));
-            if (st == S2H_OK) st = upload(ctx, dt, tiles.data(), tiles.size() * sizeof(vox::Tile));
-            if (st == S2H_OK) st = dev_alloc(ctx, dov, (1 + 2 * (uint64_t)cap) * 4);
-            uint32_t nover = 0;
-            hipError_t e = hipSuccess;
-            if (st == S2H_OK) {
-                e = hipMemsetAsync(dov.ptr, 0, 4, ctx->stream);
-                if (e == hipSuccess) {
-                    batch::cov_kernel<<<(unsigned)tiles.size(), 256, 0, ctx->stream>>>(
-                        (const vox::TriGeo *)dg.ptr, (const vox::GranuleInfo *)dgr.ptr, (const vox::Tile *)dt.ptr,
-                        bbox_min[0], bbox_min[1], bbox_min[2], pitch, (uint32_t *)sd->cov.ptr, dims[1], dims[2],
-                        (uint32_t *)dov.ptr, cap);
-                    e = hipGetLastError();
-                }
-                if (e == hipSuccess) e = hipMemcpyAsync(&nover, dov.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
-                if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-                if (e == hipSuccess && nover > cap) st = fail(S2H_ERR_UNSUPPORTED, "too many overflow rays");
-                if (e == hipSuccess && st == S2H_OK && nover) {
-                    batch::cov_overflow_kernel<<<(nover + 63) / 64, 64, 0, ctx->stream>>>(
-                        (const vox::TriGeo *)dg.ptr, (const vox::GranuleInfo *)dgr.ptr, (const vox::Tile *)dt.ptr,
-                        bbox_min[0], bbox_min[1], bbox_min[2], pitch, (uint32_t *)sd->cov.ptr, dims[1], dims[2],
-                        (const uint32_t *)dov.ptr, nover);
-                    e = hipGetLastError();
-                }
-                if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-                if (e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("coverage kernel: ") + hipGetErrorString(e));
-            }
-            dev_free(ctx, dg);
-            dev_free(ctx, dgr);
-            dev_free(ctx, dt);
-            dev_free(ctx, dov);
-            if (st != S2H_OK) return cleanup(st);
-        }
+    S2H_TRY(dev_alloc(ctx, sd->cov, cells * 4));
+    zero_u32_kernel<<<4096, 256, 0, ctx->stream>>>((uint32_t *)sd->cov.ptr, cells);
+    S2H_HIP(hipGetLastError());
+    std::vector<uint32_t> start(ngranules + 1, 0);
+    for (uint64_t f = 0; f < ntris; ++f) start[granule_of_tri[f] + 1]++;
+    for (uint32_t g = 0; g < ngranules; ++g) start[g + 1] += start[g];
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    std::vector<double> txyz(9 * ntris);
+    for (uint64_t f = 0; f < ntris; ++f) {
+        const uint64_t slot = fill[granule_of_tri[f]]++;
+        for (int c = 0; c < 3; ++c)
+            for (int a = 0; a < 3; ++a) txyz[9 * slot + 3 * c + a] = xyz[3 * (uint64_t)tri[3 * f + c] + a];
     }
-    // seed exact counts (kept: the plan's table is reused by later evaluations)
-    const double pm = pitch < 1e-9 ? 1e-9 : pitch;
-    if ((st = exact_plan_get(ctx, dims, r_max, pm, &sd->plan)) != S2H_OK) return cleanup(st);
-    if ((st = exact_run_counts(ctx, sd->vol, sd->plan)) != S2H_OK) return cleanup(st);
-    if ((st = dev_alloc(ctx, sd->table, std::max<uint64_t>(sd->plan->table_size, 1) * 8)) != S2H_OK)
-        return cleanup(st);
-    S2H_HIP(hipMemcpyAsync(sd->table.ptr, sd->plan->d_table.ptr, sd->plan->table_size * 8,
-                           hipMemcpyDeviceToDevice, ctx->stream));
-    for (const DzBlock &b : sd->plan->blocks) sd->gb_host.push_back(batch::GB{b.group, b.dz0, b.dzb});
-    if ((st = upload(ctx, sd->gbs, sd->gb_host.data(), sd->gb_host.size() * sizeof(batch::GB))) != S2H_OK)
-        return cleanup(st);
-    S2H_HIP(hipStreamSynchronize(ctx->stream));
-    *out = sd;
-    return S2H_OK;
+    std::vector<vox::TriGeo> geo(ntris);
+    std::vector<vox::GranuleInfo> gran(ngranules);
+    std::vector<vox::Tile> tiles;
+    for (uint32_t g = 0; g < ngranules; ++g) {
+        vox::GranuleInfo &gi = gran[g];
+        gi.tri_start = start[g];
+        gi.tri_end = start[g + 1];
+        gi.owner = 0;
+        gi.sign = 1;
+        int64_t lo[3], hi[3];
+        if (vox::prepare_granule(txyz.data() + 9 * (uint64_t)start[g], start[g + 1] - start[g], sd->bmin, sd->pitch,
+                                 dims, geo.data() + start[g], gi, lo, hi))
+            vox::push_tiles(g, lo, hi, tiles);
+    }
+    if (tiles.empty()) return S2H_OK;
+    DevBuf dg, dgr, dt, dov;
+    auto release = [&]() {
+        dev_free(ctx, dg);
+        dev_free(ctx, dgr);
+        dev_free(ctx, dt);
+        dev_free(ctx, dov);
+    };
+    s2h_status st = upload(ctx, dg, geo.data(), geo.size() * sizeof(vox::TriGeo));
+    if (st == S2H_OK) st = upload(ctx, dgr, gran.data(), gran.size() * sizeof(vox::GranuleInfo));
+    if (st == S2H_OK) st = upload(ctx, dt, tiles.data(), tiles.size() * sizeof(vox::Tile));
+    if (st == S2H_OK) st = dev_alloc(ctx, dov, (1 + 2 * (uint64_t)kOverflowCap) * 4);
+    if (st != S2H_OK) {
+        release();
+        return st;
+    }
+    uint32_t nover = 0;
+    hipError_t e = hipSuccess;
+    zero_u32_kernel<<<1, 64, 0, ctx->stream>>>((uint32_t *)dov.ptr, 1);
+    for (size_t t0 = 0; e == hipSuccess && t0 < tiles.size(); t0 += (1u << 30)) {
+        const size_t nt = std::min<size_t>(tiles.size() - t0, 1u << 30);
+        cov_kernel<<<(unsigned)nt, 256, 0, ctx->stream>>>(
+            (const vox::TriGeo *)dg.ptr, (const vox::GranuleInfo *)dgr.ptr, (const vox::Tile *)dt.ptr + t0,
+            sd->bmin[0], sd->bmin[1], sd->bmin[2], sd->pitch, (uint32_t *)sd->cov.ptr, dims[1], dims[2],
+            (uint32_t *)dov.ptr, kOverflowCap);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&nover, dov.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && nover > kOverflowCap) st = fail(S2H_ERR_UNSUPPORTED, "too many overflow rays");
+    if (e == hipSuccess && st == S2H_OK && nover) {
+        cov_overflow_kernel<<<(nover + 63) / 64, 64, 0, ctx->stream>>>(
+            (const vox::TriGeo *)dg.ptr, (const vox::GranuleInfo *)dgr.ptr, (const vox::Tile *)dt.ptr, sd->bmin[0],
+            sd->bmin[1], sd->bmin[2], sd->pitch, (uint32_t *)sd->cov.ptr, dims[1], dims[2], (const uint32_t *)dov.ptr,
+            nover);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("coverage kernel: ") + hipGetErrorString(e));
+    release();
+    return st;
 }
 
-void s2h_seed_free(s2h_ctx *ctx, s2h_seed *sd) {
-    if (!ctx || !sd) return;
-    if (sd->vol) s2h_vol_free(ctx, sd->vol);
-    dev_free(ctx, sd->cov);
-    dev_free(ctx, sd->table);
-    dev_free(ctx, sd->gbs);
-    delete sd;
+static s2h_status ws_reserve(s2h_ctx *ctx, DevBuf &b, uint64_t bytes) {
+    if (b.ptr && b.bytes >= bytes) return S2H_OK;
+    const uint64_t want = std::max<uint64_t>(std::max<uint64_t>(bytes, 256), b.bytes + b.bytes / 2);
+    dev_free(ctx, b);
+    return dev_alloc(ctx, b, want);
 }
 
-s2h_status s2h_seed_volume(s2h_seed *sd, s2h_vol **vol) {
-    if (!sd || !vol) return fail(S2H_ERR_ARG, "null argument");
-    *vol = sd->vol;
-    return S2H_OK;
-}
+// The host side of one batch: descriptors of every candidate, its granules at
+// both positions and their voxel tiles; device-side preparation does the rest.
+struct HostBatch {
+    std::vector<SeedDev> sdev;
+    std::vector<CandDev> cd;
+    std::vector<vox::GranuleInfo> gran;
+    std::vector<vox::Tile> tiles;
+    std::vector<double> origin;
+    std::vector<uint64_t> ad_prefix;
+    std::vector<uint32_t> fold_prefix;
+    std::vector<uint64_t> xyz_off;  // per candidate: first new vertex (3 doubles each)
+    BatchHeader hdr{};
+    uint64_t geo_total = 0, newxyz_total = 0, curve_total = 0;
+};
 
-s2h_status s2h_batch_eval(s2h_ctx *ctx, s2h_seed *const *seeds, uint32_t nseeds, const s2h_candidate *cands,
-                          uint32_t ncand, double *curves, uint64_t ncurve) {
-    using namespace s2h::batch;
-    if (!ctx || (!seeds && nseeds) || (!cands && ncand) || (!curves && ncurve)) return fail(S2H_ERR_ARG, "null argument");
-    S2H_HIP(hipSetDevice(ctx->device));
-    // ---- host: descriptors
-    std::vector<SeedDev> sdev(nseeds);
+static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, uint32_t nseeds,
+                             const s2h_candidate *cands, uint32_t ncand, uint32_t row_chunk, HostBatch &hb) {
+    hb.sdev.resize(nseeds);
     for (uint32_t i = 0; i < nseeds; ++i) {
         const s2h_seed *sd = seeds[i];
+        if (!sd) return fail(S2H_ERR_ARG, "null seed");
         const ExactPlan *p = sd->plan;
-        SeedDev &d = sdev[i];
+        SeedDev &d = hb.sdev[i];
         d.vol = (const uint32_t *)sd->vol->d_words;
         d.cov = (const uint32_t *)sd->cov.ptr;
         d.table = (const unsigned long long *)sd->table.ptr;
@@ -547,154 +649,582 @@ s2h_status s2h_batch_eval(s2h_ctx *ctx, s2h_seed *const *seeds, uint32_t nseeds,
         d.nz = (int)sd->vol->dims[2];
         d.row_dw = (int)(sd->vol->row_words * 2);
     }
-    std::vector<CandDev> cd(ncand);
-    std::vector<vox::TriGeo> geo;
-    std::vector<vox::GranuleInfo> gran;
-    std::vector<vox::Tile> tiles;
-    std::vector<double> origin(4 * (size_t)ncand);
-    std::vector<uint64_t> row_prefix(ncand + 1, 0);
-    std::vector<uint32_t> fold_prefix(ncand + 1, 0);
-    uint64_t rows_total = 0, box_total = 0, delta_total = 0, items = 0, curve_total = 0;
+    // new-vertex layout first: granule descriptors point into the device copy
+    hb.xyz_off.resize(ncand);
     for (uint32_t c = 0; c < ncand; ++c) {
         const s2h_candidate &k = cands[c];
         if (k.seed >= nseeds) return fail(S2H_ERR_ARG, "candidate seed index out of range");
         const s2h_seed *sd = seeds[k.seed];
+        if (k.particle >= sd->nparticles) return fail(S2H_ERR_ARG, "candidate particle index out of range");
+        if (!k.new_xyz) return fail(S2H_ERR_ARG, "candidate without vertices");
+        hb.xyz_off[c] = hb.newxyz_total;
+        hb.newxyz_total += (uint64_t)(sd->vstart[k.particle + 1] - sd->vstart[k.particle]);
+    }
+    S2H_TRY(ws_reserve(ctx, w.newxyz, std::max<uint64_t>(hb.newxyz_total, 1) * 24));
+    hb.cd.resize(ncand);
+    hb.origin.resize(4 * (size_t)ncand);
+    hb.ad_prefix.assign(ncand + 1, 0);
+    hb.fold_prefix.assign(ncand + 1, 0);
+    uint64_t ad_total = 0, info_total = 0, box_total = 0, delta_total = 0, items = 0;
+    std::vector<double> mn, mx, amax;
+    std::vector<uint8_t> have;
+    for (uint32_t c = 0; c < ncand; ++c) {
+        const s2h_candidate &k = cands[c];
+        const s2h_seed *sd = seeds[k.seed];
         const uint64_t *dims = sd->vol->dims;
-        for (uint32_t f = 0; f < k.ntris; ++f)
-            if (k.granule[f] >= k.ngranules) return fail(S2H_ERR_ARG, "candidate granule id out of range");
-        int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
-        prepare_mesh(k.old_tri, k.ntris, k.granule, k.ngranules, sd->bmin, sd->pitch, dims, c, -1, geo, gran, tiles, lo,
-                     hi);
-        prepare_mesh(k.new_tri, k.ntris, k.granule, k.ngranules, sd->bmin, sd->pitch, dims, c, +1, geo, gran, tiles, lo,
-                     hi);
-        CandDev &d = cd[c];
-        d.seed = k.seed;
-        if (lo[0] > hi[0]) {  // particle outside the grid at both positions: nothing changes
-            for (int a = 0; a < 3; ++a) lo[a] = hi[a] = 0;
+        const uint32_t p = k.particle, v0 = sd->vstart[p], nv = sd->vstart[p + 1] - v0, ng = sd->ngran[p];
+        int64_t blo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, bhi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+        for (int side = 0; side < 2; ++side) {
+            const double *hx = side == 0 ? sd->xyz.data() + 3 * (uint64_t)v0 : k.new_xyz;
+            const double *dx = side == 0 ? (const double *)sd->d_xyz.ptr + 3 * (uint64_t)v0
+                                         : (const double *)w.newxyz.ptr + 3 * hb.xyz_off[c];
+            // vertex bbox of every local granule (over the vertices its triangles use)
+            mn.assign(3 * (size_t)ng, 0.0);
+            mx.assign(3 * (size_t)ng, 0.0);
+            amax.assign(ng, 0.0);
+            have.assign(ng, 0);
+            for (uint32_t v = 0; v < nv; ++v) {
+                const int32_t g = sd->vgran[v0 + v];
+                if (g < 0) continue;
+                const double *q = hx + 3 * (uint64_t)v;
+                for (int a = 0; a < 3; ++a) {
+                    mn[3 * g + a] = have[g] ? vox::rmin(mn[3 * g + a], q[a]) : q[a];
+                    mx[3 * g + a] = have[g] ? vox::rmax(mx[3 * g + a], q[a]) : q[a];
+                    amax[g] = std::max(amax[g], std::fabs(q[a]));
+                }
+                have[g] = 1;
+            }
+            for (uint32_t g = 0; g < ng; ++g) {
+                const uint32_t t0 = sd->gtri[sd->pgran[p] + g], t1 = sd->gtri[sd->pgran[p] + g + 1];
+                vox::GranuleInfo gi;
+                for (int a = 0; a < 3; ++a) {
+                    gi.mn[a] = mn[3 * g + a];
+                    gi.mx[a] = mx[3 * g + a];
+                }
+                gi.tri_start = (uint32_t)hb.geo_total;
+                gi.tri_end = (uint32_t)(hb.geo_total + (t1 - t0));
+                gi.owner = c;
+                gi.sign = side == 0 ? -1 : 1;
+                gi.amax = amax[g];
+                gi.cmax = gi.smax = 0.0;
+                gi.xyz = dx;
+                gi.tri = (const uint32_t *)sd->d_tri.ptr + 3 * (uint64_t)t0;
+                int64_t lo[3], hi[3];
+                if (!have[g] || !vox::granule_bounds(gi.mn, gi.mx, sd->bmin, sd->pitch, dims, lo, hi, &gi.cmax,
+                                                     &gi.smax))
+                    continue;
+                hb.geo_total += t1 - t0;
+                vox::push_tiles((uint32_t)hb.gran.size(), lo, hi, hb.tiles);
+                hb.gran.push_back(gi);
+                for (int a = 0; a < 3; ++a) {
+                    blo[a] = std::min(blo[a], lo[a]);
+                    bhi[a] = std::max(bhi[a], hi[a]);
+                }
+            }
         }
-        d.bx0 = (int32_t)lo[0];
-        d.by0 = (int32_t)lo[1];
-        d.bz0 = (int32_t)lo[2];
-        d.bx1 = (int32_t)hi[0];
-        d.by1 = (int32_t)hi[1];
-        d.bz1 = (int32_t)hi[2];
-        const uint64_t nrows = (uint64_t)(hi[0] - lo[0]) * (uint64_t)(hi[1] - lo[1]);
-        d.nchunks = (uint32_t)((nrows + kRowChunk - 1) / kRowChunk);
-        d.rows_off = rows_total;
-        rows_total += nrows * (uint64_t)sdev[k.seed].row_dw;
-        row_prefix[c + 1] = rows_total;
+        CandDev &d = hb.cd[c];
+        d.seed = k.seed;
+        if (blo[0] > bhi[0])  // particle outside the grid at both positions: nothing changes
+            for (int a = 0; a < 3; ++a) blo[a] = bhi[a] = 0;
+        d.bx0 = (int32_t)blo[0];
+        d.by0 = (int32_t)blo[1];
+        d.bz0 = (int32_t)blo[2];
+        d.bx1 = (int32_t)bhi[0];
+        d.by1 = (int32_t)bhi[1];
+        d.bz1 = (int32_t)bhi[2];
+        const uint64_t nrows = (uint64_t)(bhi[0] - blo[0]) * (uint64_t)(bhi[1] - blo[1]);
+        const uint64_t nwd = (dims[2] + 31) / 32;
+        d.nchunks = (uint32_t)((nrows + row_chunk - 1) / row_chunk);
+        d.ad_off = ad_total;
+        ad_total += nrows * nwd;
+        hb.ad_prefix[c + 1] = ad_total;
+        d.info_off = info_total;
+        info_total += nrows;
         d.box_off = box_total;
-        box_total += (uint64_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+        box_total += (uint64_t)(bhi[0] - blo[0]) * (bhi[1] - blo[1]) * (bhi[2] - blo[2]);
         d.delta_off = delta_total;
         delta_total += sd->plan->table_size;
         d.item_start = items;
-        items += (uint64_t)d.nchunks * sdev[k.seed].ngb;
-        d.curve_off = curve_total;
-        curve_total += sd->r_max + 1;
-        fold_prefix[c + 1] = fold_prefix[c] + sd->r_max;
-        origin[4 * c] = sd->bmin[0];
-        origin[4 * c + 1] = sd->bmin[1];
-        origin[4 * c + 2] = sd->bmin[2];
-        origin[4 * c + 3] = sd->pitch;
+        items += (uint64_t)d.nchunks * hb.sdev[k.seed].ngb;
+        d.curve_off = hb.curve_total;
+        hb.curve_total += sd->r_max + 1;
+        hb.fold_prefix[c + 1] = hb.fold_prefix[c] + sd->r_max;
+        hb.origin[4 * c] = sd->bmin[0];
+        hb.origin[4 * c + 1] = sd->bmin[1];
+        hb.origin[4 * c + 2] = sd->bmin[2];
+        hb.origin[4 * c + 3] = sd->pitch;
     }
-    if (ncurve != curve_total) return fail(S2H_ERR_SIZE, "curves buffer must hold sum of r_max+1 over candidates");
-    if (ncand == 0) return S2H_OK;
-    BatchHeader hdr{items, ncand, fold_prefix[ncand]};
-    uint32_t ntiles = (uint32_t)tiles.size();
-    // ---- device buffers (grow-only scratch owned by this call)
-    DevBuf b_seeds, b_cands, b_hdr, b_geo, b_gran, b_tiles, b_ntiles, b_origin, b_rowpre, b_foldpre, b_zero, b_rows,
-        b_curve, b_over;
-    auto release = [&]() {
-        DevBuf *all[] = {&b_seeds, &b_cands, &b_hdr, &b_geo, &b_gran, &b_tiles, &b_ntiles, &b_origin, &b_rowpre,
-                         &b_foldpre, &b_zero, &b_rows, &b_curve, &b_over};
-        for (DevBuf *b : all) dev_free(ctx, *b);
-    };
-    // zeroed region: [delta tables][boxes][occ_delta][overflow counter]
-    const uint64_t zero_words = delta_total + box_total + ncand + 1;
-    const uint32_t over_cap = 1u << 16;
-    s2h_status st = S2H_OK;
-    auto up = [&](DevBuf &b, const void *src, uint64_t bytes) {
-        if (st == S2H_OK) st = upload(ctx, b, src, std::max<uint64_t>(bytes, 8));
-    };
-    up(b_seeds, sdev.data(), sdev.size() * sizeof(SeedDev));
-    up(b_cands, cd.data(), cd.size() * sizeof(CandDev));
-    up(b_hdr, &hdr, sizeof hdr);
-    up(b_geo, geo.data(), geo.size() * sizeof(vox::TriGeo));
-    up(b_gran, gran.data(), gran.size() * sizeof(vox::GranuleInfo));
-    up(b_tiles, tiles.data(), tiles.size() * sizeof(vox::Tile));
-    up(b_ntiles, &ntiles, 4);
-    up(b_origin, origin.data(), origin.size() * 8);
-    up(b_rowpre, row_prefix.data(), row_prefix.size() * 8);
-    up(b_foldpre, fold_prefix.data(), fold_prefix.size() * 4);
-    if (st == S2H_OK) st = dev_alloc(ctx, b_zero, zero_words * 4);
-    if (st == S2H_OK) st = dev_alloc(ctx, b_rows, std::max<uint64_t>(rows_total, 1) * 4);
-    if (st == S2H_OK) st = dev_alloc(ctx, b_curve, curve_total * 9 + 16);
-    if (st == S2H_OK) st = dev_alloc(ctx, b_over, (1 + 2 * (uint64_t)over_cap) * 4);
-    if (st != S2H_OK) {
-        release();
-        return st;
+    if (hb.tiles.size() >= (1ull << 31) || hb.gran.size() >= (1ull << 31) || hb.geo_total >= (1ull << 32))
+        return fail(S2H_ERR_UNSUPPORTED, "batch too large");
+    if (ad_total * 8 + 64 >= 0x7ffffe00ull) return fail(S2H_ERR_UNSUPPORTED, "changed rows of the batch above 2 GiB");
+    hb.hdr.nitems = items;
+    hb.hdr.ad_total = ad_total;
+    hb.hdr.info_total = info_total;
+    hb.hdr.box_words = box_total;
+    hb.hdr.delta_words = delta_total;
+    hb.hdr.ncand = ncand;
+    hb.hdr.nfold = hb.fold_prefix[ncand];
+    hb.hdr.ngran = (uint32_t)hb.gran.size();
+    hb.hdr.ntiles = (uint32_t)hb.tiles.size();
+    hb.hdr.row_chunk = row_chunk;
+    hb.hdr.pad = 0;
+    return S2H_OK;
+}
+
+// Recompute the item split for another row chunk (autotuning).
+static void set_row_chunk(HostBatch &hb, uint32_t row_chunk) {
+    uint64_t items = 0;
+    for (CandDev &d : hb.cd) {
+        const uint64_t nrows = (uint64_t)(d.bx1 - d.bx0) * (uint64_t)(d.by1 - d.by0);
+        d.nchunks = (uint32_t)((nrows + row_chunk - 1) / row_chunk);
+        d.item_start = items;
+        items += (uint64_t)d.nchunks * hb.sdev[d.seed].ngb;
     }
-    int32_t *delta = (int32_t *)b_zero.ptr;
-    int32_t *box = delta + delta_total;
-    int *occ_delta = box + box_total;
-    uint32_t *over = (uint32_t *)b_over.ptr;
-    hipStream_t s = ctx->stream;
-    prof_mark(ctx, 0);
-    zero_kernel<<<2048, 256, 0, s>>>((uint32_t *)b_zero.ptr, zero_words);
-    zero_kernel<<<1, 64, 0, s>>>(over, 1);
-    batch_voxel_kernel<<<8192, 256, 0, s>>>((const vox::TriGeo *)b_geo.ptr, (const vox::GranuleInfo *)b_gran.ptr,
-                                            (const vox::Tile *)b_tiles.ptr, (const uint32_t *)b_ntiles.ptr,
-                                            (const double *)b_origin.ptr, (const CandDev *)b_cands.ptr, box, over,
-                                            over_cap);
-    batch_voxel_overflow_kernel<<<64, 64, 0, s>>>((const vox::TriGeo *)b_geo.ptr, (const vox::GranuleInfo *)b_gran.ptr,
-                                                  (const vox::Tile *)b_tiles.ptr, (const double *)b_origin.ptr,
-                                                  (const CandDev *)b_cands.ptr, box, over, over_cap);
-    build_rows_kernel<<<4096, 256, 0, s>>>((const SeedDev *)b_seeds.ptr, (const CandDev *)b_cands.ptr, ncand,
-                                           (const uint64_t *)b_rowpre.ptr, box, (uint32_t *)b_rows.ptr, occ_delta);
-    delta_count_kernel<<<2048, 64 * kDeltaWaves, 0, s>>>((const SeedDev *)b_seeds.ptr, (const CandDev *)b_cands.ptr,
-                                                         (const BatchHeader *)b_hdr.ptr, (const uint32_t *)b_rows.ptr,
-                                                         delta);
-    prof_mark(ctx, 1);
-    double *d_curves = (double *)b_curve.ptr;
-    uint8_t *d_support = (uint8_t *)(d_curves + curve_total);
-    batch_fold_kernel<<<8192, 256, 0, s>>>((const SeedDev *)b_seeds.ptr, (const CandDev *)b_cands.ptr,
-                                           (const BatchHeader *)b_hdr.ptr, (const uint32_t *)b_foldpre.ptr, delta,
-                                           occ_delta, d_curves, d_support);
-    prof_mark(ctx, 2);
-    ctx->launches = 7;
-    hipError_t e = hipGetLastError();
-    std::vector<double> hc(curve_total);
-    std::vector<uint8_t> hs(curve_total);
-    std::vector<int> hocc(ncand);
+    hb.hdr.nitems = items;
+    hb.hdr.row_chunk = row_chunk;
+}
+
+// Curve buffer: values, then one support byte per value, in one allocation.
+static uint8_t *curve_support(const BatchWS &w) {
+    return (uint8_t *)w.curves.ptr + w.curves.bytes / 9 * 8;
+}
+
+// Byte size of the (A, D) buffer for its buffer resource (below the
+// out-of-range offset the kernels use for lanes without a row).
+static uint32_t ad_records(const BatchWS &w) { return (uint32_t)std::min<uint64_t>(w.ad.bytes, 0x7ffffe00u); }
+
+static void launch_sequence(hipStream_t s, BatchWS &w) {
+    const BatchHeader *hdr = (const BatchHeader *)w.hdr.ptr;
+    const SeedDev *seeds = (const SeedDev *)w.seeds.ptr;
+    const CandDev *cands = (const CandDev *)w.cands.ptr;
+    int32_t *delta = (int32_t *)w.delta.ptr;
+    int32_t *box = (int32_t *)w.box.ptr;
+    uint2 *info = (uint2 *)w.info.ptr;
+    int *occ = (int *)w.occ.ptr;
+    uint32_t *over = (uint32_t *)w.over.ptr;
+    uint2 *ad = (uint2 *)w.ad.ptr;
+    clear_kernel<<<kClearBlocks, 256, 0, s>>>(hdr, delta, box, info, occ, over);
+    prep_kernel<<<kPrepBlocks, 256, 0, s>>>((const vox::GranuleInfo *)w.gran.ptr, hdr, (vox::TriGeo *)w.geo.ptr);
+    batch_voxel_kernel<<<kVoxelBlocks, 256, 0, s>>>((const vox::TriGeo *)w.geo.ptr, (const vox::GranuleInfo *)w.gran.ptr,
+                                                    (const vox::Tile *)w.tiles.ptr, hdr, (const double *)w.origin.ptr,
+                                                    cands, box, over);
+    batch_voxel_overflow_kernel<<<kOverflowBlocks, 64, 0, s>>>(
+        (const vox::TriGeo *)w.geo.ptr, (const vox::GranuleInfo *)w.gran.ptr, (const vox::Tile *)w.tiles.ptr,
+        (const double *)w.origin.ptr, cands, box, over);
+    build_ad_kernel<<<kAdBlocks, 256, 0, s>>>(seeds, cands, hdr, (const uint64_t *)w.adpre.ptr, box, ad, info, occ);
+    delta_count_kernel<<<kDeltaBlocks, 64 * kDeltaWaves, 0, s>>>(seeds, cands, hdr, ad, ad_records(w), info,
+                                                                 delta);
+    batch_fold_kernel<<<kFoldBlocks, 256, 0, s>>>(seeds, cands, hdr, (const uint32_t *)w.foldpre.ptr, delta, occ,
+                                                  (double *)w.curves.ptr, curve_support(w));
+}
+
+static std::vector<void *> ws_ptrs(const BatchWS &w) {
+    return {w.seeds.ptr, w.cands.ptr, w.hdr.ptr, w.geo.ptr, w.gran.ptr, w.tiles.ptr, w.origin.ptr, w.adpre.ptr,
+            w.foldpre.ptr, w.delta.ptr, w.box.ptr, w.info.ptr, w.occ.ptr, w.ad.ptr, w.curves.ptr, w.over.ptr,
+            (void *)(uintptr_t)w.ad.bytes, (void *)(uintptr_t)w.curves.bytes};
+}
+
+static bool use_graph() {
+    const char *env = std::getenv("S2HIP_BATCH_GRAPH");
+    return !(env && env[0] == '0');
+}
+
+// Launch the batch sequence: replay the captured graph, capturing it first when
+// the workspace buffers moved since the last capture.
+static s2h_status run_sequence(s2h_ctx *ctx, BatchWS &w) {
+    if (!use_graph()) {
+        launch_sequence(ctx->stream, w);
+        S2H_HIP(hipGetLastError());
+        ctx->launches += 7;
+        return S2H_OK;
+    }
+    const std::vector<void *> now = ws_ptrs(w);
+    if (!w.exec || now != w.captured) {
+        if (w.exec) (void)hipGraphExecDestroy(w.exec);
+        if (w.graph) (void)hipGraphDestroy(w.graph);
+        w.exec = nullptr;
+        w.graph = nullptr;
+        S2H_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        launch_sequence(ctx->stream, w);
+        hipGraph_t g = nullptr;
+        const hipError_t le = hipGetLastError();
+        const hipError_t ee = hipStreamEndCapture(ctx->stream, &g);
+        if (le != hipSuccess || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(S2H_ERR_HIP, std::string("batch graph capture failed: ") +
+                                         hipGetErrorString(le != hipSuccess ? le : ee));
+        }
+        hipError_t ie = hipGraphInstantiate(&w.exec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            w.exec = nullptr;
+            return fail(S2H_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
+        w.graph = g;
+        w.captured = now;
+    }
+    S2H_HIP(hipGraphLaunch(w.exec, ctx->stream));
+    ctx->launches += 1;
+    return S2H_OK;
+}
+
+// Pinned staging layout: inputs from the start, outputs in the last part.
+static uint64_t out_region(const HostBatch &hb, uint32_t ncand) {
+    return ((hb.curve_total * 9 + 15) / 16 * 16) + ((uint64_t)ncand * 4 + 15) / 16 * 16 + 16;
+}
+
+// Reserve the workspace for a host batch and copy its inputs in (pinned
+// staging, one async copy per buffer, on the context's stream).
+static s2h_status upload_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, const s2h_candidate *cands,
+                               uint32_t ncand, s2h_seed *const *seeds) {
+    const uint32_t nc = std::max<uint32_t>(ncand, 1);
+    S2H_TRY(ws_reserve(ctx, w.seeds, std::max<size_t>(hb.sdev.size(), 1) * sizeof(SeedDev)));
+    S2H_TRY(ws_reserve(ctx, w.cands, nc * sizeof(CandDev)));
+    S2H_TRY(ws_reserve(ctx, w.hdr, sizeof(BatchHeader)));
+    S2H_TRY(ws_reserve(ctx, w.geo, std::max<uint64_t>(hb.geo_total, 1) * sizeof(vox::TriGeo)));
+    S2H_TRY(ws_reserve(ctx, w.gran, std::max<size_t>(hb.gran.size(), 1) * sizeof(vox::GranuleInfo)));
+    S2H_TRY(ws_reserve(ctx, w.tiles, std::max<size_t>(hb.tiles.size(), 1) * sizeof(vox::Tile)));
+    S2H_TRY(ws_reserve(ctx, w.origin, (size_t)nc * 32));
+    S2H_TRY(ws_reserve(ctx, w.adpre, ((size_t)ncand + 1) * 8));
+    S2H_TRY(ws_reserve(ctx, w.foldpre, ((size_t)ncand + 1) * 4));
+    S2H_TRY(ws_reserve(ctx, w.delta, std::max<uint64_t>(hb.hdr.delta_words, 1) * 4));
+    S2H_TRY(ws_reserve(ctx, w.box, std::max<uint64_t>(hb.hdr.box_words, 1) * 4));
+    S2H_TRY(ws_reserve(ctx, w.info, std::max<uint64_t>(hb.hdr.info_total, 1) * 8));
+    S2H_TRY(ws_reserve(ctx, w.occ, (size_t)nc * 4));
+    S2H_TRY(ws_reserve(ctx, w.ad, std::max<uint64_t>(hb.hdr.ad_total, 1) * 8 + 64));
+    if (!w.curves.ptr || w.curves.bytes / 9 < hb.curve_total) {
+        const uint64_t entries = std::max<uint64_t>(hb.curve_total, 64) * 3 / 2;
+        dev_free(ctx, w.curves);
+        S2H_TRY(dev_alloc(ctx, w.curves, entries * 9));
+    }
+    S2H_TRY(ws_reserve(ctx, w.over, (1 + 2 * (uint64_t)kOverflowCap) * 4));
+    const uint64_t in_bytes = (hb.sdev.size() * sizeof(SeedDev) + 16) + ((uint64_t)ncand * sizeof(CandDev) + 16) +
+                              (sizeof(BatchHeader) + 16) + (hb.gran.size() * sizeof(vox::GranuleInfo) + 16) +
+                              (hb.tiles.size() * sizeof(vox::Tile) + 16) + ((uint64_t)ncand * 32 + 16) +
+                              (((uint64_t)ncand + 1) * 8 + 16) + (((uint64_t)ncand + 1) * 4 + 16) +
+                              hb.newxyz_total * 24 + 64;
+    const uint64_t need = in_bytes + out_region(hb, ncand) + 64;
+    if (w.pin_bytes < need) {
+        if (w.pin) (void)hipHostFree(w.pin);
+        w.pin = nullptr;
+        w.pin_bytes = 0;
+        const uint64_t want = need * 3 / 2;
+        S2H_HIP(hipHostMalloc(&w.pin, want, hipHostMallocDefault));
+        w.pin_bytes = want;
+    }
+    uint8_t *p = (uint8_t *)w.pin;
+    auto put = [&](DevBuf &dst, const void *src, uint64_t bytes) -> s2h_status {
+        if (!bytes) return S2H_OK;
+        std::memcpy(p, src, bytes);
+        S2H_HIP(hipMemcpyAsync(dst.ptr, p, bytes, hipMemcpyHostToDevice, ctx->stream));
+        p += (bytes + 15) / 16 * 16;
+        return S2H_OK;
+    };
+    S2H_TRY(put(w.seeds, hb.sdev.data(), hb.sdev.size() * sizeof(SeedDev)));
+    S2H_TRY(put(w.cands, hb.cd.data(), (uint64_t)ncand * sizeof(CandDev)));
+    S2H_TRY(put(w.hdr, &hb.hdr, sizeof(BatchHeader)));
+    S2H_TRY(put(w.gran, hb.gran.data(), hb.gran.size() * sizeof(vox::GranuleInfo)));
+    S2H_TRY(put(w.tiles, hb.tiles.data(), hb.tiles.size() * sizeof(vox::Tile)));
+    S2H_TRY(put(w.origin, hb.origin.data(), (uint64_t)ncand * 32));
+    S2H_TRY(put(w.adpre, hb.ad_prefix.data(), ((uint64_t)ncand + 1) * 8));
+    S2H_TRY(put(w.foldpre, hb.fold_prefix.data(), ((uint64_t)ncand + 1) * 4));
+    // new vertices: one contiguous block in candidate order
+    if (hb.newxyz_total) {
+        for (uint32_t c = 0; c < ncand; ++c) {
+            const s2h_seed *sd = seeds[cands[c].seed];
+            const uint32_t nv = sd->vstart[cands[c].particle + 1] - sd->vstart[cands[c].particle];
+            std::memcpy(p + hb.xyz_off[c] * 24, cands[c].new_xyz, (size_t)nv * 24);
+        }
+        S2H_HIP(hipMemcpyAsync(w.newxyz.ptr, p, hb.newxyz_total * 24, hipMemcpyHostToDevice, ctx->stream));
+    }
+    return S2H_OK;
+}
+
+struct BatchOut {
+    std::vector<double> values;
+    std::vector<uint8_t> support;
+    std::vector<int> occ;
     uint32_t nover = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(hc.data(), d_curves, curve_total * 8, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(hs.data(), d_support, curve_total, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(hocc.data(), occ_delta, ncand * 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipMemcpyAsync(&nover, over, 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
+};
+
+static s2h_status download_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, uint32_t ncand, BatchOut &out) {
+    uint8_t *p = (uint8_t *)w.pin + (w.pin_bytes - out_region(hb, ncand)) / 16 * 16;
+    double *vals = (double *)p;
+    uint8_t *sup = p + hb.curve_total * 8;
+    int *occ = (int *)(p + (hb.curve_total * 9 + 15) / 16 * 16);
+    uint32_t *nover = (uint32_t *)(p + (hb.curve_total * 9 + 15) / 16 * 16 + ((uint64_t)ncand * 4 + 15) / 16 * 16);
+    S2H_HIP(hipMemcpyAsync(vals, w.curves.ptr, hb.curve_total * 8, hipMemcpyDeviceToHost, ctx->stream));
+    S2H_HIP(hipMemcpyAsync(sup, curve_support(w), hb.curve_total, hipMemcpyDeviceToHost, ctx->stream));
+    S2H_HIP(hipMemcpyAsync(occ, w.occ.ptr, (uint64_t)ncand * 4, hipMemcpyDeviceToHost, ctx->stream));
+    S2H_HIP(hipMemcpyAsync(nover, w.over.ptr, 4, hipMemcpyDeviceToHost, ctx->stream));
+    S2H_HIP(hipStreamSynchronize(ctx->stream));
+    out.values.assign(vals, vals + hb.curve_total);
+    out.support.assign(sup, sup + hb.curve_total);
+    out.occ.assign(occ, occ + ncand);
+    out.nover = *nover;
+    return S2H_OK;
+}
+
+static constexpr uint32_t kRowChunkDefault = 32;
+
+// Rows per delta item: the first batch over a new set of seed shapes times
+// each candidate value on the device (HIP events around the graph replay) and
+// keeps the fastest; later batches over those shapes reuse it.  Contexts with
+// autotuning off, and batches of fewer than 16 candidates, use the default.
+static s2h_status pick_row_chunk(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, uint32_t nseeds,
+                                 const s2h_candidate *cands, uint32_t ncand, HostBatch &hb) {
+    std::vector<uint64_t> key;
+    for (uint32_t i = 0; i < nseeds; ++i) {
+        const s2h_seed *sd = seeds[i];
+        for (int a = 0; a < 3; ++a) key.push_back(sd->vol->dims[a]);
+        key.push_back(sd->r_max);
+    }
+    auto it = w.row_chunk.find(key);
+    if (it != w.row_chunk.end()) {
+        set_row_chunk(hb, it->second);
+        return S2H_OK;
+    }
+    const char *env = std::getenv("S2HIP_AUTOTUNE");
+    if (!ctx->autotune || (env && env[0] == '0') || ncand < 16) return S2H_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    S2H_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(S2H_ERR_HIP, "hipEventCreate failed");
+    }
+    s2h_status st = S2H_OK;
+    float best = 1e30f;
+    uint32_t pick = kRowChunkDefault;
+    for (uint32_t rc : {8u, 16u, 32u, 64u}) {
+        set_row_chunk(hb, rc);
+        st = upload_batch(ctx, w, hb, cands, ncand, seeds);
+        if (st == S2H_OK) st = run_sequence(ctx, w);  // warm-up (and capture)
+        hipError_t e = hipSuccess;
+        if (st == S2H_OK) e = hipEventRecord(e0, ctx->stream);
+        if (st == S2H_OK && e == hipSuccess) st = run_sequence(ctx, w);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (st == S2H_OK && e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("batch tuning: ") + hipGetErrorString(e));
+        if (st != S2H_OK) break;
+        if (ms < best) {
+            best = ms;
+            pick = rc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != S2H_OK) return st;
+    w.row_chunk[key] = pick;
+    set_row_chunk(hb, pick);
+    return S2H_OK;
+}
+
+// One batch end to end; the curves are finished on the host.
+static s2h_status evaluate(s2h_ctx *ctx, s2h_seed *const *seeds, uint32_t nseeds, const s2h_candidate *cands,
+                           uint32_t ncand, double *curves, uint64_t ncurve, HostBatch &hb, BatchOut &out) {
+    if (!ctx->batch_ws) ctx->batch_ws = new BatchWS();
+    BatchWS &w = *ctx->batch_ws;
+    S2H_TRY(build_host(ctx, w, seeds, nseeds, cands, ncand, kRowChunkDefault, hb));
+    if (ncurve != hb.curve_total) return fail(S2H_ERR_SIZE, "curves buffer must hold sum of r_max+1 over candidates");
+    if (ncand == 0) return S2H_OK;
+    S2H_TRY(pick_row_chunk(ctx, w, seeds, nseeds, cands, ncand, hb));
+    S2H_TRY(upload_batch(ctx, w, hb, cands, ncand, seeds));
+    ctx->launches = 0;
+    prof_mark(ctx, 0);
+    S2H_TRY(run_sequence(ctx, w));
+    prof_mark(ctx, 1);
+    prof_mark(ctx, 2);
+    S2H_TRY(download_batch(ctx, w, hb, ncand, out));
     prof_collect(ctx);
-    release();
-    if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("batch kernels failed: ") + hipGetErrorString(e));
-    if (nover > over_cap) return fail(S2H_ERR_UNSUPPORTED, "more than 65536 candidate rays with over 16 hits");
-    // ---- host: finish each curve (vf, fill, S2(0) = vf)
+    if (out.nover > kOverflowCap) return fail(S2H_ERR_UNSUPPORTED, "more than 65536 candidate rays with over 16 hits");
+    // finish each curve: vf, fill, S2(0) = vf
     for (uint32_t c = 0; c < ncand; ++c) {
-        const s2h_seed *sd = seeds[cd[c].seed];
-        const uint64_t n = sd->r_max + 1, off = cd[c].curve_off;
+        const s2h_seed *sd = seeds[hb.cd[c].seed];
+        const uint64_t n = sd->r_max + 1, off = hb.cd[c].curve_off;
         const uint64_t cells = sd->vol->dims[0] * sd->vol->dims[1] * sd->vol->dims[2];
-        const int64_t occ = (int64_t)sd->occupied + hocc[c];
-        double *out = curves + off;
+        const int64_t occ = (int64_t)sd->occupied + out.occ[c];
+        double *o = curves + off;
         if (occ == 0) {
-            std::fill(out, out + n, 0.0);
+            std::fill(o, o + n, 0.0);
             continue;
         }
         const double vf = (double)occ / (double)cells;
-        std::memcpy(out, hc.data() + off, n * 8);
-        std::vector<uint8_t> sup(hs.begin() + (long)off, hs.begin() + (long)(off + n));
-        out[0] = vf;
+        std::memcpy(o, out.values.data() + off, n * 8);
+        std::vector<uint8_t> sup(out.support.begin() + (long)off, out.support.begin() + (long)(off + n));
+        o[0] = vf;
         sup[0] = 1;
-        fill_missing(out, sup.data(), n, vf);
-        out[0] = vf;
+        fill_missing(o, sup.data(), n, vf);
+        o[0] = vf;
     }
+    return S2H_OK;
+}
+
+}  // namespace batch
+}  // namespace s2h
+
+using namespace s2h;
+
+extern "C" {
+
+s2h_status s2h_seed_create(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,
+                           const uint32_t *granule_of_tri, uint32_t ngranules, const uint32_t *particle_vert_start,
+                           const uint32_t *particle_tri_start, uint32_t nparticles, const double bbox_min[3],
+                           double pitch, const uint64_t dims[3], uint32_t r_max, s2h_seed **out) {
+    if (!ctx || !out || !dims || !bbox_min || !particle_vert_start || !particle_tri_start ||
+        (ntris && (!xyz || !tri || !granule_of_tri)))
+        return fail(S2H_ERR_ARG, "null argument");
+    // particles: contiguous vertex and triangle ranges covering the mesh, each
+    // triangle using only its particle's vertices
+    if (particle_vert_start[0] != 0 || particle_tri_start[0] != 0 || particle_vert_start[nparticles] != nverts ||
+        particle_tri_start[nparticles] != ntris)
+        return fail(S2H_ERR_ARG, "particle ranges must cover the vertices and triangles");
+    for (uint32_t p = 0; p < nparticles; ++p) {
+        if (particle_vert_start[p + 1] < particle_vert_start[p] || particle_tri_start[p + 1] < particle_tri_start[p])
+            return fail(S2H_ERR_ARG, "particle ranges must be non-decreasing");
+        for (uint32_t f = particle_tri_start[p]; f < particle_tri_start[p + 1]; ++f)
+            for (int k = 0; k < 3; ++k)
+                if (tri[3 * (uint64_t)f + k] < particle_vert_start[p] ||
+                    tri[3 * (uint64_t)f + k] >= particle_vert_start[p + 1])
+                    return fail(S2H_ERR_ARG, "a triangle uses a vertex of another particle");
+    }
+    for (uint64_t f = 0; f < ntris; ++f)
+        if (granule_of_tri[f] >= ngranules) return fail(S2H_ERR_ARG, "granule id out of range");
+    S2H_HIP(hipSetDevice(ctx->device));
+    auto *sd = new s2h_seed();
+    auto cleanup = [&](s2h_status st) {
+        s2h_seed_free(ctx, sd);
+        return st;
+    };
+    s2h_status st = s2h_voxelize(ctx, xyz, nverts, tri, ntris, granule_of_tri, ngranules, bbox_min, pitch, dims,
+                                 &sd->vol, &sd->occupied);
+    if (st != S2H_OK) return cleanup(st);
+    for (int a = 0; a < 3; ++a) sd->bmin[a] = bbox_min[a];
+    sd->pitch = pitch;
+    sd->r_max = r_max;
+    // particle geometry: local granules (by first appearance), triangles sorted
+    // by (particle, local granule) with particle-local vertex indices
+    sd->nparticles = nparticles;
+    sd->vstart.assign(particle_vert_start, particle_vert_start + nparticles + 1);
+    sd->tstart.assign(particle_tri_start, particle_tri_start + nparticles + 1);
+    sd->xyz.assign(xyz, xyz + 3 * nverts);
+    sd->vgran.assign(nverts, -1);
+    sd->pgran.assign(nparticles + 1, 0);
+    sd->ngran.assign(nparticles, 0);
+    std::vector<uint32_t> tri_sorted(3 * ntris);
+    std::vector<int32_t> local(ngranules, -1);
+    uint64_t tpos = 0;
+    for (uint32_t p = 0; p < nparticles; ++p) {
+        const uint32_t f0 = particle_tri_start[p], f1 = particle_tri_start[p + 1], v0 = particle_vert_start[p];
+        std::vector<uint32_t> ids;
+        for (uint32_t f = f0; f < f1; ++f) {
+            const uint32_t g = granule_of_tri[f];
+            if (local[g] < 0) {
+                local[g] = (int32_t)ids.size();
+                ids.push_back(g);
+            }
+        }
+        std::vector<std::vector<uint32_t>> by(ids.size());
+        for (uint32_t f = f0; f < f1; ++f) by[local[granule_of_tri[f]]].push_back(f);
+        sd->pgran[p] = (uint32_t)sd->gtri.size();
+        sd->ngran[p] = (uint32_t)ids.size();
+        for (size_t g = 0; g < ids.size(); ++g) {
+            sd->gtri.push_back((uint32_t)tpos);
+            for (uint32_t f : by[g]) {
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t v = tri[3 * (uint64_t)f + k];
+                    tri_sorted[3 * tpos + k] = v - v0;
+                    sd->vgran[v] = (int32_t)g;
+                }
+                ++tpos;
+            }
+        }
+        sd->gtri.push_back((uint32_t)tpos);  // end of the particle's last granule
+        for (uint32_t g : ids) local[g] = -1;
+    }
+    if ((st = upload(ctx, sd->d_xyz, xyz, std::max<uint64_t>(nverts, 1) * 24)) != S2H_OK) return cleanup(st);
+    if ((st = upload(ctx, sd->d_tri, tri_sorted.data(), std::max<uint64_t>(ntris, 1) * 12)) != S2H_OK)
+        return cleanup(st);
+    if ((st = batch::seed_coverage(ctx, sd, xyz, tri, ntris, granule_of_tri, ngranules, dims)) != S2H_OK)
+        return cleanup(st);
+    // seed exact counts (kept: the plan's table is reused by later evaluations)
+    const double pm = pitch < 1e-9 ? 1e-9 : pitch;
+    if ((st = exact_plan_get(ctx, dims, r_max, pm, &sd->plan)) != S2H_OK) return cleanup(st);
+    if ((st = exact_run_counts(ctx, sd->vol, sd->plan)) != S2H_OK) return cleanup(st);
+    if ((st = dev_alloc(ctx, sd->table, std::max<uint64_t>(sd->plan->table_size, 1) * 8)) != S2H_OK)
+        return cleanup(st);
+    S2H_HIP(hipMemcpyAsync(sd->table.ptr, sd->plan->d_table.ptr, sd->plan->table_size * 8,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    for (const DzBlock &b : sd->plan->blocks) sd->gb_host.push_back(batch::GB{b.group, b.dz0, b.dzb});
+    if ((st = upload(ctx, sd->gbs, sd->gb_host.data(), std::max<size_t>(sd->gb_host.size(), 1) * sizeof(batch::GB))) !=
+        S2H_OK)
+        return cleanup(st);
+    S2H_HIP(hipStreamSynchronize(ctx->stream));
+    *out = sd;
+    return S2H_OK;
+}
+
+void s2h_seed_free(s2h_ctx *ctx, s2h_seed *sd) {
+    if (!ctx || !sd) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (sd->vol) s2h_vol_free(ctx, sd->vol);
+    dev_free(ctx, sd->cov);
+    dev_free(ctx, sd->table);
+    dev_free(ctx, sd->gbs);
+    dev_free(ctx, sd->d_xyz);
+    dev_free(ctx, sd->d_tri);
+    delete sd;
+}
+
+s2h_status s2h_seed_volume(s2h_seed *sd, s2h_vol **vol) {
+    if (!sd || !vol) return fail(S2H_ERR_ARG, "null argument");
+    *vol = sd->vol;
+    return S2H_OK;
+}
+
+s2h_status s2h_batch_eval(s2h_ctx *ctx, s2h_seed *const *seeds, uint32_t nseeds, const s2h_candidate *cands,
+                          uint32_t ncand, double *curves, uint64_t ncurve) {
+    if (!ctx || (!seeds && nseeds) || (!cands && ncand) || (!curves && ncurve)) return fail(S2H_ERR_ARG, "null argument");
+    S2H_HIP(hipSetDevice(ctx->device));
+    batch::HostBatch hb;
+    batch::BatchOut out;
+    return batch::evaluate(ctx, seeds, nseeds, cands, ncand, curves, ncurve, hb, out);
+}
+
+s2h_status s2h_seed_accept(s2h_ctx *ctx, s2h_seed *sd, uint32_t particle, const double *new_xyz, double *curve,
+                           uint64_t n) {
+    using namespace s2h::batch;
+    if (!ctx || !sd || !new_xyz) return fail(S2H_ERR_ARG, "null argument");
+    if (particle >= sd->nparticles) return fail(S2H_ERR_ARG, "particle index out of range");
+    if (curve && n != sd->r_max + 1) return fail(S2H_ERR_SIZE, "curve buffer must hold r_max+1 values");
+    S2H_HIP(hipSetDevice(ctx->device));
+    std::vector<double> tmp(sd->r_max + 1);
+    s2h_seed *const seeds[1] = {sd};
+    const s2h_candidate cand{0, particle, new_xyz};
+    HostBatch hb;
+    BatchOut out;
+    S2H_TRY(evaluate(ctx, seeds, 1, &cand, 1, tmp.data(), tmp.size(), hb, out));
+    BatchWS &w = *ctx->batch_ws;
+    const uint64_t *dims = sd->vol->dims;
+    const int nwd = (int)((dims[2] + 31) / 32);
+    apply_cov_kernel<<<1024, 256, 0, ctx->stream>>>((const CandDev *)w.cands.ptr, (const int32_t *)w.box.ptr,
+                                                    (uint32_t *)sd->cov.ptr, (int)dims[1], (int)dims[2]);
+    apply_rows_kernel<<<1024, 256, 0, ctx->stream>>>((const CandDev *)w.cands.ptr, (const uint2 *)w.ad.ptr,
+                                                     (uint32_t *)sd->vol->d_words, (int)dims[1], nwd,
+                                                     (int)(sd->vol->row_words * 2));
+    apply_table_kernel<<<1024, 256, 0, ctx->stream>>>((const CandDev *)w.cands.ptr, (const int32_t *)w.delta.ptr,
+                                                      (unsigned long long *)sd->table.ptr, sd->plan->table_size);
+    S2H_HIP(hipGetLastError());
+    const uint32_t v0 = sd->vstart[particle], nv = sd->vstart[particle + 1] - v0;
+    std::memcpy(sd->xyz.data() + 3 * (uint64_t)v0, new_xyz, (size_t)nv * 24);
+    if (nv)
+        S2H_HIP(hipMemcpyAsync((double *)sd->d_xyz.ptr + 3 * (uint64_t)v0, sd->xyz.data() + 3 * (uint64_t)v0,
+                               (size_t)nv * 24, hipMemcpyHostToDevice, ctx->stream));
+    S2H_HIP(hipStreamSynchronize(ctx->stream));
+    sd->occupied = (uint64_t)((int64_t)sd->occupied + out.occ[0]);
+    sd->vol->occupied = sd->occupied;
+    if (curve) std::memcpy(curve, tmp.data(), tmp.size() * 8);
     return S2H_OK;
 }
 

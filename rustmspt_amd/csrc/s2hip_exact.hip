@@ -45,7 +45,7 @@ __device__ __forceinline__ void count_item(const uint32_t *__restrict__ vol, con
                                            unsigned long long *__restrict__ table) {
     const int q0 = it.dz0 >> 5;
     const int nw = it.nw;  // partner dwords q0 .. q0+nw-1 hold voxels; later ones read as 0
-    const int oob = 0x7ffffff0;
+    const int oob = 0x7fffff00;  // + up to 255 bytes: past any buffer, no int overflow
 
     uint32_t cnt[DZB];
 #pragma unroll

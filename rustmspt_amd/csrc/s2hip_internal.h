@@ -133,6 +133,7 @@ struct McPlan {
     DevBuf d_hv;                   // uint64[2*(r_max+1)]
 };
 
+struct BatchWS;  // defined in s2hip_batch.hip
 struct FftPlan;  // defined in s2hip_fft.hip
 void fft_plan_release(s2h_ctx *ctx, FftPlan *p);
 
@@ -165,6 +166,7 @@ struct s2h_ctx {
     s2h::DevBuf d_scalar;   // small scratch (popcount result etc.)
     bool profiling = false;
     bool autotune = true;   // time launch-shape candidates on a new exact plan
+    s2h::BatchWS *batch_ws = nullptr;            // persistent batch workspace and graph
     std::multimap<uint64_t, void *> block_cache;  // released device blocks by size
     uint64_t cached_bytes = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -176,6 +178,7 @@ namespace s2h {
 s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes);
 void dev_free(s2h_ctx *ctx, DevBuf &b);
 void dev_cache_release(s2h_ctx *ctx);  // hipFree every cached block
+void batch_ws_release(s2h_ctx *ctx);
 s2h_status upload(s2h_ctx *ctx, DevBuf &b, const void *src, uint64_t bytes);
 s2h_status vol_alloc(s2h_ctx *ctx, const uint64_t dims[3], s2h_vol **out);
 s2h_status vol_popcount(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);

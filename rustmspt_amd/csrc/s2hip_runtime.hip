@@ -408,6 +408,7 @@ void s2h_destroy(s2h_ctx *ctx) {
     }
     for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
     dev_free(ctx, ctx->d_scalar);
+    batch_ws_release(ctx);
     dev_cache_release(ctx);
     for (hipEvent_t &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

@@ -49,9 +49,6 @@ __global__ void voxelize_overflow_kernel(const TriGeo *__restrict__ tris, const 
     voxel_overflow_ray(tris, gran, tiles, bx, by, bz, pitch, overflow, k, SetBit{words, ny, row_words});
 }
 
-// f64::min / f64::max as Rust defines them: a NaN operand is ignored.
-inline double rmin(double a, double b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a < b ? a : b)); }
-inline double rmax(double a, double b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a > b ? a : b)); }
 
 // Rust `f as isize` (saturating, NaN -> 0).
 inline int64_t f2i(double f) {
@@ -59,6 +56,32 @@ inline int64_t f2i(double f) {
     if (f >= 9.2233720368547758e18) return std::numeric_limits<int64_t>::max();
     if (f <= -9.2233720368547758e18) return std::numeric_limits<int64_t>::min();
     return (int64_t)f;
+}
+
+bool granule_bounds(const double mn[3], const double mx[3], const double bbox_min[3], double pitch,
+                    const uint64_t dims[3], int64_t lo[3], int64_t hi[3], double *cmax, double *smax) {
+    // src/geometry/s2.rs:126-147 part_voxel_ranges
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = std::max<int64_t>(0, f2i(std::floor((mn[a] - bbox_min[a]) / pitch)));
+        hi[a] = std::min<int64_t>((int64_t)dims[a], std::max<int64_t>(0, f2i(std::ceil((mx[a] - bbox_min[a]) / pitch))));
+        if (lo[a] >= hi[a]) return false;
+    }
+    // largest |coordinate| of a centre this granule tests, and the largest
+    // distance from a centre to a vertex (bounds for the cull margin)
+    double c = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        c = std::max(c, std::fabs(bbox_min[a] + ((double)lo[a] + 0.5) * pitch));
+        c = std::max(c, std::fabs(bbox_min[a] + ((double)hi[a] + 0.5) * pitch));
+    }
+    double s = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double d = std::max({std::fabs(mx[a] - (bbox_min[a] + (double)lo[a] * pitch)),
+                                   std::fabs((bbox_min[a] + (double)(hi[a] + 1) * pitch) - mn[a]), pitch});
+        s += d * d;
+    }
+    *cmax = c;
+    *smax = std::sqrt(s);
+    return true;
 }
 
 bool prepare_granule(const double *tri_xyz, uint32_t ntri, const double bbox_min[3], double pitch,
@@ -79,81 +102,15 @@ bool prepare_granule(const double *tri_xyz, uint32_t ntri, const double bbox_min
         gi.mn[a] = mn[a];
         gi.mx[a] = mx[a];
     }
+    gi.amax = amax;
+    gi.cmax = gi.smax = 0.0;
+    gi.xyz = nullptr;
+    gi.tri = nullptr;
     if (!have) return false;
-    // src/geometry/s2.rs:126-147 part_voxel_ranges
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = std::max<int64_t>(0, f2i(std::floor((mn[a] - bbox_min[a]) / pitch)));
-        hi[a] = std::min<int64_t>((int64_t)dims[a], std::max<int64_t>(0, f2i(std::ceil((mx[a] - bbox_min[a]) / pitch))));
-        if (lo[a] >= hi[a]) return false;
-    }
-    // largest |coordinate| of a centre this granule tests, and the largest
-    // distance from a centre to a vertex (bounds for the cull margin)
-    double cmax = 0.0;
-    for (int a = 0; a < 3; ++a) {
-        cmax = std::max(cmax, std::fabs(bbox_min[a] + ((double)lo[a] + 0.5) * pitch));
-        cmax = std::max(cmax, std::fabs(bbox_min[a] + ((double)hi[a] + 0.5) * pitch));
-    }
-    double smax = 0.0;
-    for (int a = 0; a < 3; ++a)
-        smax += std::pow(std::max({std::fabs(mx[a] - (bbox_min[a] + (double)lo[a] * pitch)),
-                                   std::fabs((bbox_min[a] + (double)(hi[a] + 1) * pitch) - mn[a]), pitch}),
-                         2.0);
-    smax = std::sqrt(smax);
-    const double dirx = kRayX, diry = kRayY, dirz = kRayZ;
+    if (!granule_bounds(mn, mx, bbox_min, pitch, dims, lo, hi, &gi.cmax, &gi.smax)) return false;
     for (uint32_t k = 0; k < ntri; ++k) {
-        const double *A = tri_xyz + 9 * (uint64_t)k, *B = A + 3, *C = A + 6;
-        TriGeo &t = geo[k];
-        t.ax = A[0];
-        t.ay = A[1];
-        t.az = A[2];
-        t.e1x = B[0] - A[0];
-        t.e1y = B[1] - A[1];
-        t.e1z = B[2] - A[2];
-        t.e2x = C[0] - A[0];
-        t.e2y = C[1] - A[1];
-        t.e2z = C[2] - A[2];
-        t.hx = diry * t.e2z - dirz * t.e2y;
-        t.hy = dirz * t.e2x - dirx * t.e2z;
-        t.hz = dirx * t.e2y - diry * t.e2x;
-        const double det = t.e1x * t.hx + t.e1y * t.hy + t.e1z * t.hz;
-        t.inv = 1.0 / det;
-        if (std::fabs(det) <= 1e-10 || std::isnan(det)) {
-            // the reference rejects |det| <= eps before anything else; a NaN det
-            // reaches comparisons that all fail -- never a hit
-            t.umin = t.vmin = INFINITY;
-            t.umax = t.vmax = -INFINITY;
-            continue;
-        }
-        const double *pa[3] = {A, B, C};
-        bool finite = std::isfinite(t.inv);
-        double u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY;
-        for (const double *p : pa) {
-            for (int c = 0; c < 3; ++c) finite = finite && std::isfinite(p[c]);
-            const double u = kUx * p[0] + kUy * p[1] + kUz * p[2];
-            const double v = kVx * p[0] + kVy * p[1] + kVz * p[2];
-            u0 = std::min(u0, u);
-            u1 = std::max(u1, u);
-            v0 = std::min(v0, v);
-            v1 = std::max(v1, v);
-        }
-        if (!finite || !std::isfinite(smax)) {
-            t.umin = t.vmin = -INFINITY;
-            t.umax = t.vmax = INFINITY;
-            continue;
-        }
-        const double ext = std::max(u1 - u0, v1 - v0);
-        const double len1 = std::sqrt(t.e1x * t.e1x + t.e1y * t.e1y + t.e1z * t.e1z);
-        const double len2 = std::sqrt(t.e2x * t.e2x + t.e2y * t.e2y + t.e2z * t.e2z);
-        const double L = std::max({len1, len2, ext, 1e-300});
-        // barycentric slack the predicate can admit: eps plus a bound on the
-        // rounding error of u and v (|s| |edge| / |det| * 16 ulps), times 4
-        const double tau = 1e-10 + 16.0 * 1.1102230246251565e-16 * smax * std::max(len1, len2) * std::fabs(t.inv);
-        const double margin =
-            4.0 * 3.0 * L * tau + 64.0 * 1.1102230246251565e-16 * (cmax + amax + L + smax) + 1e-12;
-        t.umin = u0 - margin;
-        t.umax = u1 + margin;
-        t.vmin = v0 - margin;
-        t.vmax = v1 + margin;
+        const double *A = tri_xyz + 9 * (uint64_t)k;
+        tri_prepare(A, A + 3, A + 6, gi.cmax, gi.smax, amax, geo[k]);
     }
     return true;
 }

@@ -4,6 +4,7 @@
 #pragma once
 #include "s2hip_internal.h"
 
+#include <cmath>
 #include <vector>
 
 namespace s2h {
@@ -28,6 +29,11 @@ struct GranuleInfo {
     uint32_t tri_start, tri_end;
     uint32_t owner;        // batch path: candidate index (unused by the full voxelizer)
     int32_t sign;          // batch path: +1 new position, -1 old position
+    // batch path, device-side triangle preparation: cull-margin bounds and the
+    // granule's triangles (3 vertex indices each, into xyz)
+    double cmax, smax, amax;
+    const double *xyz;
+    const uint32_t *tri;
 };
 
 struct Tile {
@@ -40,6 +46,72 @@ struct Tile {
 // v = dir x u (|dir . u|, |dir . v| < 1e-16; the cull margin covers it).
 constexpr double kUx = 0.27587465517742293, kUy = -0.9611936197409646, kUz = 0.0;
 constexpr double kVx = 0.18850557782307928, kVy = 0.05410347115597539, kVz = -0.9808731791583722;
+
+__host__ __device__ inline bool f64_isnan(double x) { return x != x; }
+__host__ __device__ inline bool f64_isfinite(double x) { return fabs(x) <= 1.7976931348623157e308; }
+
+// Precomputed triangle data for the predicate: edges, h = dir x e2 and
+// inv = 1/det in the reference's operation order (bit-identical to computing
+// them per test), plus the triangle's projected bound on the plane normal to
+// the ray, widened by a margin that dominates every rounding error of the
+// predicate for centres within the granule's range.  cmax is the largest
+// |coordinate| of such a centre, smax the largest centre-vertex distance and
+// amax the largest |vertex coordinate| of the granule.  Shared by the host
+// voxelizer and the batched device preparation.
+__host__ __device__ inline void tri_prepare(const double *A, const double *B, const double *C, double cmax,
+                                            double smax, double amax, TriGeo &t) {
+    t.ax = A[0];
+    t.ay = A[1];
+    t.az = A[2];
+    t.e1x = B[0] - A[0];
+    t.e1y = B[1] - A[1];
+    t.e1z = B[2] - A[2];
+    t.e2x = C[0] - A[0];
+    t.e2y = C[1] - A[1];
+    t.e2z = C[2] - A[2];
+    t.hx = kRayY * t.e2z - kRayZ * t.e2y;
+    t.hy = kRayZ * t.e2x - kRayX * t.e2z;
+    t.hz = kRayX * t.e2y - kRayY * t.e2x;
+    const double det = t.e1x * t.hx + t.e1y * t.hy + t.e1z * t.hz;
+    t.inv = 1.0 / det;
+    if (fabs(det) <= 1e-10 || f64_isnan(det)) {
+        // the reference rejects |det| <= eps before anything else; a NaN det
+        // reaches comparisons that all fail -- never a hit
+        t.umin = t.vmin = INFINITY;
+        t.umax = t.vmax = -INFINITY;
+        return;
+    }
+    const double *pa[3] = {A, B, C};
+    bool finite = f64_isfinite(t.inv);
+    double u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY;
+    for (int k = 0; k < 3; ++k) {
+        const double *p = pa[k];
+        for (int c = 0; c < 3; ++c) finite = finite && f64_isfinite(p[c]);
+        const double u = kUx * p[0] + kUy * p[1] + kUz * p[2];
+        const double v = kVx * p[0] + kVy * p[1] + kVz * p[2];
+        u0 = fmin(u0, u);
+        u1 = fmax(u1, u);
+        v0 = fmin(v0, v);
+        v1 = fmax(v1, v);
+    }
+    if (!finite || !f64_isfinite(smax)) {
+        t.umin = t.vmin = -INFINITY;
+        t.umax = t.vmax = INFINITY;
+        return;
+    }
+    const double ext = fmax(u1 - u0, v1 - v0);
+    const double len1 = sqrt(t.e1x * t.e1x + t.e1y * t.e1y + t.e1z * t.e1z);
+    const double len2 = sqrt(t.e2x * t.e2x + t.e2y * t.e2y + t.e2z * t.e2z);
+    const double L = fmax(fmax(len1, len2), fmax(ext, 1e-300));
+    // barycentric slack the predicate can admit: eps plus a bound on the
+    // rounding error of u and v (|s| |edge| / |det| * 16 ulps), times 4
+    const double tau = 1e-10 + 16.0 * 1.1102230246251565e-16 * smax * fmax(len1, len2) * fabs(t.inv);
+    const double margin = 4.0 * 3.0 * L * tau + 64.0 * 1.1102230246251565e-16 * (cmax + amax + L + smax) + 1e-12;
+    t.umin = u0 - margin;
+    t.umax = u1 + margin;
+    t.vmin = v0 - margin;
+    t.vmax = v1 + margin;
+}
 
 // Moller-Trumbore in the reference's operation order on precomputed
 // e1, e2, h = dir x e2 and inv = 1/det (identical values to computing them per
@@ -224,6 +296,15 @@ struct SetBit {
         set_voxel(words, ny, row_words, x, y, z);
     }
 };
+
+// Voxel index range [lo, hi) of a granule with vertex bbox [mn, mx]
+// (part_voxel_ranges, src/geometry/s2.rs:126-147) and the cull-margin bounds
+// cmax / smax of tri_prepare.  Returns false when the range is empty.
+bool granule_bounds(const double mn[3], const double mx[3], const double bbox_min[3], double pitch,
+                    const uint64_t dims[3], int64_t lo[3], int64_t hi[3], double *cmax, double *smax);
+// Rust's f64::min / f64::max (a NaN operand is ignored).
+inline double rmin(double a, double b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a < b ? a : b)); }
+inline double rmax(double a, double b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a > b ? a : b)); }
 
 // Host-side preparation of one granule's triangles (precomputed edges, 1/det and
 // the conservative projected bound) and its voxel range.  Returns false when the

@@ -112,9 +112,10 @@ class Context:
 class Volume:
     """A device-resident packed occupancy volume (s2h_vol)."""
 
-    def __init__(self, ctx: Context, handle):
+    def __init__(self, ctx: Context, handle, owned: bool = True):
         self.ctx = ctx
         self.handle = handle
+        self.owned = owned  # False: a view of a volume owned elsewhere (a seed's), never freed here
         d = np.zeros(3, np.uint64)
         L.check(L.lib().s2h_vol_dims(handle, L.ptr(d, L._u64p)))
         self.dims = [int(v) for v in d]
@@ -157,9 +158,9 @@ class Volume:
         return int(v.value)
 
     def free(self):
-        if self.handle:
+        if self.handle and self.owned:
             L.lib().s2h_vol_free(self.ctx.handle, self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -73,3 +73,66 @@ def test_move_out_of_grid_and_identity(ctx):
     far = parts[1].moved(np.array([100.0, 0.0, 0.0]), np.eye(3)).mesh().vertices
     got = batch.batch_eval(ctx, [sd], [(0, 1, far)])[0]
     assert np.array_equal(got, full_eval(ctx, sd, 1, far))
+
+
+def test_graph_replays_with_eager_work_between(ctx):
+    # three batches of different sizes (the workspace grows, the graph is
+    # recaptured, then replayed), with unrelated eager work in between
+    sd, parts, rng = make_seed(ctx, 44, 12, 91, 11, overlap=True)
+    other = s2.Volume.from_bytes(ctx, (np.random.default_rng(5).random((30, 31, 33)) < 0.3).astype(np.uint8))
+    ref_other = other.exact_s2(9)
+    for count in (5, 21, 9):
+        moves = []
+        for _ in range(count):
+            i = int(rng.integers(0, len(parts)))
+            moves.append((0, i, random_move(rng, parts[i])))
+        curves = batch.batch_eval(ctx, [sd], moves)
+        assert np.array_equal(other.exact_s2(9), ref_other)
+        for (si, i, newv), got in zip(moves, curves):
+            assert np.array_equal(got, full_eval(ctx, sd, i, newv)), (count, i)
+    other.free()
+
+
+def test_graph_and_eager_launches_agree(ctx, monkeypatch):
+    sd, parts, rng = make_seed(ctx, 36, 9, 17, 9)
+    moves = [(0, int(i), random_move(rng, parts[int(i)])) for i in rng.integers(0, len(parts), 17)]
+    graph = batch.batch_eval(ctx, [sd], moves)
+    monkeypatch.setenv("S2HIP_BATCH_GRAPH", "0")
+    eager = batch.batch_eval(ctx, [sd], moves)
+    for a, b in zip(graph, eager):
+        assert np.array_equal(a, b)
+
+
+def test_accept_updates_seed_exactly(ctx):
+    sd, parts, rng = make_seed(ctx, 40, 10, 23, 10, overlap=True)
+    for step in range(3):
+        i = int(rng.integers(0, len(parts)))
+        newv = random_move(rng, parts[i])
+        want = full_eval(ctx, sd, i, newv)
+        moved = sd.moved_mesh(i, newv)
+        got = sd.accept(i, newv)
+        assert np.array_equal(got, want), step
+        # the seed's volume is the moved packing's voxelization, its counts match a recount
+        ref = s2.voxelize(ctx, moved, sd.bbox, 1.0)
+        assert np.array_equal(sd.volume().read_packed(), ref.read_packed())
+        ref.free()
+        assert np.array_equal(sd.exact_s2(), want)
+        # a candidate against the updated seed still matches a full evaluation
+        j = int(rng.integers(0, len(parts)))
+        cand = sd.meshes[j].vertices + rng.uniform(-1.5, 1.5, 3)
+        assert np.array_equal(batch.batch_eval(ctx, [sd], [(0, j, cand)])[0], full_eval(ctx, sd, j, cand))
+
+
+def test_particle_with_two_components(ctx):
+    rng = synth.rng_for(77, 3)
+    parts = synth.place_particles(6, 36, rng, synth.ellipsoid_maker(2.5, 3.5), gap=1.0)
+    meshes = [p.mesh() for p in parts]
+    # particle 0 = two disjoint ellipsoids (two granules in one particle)
+    from rustmspt_amd.mesh import merge_meshes
+    pair = merge_meshes([meshes[0], meshes[1]])
+    meshes = [pair] + meshes[2:]
+    bbox = s2.BoundingBox.from_size((36, 36, 36))
+    sd = batch.Seed(ctx, meshes, bbox, 1.0, 9)
+    moves = [(0, 0, pair.vertices + np.array([1.25, -0.5, 0.75])), (0, 1, meshes[1].vertices - 1.0)]
+    for (si, i, newv), got in zip(moves, batch.batch_eval(ctx, [sd], moves)):
+        assert np.array_equal(got, full_eval(ctx, sd, i, newv))
