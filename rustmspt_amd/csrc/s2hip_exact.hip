@@ -190,16 +190,33 @@ static Offset canonical(Offset o) {
     return o;
 }
 
-// Shape-independent part: shell offsets, the half-ball columns (dense table
-// layout in (dx, dy) order), the gather map and the fixed device buffers.
-static s2h_status build_base(s2h_ctx *ctx, ExactPlan *p) {
-    shell_order_offsets(p->dims, p->r_max, p->pitch, p->offs, p->radius_start);
-    const uint64_t K = p->offs.size();
+// ---- shared shell data
+
+// Grid extent that matters for the domain filter |d_i| < n_i: beyond the
+// largest |d_i| of the shells, every offset is in domain.
+static uint64_t clip_extent(uint64_t n, uint32_t r_max, double pitch) {
+    const uint64_t lim = (uint64_t)std::ceil((double)r_max / pitch + 0.5 / pitch) + 2;
+    return std::min(n, lim);
+}
+
+using ShellKey = std::tuple<uint32_t, uint64_t, uint64_t, uint64_t, uint64_t>;
+
+static ShellKey shell_key(const uint64_t dims[3], uint32_t r_max, double pitch) {
+    uint64_t pbits;
+    std::memcpy(&pbits, &pitch, 8);
+    return ShellKey{r_max, pbits, clip_extent(dims[0], r_max, pitch), clip_extent(dims[1], r_max, pitch),
+                    clip_extent(dims[2], r_max, pitch)};
+}
+
+static std::shared_ptr<ShellBase> build_shell_base(const uint64_t dims[3], uint32_t r_max, double pitch) {
+    auto b = std::make_shared<ShellBase>();
+    shell_order_offsets(dims, r_max, pitch, b->offs, b->radius_start);
+    const uint64_t K = b->offs.size();
     int dmax = 0;
-    for (const Offset &o : p->offs) dmax = std::max({dmax, std::abs(o.dx), std::abs(o.dy)});
+    for (const Offset &o : b->offs) dmax = std::max({dmax, std::abs(o.dx), std::abs(o.dy)});
     const int span = 2 * dmax + 1;
     std::vector<int32_t> colm((size_t)span * span, -1);
-    for (const Offset &o0 : p->offs) {
+    for (const Offset &o0 : b->offs) {
         if (o0.dx == 0 && o0.dy == 0 && o0.dz == 0) continue;
         const Offset o = canonical(o0);
         int32_t &m = colm[(size_t)(o.dx + dmax) * span + (o.dy + dmax)];
@@ -207,33 +224,105 @@ static s2h_status build_base(s2h_ctx *ctx, ExactPlan *p) {
     }
     std::vector<int32_t> colidx((size_t)span * span, -1);
     uint64_t tsize = 0;
-    p->colset.clear();
     for (int dx = -dmax; dx <= dmax; ++dx)
         for (int dy = -dmax; dy <= dmax; ++dy) {
             const int m = colm[(size_t)(dx + dmax) * span + (dy + dmax)];
             if (m < 0) continue;
-            colidx[(size_t)(dx + dmax) * span + (dy + dmax)] = (int32_t)p->colset.size();
-            p->colset.push_back(ColInfo{dx, dy, m, (uint32_t)tsize});
+            colidx[(size_t)(dx + dmax) * span + (dy + dmax)] = (int32_t)b->colset.size();
+            b->colset.push_back(ColInfo{dx, dy, m, (uint32_t)tsize});
             tsize += (uint64_t)m + 1;
         }
-    if (tsize >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "offset table too large");
-    p->table_size = tsize;
-    std::vector<int32_t> gather(K);
+    b->table_size = tsize;
+    b->gather.resize(K);
     for (uint64_t i = 0; i < K; ++i) {
-        const Offset &o0 = p->offs[i];
+        const Offset &o0 = b->offs[i];
         if (o0.dx == 0 && o0.dy == 0 && o0.dz == 0) {
-            gather[i] = -1;
+            b->gather[i] = -1;
             continue;
         }
         const Offset o = canonical(o0);
         const int32_t ci = colidx[(size_t)(o.dx + dmax) * span + (o.dy + dmax)];
-        gather[i] = (int32_t)(p->colset[ci].base + (uint32_t)o.dz);
+        b->gather[i] = (int32_t)(b->colset[ci].base + (uint32_t)o.dz);
     }
-    std::vector<uint64_t> rs(p->radius_start.begin(), p->radius_start.end());
-    S2H_TRY(upload(ctx, p->d_offs, p->offs.data(), K * sizeof(Offset)));
-    S2H_TRY(upload(ctx, p->d_gather, gather.data(), K * sizeof(int32_t)));
-    S2H_TRY(upload(ctx, p->d_rstart, rs.data(), rs.size() * 8));
-    S2H_TRY(dev_alloc(ctx, p->d_table, std::max<uint64_t>(tsize, 1) * 8));
+    return b;
+}
+
+// Process-wide shell bases, kept while their total offset count stays under
+// kShellCacheOffsets (least recently used dropped first; plans keep their own
+// reference).
+static constexpr uint64_t kShellCacheOffsets = 160ull << 20;
+static std::mutex g_shell_mu;
+static std::map<ShellKey, std::pair<std::shared_ptr<const ShellBase>, uint64_t>> g_shells;  // base, last use
+static uint64_t g_shell_clock = 0, g_shell_offsets = 0;
+
+static std::shared_ptr<const ShellBase> shell_base(const uint64_t dims[3], uint32_t r_max, double pitch) {
+    const ShellKey key = shell_key(dims, r_max, pitch);
+    {
+        std::lock_guard<std::mutex> lk(g_shell_mu);
+        auto it = g_shells.find(key);
+        if (it != g_shells.end()) {
+            it->second.second = ++g_shell_clock;
+            return it->second.first;
+        }
+    }
+    // the clipped extent gives the same in-domain offsets as the real one
+    const uint64_t cd[3] = {std::get<2>(key), std::get<3>(key), std::get<4>(key)};
+    std::shared_ptr<const ShellBase> b = build_shell_base(cd, r_max, pitch);
+    std::lock_guard<std::mutex> lk(g_shell_mu);
+    auto it = g_shells.find(key);
+    if (it != g_shells.end()) return it->second.first;  // built meanwhile by another thread
+    g_shells[key] = {b, ++g_shell_clock};
+    g_shell_offsets += b->offs.size();
+    while (g_shell_offsets > kShellCacheOffsets && g_shells.size() > 1) {
+        auto old = g_shells.begin();
+        for (auto j = g_shells.begin(); j != g_shells.end(); ++j)
+            if (j->second.second < old->second.second) old = j;
+        g_shell_offsets -= old->second.first->offs.size();
+        g_shells.erase(old);
+    }
+    return b;
+}
+
+static s2h_status shell_dev(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
+                            std::shared_ptr<ShellDev> *out) {
+    const ShellKey key = shell_key(dims, r_max, pitch);
+    auto it = ctx->shell_devs.find(key);
+    if (it != ctx->shell_devs.end()) {
+        *out = it->second;
+        return S2H_OK;
+    }
+    auto d = std::make_shared<ShellDev>();
+    d->host = shell_base(dims, r_max, pitch);
+    const ShellBase &b = *d->host;
+    if (b.table_size >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "offset table too large");
+    const uint64_t K = b.offs.size();
+    s2h_status st = upload(ctx, d->d_offs, b.offs.data(), std::max<uint64_t>(K, 1) * sizeof(Offset));
+    if (st == S2H_OK) st = upload(ctx, d->d_gather, b.gather.data(), std::max<uint64_t>(K, 1) * sizeof(int32_t));
+    if (st == S2H_OK) st = upload(ctx, d->d_rstart, b.radius_start.data(), b.radius_start.size() * 8);
+    if (st != S2H_OK) {
+        dev_free(ctx, d->d_offs);
+        dev_free(ctx, d->d_gather);
+        dev_free(ctx, d->d_rstart);
+        return st;
+    }
+    ctx->shell_devs[key] = d;
+    *out = d;
+    return S2H_OK;
+}
+
+void shell_devs_release(s2h_ctx *ctx) {
+    for (auto &kv : ctx->shell_devs) {
+        dev_free(ctx, kv.second->d_offs);
+        dev_free(ctx, kv.second->d_gather);
+        dev_free(ctx, kv.second->d_rstart);
+    }
+    ctx->shell_devs.clear();
+}
+
+// Per-plan device buffers.
+static s2h_status plan_buffers(s2h_ctx *ctx, ExactPlan *p) {
+    const uint64_t K = p->offs.size();
+    S2H_TRY(dev_alloc(ctx, p->d_table, std::max<uint64_t>(p->table_size, 1) * 8));
     S2H_TRY(dev_alloc(ctx, p->d_terms, std::max<uint64_t>(K, 1) * 8));
     S2H_TRY(dev_alloc(ctx, p->d_hits, std::max<uint64_t>(K, 1) * 8));
     S2H_TRY(dev_alloc(ctx, p->d_curve, (p->r_max + 1) * 9 + 16));
@@ -260,11 +349,15 @@ struct Grouping {
     double total = 0;
 };
 
-static void group_columns(const ExactPlan *p, int band, Grouping &g) {
-    const int nx = (int)p->dims[0], ny = (int)p->dims[1], nz = (int)p->dims[2];
-    g.band = band;
-    const std::vector<ColInfo> &cs = p->colset;
-    // bands of `band` consecutive m values, counted from the largest m down
+// Columns in bands of `band` consecutive m values (counted from the largest m
+// down), by angle inside a band, padded to whole groups of 64 with idle lanes.
+static std::shared_ptr<const std::vector<ColInfo>> band_lanes(const ShellBase &b, int band) {
+    {
+        std::lock_guard<std::mutex> lk(b.lanes_mu);
+        auto it = b.lanes.find(band);
+        if (it != b.lanes.end()) return it->second;
+    }
+    const std::vector<ColInfo> &cs = b.colset;
     std::vector<int32_t> ms;
     for (const ColInfo &c : cs) ms.push_back(c.m);
     std::sort(ms.begin(), ms.end(), std::greater<int32_t>());
@@ -279,14 +372,25 @@ static void group_columns(const ExactPlan *p, int band, Grouping &g) {
     }
     std::vector<size_t> order(cs.size());
     std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-        if (bnd[a] != bnd[b]) return bnd[a] < bnd[b];
-        if (ang[a] != ang[b]) return ang[a] < ang[b];
-        return cs[a].m > cs[b].m;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t c) {
+        if (bnd[a] != bnd[c]) return bnd[a] < bnd[c];
+        if (ang[a] != ang[c]) return ang[a] < ang[c];
+        return cs[a].m > cs[c].m;
     });
-    g.ngroups = (uint32_t)((cs.size() + 63) / 64);
-    g.lanes.assign((size_t)g.ngroups * 64, ColInfo{0, 0, -1, 0});
-    for (size_t k = 0; k < order.size(); ++k) g.lanes[k] = cs[order[k]];
+    auto lanes = std::make_shared<std::vector<ColInfo>>((cs.size() + 63) / 64 * 64, ColInfo{0, 0, -1, 0});
+    for (size_t k = 0; k < order.size(); ++k) (*lanes)[k] = cs[order[k]];
+    std::lock_guard<std::mutex> lk(b.lanes_mu);
+    auto it = b.lanes.find(band);
+    if (it != b.lanes.end()) return it->second;
+    b.lanes[band] = lanes;
+    return lanes;
+}
+
+static void group_columns(const ExactPlan *p, int band, Grouping &g) {
+    const int nx = (int)p->dims[0], ny = (int)p->dims[1], nz = (int)p->dims[2];
+    g.band = band;
+    g.lanes = *band_lanes(*p->shells->host, band);
+    g.ngroups = (uint32_t)(g.lanes.size() / 64);
     const int need_dw = (nz + 31) / 32;
     const int NW = std::max(1, supported_nw(need_dw));
     g.blks.clear();
@@ -444,8 +548,7 @@ static bool autotune_enabled(const s2h_ctx *ctx) {
 }
 
 void exact_plan_release(s2h_ctx *ctx, ExactPlan *p) {
-    DevBuf *bufs[] = {&p->d_offs, &p->d_gather, &p->d_rstart, &p->d_cols, &p->d_items,
-                      &p->d_table, &p->d_terms, &p->d_hits, &p->d_curve};
+    DevBuf *bufs[] = {&p->d_cols, &p->d_items, &p->d_table, &p->d_terms, &p->d_hits, &p->d_curve};
     for (DevBuf *b : bufs) dev_free(ctx, *b);
 }
 
@@ -460,11 +563,13 @@ s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, 
         *out = itr->second.get();
         return S2H_OK;
     }
-    auto p = std::make_unique<ExactPlan>();
+    std::shared_ptr<ShellDev> sdv;
+    S2H_TRY(shell_dev(ctx, dims, r_max, pitch, &sdv));
+    auto p = std::make_unique<ExactPlan>(sdv);
     for (int a = 0; a < 3; ++a) p->dims[a] = dims[a];
     p->r_max = r_max;
     p->pitch = pitch;
-    s2h_status st = build_base(ctx, p.get());
+    s2h_status st = plan_buffers(ctx, p.get());
     if (st == S2H_OK) {
         TuneEntry e;
         if (tune_lookup(p.get(), e)) {

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -96,14 +97,43 @@ struct ExactTune {
     int32_t waves = 16384;
 };
 
+// Shell data of an exact plan that depends only on (r_max, pitch) and on the
+// grid extent where it clips the shells: the in-domain offsets in shell order,
+// the canonical half-ball columns (dense-table layout) and the gather map.
+// Built once per process and shared (read-only) by every plan and context.
+struct ShellBase {
+    std::vector<Offset> offs;             // shell order
+    std::vector<uint64_t> radius_start;   // r_max + 2
+    std::vector<ColInfo> colset;          // every half-ball column, table order
+    std::vector<int32_t> gather;          // shell order -> table index (-1 = zero offset)
+    uint64_t table_size = 0;              // dense count-table entries
+    // columns ordered 64 to a wave for a band width (filled on first use)
+    mutable std::mutex lanes_mu;
+    mutable std::map<int, std::shared_ptr<const std::vector<ColInfo>>> lanes;
+};
+
+// A context's device copy of a ShellBase, shared by its plans.
+struct ShellDev {
+    std::shared_ptr<const ShellBase> host;
+    DevBuf d_offs;     // Offset[K]
+    DevBuf d_gather;   // int32[K]
+    DevBuf d_rstart;   // uint64[r_max+2]
+};
+
 struct ExactPlan {
+    explicit ExactPlan(std::shared_ptr<ShellDev> s)
+        : shells(std::move(s)), offs(shells->host->offs), radius_start(shells->host->radius_start),
+          colset(shells->host->colset), table_size(shells->host->table_size), d_offs(shells->d_offs),
+          d_gather(shells->d_gather), d_rstart(shells->d_rstart) {}
     uint64_t dims[3];
     uint32_t r_max;
     double pitch;
-    std::vector<Offset> offs;             // shell order
-    std::vector<uint64_t> radius_start;   // r_max + 2
-    uint64_t table_size = 0;              // dense count-table entries
-    std::vector<ColInfo> colset;          // every half-ball column, table order
+    std::shared_ptr<ShellDev> shells;
+    const std::vector<Offset> &offs;
+    const std::vector<uint64_t> &radius_start;
+    const std::vector<ColInfo> &colset;
+    const uint64_t table_size;
+    const DevBuf &d_offs, &d_gather, &d_rstart;
     // tuned part: groups, blocks and items for `tune`
     ExactTune tune;
     int tuned = 0;                        // 0 not yet, 1 timed on the device, -1 model pick
@@ -112,10 +142,7 @@ struct ExactPlan {
     uint32_t ngroups = 0;
     std::vector<DzBlock> blocks;
     std::vector<WorkItem> items;          // heaviest first
-    // device copies
-    DevBuf d_offs;     // Offset[K]
-    DevBuf d_gather;   // int32[K] table index (-1 = zero offset)
-    DevBuf d_rstart;   // uint64[r_max+2]
+    // per-plan device buffers
     DevBuf d_cols;     // ColInfo[ngroups*64]
     DevBuf d_items;    // WorkItem[items]
     DevBuf d_table;    // uint64[table_size]
@@ -166,6 +193,8 @@ struct s2h_ctx {
     s2h::DevBuf d_scalar;   // small scratch (popcount result etc.)
     bool profiling = false;
     bool autotune = true;   // time launch-shape candidates on a new exact plan
+    std::map<std::tuple<uint32_t, uint64_t, uint64_t, uint64_t, uint64_t>, std::shared_ptr<s2h::ShellDev>>
+        shell_devs;   // device copies of shared shell data, by (r_max, pitch, clipped extent)
     s2h::BatchWS *batch_ws = nullptr;            // persistent batch workspace and graph
     std::multimap<uint64_t, void *> block_cache;  // released device blocks by size
     uint64_t cached_bytes = 0;
@@ -186,6 +215,7 @@ s2h_status vol_popcount(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);
 s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
                           ExactPlan **out);
 void exact_plan_release(s2h_ctx *ctx, ExactPlan *plan);
+void shell_devs_release(s2h_ctx *ctx);  // device copies of shared shell data
 // Launch the count kernels for vol into plan->d_table (zeroed first) on ctx->stream.
 s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan);
 // Gather table -> per-offset hits and f64 terms, fold per radius into d_curve.

@@ -401,6 +401,8 @@ void s2h_destroy(s2h_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->exact_plans) exact_plan_release(ctx, kv.second.get());
+    ctx->exact_plans.clear();
+    shell_devs_release(ctx);
     for (auto &kv : ctx->mc_plans) {
         dev_free(ctx, kv.second->d_shell);
         dev_free(ctx, kv.second->d_start);

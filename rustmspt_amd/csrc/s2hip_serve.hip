@@ -212,6 +212,7 @@ void exact_plans_clear(s2h_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->exact_plans) exact_plan_release(ctx, kv.second.get());
     ctx->exact_plans.clear();
+    shell_devs_release(ctx);
     for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
     ctx->fft_plans.clear();
 }

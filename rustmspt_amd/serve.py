@@ -167,6 +167,7 @@ def run_stream(server: Server, pool: List[Volume], schedule: List[Arrival], time
             v = pool[a.volume]
             tickets[i] = server.submit(v.words, v.dims, a.r_max, a.method, tag=i)
 
+    st0 = server.stats()
     th = threading.Thread(target=submitter)
     th.start()
     pending = set()
@@ -198,5 +199,5 @@ def run_stream(server: Server, pool: List[Volume], schedule: List[Arrival], time
             "throughput_evals_per_s": len(done) / max(t_last - t_first, 1e-9),
             "latency_p50_ms": float(np.percentile(lat, 50) * 1e3), "latency_p99_ms": float(np.percentile(lat, 99) * 1e3),
             "latency_max_ms": float(lat.max() * 1e3), "slot_busy_s": busy, "makespan_s": t_last - t_first,
-            "admitted": st["admitted"], "deferred": st["deferred"],
-            "failed": st["failed"], "peak_bytes": st["peak_bytes"], "budget": st["budget"]}
+            "admitted": st["admitted"] - st0["admitted"], "deferred": st["deferred"] - st0["deferred"],
+            "failed": st["failed"] - st0["failed"], "peak_bytes": st["peak_bytes"], "budget": st["budget"]}
