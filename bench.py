@@ -30,39 +30,60 @@ VALU_PEAK_OPS = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 G
 COUNT_PHASE_TRAFFIC = 12549 * 1024 * 2.0 + 1696499 * 1024 * 1.0
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(occ_words, dims, r_max, offs, pair_work_total, target_s):
     """Time the oracle's packed direct kernel (C restatement of
     direct_pair_counts_bits, OpenMP over offsets) on every k-th offset in shell
-    order and extrapolate by pair work."""
+    order and extrapolate by pair work -- with all the threads OpenMP gets on
+    this host, then with 8 threads.  The reference's own planner picks this
+    direct kernel at 256^3 r_max=64 (its FFT working set exceeds the 768 MiB
+    CPU budget)."""
     import numpy as np
 
     from oracle import oracle as o
 
     valid = np.prod(np.array(dims, dtype=np.float64)[None, :] - np.abs(offs), axis=1)
+
+    def timed(k):
+        sample = offs[::k]
+        t0 = time.perf_counter()
+        o.hits_list(occ_words, dims, sample)
+        return time.perf_counter() - t0, float(valid[::k].sum()), len(sample)
+
     # calibrate: a small stride-sample first, then size the real sample
-    k = 4001
-    sample = offs[::k]
-    t0 = time.perf_counter()
-    o.hits_list(occ_words, dims, sample)
-    dt = max(time.perf_counter() - t0, 1e-4)
-    rate = valid[::k].sum() / dt
-    want_pairs = rate * target_s
-    k = max(1, int(pair_work_total / max(want_pairs, 1.0)))
-    sample = offs[::k]
-    t0 = time.perf_counter()
-    o.hits_list(occ_words, dims, sample)
-    dt = time.perf_counter() - t0
-    pairs = float(valid[::k].sum())
+    dt, pairs, _ = timed(4001)
+    rate = pairs / max(dt, 1e-4)
+    k = max(1, int(pair_work_total / max(rate * target_s, 1.0)))
+    dt, pairs, n = timed(k)
     rate = pairs / dt
-    return {
+    threads = o.num_threads()
+    out = {
         "value": rate / pair_work_total,
         "unit": "evals/s",
-        "cores": o.num_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": f"every {k}th of {len(offs)} in-domain offsets in shell order ({len(sample)} offsets, "
+        "sample": f"every {k}th of {len(offs)} in-domain offsets in shell order ({n} offsets, "
                   f"{pairs:.3e} of {pair_work_total:.3e} pair-tests) timed in {dt:.1f} s, extrapolated by pair work",
         "seconds_per_eval": pair_work_total / rate,
+        "cpu_model": _cpu_model(),
+        "host_cpus": os.cpu_count(),
     }
+    if threads != 8:
+        o.set_threads(8)
+        k8 = max(1, int(k * threads / 8))
+        dt8, pairs8, n8 = timed(k8)
+        o.set_threads(threads)
+        out["seconds_per_eval_8_threads"] = pair_work_total / (pairs8 / dt8)
+    return out
 
 
 def batched_leg(ctx, ncand: int, seed: int, plain_sample: int):
@@ -150,7 +171,15 @@ def fft_leg(ctx, vol, r_max, hits, offs, rs, curve_exact, steps):
     per_bin_rel = [0.0] + [float(err[rs[r]:rs[r + 1]].max() / max(1.0, hits[rs[r]:rs[r + 1]].max()))
                            for r in range(1, r_max + 1)]
     d_s2 = np.abs(curve - curve_exact)
+    from rustmspt_amd.config import smooth_fft_length
+
+    f = [smooth_fft_length(2 * n - 1) for n in vol.dims]
+    alg_bytes = 12.0 * f[0] * f[1] * (f[2] // 2 + 1) * 8.0
+    dev_s = float(np.mean(dev)) * 1e-3
     return {"evals_per_s": steps / el, "device_ms_per_eval": float(np.mean(dev)),
+            "roofline": {"bound": "hbm", "achieved": alg_bytes / dev_s / 1e9, "peak": 8000.0, "unit": "GB/s",
+                         "frac": alg_bytes / dev_s / 8e12, "algorithmic_bytes_per_eval": alg_bytes,
+                         "padded": f},
             "max_abs_count_error": float(err.max()), "max_rel_count_error": float(max(per_bin_rel)),
             "rounds_to_exact": bool(np.all(np.rint(fc) == hits)),
             "max_abs_s2_error": float(d_s2.max()),

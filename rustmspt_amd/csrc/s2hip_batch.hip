@@ -965,7 +965,7 @@ static s2h_status download_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, 
     return S2H_OK;
 }
 
-static constexpr uint32_t kRowChunkDefault = 32;
+static constexpr uint32_t kRowChunkDefault = 128;
 
 // Rows per delta item: the first batch over a new set of seed shapes times
 // each candidate value on the device (HIP events around the graph replay) and
@@ -995,7 +995,7 @@ static s2h_status pick_row_chunk(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seed
     s2h_status st = S2H_OK;
     float best = 1e30f;
     uint32_t pick = kRowChunkDefault;
-    for (uint32_t rc : {8u, 16u, 32u, 64u}) {
+    for (uint32_t rc : {32u, 64u, 128u, 256u, 512u}) {
         set_row_chunk(hb, rc);
         st = upload_batch(ctx, w, hb, cands, ncand, seeds);
         if (st == S2H_OK) st = run_sequence(ctx, w);  // warm-up (and capture)

@@ -233,6 +233,11 @@ static std::shared_ptr<ShellBase> build_shell_base(const uint64_t dims[3], uint3
             tsize += (uint64_t)m + 1;
         }
     b->table_size = tsize;
+    for (const Offset &o : b->offs) {
+        b->extent[0] = std::max(b->extent[0], std::abs(o.dx));
+        b->extent[1] = std::max(b->extent[1], std::abs(o.dy));
+        b->extent[2] = std::max(b->extent[2], std::abs(o.dz));
+    }
     b->gather.resize(K);
     for (uint64_t i = 0; i < K; ++i) {
         const Offset &o0 = b->offs[i];

@@ -24,6 +24,7 @@
 // that order, output natural), so no reordering pass is needed; the pointwise
 // |F|^2 is order-independent.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 
@@ -50,6 +51,7 @@ struct FftPlan {
     DevBuf rev_z;    // int32[fz]: natural frequency held at each DIF output position
     DevBuf pos_z;    // int32[fz]: DIF output position of each natural frequency
     DevBuf g1, g2, g3, cube, fcounts;
+    int tile = 0;    // kz columns per workgroup in the y and x passes (0: not chosen yet)
 };
 
 void fft_plan_release(s2h_ctx *ctx, FftPlan *p) {
@@ -479,9 +481,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     const int nxo = 2 * Rx + 1, nyo = 2 * Ry + 1;
     const Fac fcx = to_fac(p->ax[0]), fcy = to_fac(p->ax[1]), fcz = to_fac(p->ax[2]);
     hipStream_t s = ctx->stream;
-    const int lmax = std::max(fx, fy);
-    int tile = kTileMax;
-    while (tile > 1 && lds_bytes(lmax, tile, lmax + 1) > 150 * 1024) tile /= 2;
+    const int tile = p->tile;
     const int tiles = (H + tile - 1) / tile;
     // z forward
     {
@@ -535,21 +535,67 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     return S2H_OK;
 }
 
+// kz columns per workgroup of the y and x passes: the largest power of two up
+// to kTileMax whose LDS tile fits, or -- on a context that autotunes -- the
+// fastest of those candidates timed on the device (HIP events around one run
+// each, after a warm-up).  Every run rewrites all of the plan's buffers, so
+// the trial runs leave nothing behind.
+s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
+    const int lmax = (int)std::max(p->f[0], p->f[1]);
+    std::vector<int> cands;
+    for (int t = kTileMax; t >= 1; t /= 2)
+        if (lds_bytes(lmax, t, lmax + 1) <= 150 * 1024) cands.push_back(t);
+    if (cands.empty()) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
+    p->tile = cands.front();
+    const char *env = std::getenv("S2HIP_AUTOTUNE");
+    const double cells = (double)p->f[0] * (double)p->f[1] * (double)p->f[2];
+    // tune only where a run is short (up to a 1024^3 padded grid)
+    if (!ctx->autotune || (env && env[0] == '0') || cands.size() < 2 || cells > 1.1e9) return S2H_OK;
+    if (cands.size() > 3) cands.resize(3);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    S2H_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(S2H_ERR_HIP, "hipEventCreate failed");
+    }
+    s2h_status st = S2H_OK;
+    float best = 1e30f;
+    int pick = cands.front();
+    for (int t : cands) {
+        p->tile = t;
+        st = fft_run(ctx, vol, p);  // warm-up
+        hipError_t e = hipSuccess;
+        if (st == S2H_OK) e = hipEventRecord(e0, ctx->stream);
+        if (st == S2H_OK && e == hipSuccess) st = fft_run(ctx, vol, p);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (st == S2H_OK && e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
+        if (st != S2H_OK) break;
+        if (ms < best) {
+            best = ms;
+            pick = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->tile = pick;
+    return st;
+}
+
 // FP32 counts of every shell offset into p->fcounts (device) using the exact plan's offset list.
 s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, ExactPlan **ep_out,
                              FftPlan **fp_out) {
     ExactPlan *ep = nullptr;
     S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pitch, &ep));
-    int R[3] = {0, 0, 0};
-    for (const Offset &o : ep->offs) {
-        R[0] = std::max(R[0], std::abs(o.dx));
-        R[1] = std::max(R[1], std::abs(o.dy));
-        R[2] = std::max(R[2], std::abs(o.dz));
-    }
+    const int *ext = ep->shells->host->extent;
+    int R[3] = {ext[0], ext[1], ext[2]};
     FftPlan *fp = nullptr;
     S2H_TRY(fft_plan_get(ctx, vol->dims, R, &fp));
     const uint64_t K = ep->offs.size();
     S2H_TRY(dev_alloc(ctx, fp->fcounts, std::max<uint64_t>(K, 1) * 4));
+    if (fp->tile == 0) S2H_TRY(fft_pick_tile(ctx, vol, fp));
     prof_mark(ctx, 0);
     S2H_TRY(fft_run(ctx, vol, fp));
     if (K) {

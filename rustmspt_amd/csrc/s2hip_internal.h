@@ -107,6 +107,7 @@ struct ShellBase {
     std::vector<ColInfo> colset;          // every half-ball column, table order
     std::vector<int32_t> gather;          // shell order -> table index (-1 = zero offset)
     uint64_t table_size = 0;              // dense count-table entries
+    int extent[3] = {0, 0, 0};            // largest |dx|, |dy|, |dz| of the offsets
     // columns ordered 64 to a wave for a band width (filled on first use)
     mutable std::mutex lanes_mu;
     mutable std::map<int, std::shared_ptr<const std::vector<ColInfo>>> lanes;
