@@ -179,6 +179,27 @@ s2h_status s2h_mc_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitc
 s2h_status s2h_mc_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, uint64_t samples,
                      uint64_t seed, double *s2, uint64_t n);
 
+/* Continuous mesh Monte Carlo (calculate_s2_mesh_mc_seeded, src/geometry/s2.rs:1273-1320;
+ * the path the reference takes for a non-exact method with voxel_pitch <= 0,
+ * src/geometry/s2.rs:1351-1356): per radius, max(samples, 200) point pairs p, p + r d
+ * with p uniform in the box [bbox_min, bbox_max] and d uniform on the sphere; pairs
+ * leaving the box are rejected; hits count pairs with both points inside the mesh
+ * by its whole-mesh parity query (PreparedMeshQuery::contains_point,
+ * src/geometry/mesh_query.rs:73-116), bit-exactly.  Philox4x32-10 stream keyed by
+ * seed (the reference's ChaCha12 streams are not reproduced; the estimand is).
+ * hits_valid: 2*(r_max+1) values, (hits, valid) per radius, radius 0 zero. */
+s2h_status s2h_mesh_mc_counts(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri,
+                              uint64_t ntris, const double bbox_min[3], const double bbox_max[3],
+                              uint32_t r_max, uint64_t samples, uint64_t seed, uint64_t *hits_valid,
+                              uint64_t n);
+/* The curve: s2[0] = vf (the caller's geometric volume fraction in the box,
+ * volume_fraction_in_bbox, src/geometry/volume.rs:437-469), s2[r] = hits/valid
+ * (0 when no pair was valid), no fill -- as the reference. */
+s2h_status s2h_mesh_mc_s2(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri,
+                          uint64_t ntris, const double bbox_min[3], const double bbox_max[3],
+                          uint32_t r_max, uint64_t samples, uint64_t seed, double vf, double *s2,
+                          uint64_t n);
+
 /* ------------------------------------------------------------ batched candidates */
 
 /* A resident seed packing for batched simulated-annealing evaluation: packed

@@ -56,6 +56,8 @@ def _load():
         "orc_fft_hits": (None, [_f64p, _u64p, _i64p, C.c_uint64, _u64p]),
         "orc_philox4x32_10": (None, [_u32p, _u32p, _u32p]),
         "orc_mc_counts": (None, [_u64p, _u64p, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _u64p]),
+        "orc_mesh_mc_counts": (None, [_f64p, C.c_uint64, _u32p, C.c_uint64, _f64p, _f64p, C.c_uint32, C.c_uint64,
+                                      C.c_uint64, _u64p]),
         "orc_num_threads": (C.c_int, []),
         "orc_set_threads": (None, [C.c_int]),
     }
@@ -264,6 +266,18 @@ def mc_counts(words, dims, r_max, pitch, samples, seed):
     lib().orc_mc_counts(_p(words, _u64p), _p(_dims(dims), _u64p), int(r_max), float(pitch), int(samples),
                         int(seed) & 0xFFFFFFFFFFFFFFFF, _p(hv, _u64p))
     return hv.reshape(-1, 2)
+
+
+def mesh_mc_counts(verts, tri, bmin, bmax, r_max, samples, seed):
+    """Mesh Monte Carlo (hits, valid) per radius, rows 0..r_max (row 0 zero)."""
+    v = np.ascontiguousarray(verts, np.float64).reshape(-1)
+    t = np.ascontiguousarray(tri, np.uint32).reshape(-1)
+    out = np.zeros(2 * (int(r_max) + 1), np.uint64)
+    lib().orc_mesh_mc_counts(_p(v, _f64p), len(v) // 3, _p(t, _u32p), len(t) // 3,
+                             _p(np.ascontiguousarray(bmin, np.float64), _f64p),
+                             _p(np.ascontiguousarray(bmax, np.float64), _f64p), int(r_max), int(samples),
+                             int(seed) & 0xFFFFFFFFFFFFFFFF, _p(out, _u64p))
+    return out.reshape(-1, 2)
 
 
 def set_threads(n):

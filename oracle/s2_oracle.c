@@ -790,6 +790,92 @@ void orc_mc_counts(const uint64_t *words, const uint64_t *dims, uint32_t r_max, 
     }
 }
 
+/* Continuous mesh Monte Carlo estimator of src/geometry/s2.rs:1273-1320 with this
+ * engine's Philox stream: sample i of radius r draws
+ * philox(ctr={i_lo, i_hi, r, 0x4D455348 + k}, key={seed_lo, seed_hi}) for call k.
+ * Calls 0-1 give p (53-bit uniforms u = ((a >> 5) * 2^26 + (b >> 6)) / 2^53 from
+ * word pairs (0,1), (2,3) of call 0 and (0,1) of call 1; coordinate
+ * min + (max - min) u); direction attempt a uses calls 2+2a and 3+2a the same way
+ * for x, y, z in [-1, 1), accepted when 1e-12 < |d|^2 <= 1 and normalized by
+ * 1/sqrt(|d|^2); after 64 rejections the direction is +x.  q = p + r d; a q
+ * outside the box is rejected, else valid += 1, and hits += 1 when both points
+ * are inside the mesh: the whole-mesh contains_point of
+ * src/geometry/mesh_query.rs:73-116 (bbox of all vertices +-1e-9, full scan).
+ * max(samples, 200) samples per radius.  hv: 2*(r_max+1) u64. */
+static double u53(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+static inline double rmin(double a, double b) { return isnan(a) ? b : (isnan(b) ? a : (a < b ? a : b)); }
+static inline double rmax(double a, double b) { return isnan(a) ? b : (isnan(b) ? a : (a > b ? a : b)); }
+
+void orc_mesh_mc_counts(const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntri,
+                        const double *bmin, const double *bmax, uint32_t r_max, uint64_t samples,
+                        uint64_t seed, uint64_t *hv) {
+    memset(hv, 0, 2 * (r_max + 1) * sizeof(uint64_t));
+    const uint64_t spr = samples < 200 ? 200 : samples;
+    bbox3 bb;
+    bb.valid = nverts > 0;
+    for (uint64_t v = 0; v < nverts; v++)
+        for (int a = 0; a < 3; a++) {
+            double c = xyz[3 * v + a];
+            bb.mn[a] = v ? rmin(bb.mn[a], c) : c;
+            bb.mx[a] = v ? rmax(bb.mx[a], c) : c;
+        }
+    double *tris = malloc((ntri ? ntri : 1) * 9 * sizeof(double));
+    for (uint64_t f = 0; f < ntri; f++)
+        for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++) tris[9 * f + 3 * k + a] = xyz[3 * (uint64_t)tri[3 * f + k] + a];
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    for (uint32_t r = 1; r <= r_max; r++) {
+        uint64_t hits = 0, valid = 0;
+        const double rr = (double)r;
+#pragma omp parallel reduction(+ : hits, valid)
+        {
+            double *scratch = malloc((ntri ? ntri : 1) * sizeof(double));
+#pragma omp for schedule(static)
+            for (uint64_t i = 0; i < spr; i++) {
+                uint32_t a[4], b[4];
+                uint32_t c0[4] = {(uint32_t)i, (uint32_t)(i >> 32), r, 0x4D455348u};
+                uint32_t c1[4] = {(uint32_t)i, (uint32_t)(i >> 32), r, 0x4D455348u + 1u};
+                orc_philox4x32_10(c0, key, a);
+                orc_philox4x32_10(c1, key, b);
+                v3 p = {bmin[0] + (bmax[0] - bmin[0]) * u53(a[0], a[1]),
+                        bmin[1] + (bmax[1] - bmin[1]) * u53(a[2], a[3]),
+                        bmin[2] + (bmax[2] - bmin[2]) * u53(b[0], b[1])};
+                v3 d = {1.0, 0.0, 0.0};
+                for (uint32_t t = 0; t < 64; t++) {
+                    uint32_t e0[4] = {(uint32_t)i, (uint32_t)(i >> 32), r, 0x4D455348u + 2u + 2u * t};
+                    uint32_t e1[4] = {(uint32_t)i, (uint32_t)(i >> 32), r, 0x4D455348u + 3u + 2u * t};
+                    orc_philox4x32_10(e0, key, a);
+                    orc_philox4x32_10(e1, key, b);
+                    double x = -1.0 + 2.0 * u53(a[0], a[1]), y = -1.0 + 2.0 * u53(a[2], a[3]);
+                    double z = -1.0 + 2.0 * u53(b[0], b[1]);
+                    double n2 = x * x + y * y + z * z;
+                    if (n2 > 1e-12 && n2 <= 1.0) {
+                        double inv = 1.0 / sqrt(n2);
+                        d.x = x * inv;
+                        d.y = y * inv;
+                        d.z = z * inv;
+                        break;
+                    }
+                }
+                v3 q = {p.x + d.x * rr, p.y + d.y * rr, p.z + d.z * rr};
+                if (q.x < bmin[0] || q.x > bmax[0] || q.y < bmin[1] || q.y > bmax[1] || q.z < bmin[2] ||
+                    q.z > bmax[2])
+                    continue;
+                valid++;
+                if (granule_contains(tris, ntri, &bb, p, scratch) && granule_contains(tris, ntri, &bb, q, scratch))
+                    hits++;
+            }
+            free(scratch);
+        }
+        hv[2 * r] = hits;
+        hv[2 * r + 1] = valid;
+    }
+    free(tris);
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

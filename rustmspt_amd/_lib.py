@@ -33,6 +33,10 @@ SIGNATURES = {
     "s2h_profile": (C.c_int32, [_vp, C.c_int]),
     "s2h_profile_read": (C.c_int32, [_vp, _f64p, C.c_uint64]),
     "s2h_valu_probe": (C.c_int32, [_vp, _f64p]),
+    "s2h_mesh_mc_counts": (C.c_int32, [_vp, _f64p, C.c_uint64, _u32p, C.c_uint64, _f64p, _f64p, C.c_uint32,
+                                       C.c_uint64, C.c_uint64, _u64p, C.c_uint64]),
+    "s2h_mesh_mc_s2": (C.c_int32, [_vp, _f64p, C.c_uint64, _u32p, C.c_uint64, _f64p, _f64p, C.c_uint32,
+                                   C.c_uint64, C.c_uint64, C.c_double, _f64p, C.c_uint64]),
     "s2h_vol_from_packed": (C.c_int32, [_vp, _u64p, C.c_uint64, _u64p, C.POINTER(_vp)]),
     "s2h_vol_from_bytes": (C.c_int32, [_vp, _u8p, _u64p, C.POINTER(_vp)]),
     "s2h_vol_read_packed": (C.c_int32, [_vp, _vp, _u64p, C.c_uint64]),

@@ -33,6 +33,7 @@ UNITS = {
     "s2hip_batch.hip": ["-ffp-contract=off"],
     "s2hip_serve.hip": ["-ffp-contract=off"],
     "s2hip_probe.hip": [],
+    "s2hip_meshmc.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = {"s2hip_host.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]}
 

@@ -1,0 +1,347 @@
+// Continuous mesh Monte Carlo S2 (no voxelization).
+//
+// Estimator of calculate_s2_mesh_mc_seeded_with_vf (src/geometry/s2.rs:1273-1320):
+// for each radius r = 1..r_max and each of max(samples, 200) samples, a point p
+// uniform in the measurement box and a direction uniform on the sphere (by
+// rejection in the cube: accept 1e-12 < |d|^2 <= 1, then normalize); q = p + r d.
+// A q outside the box is rejected; otherwise valid += 1, and hits += 1 when both
+// p and q are inside the mesh.  Inside means the whole-mesh parity query of
+// PreparedMeshQuery::contains_point (src/geometry/mesh_query.rs:73-116): outside
+// the mesh's vertex bbox +-1e-9 -> out; otherwise every forward hit t of the ray
+// along RAY_DIR_GPU (the predicate of src/geometry/s2.rs:27-56), sorted, counted
+// with the anchored 1e-8 rule, odd -> in.
+//
+// Stream (the reference's ChaCha12 streams cannot be reproduced; the estimand
+// is the same): sample i of radius r draws Philox4x32-10 with key
+// {seed_lo, seed_hi} and counter {i_lo, i_hi, r, 0x4D455348 + k} for call k.
+// Calls 0-1 give p (53-bit uniforms from word pairs (0,1), (2,3) of call 0 and
+// (0,1) of call 1, each u = ((a >> 5) * 2^26 + (b >> 6)) / 2^53, coordinate
+// min + (max - min) u).  Direction attempt a uses calls 2+2a and 3+2a the same
+// way for x, y, z in [-1, 1); after 64 rejected attempts the direction is +x.
+//
+// Acceleration: every query ray has the same direction, so a triangle can only
+// be hit when the query point's projection on the plane normal to the ray lies
+// in the triangle's projected bound (widened by a margin that dominates every
+// rounding error of the predicate -- the voxelizer's conservative cull).  The
+// triangles are binned by that bound on a 2-D grid over the plane; a query
+// tests only its cell's list, so the hit multiset equals a full scan.
+// Compiled with -ffp-contract=off: every f64 operation keeps its order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "s2hip_internal.h"
+#include "s2hip_voxel_dev.h"
+
+namespace s2h {
+namespace meshmc {
+
+constexpr uint32_t kTag = 0x4D455348u;
+constexpr int kDirAttempts = 64;
+constexpr int kThreads = 256;
+constexpr int kPerThread = 8;
+
+struct Grid {
+    const vox::TriGeo *tris;
+    const uint32_t *start;   // gu * gv + 1
+    const uint32_t *ids;     // triangle index per cell entry
+    double u0, v0, inv_du, inv_dv;
+    int gu, gv;
+    int have_bbox;
+    double mn[3], mx[3];     // whole-mesh vertex bbox
+};
+
+__device__ __forceinline__ double u53(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Exact parity for a ray with more than kMaxHits hits: repeated minimum over
+// the cell's list (the anchored rule keeps t_0 = min, then the smallest t with
+// t - last > 1e-8).
+__device__ int overflow_count(const Grid &g, uint32_t c0, uint32_t c1, double px, double py, double pz) {
+    double last = -INFINITY;
+    int unique = 0;
+    while (true) {
+        double best = INFINITY;
+        bool found = false;
+        for (uint32_t k = c0; k < c1; ++k) {
+            const double t = vox::mt_hit(g.tris[g.ids[k]], px, py, pz);
+            if (t < 0.0) continue;
+            if (t - last > 1e-8 && (!found || t < best)) {
+                best = t;
+                found = true;
+            }
+        }
+        if (!found) break;
+        ++unique;
+        last = best;
+    }
+    return unique;
+}
+
+__device__ bool contains(const Grid &g, double px, double py, double pz) {
+    if (!g.have_bbox) return false;
+    const double eps = 1e-9;
+    if (px < g.mn[0] - eps || px > g.mx[0] + eps || py < g.mn[1] - eps || py > g.mx[1] + eps ||
+        pz < g.mn[2] - eps || pz > g.mx[2] + eps)
+        return false;
+    const double pu = vox::kUx * px + vox::kUy * py + vox::kUz * pz;
+    const double pv = vox::kVx * px + vox::kVy * py + vox::kVz * pz;
+    int iu = (int)floor((pu - g.u0) * g.inv_du), iv = (int)floor((pv - g.v0) * g.inv_dv);
+    iu = min(max(iu, 0), g.gu - 1);
+    iv = min(max(iv, 0), g.gv - 1);
+    const uint32_t cell = (uint32_t)iu * g.gv + iv;
+    const uint32_t c0 = g.start[cell], c1 = g.start[cell + 1];
+    double hits[vox::kMaxHits];
+    int nh = 0;
+    bool over = false;
+    for (uint32_t k = c0; k < c1; ++k) {
+        const vox::TriGeo &t = g.tris[g.ids[k]];
+        if (pu < t.umin || pu > t.umax || pv < t.vmin || pv > t.vmax) continue;
+        const double th = vox::mt_hit(t, px, py, pz);
+        if (th < 0.0) continue;
+        if (nh >= vox::kMaxHits) {
+            over = true;
+            break;
+        }
+        double cur = th;
+#pragma unroll
+        for (int s = 0; s < vox::kMaxHits; ++s) {
+            if (s < nh) {
+                const double lo = hits[s] < cur ? hits[s] : cur;
+                const double hi = hits[s] < cur ? cur : hits[s];
+                hits[s] = lo;
+                cur = hi;
+            } else if (s == nh) {
+                hits[s] = cur;
+            }
+        }
+        ++nh;
+    }
+    int unique = 0;
+    if (over) {
+        unique = overflow_count(g, c0, c1, px, py, pz);
+    } else {
+        double last = -INFINITY;
+#pragma unroll
+        for (int s = 0; s < vox::kMaxHits; ++s) {
+            if (s < nh && fabs(hits[s] - last) > 1e-8) {
+                ++unique;
+                last = hits[s];
+            }
+        }
+    }
+    return unique & 1;
+}
+
+__global__ __launch_bounds__(kThreads) void meshmc_kernel(Grid g, double bx0, double by0, double bz0, double bx1,
+                                                          double by1, double bz1, uint64_t samples, uint32_t seed_lo,
+                                                          uint32_t seed_hi, uint32_t blocks_per_r,
+                                                          unsigned long long *__restrict__ hv) {
+    const uint32_t r = blockIdx.x / blocks_per_r + 1;
+    const uint32_t blk = blockIdx.x % blocks_per_r;
+    const double rr = (double)r;
+    uint64_t hits = 0, valid = 0;
+    const uint64_t base = (uint64_t)blk * kThreads * kPerThread + threadIdx.x;
+    for (int k = 0; k < kPerThread; ++k) {
+        const uint64_t i = base + (uint64_t)k * kThreads;
+        if (i >= samples) break;
+        const uint32_t ilo = (uint32_t)i, ihi = (uint32_t)(i >> 32);
+        uint32_t a[4], b[4];
+        philox4x32_10(ilo, ihi, r, kTag + 0u, seed_lo, seed_hi, a);
+        philox4x32_10(ilo, ihi, r, kTag + 1u, seed_lo, seed_hi, b);
+        const double px = bx0 + (bx1 - bx0) * u53(a[0], a[1]);
+        const double py = by0 + (by1 - by0) * u53(a[2], a[3]);
+        const double pz = bz0 + (bz1 - bz0) * u53(b[0], b[1]);
+        double dx = 1.0, dy = 0.0, dz = 0.0;
+        for (int t = 0; t < kDirAttempts; ++t) {
+            philox4x32_10(ilo, ihi, r, kTag + 2u + 2u * (uint32_t)t, seed_lo, seed_hi, a);
+            philox4x32_10(ilo, ihi, r, kTag + 3u + 2u * (uint32_t)t, seed_lo, seed_hi, b);
+            const double x = -1.0 + 2.0 * u53(a[0], a[1]);
+            const double y = -1.0 + 2.0 * u53(a[2], a[3]);
+            const double z = -1.0 + 2.0 * u53(b[0], b[1]);
+            const double n2 = x * x + y * y + z * z;
+            if (n2 > 1e-12 && n2 <= 1.0) {
+                const double inv = 1.0 / sqrt(n2);
+                dx = x * inv;
+                dy = y * inv;
+                dz = z * inv;
+                break;
+            }
+        }
+        const double qx = px + dx * rr, qy = py + dy * rr, qz = pz + dz * rr;
+        if (qx < bx0 || qx > bx1 || qy < by0 || qy > by1 || qz < bz0 || qz > bz1) continue;
+        ++valid;
+        if (contains(g, px, py, pz) && contains(g, qx, qy, qz)) ++hits;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        hits += __shfl_xor(hits, o, 64);
+        valid += __shfl_xor(valid, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (hits) atomicAdd(&hv[2 * r], (unsigned long long)hits);
+        if (valid) atomicAdd(&hv[2 * r + 1], (unsigned long long)valid);
+    }
+}
+
+__global__ void clear_u64_kernel(unsigned long long *p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+}  // namespace meshmc
+}  // namespace s2h
+
+using namespace s2h;
+
+extern "C" {
+
+s2h_status s2h_mesh_mc_counts(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,
+                              const double bbox_min[3], const double bbox_max[3], uint32_t r_max, uint64_t samples,
+                              uint64_t seed, uint64_t *hits_valid, uint64_t n) {
+    using namespace s2h::meshmc;
+    if (!ctx || !bbox_min || !bbox_max || !hits_valid || (ntris && (!xyz || !tri)) || (nverts && !xyz))
+        return fail(S2H_ERR_ARG, "null argument");
+    if (n != 2 * ((uint64_t)r_max + 1)) return fail(S2H_ERR_SIZE, "hits_valid must hold 2*(r_max+1) values");
+    for (uint64_t f = 0; f < 3 * ntris; ++f)
+        if (tri[f] >= nverts) return fail(S2H_ERR_ARG, "vertex index out of range");
+    for (int a = 0; a < 3; ++a)
+        if (!(bbox_max[a] >= bbox_min[a]) || !std::isfinite(bbox_min[a]) || !std::isfinite(bbox_max[a]))
+            return fail(S2H_ERR_ARG, "bounding box must be finite with max >= min");
+    S2H_HIP(hipSetDevice(ctx->device));
+    std::fill(hits_valid, hits_valid + n, 0);
+    if (r_max == 0) return S2H_OK;
+    const uint64_t spr = std::max<uint64_t>(samples, 200);
+    // whole-mesh vertex bbox (mesh_bbox, src/geometry/bbox.rs:8-25), amax
+    Grid g{};
+    double amax = 0.0;
+    for (uint64_t v = 0; v < nverts; ++v)
+        for (int a = 0; a < 3; ++a) {
+            const double c = xyz[3 * v + a];
+            g.mn[a] = v ? vox::rmin(g.mn[a], c) : c;
+            g.mx[a] = v ? vox::rmax(g.mx[a], c) : c;
+            amax = std::max(amax, std::fabs(c));
+        }
+    g.have_bbox = nverts > 0;
+    // cull-margin bounds: any query point lies in the measurement box
+    double cmax = 0.0, span2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        cmax = std::max({cmax, std::fabs(bbox_min[a]), std::fabs(bbox_max[a])});
+        const double lo = std::min(bbox_min[a], g.have_bbox ? g.mn[a] : bbox_min[a]);
+        const double hi = std::max(bbox_max[a], g.have_bbox ? g.mx[a] : bbox_max[a]);
+        span2 += (hi - lo) * (hi - lo);
+    }
+    const double smax = std::sqrt(span2) + 1.0;
+    std::vector<vox::TriGeo> geo(ntris);
+    for (uint64_t f = 0; f < ntris; ++f)
+        vox::tri_prepare(xyz + 3 * (uint64_t)tri[3 * f], xyz + 3 * (uint64_t)tri[3 * f + 1],
+                         xyz + 3 * (uint64_t)tri[3 * f + 2], cmax, smax, amax, geo[f]);
+    // grid over the projection of the measurement box (query points lie in it)
+    double u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY;
+    for (int c = 0; c < 8; ++c) {
+        const double x = (c & 1) ? bbox_max[0] : bbox_min[0], y = (c & 2) ? bbox_max[1] : bbox_min[1],
+                     z = (c & 4) ? bbox_max[2] : bbox_min[2];
+        const double u = vox::kUx * x + vox::kUy * y + vox::kUz * z, v = vox::kVx * x + vox::kVy * y + vox::kVz * z;
+        u0 = std::min(u0, u);
+        u1 = std::max(u1, u);
+        v0 = std::min(v0, v);
+        v1 = std::max(v1, v);
+    }
+    const double pad = 1e-6 * std::max(1.0, std::max(u1 - u0, v1 - v0));
+    u0 -= pad;
+    u1 += pad;
+    v0 -= pad;
+    v1 += pad;
+    const int G = (int)std::min<double>(1024.0, std::max(8.0, std::sqrt((double)ntris / 2.0)));
+    g.gu = g.gv = G;
+    g.u0 = u0;
+    g.v0 = v0;
+    g.inv_du = G / std::max(u1 - u0, 1e-300);
+    g.inv_dv = G / std::max(v1 - v0, 1e-300);
+    auto cell_range = [&](double lo, double hi, double o, double inv, int &c0, int &c1) {
+        // cells whose span meets [lo, hi]; empty when the bound misses the grid
+        if (!(hi >= lo)) {
+            c0 = 1;
+            c1 = 0;
+            return;
+        }
+        const double a = (lo - o) * inv, b = (hi - o) * inv;
+        c0 = a <= 0.0 ? 0 : (a >= G ? G : (int)std::floor(a));
+        c1 = b < 0.0 ? -1 : (b >= G - 1 ? G - 1 : (int)std::floor(b));
+    };
+    std::vector<uint32_t> count((size_t)G * G + 1, 0);
+    for (uint64_t f = 0; f < ntris; ++f) {
+        int a0, a1, b0, b1;
+        cell_range(geo[f].umin, geo[f].umax, u0, g.inv_du, a0, a1);
+        cell_range(geo[f].vmin, geo[f].vmax, v0, g.inv_dv, b0, b1);
+        for (int i = a0; i <= a1; ++i)
+            for (int j = b0; j <= b1; ++j) count[(size_t)i * G + j + 1]++;
+    }
+    for (size_t c = 0; c < (size_t)G * G; ++c) count[c + 1] += count[c];
+    if (count[(size_t)G * G] >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "mesh too large for the query grid");
+    std::vector<uint32_t> ids(std::max<uint32_t>(count[(size_t)G * G], 1));
+    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
+    for (uint64_t f = 0; f < ntris; ++f) {
+        int a0, a1, b0, b1;
+        cell_range(geo[f].umin, geo[f].umax, u0, g.inv_du, a0, a1);
+        cell_range(geo[f].vmin, geo[f].vmax, v0, g.inv_dv, b0, b1);
+        for (int i = a0; i <= a1; ++i)
+            for (int j = b0; j <= b1; ++j) ids[fill[(size_t)i * G + j]++] = (uint32_t)f;
+    }
+    DevBuf d_geo, d_start, d_ids, d_hv;
+    auto release = [&]() {
+        dev_free(ctx, d_geo);
+        dev_free(ctx, d_start);
+        dev_free(ctx, d_ids);
+        dev_free(ctx, d_hv);
+    };
+    s2h_status st = upload(ctx, d_geo, geo.data(), std::max<uint64_t>(ntris, 1) * sizeof(vox::TriGeo));
+    if (st == S2H_OK) st = upload(ctx, d_start, count.data(), count.size() * 4);
+    if (st == S2H_OK) st = upload(ctx, d_ids, ids.data(), ids.size() * 4);
+    if (st == S2H_OK) st = dev_alloc(ctx, d_hv, n * 8);
+    if (st != S2H_OK) {
+        release();
+        return st;
+    }
+    g.tris = (const vox::TriGeo *)d_geo.ptr;
+    g.start = (const uint32_t *)d_start.ptr;
+    g.ids = (const uint32_t *)d_ids.ptr;
+    const uint64_t per_block = (uint64_t)kThreads * kPerThread;
+    const uint64_t bpr = (spr + per_block - 1) / per_block;
+    if (bpr * r_max >= (1ull << 31)) {
+        release();
+        return fail(S2H_ERR_UNSUPPORTED, "too many samples");
+    }
+    prof_mark(ctx, 0);
+    clear_u64_kernel<<<1, 256, 0, ctx->stream>>>((unsigned long long *)d_hv.ptr, n);
+    meshmc_kernel<<<(unsigned)(bpr * r_max), kThreads, 0, ctx->stream>>>(
+        g, bbox_min[0], bbox_min[1], bbox_min[2], bbox_max[0], bbox_max[1], bbox_max[2], spr, (uint32_t)seed,
+        (uint32_t)(seed >> 32), (uint32_t)bpr, (unsigned long long *)d_hv.ptr);
+    ctx->launches = 2;
+    hipError_t e = hipGetLastError();
+    prof_mark(ctx, 1);
+    prof_mark(ctx, 2);
+    if (e == hipSuccess) e = hipMemcpyAsync(hits_valid, d_hv.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    prof_collect(ctx);
+    release();
+    if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("mesh Monte Carlo: ") + hipGetErrorString(e));
+    return S2H_OK;
+}
+
+s2h_status s2h_mesh_mc_s2(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,
+                          const double bbox_min[3], const double bbox_max[3], uint32_t r_max, uint64_t samples,
+                          uint64_t seed, double vf, double *s2, uint64_t n) {
+    if (!s2) return fail(S2H_ERR_ARG, "null argument");
+    if (n != (uint64_t)r_max + 1) return fail(S2H_ERR_SIZE, "s2 buffer must hold r_max+1 values");
+    std::vector<uint64_t> hv(2 * ((uint64_t)r_max + 1));
+    S2H_TRY(s2h_mesh_mc_counts(ctx, xyz, nverts, tri, ntris, bbox_min, bbox_max, r_max, samples, seed, hv.data(),
+                               hv.size()));
+    s2[0] = vf;
+    for (uint32_t r = 1; r <= r_max; ++r)
+        s2[r] = hv[2 * r + 1] == 0 ? 0.0 : (double)hv[2 * r] / (double)hv[2 * r + 1];
+    return S2H_OK;
+}
+
+}  // extern "C"

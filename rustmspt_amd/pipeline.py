@@ -63,26 +63,14 @@ def l2_error(a, b) -> float:
 
 def volume_fraction_in_bbox(mesh, bbox: BoundingBox, occupancy_vf: float):
     """src/geometry/volume.rs:437-469 for granules wholly inside the box
-    (sum of |signed volume| per granule); a granule crossing the box would need
-    the reference's clipping, which is outside this engine -- the occupancy VF
-    is reported instead, with a warning."""
-    v = mesh.vertices
-    inside = np.all((v >= np.asarray(bbox.min)) & (v <= np.asarray(bbox.max)), axis=1)
-    box_vol = max(np.prod(np.maximum(np.asarray(bbox.max) - np.asarray(bbox.min), 0.0)), 1e-12)
-    if len(mesh.faces) == 0:
-        return 0.0
-    if not np.all(inside[mesh.faces.reshape(-1)]):
+    (s2.geometric_volume_fraction); a granule crossing the box would need the
+    reference's clipping, which is outside this engine -- the occupancy VF is
+    reported instead, with a warning."""
+    vf = s2.geometric_volume_fraction(mesh, bbox)
+    if vf is None:
         print("[Warning] particles cross the measurement box; Volume Fraction reports the occupancy fraction")
         return occupancy_vf
-    gid, n = granules(mesh)
-    a, b, c = (v[mesh.faces[:, k]] for k in range(3))
-    cross = np.stack([b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1], b[:, 2] * c[:, 0] - b[:, 0] * c[:, 2],
-                      b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0]], 1)
-    tet = (a[:, 0] * cross[:, 0] + a[:, 1] * cross[:, 1] + a[:, 2] * cross[:, 2]) / 6.0
-    total = 0.0
-    for g in range(n):
-        total += abs(float(np.sum(tet[gid == g])))
-    return min(max(total / box_vol, 0.0), 1.0)
+    return vf
 
 
 class MeasurePipeline:

@@ -282,13 +282,67 @@ class VoxelS2Hip:
         return self.volume.mc_s2(r_max, self.pitch, max(int(samples), 200), 0 if seed is None else int(seed))
 
 
+def geometric_volume_fraction(mesh: Mesh, bbox: BoundingBox) -> Optional[float]:
+    """volume_fraction_in_bbox (src/geometry/volume.rs:437-469) when every granule
+    lies wholly inside the box: per granule |sum of a . (b x c) / 6| (mesh_volume,
+    src/geometry/volume.rs:6-15, summed in face order), granules summed in order,
+    over the box volume, clamped to [0, 1].  A granule crossing the box needs the
+    reference's mesh clipping, which this engine does not restate: None."""
+    v = np.asarray(mesh.vertices, np.float64)
+    lo, hi = np.asarray(bbox.min, np.float64), np.asarray(bbox.max, np.float64)
+    box_vol = max(float(np.prod(hi - lo)), 1e-12)
+    if len(mesh.faces) == 0:
+        return 0.0
+    if not np.all((v[mesh.faces.reshape(-1)] >= lo) & (v[mesh.faces.reshape(-1)] <= hi)):
+        return None
+    gid, n = granules(mesh)
+    a, b, c = (v[mesh.faces[:, k]] for k in range(3))
+    cross = np.stack([b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1], b[:, 2] * c[:, 0] - b[:, 0] * c[:, 2],
+                      b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0]], 1)
+    tet = ((a[:, 0] * cross[:, 0] + a[:, 1] * cross[:, 1]) + a[:, 2] * cross[:, 2]) / 6.0
+    total = 0.0
+    for g in range(n):
+        part = tet[gid == g]
+        total += abs(float(np.cumsum(part)[-1])) if len(part) else 0.0  # sequential, in face order
+    return min(max(total / box_vol, 0.0), 1.0)
+
+
+def mesh_mc_counts(ctx: Context, mesh: Mesh, bbox: BoundingBox, r_max: int, samples: int, seed: int) -> np.ndarray:
+    """Continuous mesh Monte Carlo (hits, valid) per radius (s2h_mesh_mc_counts)."""
+    out = np.zeros(2 * (int(r_max) + 1), np.uint64)
+    v = np.ascontiguousarray(mesh.vertices, np.float64)
+    f = np.ascontiguousarray(mesh.faces, np.uint32)
+    L.check(L.lib().s2h_mesh_mc_counts(ctx.handle, L.ptr(v.reshape(-1), L._f64p), len(v), L.ptr(f.reshape(-1), L._u32p),
+                                       len(f), L.ptr(np.asarray(bbox.min, np.float64), L._f64p),
+                                       L.ptr(np.asarray(bbox.max, np.float64), L._f64p), int(r_max), int(samples),
+                                       int(seed) & 0xFFFFFFFFFFFFFFFF, L.ptr(out, L._u64p), len(out)))
+    return out.reshape(-1, 2)
+
+
+def mesh_mc_s2(ctx: Context, mesh: Mesh, bbox: BoundingBox, r_max: int, samples: int, seed: int) -> np.ndarray:
+    """calculate_s2_mesh_mc_seeded (src/geometry/s2.rs:1273-1320): S2(0) is the
+    geometric volume fraction in the box, S2(r) = hits / valid."""
+    vf = geometric_volume_fraction(mesh, bbox)
+    if vf is None:
+        raise L.S2HipError(5, "mesh Monte Carlo: a granule crosses the measurement box (its clipped volume "
+                              "is not computed by this engine)")
+    out = np.zeros(int(r_max) + 1, np.float64)
+    v = np.ascontiguousarray(mesh.vertices, np.float64)
+    f = np.ascontiguousarray(mesh.faces, np.uint32)
+    L.check(L.lib().s2h_mesh_mc_s2(ctx.handle, L.ptr(v.reshape(-1), L._f64p), len(v), L.ptr(f.reshape(-1), L._u32p),
+                                   len(f), L.ptr(np.asarray(bbox.min, np.float64), L._f64p),
+                                   L.ptr(np.asarray(bbox.max, np.float64), L._f64p), int(r_max), int(samples),
+                                   int(seed) & 0xFFFFFFFFFFFFFFFF, float(vf), L.ptr(out, L._f64p), len(out)))
+    return out
+
+
 def calculate_s2(ctx: Context, mesh: Mesh, bbox: BoundingBox, r_max: int, voxel_pitch: float, method: str,
                  samples: int, seed: Optional[int] = None) -> np.ndarray:
-    """calculate_s2_seeded (src/geometry/s2.rs:1345-1368) on the HIP engine.  The
-    reference's continuous mesh Monte Carlo (non-exact with pitch <= 0) is not
-    part of this engine."""
+    """calculate_s2_seeded (src/geometry/s2.rs:1345-1368) on the HIP engine: a
+    non-exact method with voxel_pitch <= 0 is the continuous mesh Monte Carlo
+    (seed None uses seed 0: the device stream is counter-based)."""
     if method != "exact" and voxel_pitch <= 0.0:
-        raise L.S2HipError(5, "mesh Monte Carlo (voxel_pitch <= 0) is not implemented by the HIP engine")
+        return mesh_mc_s2(ctx, mesh, bbox, r_max, samples, 0 if seed is None else int(seed))
     pitch = voxel_pitch
     if voxel_pitch <= 0.0:
         print("[Warning] exact S2 requested with voxel_pitch <= 0; falling back to voxel_pitch=1.0")
