@@ -216,6 +216,25 @@ def mc_leg(ctx, vol, r_max, hits, offs, rs, total_samples, seed):
             "radii_beyond_3sigma": int((z > 3).sum()), "fraction_beyond_2sigma": float((z > 2).mean())}
 
 
+def mesh_mc_leg(ctx, mesh, bbox, r_max, total_samples, seed):
+    """Continuous mesh Monte Carlo on the same 500-particle mesh (no voxels):
+    total_samples point pairs spread evenly over the r_max radii."""
+    import numpy as np
+
+    from rustmspt_amd import s2
+
+    per_r = max(200, total_samples // r_max)
+    s2.mesh_mc_counts(ctx, mesh, bbox, r_max, 1000, seed)  # warm-up
+    t0 = time.perf_counter()
+    hv = s2.mesh_mc_counts(ctx, mesh, bbox, r_max, per_r, seed).astype(np.float64)
+    el = time.perf_counter() - t0
+    dev = ctx.profile_read()["device_ms"]
+    vf = s2.geometric_volume_fraction(mesh, bbox)
+    return {"samples_total": per_r * r_max, "samples_per_radius": per_r, "seconds": el, "device_ms": dev,
+            "pair_samples_per_s": per_r * r_max / el, "triangles": int(len(mesh.faces)),
+            "geometric_vf": vf, "s2_r1_2_4_8": [round(float(hv[r, 0] / hv[r, 1]), 6) for r in (1, 2, 4, 8)]}
+
+
 def served_leg(rate: float, duration: float, pool_size: int, slots: int, budget_gb: float, seed: int):
     """Poisson stream of mixed requests (grid 64^3-384^3 log-uniform, VF
     U[0.1, 0.5], r_max U{16..96}, 70 % exact / 30 % FP32 FFT) through the serving
@@ -356,6 +375,8 @@ def main():
             out["fft32"] = fft_leg(ctx, vol, r_max, hits, offs, rs, curve, steps=5)
         if not args.no_mc:
             out["monte_carlo"] = mc_leg(ctx, vol, r_max, hits, offs, rs, args.mc_samples, seed=7)
+    if not args.no_mc:
+        out["mesh_monte_carlo"] = mesh_mc_leg(ctx, mesh, bbox, r_max, args.mc_samples, seed=11)
     if not args.no_batch:
         out["batched"] = batched_leg(ctx, args.batch_candidates, seed=3, plain_sample=32)
     if not args.no_serve:
