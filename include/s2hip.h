@@ -92,6 +92,63 @@ s2h_status s2h_vol_dims(const s2h_vol *vol, uint64_t dims[3]);
 s2h_status s2h_vol_occupied(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);
 void s2h_vol_free(s2h_ctx *ctx, s2h_vol *vol);
 
+/* ------------------------------------------------------------ volume ingest */
+
+/* CT-style sample volumes: a folder of RAW slice files or a TIFF stack.
+ * Samples are z-major as Volume3D (idx = z*width*height + y*width + x,
+ * src/io/volume.rs:45-53).  The occupancy grid maps file axes one to one:
+ * S2 (x, y, z) = file (x, y, slice), so dims = {width, height, depth} and
+ * each S2 row (x, y) runs along the slices. */
+#define S2H_VOLUME_RAW 0
+#define S2H_VOLUME_TIFF 1
+typedef struct {
+    const char *path;       /* RAW: folder of slice files; TIFF: a multi-page file or a folder of TIFF files */
+    int32_t kind;           /* S2H_VOLUME_RAW or S2H_VOLUME_TIFF */
+    int32_t reserved;
+    int64_t slice_start;    /* inclusive; -1 = first (resolve_slice_range, src/io/volume.rs:258-285) */
+    int64_t slice_end;      /* inclusive; -1 = last */
+    uint64_t width, height; /* RAW only (RawFolderSpec, src/io/volume.rs:193-203) */
+    uint32_t bits;          /* RAW only: 8, 16 or 32 */
+    int32_t is_signed;      /* RAW only */
+    int32_t big_endian;     /* RAW only */
+} s2h_volume_source;
+typedef struct {
+    uint64_t width, height, depth;
+    uint32_t bits;       /* 8, 16 or 32 */
+    int32_t is_signed;
+    int32_t big_endian;  /* byte order of the staged samples */
+    int32_t reserved;
+} s2h_volume_info;
+
+/* Which samples are occupied. */
+#define S2H_PHASE_RANGE 0           /* lo <= v <= hi (a threshold window) */
+#define S2H_PHASE_FOREGROUND 1      /* v != background */
+#define S2H_PHASE_FOREGROUND_AUTO 2 /* v != the modal boundary value, smallest on ties
+                                       (detect_background_mode, src/pipeline/crop.rs:338-373) */
+typedef struct {
+    int32_t mode;
+    int32_t reserved;
+    int64_t lo, hi, background;
+} s2h_phase;
+
+/* Host only (no device): the volume's dimensions and staged layout after the
+ * slice range (load_raw_folder_typed, src/io/volume.rs:344-403;
+ * load_tiff_or_folder_typed_with_range, src/io/volume.rs:521-581). */
+s2h_status s2h_volume_probe(const s2h_volume_source *src, s2h_volume_info *info);
+/* Host only: decode every slice into samples (width*height*depth*bits/8 bytes,
+ * in the staged layout). */
+s2h_status s2h_volume_read(const s2h_volume_source *src, void *samples, uint64_t nbytes, s2h_volume_info *info);
+/* Classify and bit-pack a z-major sample buffer on the device.  background_used
+ * (optional) receives the background of the FOREGROUND modes; occupied
+ * (optional) the occupied voxel count. */
+s2h_status s2h_vol_from_samples(s2h_ctx *ctx, const void *samples, uint64_t nbytes, const s2h_volume_info *layout,
+                                const s2h_phase *phase, s2h_vol **out, int64_t *background_used,
+                                uint64_t *occupied);
+/* Read a RAW folder or TIFF stack, streaming its slices through pinned staging
+ * to the device, then classify and bit-pack there. */
+s2h_status s2h_vol_load(s2h_ctx *ctx, const s2h_volume_source *src, const s2h_phase *phase, s2h_vol **out,
+                        s2h_volume_info *info, int64_t *background_used, uint64_t *occupied);
+
 /* --------------------------------------------------------------- voxelization */
 
 /* Replaces build_bbox_occupancy / VoxelS2::new (src/geometry/s2.rs:160-201, 1475)

@@ -308,3 +308,69 @@ def brute_pair_counts(occ, off):
     A = occ[tuple(sl_a)]
     B = occ[tuple(sl_b)]
     return int(np.count_nonzero(A & B)), int(A.size)
+
+
+# ------------------------------------------------------------------ volume ingest
+
+_RAW_DTYPES = {(8, False): "u1", (8, True): "i1", (16, False): "u2", (16, True): "i2", (32, False): "u4",
+               (32, True): "i4"}
+
+
+def decode_raw_slice(data: bytes, bits: int, signed: bool, big_endian: bool) -> np.ndarray:
+    """decode_raw_slice (src/io/volume.rs:287-332): samples of the slice's own
+    width and sign in the given byte order; misaligned lengths and other widths
+    are errors."""
+    if (bits, bool(signed)) not in _RAW_DTYPES:
+        raise ValueError("RAW bits must be 8/16/32")
+    dt = np.dtype(_RAW_DTYPES[(bits, bool(signed))]).newbyteorder(">" if big_endian else "<")
+    if len(data) % dt.itemsize:
+        raise ValueError(f"RAW byte length is not aligned to {bits}-bit samples")
+    return np.frombuffer(data, dtype=dt).astype(np.int64)
+
+
+def resolve_slice_range(total: int, start: int, end: int):
+    """resolve_slice_range (src/io/volume.rs:258-285): inclusive, -1 = first/last."""
+    if total == 0:
+        raise ValueError("Input folder has no files")
+    s = 0 if start < 0 else start
+    e = total - 1 if end < 0 else end
+    if s >= total or e >= total or s > e:
+        raise ValueError(f"Invalid slice range [{start}, {end}] for {total} files")
+    return s, e
+
+
+def boundary_values(vol_zyx: np.ndarray) -> np.ndarray:
+    """for_each_boundary_value (src/pipeline/crop.rs:312-330), written as a
+    loop over z and y in the reference's order (each boundary voxel once)."""
+    D, H, W = vol_zyx.shape
+    out = []
+    for z in range(D):
+        if z == 0 or z + 1 == D:
+            out.append(vol_zyx[z].reshape(-1))
+            continue
+        for y in range(H):
+            if y == 0 or y + 1 == H:
+                out.append(vol_zyx[z, y])
+            else:
+                out.append(vol_zyx[z, y, :1])
+                if W > 1:
+                    out.append(vol_zyx[z, y, W - 1:])
+    return np.concatenate(out).astype(np.int64) if out else np.zeros(0, np.int64)
+
+
+def boundary_mode(vol_zyx: np.ndarray) -> int:
+    """detect_background_mode (src/pipeline/crop.rs:338-373): the most frequent
+    boundary value, the smallest one on ties; 0 for an empty volume."""
+    if vol_zyx.size == 0:
+        return 0
+    vals, counts = np.unique(boundary_values(vol_zyx), return_counts=True)  # ascending values
+    return int(vals[int(np.argmax(counts))])  # argmax takes the first (smallest) maximum
+
+
+def classify_pack(vol_zyx: np.ndarray, mode: str, lo=0, hi=0, background=0):
+    """Occupancy of a z-major sample volume in the S2 layout (x-major, z
+    fastest; S2 (x, y, z) = file (x, y, slice)), packed as OccupancyBits.
+    mode: "range" (lo <= v <= hi) or "foreground" (v != background)."""
+    v = np.asarray(vol_zyx).astype(np.int64)
+    occ = (v >= lo) & (v <= hi) if mode == "range" else v != background
+    return pack(np.transpose(occ, (2, 1, 0)).astype(np.uint8)), int(np.count_nonzero(occ))

@@ -37,6 +37,10 @@ SIGNATURES = {
                                        C.c_uint64, C.c_uint64, _u64p, C.c_uint64]),
     "s2h_mesh_mc_s2": (C.c_int32, [_vp, _f64p, C.c_uint64, _u32p, C.c_uint64, _f64p, _f64p, C.c_uint32,
                                    C.c_uint64, C.c_uint64, C.c_double, _f64p, C.c_uint64]),
+    "s2h_volume_probe": (C.c_int32, [_vp, _vp]),
+    "s2h_volume_read": (C.c_int32, [_vp, _vp, C.c_uint64, _vp]),
+    "s2h_vol_from_samples": (C.c_int32, [_vp, _vp, C.c_uint64, _vp, _vp, C.POINTER(_vp), _i64p, _u64p]),
+    "s2h_vol_load": (C.c_int32, [_vp, _vp, _vp, C.POINTER(_vp), _vp, _i64p, _u64p]),
     "s2h_vol_from_packed": (C.c_int32, [_vp, _u64p, C.c_uint64, _u64p, C.POINTER(_vp)]),
     "s2h_vol_from_bytes": (C.c_int32, [_vp, _u8p, _u64p, C.POINTER(_vp)]),
     "s2h_vol_read_packed": (C.c_int32, [_vp, _vp, _u64p, C.c_uint64]),
@@ -90,6 +94,22 @@ class Result(C.Structure):
     _fields_ = [("ready", C.c_int32), ("status", C.c_int32), ("deferrals", C.c_uint32), ("reserved", C.c_uint32),
                 ("tag", C.c_uint64), ("footprint", C.c_uint64), ("t_submit", C.c_double), ("t_start", C.c_double),
                 ("t_done", C.c_double)]
+
+
+class VolumeSource(C.Structure):
+    _fields_ = [("path", C.c_char_p), ("kind", C.c_int32), ("reserved", C.c_int32), ("slice_start", C.c_int64),
+                ("slice_end", C.c_int64), ("width", C.c_uint64), ("height", C.c_uint64), ("bits", C.c_uint32),
+                ("is_signed", C.c_int32), ("big_endian", C.c_int32)]
+
+
+class VolumeInfo(C.Structure):
+    _fields_ = [("width", C.c_uint64), ("height", C.c_uint64), ("depth", C.c_uint64), ("bits", C.c_uint32),
+                ("is_signed", C.c_int32), ("big_endian", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Phase(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("reserved", C.c_int32), ("lo", C.c_int64), ("hi", C.c_int64),
+                ("background", C.c_int64)]
 
 
 class S2HipError(RuntimeError):

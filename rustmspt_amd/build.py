@@ -34,8 +34,10 @@ UNITS = {
     "s2hip_serve.hip": ["-ffp-contract=off"],
     "s2hip_probe.hip": [],
     "s2hip_meshmc.hip": ["-ffp-contract=off"],
+    "s2hip_volin.hip": [],
 }
-HOST_UNITS = {"s2hip_host.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]}
+HOST_UNITS = {"s2hip_host.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"],
+              "s2hip_volio.cpp": ["-O2", "-std=c++17", "-fPIC"]}
 
 
 def _run(cmd):
@@ -61,7 +63,7 @@ def build(verbose: bool = False) -> Path:
                 print(r.stdout, r.stderr, file=sys.stderr)
     objs = [str(OBJ / (s + ".o")) for s in list(UNITS) + list(HOST_UNITS)]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-lpthread", "-o", str(tmp)])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-lpthread", "-lz", "-o", str(tmp)])
     os.replace(tmp, LIB)
     return LIB
 
