@@ -163,6 +163,37 @@ s2h_status s2h_voxelize(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const 
                         const double bbox_min[3], double pitch, const uint64_t dims[3],
                         s2h_vol **out, uint64_t *occupied);
 
+/* ------------------------------------------------------- device-resident meshes */
+
+/* A welded, granule-split mesh kept on the device (vertices f64, faces u32,
+ * granule id per face), so that voxelization needs no host preparation. */
+typedef struct s2h_mesh s2h_mesh;
+/* Replaces load_stl / load_stl_or_merge_folder (src/io/stl.rs:214-239, 282-302)
+ * followed by split_mesh_into_granules (src/geometry/mesh_ops.rs:48): one STL
+ * byte stream per file (ASCII sniffed and parsed on the host, binary decoded on
+ * the device), welded on the device with the reference's 1e-6 key, the first
+ * occurrence keeping its index, never across files; files concatenate in the
+ * order given.  The vertices, faces and granule ids equal s2h_stl_load +
+ * s2h_mesh_granules on the same bytes. */
+s2h_status s2h_mesh_from_stl(s2h_ctx *ctx, const void *const *files, const uint64_t *sizes, uint32_t nfiles,
+                             s2h_mesh **out);
+/* As s2h_mesh_from_stl, reading the files (in the order given). */
+s2h_status s2h_mesh_load_stl(s2h_ctx *ctx, const char *const *paths, uint32_t npaths, s2h_mesh **out);
+/* An already welded mesh; granules are split on the device. */
+s2h_status s2h_mesh_from_arrays(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri,
+                                uint64_t ntris, s2h_mesh **out);
+/* Sizes and the vertex bbox (mesh_bbox, src/geometry/bbox.rs:8-25: f64::min/max,
+ * NaN ignored) of a resident mesh. */
+s2h_status s2h_mesh_info(const s2h_mesh *mesh, uint64_t *nverts, uint64_t *ntris, uint32_t *ngranules,
+                         double bbox_min[3], double bbox_max[3]);
+/* Copy the mesh back: xyz 3*nverts, tri 3*ntris, gid ntris (any may be null). */
+s2h_status s2h_mesh_read(s2h_ctx *ctx, const s2h_mesh *mesh, double *xyz, uint64_t nxyz, uint32_t *tri,
+                         uint64_t ntri, uint32_t *gid, uint64_t ngid);
+/* s2h_voxelize on the resident mesh (same result bits). */
+s2h_status s2h_mesh_voxelize(s2h_ctx *ctx, const s2h_mesh *mesh, const double bbox_min[3], double pitch,
+                             const uint64_t dims[3], s2h_vol **out, uint64_t *occupied);
+void s2h_mesh_free(s2h_ctx *ctx, s2h_mesh *mesh);
+
 /* ------------------------------------------------------------------ shells */
 
 /* Number of in-domain offsets of radii 1..r_max in shell order (sizes the outputs

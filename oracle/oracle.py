@@ -374,3 +374,46 @@ def classify_pack(vol_zyx: np.ndarray, mode: str, lo=0, hi=0, background=0):
     v = np.asarray(vol_zyx).astype(np.int64)
     occ = (v >= lo) & (v <= hi) if mode == "range" else v != background
     return pack(np.transpose(occ, (2, 1, 0)).astype(np.uint8)), int(np.count_nonzero(occ))
+
+
+# ------------------------------------------------------------------ STL weld
+
+def _round_half_away(v: np.ndarray) -> np.ndarray:
+    t = np.trunc(v)
+    frac = v - t  # exact
+    return t + np.where(np.abs(frac) >= 0.5, np.sign(v), 0.0)
+
+
+def weld_key(v: np.ndarray) -> np.ndarray:
+    """quantize_key (src/io/stl.rs:55-62): (v * 1e6).round() as i64 per axis --
+    round half away from zero, then Rust's saturating float-to-int cast (NaN -> 0)."""
+    r = _round_half_away(np.asarray(v, np.float64) * 1000000.0)
+    out = np.zeros(r.shape, np.int64)
+    ok = ~np.isnan(r)
+    hi, lo = r >= 9.2233720368547758e18, r <= -9.2233720368547758e18
+    mid = ok & ~hi & ~lo
+    out[mid] = r[mid].astype(np.int64)
+    out[hi] = np.iinfo(np.int64).max
+    out[lo] = np.iinfo(np.int64).min
+    return out
+
+
+def weld(corners: np.ndarray, part=None):
+    """dedup_vertex (src/io/stl.rs:64-80) over corners in order: the first corner
+    with a key keeps the next vertex index; later corners with that key reuse it.
+    part (one id per corner) keeps files apart, as load_stl_or_merge_folder
+    welds each file separately.  Returns (vertices (V, 3), tri (N,))."""
+    c = np.asarray(corners, np.float64).reshape(-1, 3)
+    keys = weld_key(c)
+    p = np.zeros(len(c), np.int64) if part is None else np.asarray(part, np.int64)
+    index = {}
+    tri = np.empty(len(c), np.uint32)
+    verts = []
+    for i in range(len(c)):
+        k = (int(p[i]), int(keys[i, 0]), int(keys[i, 1]), int(keys[i, 2]))
+        j = index.get(k)
+        if j is None:
+            j = index[k] = len(verts)
+            verts.append(c[i])
+        tri[i] = j
+    return np.array(verts, np.float64).reshape(-1, 3), tri

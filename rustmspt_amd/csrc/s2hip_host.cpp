@@ -75,25 +75,32 @@ bool looks_ascii(const std::vector<unsigned char> &b) {
     return s.find("facet") != std::string::npos && s.find("vertex") != std::string::npos;
 }
 
-// src/io/stl.rs:157-189 parse_binary_reader (f32 little-endian -> f64).
-bool parse_binary(const std::vector<unsigned char> &b, Welder &w, std::vector<uint32_t> &tri, std::string &err) {
-    if (b.size() < 84) {
+// src/io/stl.rs:157-189 parse_binary_reader: the face count at byte 80 and a
+// file long enough for it (trailing bytes are ignored).
+bool binary_count(const unsigned char *b, size_t n, uint32_t &count, std::string &err) {
+    if (n < 84) {
         err = "Binary STL size mismatch";
         return false;
     }
+    std::memcpy(&count, b + 80, 4);
+    if (n < 84 + (uint64_t)count * 50) {
+        err = "Binary STL size mismatch";
+        return false;
+    }
+    return true;
+}
+
+// f32 little-endian vertices widened to f64, three corners per face.
+template <class Add>
+bool parse_binary(const std::vector<unsigned char> &b, Add &&add, std::string &err) {
     uint32_t count;
-    std::memcpy(&count, b.data() + 80, 4);
-    if (b.size() < 84 + (uint64_t)count * 50) {
-        err = "Binary STL size mismatch";
-        return false;
-    }
-    tri.reserve((size_t)count * 3);
+    if (!binary_count(b.data(), b.size(), count, err)) return false;
     for (uint32_t f = 0; f < count; ++f) {
         const unsigned char *r = b.data() + 84 + (size_t)f * 50;
         for (int k = 0; k < 3; ++k) {
             float v[3];
             std::memcpy(v, r + 12 + 12 * k, 12);
-            tri.push_back(w.add((double)v[0], (double)v[1], (double)v[2]));
+            add((double)v[0], (double)v[1], (double)v[2]);
         }
     }
     return true;
@@ -101,8 +108,9 @@ bool parse_binary(const std::vector<unsigned char> &b, Welder &w, std::vector<ui
 
 // src/io/stl.rs:82-153: line-by-line "vertex x y z" (exactly four tokens), a face
 // per three vertices; no triangle -> binary parse of the same bytes.
-bool parse_ascii(const std::vector<unsigned char> &b, Welder &w, std::vector<uint32_t> &tri) {
-    size_t pos = 0;
+template <class Add>
+bool parse_ascii(const std::vector<unsigned char> &b, Add &&add) {
+    size_t pos = 0, corners = 0;
     double pending[9];
     int np = 0;
     while (pos < b.size()) {
@@ -136,14 +144,33 @@ bool parse_ascii(const std::vector<unsigned char> &b, Welder &w, std::vector<uin
         if (!ok) continue;
         for (int k = 0; k < 3; ++k) pending[3 * np + k] = v[k];
         if (++np == 3) {
-            for (int k = 0; k < 3; ++k) tri.push_back(w.add(pending[3 * k], pending[3 * k + 1], pending[3 * k + 2]));
+            for (int k = 0; k < 3; ++k) add(pending[3 * k], pending[3 * k + 1], pending[3 * k + 2]);
+            corners += 3;
             np = 0;
         }
     }
-    return !tri.empty();
+    return corners > 0;
 }
 
 }  // namespace
+
+namespace s2h {
+namespace stlio {
+bool read_file(const char *path, std::vector<unsigned char> &out) { return ::read_file(path, out); }
+bool looks_ascii(const std::vector<unsigned char> &b) { return ::looks_ascii(b); }
+bool ascii_corners(const std::vector<unsigned char> &b, std::vector<double> &xyz) {
+    xyz.clear();
+    return ::parse_ascii(b, [&](double x, double y, double z) {
+        xyz.push_back(x);
+        xyz.push_back(y);
+        xyz.push_back(z);
+    });
+}
+bool binary_count(const unsigned char *b, size_t n, uint32_t &count, std::string &err) {
+    return ::binary_count(b, n, count, err);
+}
+}  // namespace stlio
+}  // namespace s2h
 
 extern "C" {
 
@@ -161,11 +188,12 @@ s2h_status s2h_stl_load(const char *path, double **xyz, uint64_t *nverts, uint32
     std::vector<uint32_t> t;
     std::string err;
     bool ok = false;
-    if (looks_ascii(bytes)) ok = parse_ascii(bytes, w, t);
+    auto add = [&](double x, double y, double z) { t.push_back(w.add(x, y, z)); };
+    if (looks_ascii(bytes)) ok = parse_ascii(bytes, add);
     if (!ok) {
         w = Welder();
         t.clear();
-        if (!parse_binary(bytes, w, t, err)) {
+        if (!parse_binary(bytes, add, err)) {
             s2h::set_error(err + " in " + path);
             return S2H_ERR_IO;
         }

@@ -52,6 +52,7 @@ __device__ __forceinline__ bool occupied(int64_t v, const Rule &r) {
     return r.range ? (r.lo <= v && v <= r.hi) : (v != r.bg);
 }
 
+// Generic shape: one sample per lane.
 template <int B>
 __global__ __launch_bounds__(64) void pack_kernel(const uint8_t *__restrict__ s, uint64_t W, uint64_t H, uint64_t D,
                                                   int sgn, int big, Rule rule, uint64_t row_words, uint64_t nblocks,
@@ -68,6 +69,37 @@ __global__ __launch_bounds__(64) void pack_kernel(const uint8_t *__restrict__ s,
         for (uint32_t k = 0; k < nz; ++k, i += plane)
             if (occupied(sample<B>(s, i, sgn, big), rule)) bits |= 1ull << k;
         words[(x * H + y) * row_words + w] = bits;
+    }
+}
+
+// Rows of 8-byte multiples: each lane loads 8 bytes (8 / B samples at
+// consecutive x) per slice, so a wave reads 512 contiguous bytes per load.
+template <int B>
+__global__ __launch_bounds__(64) void pack_wide_kernel(const uint8_t *__restrict__ s, uint64_t W, uint64_t H,
+                                                       uint64_t D, int sgn, int big, Rule rule, uint64_t row_words,
+                                                       uint64_t nblocks, unsigned long long *__restrict__ words) {
+    constexpr int S = 8 / B;
+    const uint64_t plane = W * H;
+    for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const uint64_t w = b % row_words, t = b / row_words;
+        const uint64_t y = t % H, xs = ((t / H) * 64 + threadIdx.x) * S;
+        if (xs >= W) continue;
+        const uint64_t z0 = w * 64;
+        const uint32_t nz = z0 < D ? (uint32_t)std::min<uint64_t>(64, D - z0) : 0;
+        uint64_t bits[S];
+#pragma unroll
+        for (int j = 0; j < S; ++j) bits[j] = 0;
+        const uint64_t base = z0 * plane + y * W + xs;  // in samples; W * B % 8 == 0 keeps this 8-byte aligned
+        for (uint32_t k = 0; k < nz; ++k) {
+            const uint2 v = *(const uint2 *)(s + (base + k * plane) * B);
+            const uint8_t *vb = (const uint8_t *)&v;
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                if (occupied(sample<B>(vb, j, sgn, big), rule)) bits[j] |= 1ull << k;
+        }
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            if (xs + j < W) words[((xs + j) * H + y) * row_words + w] = bits[j];
     }
 }
 
@@ -233,25 +265,49 @@ s2h_status pack_samples(s2h_ctx *ctx, const uint8_t *d_samples, const s2h_volume
     const uint64_t dims[3] = {l->width, l->height, l->depth};
     s2h_vol *vol = nullptr;
     S2H_TRY(vol_alloc(ctx, dims, &vol));
-    const uint64_t nblocks = (l->width + 63) / 64 * l->height * vol->row_words;
-    const unsigned grid = (unsigned)std::min<uint64_t>(nblocks, 1u << 22);
     auto *words = (unsigned long long *)vol->d_words;
-    if (l->bits == 8)
-        volin::pack_kernel<1><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth, l->is_signed,
-                                                            l->big_endian, rule, vol->row_words, nblocks, words);
-    else if (l->bits == 16)
-        volin::pack_kernel<2><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth, l->is_signed,
-                                                            l->big_endian, rule, vol->row_words, nblocks, words);
-    else
-        volin::pack_kernel<4><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth, l->is_signed,
-                                                            l->big_endian, rule, vol->row_words, nblocks, words);
+    prof_mark(ctx, 0);
+    const uint32_t bytes = l->bits / 8;
+    if ((l->width * bytes) % 8 == 0 && ((uintptr_t)d_samples & 7) == 0) {
+        const uint64_t per_block = 64 * (8 / bytes);
+        const uint64_t nblocks = (l->width + per_block - 1) / per_block * l->height * vol->row_words;
+        const unsigned grid = (unsigned)std::min<uint64_t>(nblocks, 1u << 22);
+        if (bytes == 1)
+            volin::pack_wide_kernel<1><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
+                                                                     l->is_signed, l->big_endian, rule,
+                                                                     vol->row_words, nblocks, words);
+        else if (bytes == 2)
+            volin::pack_wide_kernel<2><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
+                                                                     l->is_signed, l->big_endian, rule,
+                                                                     vol->row_words, nblocks, words);
+        else
+            volin::pack_wide_kernel<4><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
+                                                                     l->is_signed, l->big_endian, rule,
+                                                                     vol->row_words, nblocks, words);
+    } else {
+        const uint64_t nblocks = (l->width + 63) / 64 * l->height * vol->row_words;
+        const unsigned grid = (unsigned)std::min<uint64_t>(nblocks, 1u << 22);
+        if (l->bits == 8)
+            volin::pack_kernel<1><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth, l->is_signed,
+                                                                l->big_endian, rule, vol->row_words, nblocks, words);
+        else if (l->bits == 16)
+            volin::pack_kernel<2><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth, l->is_signed,
+                                                                l->big_endian, rule, vol->row_words, nblocks, words);
+        else
+            volin::pack_kernel<4><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth, l->is_signed,
+                                                                l->big_endian, rule, vol->row_words, nblocks, words);
+    }
+    ctx->launches = 1;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         s2h_vol_free(ctx, vol);
         return fail(S2H_ERR_HIP, std::string("pack kernel: ") + hipGetErrorString(e));
     }
+    prof_mark(ctx, 1);
+    prof_mark(ctx, 2);
     uint64_t occ = 0;
     s2h_status st = vol_popcount(ctx, vol, &occ);
+    prof_collect(ctx);
     if (st != S2H_OK) {
         s2h_vol_free(ctx, vol);
         return st;

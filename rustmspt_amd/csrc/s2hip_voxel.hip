@@ -50,38 +50,38 @@ __global__ void voxelize_overflow_kernel(const TriGeo *__restrict__ tris, const 
 }
 
 
-// Rust `f as isize` (saturating, NaN -> 0).
-inline int64_t f2i(double f) {
-    if (std::isnan(f)) return 0;
-    if (f >= 9.2233720368547758e18) return std::numeric_limits<int64_t>::max();
-    if (f <= -9.2233720368547758e18) return std::numeric_limits<int64_t>::min();
-    return (int64_t)f;
-}
-
-bool granule_bounds(const double mn[3], const double mx[3], const double bbox_min[3], double pitch,
-                    const uint64_t dims[3], int64_t lo[3], int64_t hi[3], double *cmax, double *smax) {
-    // src/geometry/s2.rs:126-147 part_voxel_ranges
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = std::max<int64_t>(0, f2i(std::floor((mn[a] - bbox_min[a]) / pitch)));
-        hi[a] = std::min<int64_t>((int64_t)dims[a], std::max<int64_t>(0, f2i(std::ceil((mx[a] - bbox_min[a]) / pitch))));
-        if (lo[a] >= hi[a]) return false;
+s2h_status run_voxelize(s2h_ctx *ctx, const TriGeo *d_geo, const GranuleInfo *d_gran, const Tile *d_tiles,
+                        uint64_t ntiles, const double bbox_min[3], double pitch, s2h_vol *vol) {
+    if (ntiles == 0) return S2H_OK;
+    const uint32_t over_cap = 1u << 16;
+    DevBuf d_over;
+    S2H_TRY(dev_alloc(ctx, d_over, (1 + 2 * (uint64_t)over_cap) * 4));
+    hipError_t e = hipMemsetAsync(d_over.ptr, 0, 4, ctx->stream);
+    for (uint64_t t0 = 0; e == hipSuccess && t0 < ntiles; t0 += (1u << 30)) {
+        const uint64_t nt = std::min<uint64_t>(ntiles - t0, 1u << 30);
+        voxelize_kernel<<<(unsigned)nt, 256, 0, ctx->stream>>>(d_geo, d_gran, d_tiles + t0, bbox_min[0], bbox_min[1],
+                                                              bbox_min[2], pitch, vol->dims[1], vol->row_words,
+                                                              (unsigned long long *)vol->d_words,
+                                                              (uint32_t *)d_over.ptr, over_cap);
+        e = hipGetLastError();
     }
-    // largest |coordinate| of a centre this granule tests, and the largest
-    // distance from a centre to a vertex (bounds for the cull margin)
-    double c = 0.0;
-    for (int a = 0; a < 3; ++a) {
-        c = std::max(c, std::fabs(bbox_min[a] + ((double)lo[a] + 0.5) * pitch));
-        c = std::max(c, std::fabs(bbox_min[a] + ((double)hi[a] + 0.5) * pitch));
+    uint32_t nover = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&nover, d_over.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && nover > over_cap) {
+        dev_free(ctx, d_over);
+        return fail(S2H_ERR_UNSUPPORTED, "more than 65536 voxel rays with over 16 hits");
     }
-    double s = 0.0;
-    for (int a = 0; a < 3; ++a) {
-        const double d = std::max({std::fabs(mx[a] - (bbox_min[a] + (double)lo[a] * pitch)),
-                                   std::fabs((bbox_min[a] + (double)(hi[a] + 1) * pitch) - mn[a]), pitch});
-        s += d * d;
+    if (e == hipSuccess && nover) {
+        voxelize_overflow_kernel<<<(nover + 63) / 64, 64, 0, ctx->stream>>>(
+            d_geo, d_gran, d_tiles, bbox_min[0], bbox_min[1], bbox_min[2], pitch, vol->dims[1], vol->row_words,
+            (unsigned long long *)vol->d_words, (const uint32_t *)d_over.ptr, nover);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     }
-    *cmax = c;
-    *smax = std::sqrt(s);
-    return true;
+    dev_free(ctx, d_over);
+    if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("voxelize kernel failed: ") + hipGetErrorString(e));
+    return S2H_OK;
 }
 
 bool prepare_granule(const double *tri_xyz, uint32_t ntri, const double bbox_min[3], double pitch,
@@ -181,50 +181,21 @@ extern "C" s2h_status s2h_voxelize(s2h_ctx *ctx, const double *xyz, uint64_t nve
                                  dims, geo.data() + start[g], gi, lo, hi))
             vox::push_tiles(g, lo, hi, tiles);
     }
-    DevBuf d_geo, d_gran, d_tiles, d_over;
+    DevBuf d_geo, d_gran, d_tiles;
     auto release = [&]() {
         dev_free(ctx, d_geo);
         dev_free(ctx, d_gran);
         dev_free(ctx, d_tiles);
-        dev_free(ctx, d_over);
     };
     s2h_status st = S2H_OK;
-    const uint32_t over_cap = 1u << 16;
     if (!tiles.empty()) {
         if ((st = upload(ctx, d_geo, geo.data(), geo.size() * sizeof(vox::TriGeo))) != S2H_OK ||
             (st = upload(ctx, d_gran, gran.data(), gran.size() * sizeof(vox::GranuleInfo))) != S2H_OK ||
             (st = upload(ctx, d_tiles, tiles.data(), tiles.size() * sizeof(vox::Tile))) != S2H_OK ||
-            (st = dev_alloc(ctx, d_over, (1 + 2 * (uint64_t)over_cap) * 4)) != S2H_OK) {
+            (st = vox::run_voxelize(ctx, (const vox::TriGeo *)d_geo.ptr, (const vox::GranuleInfo *)d_gran.ptr,
+                                    (const vox::Tile *)d_tiles.ptr, tiles.size(), bbox_min, pitch, vol)) != S2H_OK) {
             release();
             return cleanup(st);
-        }
-        hipError_t e = hipMemsetAsync(d_over.ptr, 0, 4, ctx->stream);
-        for (size_t t0 = 0; e == hipSuccess && t0 < tiles.size(); t0 += (1u << 30)) {
-            const size_t nt = std::min<size_t>(tiles.size() - t0, 1u << 30);
-            vox::voxelize_kernel<<<(unsigned)nt, 256, 0, ctx->stream>>>(
-                (const vox::TriGeo *)d_geo.ptr, (const vox::GranuleInfo *)d_gran.ptr, (const vox::Tile *)d_tiles.ptr + t0,
-                bbox_min[0], bbox_min[1], bbox_min[2], pitch, dims[1], vol->row_words,
-                (unsigned long long *)vol->d_words, (uint32_t *)d_over.ptr, over_cap);
-            e = hipGetLastError();
-        }
-        uint32_t nover = 0;
-        if (e == hipSuccess) e = hipMemcpyAsync(&nover, d_over.ptr, 4, hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        if (e == hipSuccess && nover > over_cap) {
-            release();
-            return cleanup(fail(S2H_ERR_UNSUPPORTED, "more than 65536 voxel rays with over 16 hits"));
-        }
-        if (e == hipSuccess && nover) {
-            vox::voxelize_overflow_kernel<<<(nover + 63) / 64, 64, 0, ctx->stream>>>(
-                (const vox::TriGeo *)d_geo.ptr, (const vox::GranuleInfo *)d_gran.ptr, (const vox::Tile *)d_tiles.ptr,
-                bbox_min[0], bbox_min[1], bbox_min[2], pitch, dims[1], vol->row_words,
-                (unsigned long long *)vol->d_words, (const uint32_t *)d_over.ptr, nover);
-            e = hipGetLastError();
-            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-        }
-        if (e != hipSuccess) {
-            release();
-            return cleanup(fail(S2H_ERR_HIP, std::string("voxelize kernel failed: ") + hipGetErrorString(e)));
         }
     }
     release();

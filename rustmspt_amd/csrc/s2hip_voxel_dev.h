@@ -297,14 +297,57 @@ struct SetBit {
     }
 };
 
+// Rust `f as isize` (saturating, NaN -> 0).
+__host__ __device__ inline int64_t f2i_sat(double f) {
+    if (f64_isnan(f)) return 0;
+    if (f >= 9.2233720368547758e18) return INT64_MAX;
+    if (f <= -9.2233720368547758e18) return INT64_MIN;
+    return (int64_t)f;
+}
+// Rust's f64::min / f64::max (a NaN operand is ignored).
+__host__ __device__ inline double rmin(double a, double b) { return f64_isnan(a) ? b : (f64_isnan(b) ? a : (a < b ? a : b)); }
+__host__ __device__ inline double rmax(double a, double b) { return f64_isnan(a) ? b : (f64_isnan(b) ? a : (a > b ? a : b)); }
+// std::max on doubles: (a < b) ? b : a.
+__host__ __device__ inline double smax2(double a, double b) { return a < b ? b : a; }
+
 // Voxel index range [lo, hi) of a granule with vertex bbox [mn, mx]
 // (part_voxel_ranges, src/geometry/s2.rs:126-147) and the cull-margin bounds
-// cmax / smax of tri_prepare.  Returns false when the range is empty.
-bool granule_bounds(const double mn[3], const double mx[3], const double bbox_min[3], double pitch,
-                    const uint64_t dims[3], int64_t lo[3], int64_t hi[3], double *cmax, double *smax);
-// Rust's f64::min / f64::max (a NaN operand is ignored).
-inline double rmin(double a, double b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a < b ? a : b)); }
-inline double rmax(double a, double b) { return std::isnan(a) ? b : (std::isnan(b) ? a : (a > b ? a : b)); }
+// cmax (largest |coordinate| of a centre the granule tests) and smax (largest
+// centre-vertex distance) of tri_prepare.  Returns false when the range is
+// empty.  Shared by the host preparation and the device mesh ingest.
+__host__ __device__ inline bool granule_bounds(const double mn[3], const double mx[3], const double bbox_min[3],
+                                               double pitch, const uint64_t dims[3], int64_t lo[3], int64_t hi[3],
+                                               double *cmax, double *smax) {
+    for (int a = 0; a < 3; ++a) {
+        const int64_t l = f2i_sat(floor((mn[a] - bbox_min[a]) / pitch));
+        const int64_t h = f2i_sat(ceil((mx[a] - bbox_min[a]) / pitch));
+        lo[a] = l > 0 ? l : 0;
+        const int64_t hp = h > 0 ? h : 0;
+        hi[a] = hp < (int64_t)dims[a] ? hp : (int64_t)dims[a];
+        if (lo[a] >= hi[a]) return false;
+    }
+    double c = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        c = smax2(c, fabs(bbox_min[a] + ((double)lo[a] + 0.5) * pitch));
+        c = smax2(c, fabs(bbox_min[a] + ((double)hi[a] + 0.5) * pitch));
+    }
+    double s = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        const double d = smax2(smax2(fabs(mx[a] - (bbox_min[a] + (double)lo[a] * pitch)),
+                                     fabs((bbox_min[a] + (double)(hi[a] + 1) * pitch) - mn[a])),
+                               pitch);
+        s += d * d;
+    }
+    *cmax = c;
+    *smax = sqrt(s);
+    return true;
+}
+
+// Launch the voxelize kernels (and the exact overflow pass) for prepared
+// triangles, granules and tiles on ctx->stream, OR-ing into vol; waits for
+// completion.  Shared by s2h_voxelize and the resident-mesh voxelizer.
+s2h_status run_voxelize(s2h_ctx *ctx, const TriGeo *d_geo, const GranuleInfo *d_gran, const Tile *d_tiles,
+                        uint64_t ntiles, const double bbox_min[3], double pitch, s2h_vol *vol);
 
 // Host-side preparation of one granule's triangles (precomputed edges, 1/det and
 // the conservative projected bound) and its voxel range.  Returns false when the

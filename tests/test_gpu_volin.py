@@ -24,10 +24,11 @@ def packed(lv):
     return words
 
 
+@pytest.mark.parametrize("W", [70, 96])   # rows of 8-byte multiples take the wide-load kernel
 @pytest.mark.parametrize("dt", ["u1", "i1", "<u2", ">u2", "<i2", ">i2", "<u4", ">u4", "<i4", ">i4"])
-def test_pack_matches_oracle_all_sample_types(ctx, dt):
-    rng = np.random.default_rng(sum(map(ord, dt)))
-    D, H, W = 130, 33, 70   # z words 64 + 64 + 2, x blocks of 64 + 6
+def test_pack_matches_oracle_all_sample_types(ctx, dt, W):
+    rng = np.random.default_rng(sum(map(ord, dt)) + W)
+    D, H = 130, 33   # z words 64 + 64 + 2
     info = np.iinfo(np.dtype(dt))
     vol = rng.integers(max(info.min, -1000), min(info.max, 1000), (D, H, W), endpoint=True).astype(dt)
     lo, hi = -37, 412
