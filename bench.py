@@ -281,6 +281,49 @@ def ingest_leg(ctx, n, seed):
                               "algorithmic_bytes": alg}}
 
 
+def stl_ingest_leg(ctx, mesh, bbox, reps=3):
+    """The single-measure packing as a binary STL (32 MB, 640,000 faces) to a
+    256^3 occupancy grid: host weld + granules + host-prepared voxelization vs
+    weld, granules and voxelization on the device (same bits)."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+
+    from rustmspt_amd import mesh as M
+    from rustmspt_amd import s2
+    from rustmspt_amd.devmesh import DeviceMesh
+
+    d = tempfile.mkdtemp(prefix="stl_")
+    try:
+        path = os.path.join(d, "packing.stl")
+        M.save_stl(path, mesh)
+        host_s, dev_s, dev_load_s = [], [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            hm = M.load_stl(path)
+            hv = s2.voxelize(ctx, hm, bbox, 1.0)   # granules + host preparation inside
+            hw = hv.read_packed() if not host_s else None
+            hv.free()
+            host_s.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            dm = DeviceMesh.load_stl(ctx, path)
+            t1 = time.perf_counter()
+            dv = dm.voxelize(bbox, 1.0)          # ends with a device popcount: synchronous
+            dev_s.append(time.perf_counter() - t0)
+            dev_load_s.append(t1 - t0)
+            if hw is not None:
+                same = bool(np.array_equal(dv.read_packed(), hw))
+            dv.free()
+            dm.free()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"faces": int(len(mesh.faces)), "file_bytes": 84 + 50 * int(len(mesh.faces)),
+            "host_stl_to_grid_s": min(host_s), "device_stl_to_grid_s": min(dev_s),
+            "device_load_weld_granules_s": min(dev_load_s), "speedup": min(host_s) / min(dev_s),
+            "grids_identical": same}
+
+
 def served_leg(rate: float, duration: float, pool_size: int, slots: int, budget_gb: float, seed: int):
     """Poisson stream of mixed requests (grid 64^3-384^3 log-uniform, VF
     U[0.1, 0.5], r_max U{16..96}, 70 % exact / 30 % FP32 FFT) through the serving
@@ -426,6 +469,7 @@ def main():
     if not args.no_mc:
         out["mesh_monte_carlo"] = mesh_mc_leg(ctx, mesh, bbox, r_max, args.mc_samples, seed=11)
     if not args.no_ingest:
+        out["stl_ingest"] = stl_ingest_leg(ctx, mesh, bbox)
         out["ingest"] = ingest_leg(ctx, args.ingest_grid, seed=5)
     if not args.no_batch:
         out["batched"] = batched_leg(ctx, args.batch_candidates, seed=3, plain_sample=32)

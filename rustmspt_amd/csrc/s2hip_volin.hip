@@ -74,6 +74,9 @@ __global__ __launch_bounds__(64) void pack_kernel(const uint8_t *__restrict__ s,
 
 // Rows of 8-byte multiples: each lane loads 8 bytes (8 / B samples at
 // consecutive x) per slice, so a wave reads 512 contiguous bytes per load.
+// One word per lane and x: the 8-byte row words land 104+ bytes apart, which
+// the memory system writes as partial sectors (about 3.7x the packed bytes at
+// 768^3); the kernel is still far below the host read and PCIe copy it follows.
 template <int B>
 __global__ __launch_bounds__(64) void pack_wide_kernel(const uint8_t *__restrict__ s, uint64_t W, uint64_t H,
                                                        uint64_t D, int sgn, int big, Rule rule, uint64_t row_words,

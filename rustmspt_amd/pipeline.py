@@ -20,6 +20,7 @@ import numpy as np
 
 from . import config as cfg
 from . import s2
+from .devmesh import DeviceMesh
 from .mesh import BoundingBox, granules, load_stl_or_merge_folder, mesh_bbox
 
 DEFAULT_CPU_EXACT_BUDGET_BYTES = 768 * 1024 * 1024
@@ -83,12 +84,38 @@ class MeasurePipeline:
         print(msg, file=self.out)
 
     def run(self):
+        self._res = {"grid": None, "dmesh": None, "ctx": None}
+        try:
+            return self._run()
+        finally:
+            r = self._res
+            if r["grid"] is not None:
+                r["grid"].volume.free()
+            if r["dmesh"] is not None:
+                r["dmesh"].free()
+            if r["ctx"] is not None:
+                r["ctx"].close()
+
+    def _run(self):
         p = self.p
+        res = self._res
         if not math.isfinite(p.voxel_pitch):
             raise cfg.ConfigError("measurement requires finite pitch and representable radius count")
         requested = cfg.configured_mode(p.acceleration, self.env)
         timer = StageTimer("measure", self.out)
-        mesh = load_stl_or_merge_folder(p.stl_path)
+        ctx, dmesh = None, None
+        # With a HIP-capable mode the STL is welded and split into granules on the
+        # device (the resident mesh is voxelized without host preparation); the
+        # host copy read back serves the volume fraction and the mesh MC.
+        if requested != cfg.AccelerationMode.CPU and self.env.get("S2HIP_HOST_INGEST", "0") != "1":
+            budget = (p.acceleration.gpu_memory_limit_mb or 0) * 1024 * 1024
+            ctx = res["ctx"] = s2.Context(0, budget)
+            dmesh = res["dmesh"] = DeviceMesh.load_stl(ctx, p.stl_path)
+            mesh, _ = dmesh.read()
+            particle_count = dmesh.ngranules
+        else:
+            mesh = load_stl_or_merge_folder(p.stl_path)
+            _, particle_count = granules(mesh)
         timer.stage("load")
         bbox = None
         for box in (p.bounding_box, p.stl_bounding_box):
@@ -96,11 +123,10 @@ class MeasurePipeline:
                 bbox = cfg.parse_box_dimensions(box)
                 break
         if bbox is None:
-            bbox = mesh_bbox(mesh) or BoundingBox.from_size((1.0, 1.0, 1.0))
+            bbox = (dmesh.bbox if dmesh is not None else mesh_bbox(mesh)) or BoundingBox.from_size((1.0, 1.0, 1.0))
         self.log(f"[Info] STL file(s) loaded from: {p.stl_path}")
         self.log("[Info] Measurement bbox: min=({:.4f},{:.4f},{:.4f}), max=({:.4f},{:.4f},{:.4f})".format(
             *bbox.min, *bbox.max))
-        _, particle_count = granules(mesh)
         raw = p.mc_method.strip().lower()
         method = "exact" if raw == "exact" else ("both" if raw == "both" else ("fft32" if raw == "fft32" else
                                                                               "monte_carlo"))
@@ -118,40 +144,53 @@ class MeasurePipeline:
             voxel_count *= int(n)
         methods = ["exact", "monte_carlo"] if method == "both" else [method]
         curves, backends, grid = [], [], None
-        ctx = None
+        vf_geo = s2.geometric_volume_fraction(mesh, bbox)
         timer.restart()
-        try:
-            for m in methods:
-                exactish = m in ("exact", "fft32")
-                est = cfg.hip_exact_bytes(dims, p.r_max, pitch, len(mesh.faces), fft=(m == "fft32")) if exactish \
-                    else dims[0] * dims[1] * s2.row_words(dims[2]) * 8 + len(mesh.faces) * 136 + 64 * (p.r_max + 2)
-                sel = cfg.resolve_execution(p.acceleration, requested, voxel_count, p.acceleration.gpu_min_voxels,
-                                            p.voxel_pitch > 0.0 or exactish, est)
-                if sel.fallback_reason:
-                    print(f"[Info] {m}: {sel.fallback_reason}", file=sys.stderr)
-                if sel.backend != "hip":
-                    raise CpuPathUnavailable(
-                        f"measure {m}: the policy selected the CPU backend, which is the host crate's own "
-                        f"implementation and not part of this engine")
-                if ctx is None:
-                    budget = (p.acceleration.gpu_memory_limit_mb or 0) * 1024 * 1024
-                    ctx = s2.Context(0, budget)
+        for m in methods:
+            exactish = m in ("exact", "fft32")
+            # monte_carlo with voxel_pitch <= 0 is the continuous mesh MC, as calculate_s2
+            # routes it (src/geometry/s2.rs:1351-1356; measure, src/pipeline/measure.rs:182-215)
+            mesh_mc = m == "monte_carlo" and p.voxel_pitch <= 0.0
+            est = cfg.hip_exact_bytes(dims, p.r_max, pitch, len(mesh.faces), fft=(m == "fft32")) if exactish \
+                else (len(mesh.faces) * 144 + len(mesh.vertices) * 24 + 16 * (p.r_max + 1) if mesh_mc else
+                      dims[0] * dims[1] * s2.row_words(dims[2]) * 8 + len(mesh.faces) * 136 + 64 * (p.r_max + 2))
+            sel = cfg.resolve_execution(p.acceleration, requested, voxel_count, p.acceleration.gpu_min_voxels,
+                                        True, est)
+            if sel.fallback_reason:
+                print(f"[Info] {m}: {sel.fallback_reason}", file=sys.stderr)
+            if sel.backend != "hip":
+                raise CpuPathUnavailable(
+                    f"measure {m}: the policy selected the CPU backend, which is the host crate's own "
+                    f"implementation and not part of this engine")
+            if ctx is None:
+                budget = (p.acceleration.gpu_memory_limit_mb or 0) * 1024 * 1024
+                ctx = res["ctx"] = s2.Context(0, budget)
+            seed = int(self.env.get("RUSTMSPT_S2_SEED", "0"))
+            if mesh_mc:
+                if vf_geo is None:
+                    raise cfg.ConfigError("continuous mesh MC needs the geometric volume fraction, which is "
+                                          "defined here only for particles wholly inside the measurement box")
+                hv = s2.mesh_mc_counts(ctx, mesh, bbox, p.r_max, samples, seed).astype(np.float64)
+                curve = np.array([vf_geo] + [hv[r, 0] / hv[r, 1] if hv[r, 1] else 0.0
+                                             for r in range(1, p.r_max + 1)])
+            else:
                 if grid is None:
                     self.log("[Info] Preparing shared HIP S2 occupancy grid")
-                    grid = s2.VoxelS2Hip(ctx, mesh, bbox, pitch)
-                seed = int(self.env.get("RUSTMSPT_S2_SEED", "0"))
+                    if dmesh is not None:
+                        grid = s2.VoxelS2Hip.from_volume(dmesh.voxelize(bbox, max(pitch, 1e-9)), pitch)
+                    else:
+                        grid = s2.VoxelS2Hip(ctx, mesh, bbox, pitch)
+                    res["grid"] = grid
                 curve = grid.calculate_seeded(p.r_max, m, samples, seed)
-                backends.append(f"{m}=hip/{ctx.device_name}")
-                self.log(f"[Info] Measure execution: method={m}, backend=hip, device={ctx.device_name}")
-                curves.append(curve)
-                timer.stage(f"s2_{m}")
+            backends.append(f"{m}=hip/{ctx.device_name}")
+            self.log(f"[Info] Measure execution: method={m}, backend=hip, device={ctx.device_name}")
+            curves.append(curve)
+            timer.stage(f"s2_{m}")
+        if vf_geo is not None:
+            vf = vf_geo
+        else:
             occ_vf = grid.volume.occupied / float(np.prod(grid.dims)) if grid else 0.0
             vf = volume_fraction_in_bbox(mesh, bbox, occ_vf)
-        finally:
-            if grid is not None:
-                grid.volume.free()
-            if ctx is not None:
-                ctx.close()
         lines = [f"Volume Fraction: {vf:.6f}", f"Compute Backend: {', '.join(backends)}", f"Method: {method}"]
         for m, curve in zip(methods, curves):
             lines.append(f"S2(0)-VF diff [{m}]: {abs(curve[0] - vf):.6f}")

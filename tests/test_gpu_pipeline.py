@@ -60,3 +60,31 @@ def test_both_methods_match_oracle(tmp_path):
     for r, v in enumerate(want):
         assert f"{r}: {v:.6f}" in block
     assert "L2 error [exact vs monte_carlo]:" in text and "monte_carlo=hip/" in text
+
+
+def test_device_and_host_ingest_reports_match(tmp_path):
+    mesh, bbox, _ = synth.config_plain(seed=9, n=10, grid=36)
+    kw = dict(r_max=6, voxel_pitch=1.0, bounding_box=[36, 36, 36], mc_method="both", mc_samples=20000)
+    dev = run(tmp_path, mesh, env={"RUSTMSPT_S2_SEED": "4"}, **kw)
+    host = run(tmp_path, mesh, env={"RUSTMSPT_S2_SEED": "4", "S2HIP_HOST_INGEST": "1"}, **kw)
+    assert dev == host and "exact=hip/" in dev
+
+
+def test_monte_carlo_with_pitch_zero_is_mesh_mc(tmp_path):
+    # calculate_s2 routes monte_carlo with voxel_pitch <= 0 to the continuous mesh MC
+    # (src/geometry/s2.rs:1351-1356); S2(0) is the geometric volume fraction
+    mesh = M.merge_meshes([M.icosphere_mesh(np.array([1.2, 1.2, 1.0]), 0.6, 3),
+                           M.icosphere_mesh(np.array([2.6, 2.8, 3.0]), 0.8, 3)])
+    text = run(tmp_path, mesh, env={"RUSTMSPT_S2_SEED": "21"}, r_max=3, voxel_pitch=0.0, mc_method="monte_carlo",
+               mc_samples=5000)
+    from rustmspt_amd import s2
+    ctx = s2.Context(0)
+    try:
+        back = M.load_stl(tmp_path / "in.stl")
+        box = M.BoundingBox((0.0, 0.0, 0.0), (4.0, 4.0, 4.0))
+        want = s2.mesh_mc_s2(ctx, back, box, 3, 5000, 21)
+    finally:
+        ctx.close()
+    assert f"Volume Fraction: {want[0]:.6f}" in text
+    for r, v in enumerate(want):
+        assert f"{r}: {v:.6f}" in text
