@@ -11,7 +11,6 @@ from __future__ import annotations
 
 import math
 import os
-import resource
 import sys
 import time
 from pathlib import Path
@@ -19,6 +18,7 @@ from pathlib import Path
 import numpy as np
 
 from . import config as cfg
+from . import report
 from . import s2
 from .devmesh import DeviceMesh
 from .mesh import BoundingBox, granules, load_stl_or_merge_folder, mesh_bbox
@@ -30,36 +30,8 @@ class CpuPathUnavailable(RuntimeError):
     """The policy chose the CPU backend, which lives in the host crate."""
 
 
-class StageTimer:
-    """src/pipeline/timing.rs:3-81: `[Timing] <pipeline> stage=<name> seconds=<s>`."""
-
-    def __init__(self, pipeline: str, out=sys.stdout):
-        self.pipeline = pipeline
-        self.out = out
-        self.t0 = self.last = time.perf_counter()
-
-    def restart(self):
-        self.last = time.perf_counter()
-
-    def stage(self, name: str):
-        now = time.perf_counter()
-        print(f"[Timing] {self.pipeline} stage={name} seconds={now - self.last:.6f}", file=self.out)
-        self.last = now
-
-    def total(self, name: str):
-        print(f"[Timing] {self.pipeline} stage={name} seconds={time.perf_counter() - self.t0:.6f}", file=self.out)
-
-    def report_resources(self):
-        rss = resource.getrusage(resource.RUSAGE_SELF).ru_maxrss * 1024
-        print(f"[Timing] {self.pipeline} peak_rss_bytes={rss}", file=self.out)
-
-
-def l2_error(a, b) -> float:
-    """src/pipeline/measure.rs:27-39."""
-    n = min(len(a), len(b))
-    if n == 0:
-        return 0.0
-    return math.sqrt(sum((float(a[i]) - float(b[i])) ** 2 for i in range(n)))
+StageTimer = report.StageTimer
+l2_error = report.l2_error
 
 
 def volume_fraction_in_bbox(mesh, bbox: BoundingBox, occupancy_vf: float):
@@ -191,16 +163,9 @@ class MeasurePipeline:
         else:
             occ_vf = grid.volume.occupied / float(np.prod(grid.dims)) if grid else 0.0
             vf = volume_fraction_in_bbox(mesh, bbox, occ_vf)
-        lines = [f"Volume Fraction: {vf:.6f}", f"Compute Backend: {', '.join(backends)}", f"Method: {method}"]
-        for m, curve in zip(methods, curves):
-            lines.append(f"S2(0)-VF diff [{m}]: {abs(curve[0] - vf):.6f}")
-            lines.append("S2 Values:" if len(methods) == 1 else f"S2 Values [{m}]:")
-            lines += [f"{r}: {v:.6f}" for r, v in enumerate(curve)]
-        if len(curves) == 2:
-            lines.append(f"L2 error [exact vs monte_carlo]: {l2_error(curves[0], curves[1]):.6f}")
         out = Path(p.output_path)
         out.parent.mkdir(parents=True, exist_ok=True)
-        out.write_text("\n".join(lines) + "\n")
+        out.write_text(report.measure_report(vf, backends, method, methods, curves))
         timer.stage("write")
         self.log(f"[Info] Measurement completed: {particle_count} particles, VF {vf:.6f}; output {p.output_path}")
         timer.total("total_in_pool")
