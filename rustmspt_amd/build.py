@@ -34,7 +34,7 @@ UNITS = {
     "s2hip_serve.hip": ["-ffp-contract=off"],
     "s2hip_probe.hip": [],
     "s2hip_meshmc.hip": ["-ffp-contract=off"],
-    "s2hip_volin.hip": [],
+    "s2hip_volin.hip": ["-ffp-contract=off"],
     "s2hip_meshin.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = {"s2hip_host.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"],
