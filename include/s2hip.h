@@ -189,6 +189,11 @@ s2h_status s2h_mesh_info(const s2h_mesh *mesh, uint64_t *nverts, uint64_t *ntris
 /* Copy the mesh back: xyz 3*nverts, tri 3*ntris, gid ntris (any may be null). */
 s2h_status s2h_mesh_read(s2h_ctx *ctx, const s2h_mesh *mesh, double *xyz, uint64_t nxyz, uint32_t *tri,
                          uint64_t ntri, uint32_t *gid, uint64_t ngid);
+/* Overwrite vertices [vstart, vstart+n) of a resident mesh (xyz: 3n doubles):
+ * a rigid particle move in the optimize loop, whose topology and granules do
+ * not change.  Replaces GpuS2Pipeline::update_mesh's partial upload of changed
+ * faces (src/gpu/s2.rs:480-560); the bbox is recomputed. */
+s2h_status s2h_mesh_set_vertices(s2h_ctx *ctx, s2h_mesh *mesh, uint64_t vstart, const double *xyz, uint64_t n);
 /* s2h_voxelize on the resident mesh (same result bits). */
 s2h_status s2h_mesh_voxelize(s2h_ctx *ctx, const s2h_mesh *mesh, const double bbox_min[3], double pitch,
                              const uint64_t dims[3], s2h_vol **out, uint64_t *occupied);
@@ -280,6 +285,12 @@ s2h_status s2h_mesh_mc_counts(s2h_ctx *ctx, const double *xyz, uint64_t nverts, 
                               uint64_t ntris, const double bbox_min[3], const double bbox_max[3],
                               uint32_t r_max, uint64_t samples, uint64_t seed, uint64_t *hits_valid,
                               uint64_t n);
+/* s2h_mesh_mc_counts on a resident mesh (GpuS2Pipeline::calculate_s2_gpu_seeded,
+ * src/gpu/s2.rs:640-670, after update_mesh): the same counts as the host-array
+ * entry on the same vertices and faces. */
+s2h_status s2h_mesh_mc_counts_resident(s2h_ctx *ctx, const s2h_mesh *mesh, const double bbox_min[3],
+                                       const double bbox_max[3], uint32_t r_max, uint64_t samples, uint64_t seed,
+                                       uint64_t *hits_valid, uint64_t n);
 /* The curve: s2[0] = vf (the caller's geometric volume fraction in the box,
  * volume_fraction_in_bbox, src/geometry/volume.rs:437-469), s2[r] = hits/valid
  * (0 when no pair was valid), no fill -- as the reference. */

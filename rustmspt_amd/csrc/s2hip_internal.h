@@ -225,6 +225,15 @@ struct s2h_ctx {
     int launches = 0;
 };
 
+// A welded, granule-split mesh resident on the device (s2hip_meshin.hip).
+struct s2h_mesh {
+    uint64_t nverts = 0, ntris = 0;
+    uint32_t ngranules = 0;
+    double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};  // mesh_bbox (f64::min / max)
+    double amax = 0.0;                                 // largest |vertex coordinate|
+    s2h::DevBuf xyz, tri, gid;  // f64 x3 per vertex, u32 x3 per face, u32 per face
+};
+
 namespace s2h {
 s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes);
 void dev_free(s2h_ctx *ctx, DevBuf &b);
@@ -232,6 +241,8 @@ void dev_cache_release(s2h_ctx *ctx);  // hipFree every cached block
 void batch_ws_release(s2h_ctx *ctx);
 s2h_status upload(s2h_ctx *ctx, DevBuf &b, const void *src, uint64_t bytes);
 s2h_status vol_alloc(s2h_ctx *ctx, const uint64_t dims[3], s2h_vol **out);
+// Exclusive prefix sum of n (1..2^31-1) u32 on ctx->stream; waits and returns the total.
+s2h_status exclusive_sum(s2h_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *total);
 s2h_status vol_popcount(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);
 
 s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,

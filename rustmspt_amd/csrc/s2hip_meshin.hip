@@ -32,13 +32,6 @@
 #include "s2hip_stlio.h"
 #include "s2hip_voxel_dev.h"
 
-struct s2h_mesh {
-    uint64_t nverts = 0, ntris = 0;
-    uint32_t ngranules = 0;
-    double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
-    s2h::DevBuf xyz, tri, gid;  // f64 x3 per vertex, u32 x3 per face, u32 per face
-};
-
 namespace s2h {
 namespace meshin {
 
@@ -224,29 +217,34 @@ __global__ void gid_kernel(const uint32_t *__restrict__ tri, const uint32_t *__r
         gid[f] = grank[minf[root[tri[3 * f]]]];
 }
 
-// ---- mesh bbox: f64::min / f64::max over all vertices (mesh_bbox, src/geometry/bbox.rs:8-25)
+// ---- mesh bbox: f64::min / f64::max over all vertices (mesh_bbox, src/geometry/bbox.rs:8-25),
+// and the largest |coordinate| (std::max from 0, NaN ignored)
 
 __global__ __launch_bounds__(256) void bbox_kernel(const double *__restrict__ xyz, uint64_t nverts,
-                                                   double *__restrict__ partial /* 6 per block */) {
-    __shared__ double red[6][256];
-    double v[6] = {NAN, NAN, NAN, NAN, NAN, NAN};
+                                                   double *__restrict__ partial /* 7 per block */) {
+    __shared__ double red[7][256];
+    double v[7] = {NAN, NAN, NAN, NAN, NAN, NAN, 0.0};
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < nverts;
          i += (uint64_t)gridDim.x * blockDim.x)
         for (int a = 0; a < 3; ++a) {
-            v[a] = vox::rmin(v[a], xyz[3 * i + a]);
-            v[3 + a] = vox::rmax(v[3 + a], xyz[3 * i + a]);
+            const double c = xyz[3 * i + a];
+            v[a] = vox::rmin(v[a], c);
+            v[3 + a] = vox::rmax(v[3 + a], c);
+            v[6] = vox::smax2(v[6], fabs(c));
         }
-    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+    for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = v[k];
     __syncthreads();
     for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if (threadIdx.x < s) {
             for (int a = 0; a < 3; ++a) {
                 red[a][threadIdx.x] = vox::rmin(red[a][threadIdx.x], red[a][threadIdx.x + s]);
                 red[3 + a][threadIdx.x] = vox::rmax(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + s]);
             }
+            red[6][threadIdx.x] = vox::smax2(red[6][threadIdx.x], red[6][threadIdx.x + s]);
+        }
         __syncthreads();
     }
-    if (threadIdx.x < 6) partial[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 7) partial[7 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // ---- voxelization from the resident mesh
@@ -378,24 +376,6 @@ unsigned grid_for(uint64_t n, unsigned block = 256) {
     return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + block - 1) / block, 1u << 20));
 }
 
-// Exclusive prefix sum of n u32 flags/counts into out; returns the total.
-s2h_status exclusive_sum(s2h_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *total) {
-    size_t bytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream) != hipSuccess)
-        return fail(S2H_ERR_HIP, "scan sizing failed");
-    DevBuf tmp;
-    S2H_TRY(dev_alloc(ctx, tmp, std::max<size_t>(bytes, 8)));
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp.ptr, bytes, in, out, (int)n, ctx->stream);
-    uint32_t last[2] = {0, 0};
-    if (e == hipSuccess) e = hipMemcpyAsync(&last[0], out + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(&last[1], in + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    dev_free(ctx, tmp);
-    if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("scan failed: ") + hipGetErrorString(e));
-    *total = (uint64_t)last[0] + last[1];
-    return S2H_OK;
-}
-
 struct Bufs {
     s2h_ctx *ctx;
     std::vector<DevBuf *> held;
@@ -416,12 +396,65 @@ struct Bufs {
         if (e_ != hipSuccess) return fail(S2H_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
     } while (0)
 
+}  // namespace
+
+// Exclusive prefix sum of n u32 flags/counts into out; returns the total.
+s2h_status exclusive_sum(s2h_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *total) {
+    if (n == 0 || n > 0x7FFFFFFFull) return fail(S2H_ERR_UNSUPPORTED, "scan length out of range");
+    size_t bytes = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream) != hipSuccess)
+        return fail(S2H_ERR_HIP, "scan sizing failed");
+    DevBuf tmp;
+    S2H_TRY(dev_alloc(ctx, tmp, std::max<size_t>(bytes, 8)));
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp.ptr, bytes, in, out, (int)n, ctx->stream);
+    uint32_t last[2] = {0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(&last[0], out + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&last[1], in + n - 1, 4, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, tmp);
+    if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("scan failed: ") + hipGetErrorString(e));
+    *total = (uint64_t)last[0] + last[1];
+    return S2H_OK;
+}
+
+// Vertex bbox and largest |coordinate| of a resident mesh, reduced on the device.
+s2h_status mesh_bounds(s2h_ctx *ctx, s2h_mesh *m) {
+    using namespace meshin;
+    const uint64_t nv = m->nverts;
+    if (nv == 0) return S2H_OK;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((nv + 255) / 256, 1024);
+    DevBuf part;
+    S2H_TRY(dev_alloc(ctx, part, (uint64_t)blocks * 7 * 8));
+    bbox_kernel<<<blocks, 256, 0, ctx->stream>>>((const double *)m->xyz.ptr, nv, (double *)part.ptr);
+    std::vector<double> h((size_t)blocks * 7);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), part.ptr, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(ctx, part);
+    if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("mesh bounds: ") + hipGetErrorString(e));
+    double amax = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        double lo = NAN, hi = NAN;
+        for (unsigned b = 0; b < blocks; ++b) {
+            lo = vox::rmin(lo, h[7 * b + a]);
+            hi = vox::rmax(hi, h[7 * b + 3 + a]);
+        }
+        m->bmin[a] = lo;
+        m->bmax[a] = hi;
+    }
+    for (unsigned b = 0; b < blocks; ++b) amax = vox::smax2(amax, h[7 * b + 6]);
+    m->amax = amax;
+    return S2H_OK;
+}
+
+namespace {
+
 // Granules and bbox of a mesh whose xyz/tri are resident.
 s2h_status finish_mesh(s2h_ctx *ctx, s2h_mesh *m) {
     using namespace meshin;
     const uint64_t nv = m->nverts, nf = m->ntris;
     if (nf > 0x7FFFFFFEull || nv > 0xFFFFFFFEull) return fail(S2H_ERR_UNSUPPORTED, "mesh too large");
-    DevBuf parent, root, minf, flag, grank, part;  // declared before tmp, which frees them
+    DevBuf parent, root, minf, flag, grank;  // declared before tmp, which frees them
     Bufs tmp(ctx);
     S2H_TRY(dev_alloc(ctx, m->gid, std::max<uint64_t>(nf, 1) * 4));
     if (nf > 0) {
@@ -446,21 +479,7 @@ s2h_status finish_mesh(s2h_ctx *ctx, s2h_mesh *m) {
         LAUNCH_OK();
         m->ngranules = (uint32_t)ng;
     }
-    if (nv > 0) {
-        const unsigned blocks = (unsigned)std::min<uint64_t>((nv + 255) / 256, 1024);
-        S2H_TRY(tmp.get(part, (uint64_t)blocks * 6 * 8));
-        bbox_kernel<<<blocks, 256, 0, ctx->stream>>>((const double *)m->xyz.ptr, nv, (double *)part.ptr);
-        LAUNCH_OK();
-        std::vector<double> h((size_t)blocks * 6);
-        S2H_HIP(hipMemcpyAsync(h.data(), part.ptr, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-        S2H_HIP(hipStreamSynchronize(ctx->stream));
-        for (int a = 0; a < 6; ++a) {
-            double r = NAN;
-            for (unsigned b = 0; b < blocks; ++b)
-                r = a < 3 ? vox::rmin(r, h[6 * b + a]) : vox::rmax(r, h[6 * b + a]);
-            (a < 3 ? m->bmin[a] : m->bmax[a - 3]) = r;
-        }
-    }
+    if (nv > 0) S2H_TRY(mesh_bounds(ctx, m));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
     return S2H_OK;
 }
@@ -730,6 +749,14 @@ s2h_status s2h_mesh_voxelize(s2h_ctx *ctx, const s2h_mesh *m, const double bbox_
     if (occupied) *occupied = occ;
     *out = vol;
     return S2H_OK;
+}
+
+s2h_status s2h_mesh_set_vertices(s2h_ctx *ctx, s2h_mesh *m, uint64_t vstart, const double *xyz, uint64_t n) {
+    if (!ctx || !m || (n && !xyz)) return fail(S2H_ERR_ARG, "null argument");
+    if (vstart > m->nverts || n > m->nverts - vstart) return fail(S2H_ERR_SIZE, "vertex range outside the mesh");
+    S2H_HIP(hipSetDevice(ctx->device));
+    if (n) S2H_HIP(hipMemcpyAsync((double *)m->xyz.ptr + 3 * vstart, xyz, n * 24, hipMemcpyHostToDevice, ctx->stream));
+    return mesh_bounds(ctx, m);
 }
 
 void s2h_mesh_free(s2h_ctx *ctx, s2h_mesh *m) {

@@ -1,6 +1,6 @@
 // Continuous mesh Monte Carlo S2 (no voxelization).
 //
-// Estimator of calculate_s2_mesh_mc_seeded_with_vf (src/geometry/s2.rs:1273-1320):
+// Estimator of calculate_s2_mesh_mc_seeded_with_vf (src/geometry/s2.rs:1274-1320):
 // for each radius r = 1..r_max and each of max(samples, 200) samples, a point p
 // uniform in the measurement box and a direction uniform on the sphere (by
 // rejection in the cube: accept 1e-12 < |d|^2 <= 1, then normalize); q = p + r d.
@@ -23,8 +23,9 @@
 // be hit when the query point's projection on the plane normal to the ray lies
 // in the triangle's projected bound (widened by a margin that dominates every
 // rounding error of the predicate -- the voxelizer's conservative cull).  The
-// triangles are binned by that bound on a 2-D grid over the plane; a query
-// tests only its cell's list, so the hit multiset equals a full scan.
+// triangles are prepared and binned by that bound on a 2-D grid over the plane
+// on the device (count per cell, prefix sum, scatter); a query tests only its
+// cell's list, so the hit multiset equals a full scan.
 // Compiled with -ffp-contract=off: every f64 operation keeps its order.
 #include <algorithm>
 #include <cmath>
@@ -190,58 +191,83 @@ __global__ void clear_u64_kernel(unsigned long long *p, uint64_t n) {
         p[i] = 0;
 }
 
-}  // namespace meshmc
-}  // namespace s2h
+__global__ void fill_u32_kernel(uint32_t *__restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
 
-using namespace s2h;
+// Cells of a G-cell axis whose span meets [lo, hi]; empty when the bound misses
+// the grid (c0 > c1).
+__device__ __forceinline__ void cell_range(double lo, double hi, double o, double inv, int G, int &c0, int &c1) {
+    if (!(hi >= lo)) {
+        c0 = 1;
+        c1 = 0;
+        return;
+    }
+    const double a = (lo - o) * inv, b = (hi - o) * inv;
+    c0 = a <= 0.0 ? 0 : (a >= G ? G : (int)floor(a));
+    c1 = b < 0.0 ? -1 : (b >= G - 1 ? G - 1 : (int)floor(b));
+}
 
-extern "C" {
+// Per triangle: the predicate's precomputed data and conservative projected
+// bound (tri_prepare), its cell range, and one count per covered cell.
+__global__ void bin_prepare_kernel(const double *__restrict__ xyz, const uint32_t *__restrict__ tri, uint64_t ntris,
+                                   double cmax, double smax, double amax, double u0, double v0, double inv_du,
+                                   double inv_dv, int G, vox::TriGeo *__restrict__ geo, int4 *__restrict__ range,
+                                   uint32_t *count) {
+    for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < ntris; f += (uint64_t)gridDim.x * blockDim.x) {
+        vox::TriGeo t;
+        vox::tri_prepare(xyz + 3 * (uint64_t)tri[3 * f], xyz + 3 * (uint64_t)tri[3 * f + 1],
+                         xyz + 3 * (uint64_t)tri[3 * f + 2], cmax, smax, amax, t);
+        geo[f] = t;
+        int a0, a1, b0, b1;
+        cell_range(t.umin, t.umax, u0, inv_du, G, a0, a1);
+        cell_range(t.vmin, t.vmax, v0, inv_dv, G, b0, b1);
+        range[f] = make_int4(a0, a1, b0, b1);
+        for (int i = a0; i <= a1; ++i)
+            for (int j = b0; j <= b1; ++j) atomicAdd(&count[(uint64_t)i * G + j], 1u);
+    }
+}
 
-s2h_status s2h_mesh_mc_counts(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,
-                              const double bbox_min[3], const double bbox_max[3], uint32_t r_max, uint64_t samples,
-                              uint64_t seed, uint64_t *hits_valid, uint64_t n) {
-    using namespace s2h::meshmc;
-    if (!ctx || !bbox_min || !bbox_max || !hits_valid || (ntris && (!xyz || !tri)) || (nverts && !xyz))
-        return fail(S2H_ERR_ARG, "null argument");
-    if (n != 2 * ((uint64_t)r_max + 1)) return fail(S2H_ERR_SIZE, "hits_valid must hold 2*(r_max+1) values");
-    for (uint64_t f = 0; f < 3 * ntris; ++f)
-        if (tri[f] >= nverts) return fail(S2H_ERR_ARG, "vertex index out of range");
-    for (int a = 0; a < 3; ++a)
-        if (!(bbox_max[a] >= bbox_min[a]) || !std::isfinite(bbox_min[a]) || !std::isfinite(bbox_max[a]))
-            return fail(S2H_ERR_ARG, "bounding box must be finite with max >= min");
-    S2H_HIP(hipSetDevice(ctx->device));
-    std::fill(hits_valid, hits_valid + n, 0);
-    if (r_max == 0) return S2H_OK;
-    const uint64_t spr = std::max<uint64_t>(samples, 200);
-    // whole-mesh vertex bbox (mesh_bbox, src/geometry/bbox.rs:8-25), amax
-    Grid g{};
-    double amax = 0.0;
-    for (uint64_t v = 0; v < nverts; ++v)
-        for (int a = 0; a < 3; ++a) {
-            const double c = xyz[3 * v + a];
-            g.mn[a] = v ? vox::rmin(g.mn[a], c) : c;
-            g.mx[a] = v ? vox::rmax(g.mx[a], c) : c;
-            amax = std::max(amax, std::fabs(c));
-        }
-    g.have_bbox = nverts > 0;
-    // cull-margin bounds: any query point lies in the measurement box
-    double cmax = 0.0, span2 = 0.0;
+// Scatter triangle indices into their cells' lists.  The order inside a list
+// does not matter: a query sorts its hits, so its result is the same for every
+// order.
+__global__ void bin_fill_kernel(const int4 *__restrict__ range, uint64_t ntris, int G,
+                                const uint32_t *__restrict__ start, uint32_t *cursor, uint32_t *__restrict__ ids) {
+    for (uint64_t f = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; f < ntris; f += (uint64_t)gridDim.x * blockDim.x) {
+        const int4 rg = range[f];
+        for (int i = rg.x; i <= rg.y; ++i)
+            for (int j = rg.z; j <= rg.w; ++j) {
+                const uint64_t cell = (uint64_t)i * G + j;
+                ids[start[cell] + atomicAdd(&cursor[cell], 1u)] = (uint32_t)f;
+            }
+    }
+}
+
+// Grid geometry and cull-margin bounds for a mesh with vertex bbox [mn, mx]
+// (have = false: no vertices) and largest |coordinate| amax, queried inside
+// [bmin, bmax].
+void setup_grid(const double bmin[3], const double bmax[3], bool have, const double mn[3], const double mx[3],
+                uint64_t ntris, Grid &g, double &cmax, double &smax) {
+    g.have_bbox = have;
     for (int a = 0; a < 3; ++a) {
-        cmax = std::max({cmax, std::fabs(bbox_min[a]), std::fabs(bbox_max[a])});
-        const double lo = std::min(bbox_min[a], g.have_bbox ? g.mn[a] : bbox_min[a]);
-        const double hi = std::max(bbox_max[a], g.have_bbox ? g.mx[a] : bbox_max[a]);
+        g.mn[a] = have ? mn[a] : 0.0;
+        g.mx[a] = have ? mx[a] : 0.0;
+    }
+    // any query point lies in the measurement box
+    cmax = 0.0;
+    double span2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        cmax = std::max({cmax, std::fabs(bmin[a]), std::fabs(bmax[a])});
+        const double lo = std::min(bmin[a], have ? mn[a] : bmin[a]);
+        const double hi = std::max(bmax[a], have ? mx[a] : bmax[a]);
         span2 += (hi - lo) * (hi - lo);
     }
-    const double smax = std::sqrt(span2) + 1.0;
-    std::vector<vox::TriGeo> geo(ntris);
-    for (uint64_t f = 0; f < ntris; ++f)
-        vox::tri_prepare(xyz + 3 * (uint64_t)tri[3 * f], xyz + 3 * (uint64_t)tri[3 * f + 1],
-                         xyz + 3 * (uint64_t)tri[3 * f + 2], cmax, smax, amax, geo[f]);
-    // grid over the projection of the measurement box (query points lie in it)
+    smax = std::sqrt(span2) + 1.0;
+    // grid over the projection of the measurement box
     double u0 = INFINITY, u1 = -INFINITY, v0 = INFINITY, v1 = -INFINITY;
     for (int c = 0; c < 8; ++c) {
-        const double x = (c & 1) ? bbox_max[0] : bbox_min[0], y = (c & 2) ? bbox_max[1] : bbox_min[1],
-                     z = (c & 4) ? bbox_max[2] : bbox_min[2];
+        const double x = (c & 1) ? bmax[0] : bmin[0], y = (c & 2) ? bmax[1] : bmin[1], z = (c & 4) ? bmax[2] : bmin[2];
         const double u = vox::kUx * x + vox::kUy * y + vox::kUz * z, v = vox::kVx * x + vox::kVy * y + vox::kVz * z;
         u0 = std::min(u0, u);
         u1 = std::max(u1, u);
@@ -259,68 +285,72 @@ s2h_status s2h_mesh_mc_counts(s2h_ctx *ctx, const double *xyz, uint64_t nverts, 
     g.v0 = v0;
     g.inv_du = G / std::max(u1 - u0, 1e-300);
     g.inv_dv = G / std::max(v1 - v0, 1e-300);
-    auto cell_range = [&](double lo, double hi, double o, double inv, int &c0, int &c1) {
-        // cells whose span meets [lo, hi]; empty when the bound misses the grid
-        if (!(hi >= lo)) {
-            c0 = 1;
-            c1 = 0;
-            return;
-        }
-        const double a = (lo - o) * inv, b = (hi - o) * inv;
-        c0 = a <= 0.0 ? 0 : (a >= G ? G : (int)std::floor(a));
-        c1 = b < 0.0 ? -1 : (b >= G - 1 ? G - 1 : (int)std::floor(b));
-    };
-    std::vector<uint32_t> count((size_t)G * G + 1, 0);
-    for (uint64_t f = 0; f < ntris; ++f) {
-        int a0, a1, b0, b1;
-        cell_range(geo[f].umin, geo[f].umax, u0, g.inv_du, a0, a1);
-        cell_range(geo[f].vmin, geo[f].vmax, v0, g.inv_dv, b0, b1);
-        for (int i = a0; i <= a1; ++i)
-            for (int j = b0; j <= b1; ++j) count[(size_t)i * G + j + 1]++;
-    }
-    for (size_t c = 0; c < (size_t)G * G; ++c) count[c + 1] += count[c];
-    if (count[(size_t)G * G] >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "mesh too large for the query grid");
-    std::vector<uint32_t> ids(std::max<uint32_t>(count[(size_t)G * G], 1));
-    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
-    for (uint64_t f = 0; f < ntris; ++f) {
-        int a0, a1, b0, b1;
-        cell_range(geo[f].umin, geo[f].umax, u0, g.inv_du, a0, a1);
-        cell_range(geo[f].vmin, geo[f].vmax, v0, g.inv_dv, b0, b1);
-        for (int i = a0; i <= a1; ++i)
-            for (int j = b0; j <= b1; ++j) ids[fill[(size_t)i * G + j]++] = (uint32_t)f;
-    }
-    DevBuf d_geo, d_start, d_ids, d_hv;
+}
+
+unsigned grid_for(uint64_t n) { return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 1u << 20)); }
+
+// Bin the resident triangles on the device and run the estimator.
+s2h_status bin_and_run(s2h_ctx *ctx, Grid g, double cmax, double smax, double amax, const double *d_xyz,
+                       const uint32_t *d_tri, uint64_t ntris, const double bbox_min[3], const double bbox_max[3],
+                       uint32_t r_max, uint64_t samples, uint64_t seed, uint64_t *hits_valid, uint64_t n) {
+    const uint64_t spr = std::max<uint64_t>(samples, 200);
+    const uint64_t per_block = (uint64_t)kThreads * kPerThread;
+    const uint64_t bpr = (spr + per_block - 1) / per_block;
+    if (bpr * r_max >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "too many samples");
+    const uint64_t cells = (uint64_t)g.gu * g.gv;
+    DevBuf d_geo, d_range, d_count, d_start, d_cursor, d_ids, d_hv;
     auto release = [&]() {
-        dev_free(ctx, d_geo);
-        dev_free(ctx, d_start);
-        dev_free(ctx, d_ids);
-        dev_free(ctx, d_hv);
+        for (DevBuf *b : {&d_geo, &d_range, &d_count, &d_start, &d_cursor, &d_ids, &d_hv}) dev_free(ctx, *b);
     };
-    s2h_status st = upload(ctx, d_geo, geo.data(), std::max<uint64_t>(ntris, 1) * sizeof(vox::TriGeo));
-    if (st == S2H_OK) st = upload(ctx, d_start, count.data(), count.size() * 4);
-    if (st == S2H_OK) st = upload(ctx, d_ids, ids.data(), ids.size() * 4);
+    s2h_status st = dev_alloc(ctx, d_geo, std::max<uint64_t>(ntris, 1) * sizeof(vox::TriGeo));
+    if (st == S2H_OK) st = dev_alloc(ctx, d_range, std::max<uint64_t>(ntris, 1) * sizeof(int4));
+    if (st == S2H_OK) st = dev_alloc(ctx, d_count, cells * 4);
+    if (st == S2H_OK) st = dev_alloc(ctx, d_start, (cells + 1) * 4);
+    if (st == S2H_OK) st = dev_alloc(ctx, d_cursor, cells * 4);
     if (st == S2H_OK) st = dev_alloc(ctx, d_hv, n * 8);
     if (st != S2H_OK) {
         release();
         return st;
     }
-    g.tris = (const vox::TriGeo *)d_geo.ptr;
-    g.start = (const uint32_t *)d_start.ptr;
-    g.ids = (const uint32_t *)d_ids.ptr;
-    const uint64_t per_block = (uint64_t)kThreads * kPerThread;
-    const uint64_t bpr = (spr + per_block - 1) / per_block;
-    if (bpr * r_max >= (1ull << 31)) {
-        release();
-        return fail(S2H_ERR_UNSUPPORTED, "too many samples");
-    }
     prof_mark(ctx, 0);
-    clear_u64_kernel<<<1, 256, 0, ctx->stream>>>((unsigned long long *)d_hv.ptr, n);
-    meshmc_kernel<<<(unsigned)(bpr * r_max), kThreads, 0, ctx->stream>>>(
-        g, bbox_min[0], bbox_min[1], bbox_min[2], bbox_max[0], bbox_max[1], bbox_max[2], spr, (uint32_t)seed,
-        (uint32_t)(seed >> 32), (uint32_t)bpr, (unsigned long long *)d_hv.ptr);
-    ctx->launches = 2;
+    auto *count = (uint32_t *)d_count.ptr, *start = (uint32_t *)d_start.ptr, *cursor = (uint32_t *)d_cursor.ptr;
+    fill_u32_kernel<<<grid_for(cells), 256, 0, ctx->stream>>>(count, cells);
+    fill_u32_kernel<<<grid_for(cells), 256, 0, ctx->stream>>>(cursor, cells);
+    if (ntris)
+        bin_prepare_kernel<<<grid_for(ntris), 256, 0, ctx->stream>>>(
+            d_xyz, d_tri, ntris, cmax, smax, amax, g.u0, g.v0, g.inv_du, g.inv_dv, g.gu, (vox::TriGeo *)d_geo.ptr,
+            (int4 *)d_range.ptr, count);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        release();
+        return fail(S2H_ERR_HIP, std::string("mesh MC binning: ") + hipGetErrorString(e));
+    }
+    uint64_t total = 0;
+    st = exclusive_sum(ctx, count, start, cells, &total);
+    if (st == S2H_OK && total >= (1ull << 31)) st = fail(S2H_ERR_UNSUPPORTED, "mesh too large for the query grid");
+    if (st == S2H_OK) st = dev_alloc(ctx, d_ids, std::max<uint64_t>(total, 1) * 4);
+    if (st != S2H_OK) {
+        release();
+        return st;
+    }
+    uint32_t tot32 = (uint32_t)total;
+    e = hipMemcpyAsync(start + cells, &tot32, 4, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && ntris)
+        bin_fill_kernel<<<grid_for(ntris), 256, 0, ctx->stream>>>((const int4 *)d_range.ptr, ntris, g.gu, start,
+                                                                   cursor, (uint32_t *)d_ids.ptr);
+    if (e == hipSuccess) e = hipGetLastError();
+    g.tris = (const vox::TriGeo *)d_geo.ptr;
+    g.start = start;
+    g.ids = (const uint32_t *)d_ids.ptr;
     prof_mark(ctx, 1);
+    if (e == hipSuccess) {
+        clear_u64_kernel<<<1, 256, 0, ctx->stream>>>((unsigned long long *)d_hv.ptr, n);
+        meshmc_kernel<<<(unsigned)(bpr * r_max), kThreads, 0, ctx->stream>>>(
+            g, bbox_min[0], bbox_min[1], bbox_min[2], bbox_max[0], bbox_max[1], bbox_max[2], spr, (uint32_t)seed,
+            (uint32_t)(seed >> 32), (uint32_t)bpr, (unsigned long long *)d_hv.ptr);
+        e = hipGetLastError();
+    }
+    ctx->launches = 7;
     prof_mark(ctx, 2);
     if (e == hipSuccess) e = hipMemcpyAsync(hits_valid, d_hv.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -328,6 +358,74 @@ s2h_status s2h_mesh_mc_counts(s2h_ctx *ctx, const double *xyz, uint64_t nverts, 
     release();
     if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("mesh Monte Carlo: ") + hipGetErrorString(e));
     return S2H_OK;
+}
+
+s2h_status check_mc_args(const double bbox_min[3], const double bbox_max[3], uint32_t r_max, uint64_t *hits_valid,
+                         uint64_t n) {
+    if (!bbox_min || !bbox_max || !hits_valid) return fail(S2H_ERR_ARG, "null argument");
+    if (n != 2 * ((uint64_t)r_max + 1)) return fail(S2H_ERR_SIZE, "hits_valid must hold 2*(r_max+1) values");
+    for (int a = 0; a < 3; ++a)
+        if (!(bbox_max[a] >= bbox_min[a]) || !std::isfinite(bbox_min[a]) || !std::isfinite(bbox_max[a]))
+            return fail(S2H_ERR_ARG, "bounding box must be finite with max >= min");
+    return S2H_OK;
+}
+
+}  // namespace meshmc
+}  // namespace s2h
+
+using namespace s2h;
+
+extern "C" {
+
+s2h_status s2h_mesh_mc_counts(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,
+                              const double bbox_min[3], const double bbox_max[3], uint32_t r_max, uint64_t samples,
+                              uint64_t seed, uint64_t *hits_valid, uint64_t n) {
+    using namespace s2h::meshmc;
+    if (!ctx || (ntris && (!xyz || !tri)) || (nverts && !xyz)) return fail(S2H_ERR_ARG, "null argument");
+    S2H_TRY(check_mc_args(bbox_min, bbox_max, r_max, hits_valid, n));
+    for (uint64_t f = 0; f < 3 * ntris; ++f)
+        if (tri[f] >= nverts) return fail(S2H_ERR_ARG, "vertex index out of range");
+    if (ntris > 0x7FFFFFFFull) return fail(S2H_ERR_UNSUPPORTED, "mesh too large");
+    S2H_HIP(hipSetDevice(ctx->device));
+    std::fill(hits_valid, hits_valid + n, 0);
+    if (r_max == 0) return S2H_OK;
+    // whole-mesh vertex bbox (mesh_bbox, src/geometry/bbox.rs:8-25) and amax
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0}, amax = 0.0;
+    for (uint64_t v = 0; v < nverts; ++v)
+        for (int a = 0; a < 3; ++a) {
+            const double c = xyz[3 * v + a];
+            mn[a] = v ? vox::rmin(mn[a], c) : c;
+            mx[a] = v ? vox::rmax(mx[a], c) : c;
+            amax = std::max(amax, std::fabs(c));
+        }
+    Grid g{};
+    double cmax, smax;
+    setup_grid(bbox_min, bbox_max, nverts > 0, mn, mx, ntris, g, cmax, smax);
+    DevBuf d_xyz, d_tri;
+    s2h_status st = upload(ctx, d_xyz, xyz, std::max<uint64_t>(nverts, 1) * 24);
+    if (st == S2H_OK) st = upload(ctx, d_tri, tri, std::max<uint64_t>(ntris, 1) * 12);
+    if (st == S2H_OK)
+        st = bin_and_run(ctx, g, cmax, smax, amax, (const double *)d_xyz.ptr, (const uint32_t *)d_tri.ptr, ntris,
+                         bbox_min, bbox_max, r_max, samples, seed, hits_valid, n);
+    dev_free(ctx, d_xyz);
+    dev_free(ctx, d_tri);
+    return st;
+}
+
+s2h_status s2h_mesh_mc_counts_resident(s2h_ctx *ctx, const s2h_mesh *m, const double bbox_min[3],
+                                       const double bbox_max[3], uint32_t r_max, uint64_t samples, uint64_t seed,
+                                       uint64_t *hits_valid, uint64_t n) {
+    using namespace s2h::meshmc;
+    if (!ctx || !m) return fail(S2H_ERR_ARG, "null argument");
+    S2H_TRY(check_mc_args(bbox_min, bbox_max, r_max, hits_valid, n));
+    S2H_HIP(hipSetDevice(ctx->device));
+    std::fill(hits_valid, hits_valid + n, 0);
+    if (r_max == 0) return S2H_OK;
+    Grid g{};
+    double cmax, smax;
+    setup_grid(bbox_min, bbox_max, m->nverts > 0, m->bmin, m->bmax, m->ntris, g, cmax, smax);
+    return bin_and_run(ctx, g, cmax, smax, m->amax, (const double *)m->xyz.ptr, (const uint32_t *)m->tri.ptr, m->ntris,
+                       bbox_min, bbox_max, r_max, samples, seed, hits_valid, n);
 }
 
 s2h_status s2h_mesh_mc_s2(s2h_ctx *ctx, const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,

@@ -84,6 +84,25 @@ class DeviceMesh:
                                           L.ptr(_dims(dims), L._u64p), C.byref(h), C.byref(occ)))
         return Volume(self.ctx, h)
 
+    def set_vertices(self, vstart: int, xyz) -> None:
+        """Move vertices [vstart, vstart + len(xyz)) (a rigid particle move)."""
+        v = np.ascontiguousarray(np.asarray(xyz, np.float64).reshape(-1, 3))
+        L.check(L.lib().s2h_mesh_set_vertices(self.ctx.handle, self.handle, int(vstart), L.ptr(v.reshape(-1), L._f64p),
+                                              len(v)))
+        bmin, bmax = np.zeros(3), np.zeros(3)
+        L.check(L.lib().s2h_mesh_info(self.handle, None, None, None, L.ptr(bmin, L._f64p), L.ptr(bmax, L._f64p)))
+        self.bbox = BoundingBox(tuple(bmin.tolist()), tuple(bmax.tolist())) if self.nverts else None
+
+    def mesh_mc_counts(self, bbox: BoundingBox, r_max: int, samples: int, seed: int) -> np.ndarray:
+        """Continuous mesh MC (hits, valid) per radius on the resident mesh."""
+        out = np.zeros(2 * (r_max + 1), np.uint64)
+        bmin = np.asarray(bbox.min, np.float64)
+        bmax = np.asarray(bbox.max, np.float64)
+        L.check(L.lib().s2h_mesh_mc_counts_resident(self.ctx.handle, self.handle, L.ptr(bmin, L._f64p),
+                                                    L.ptr(bmax, L._f64p), int(r_max), int(samples),
+                                                    int(seed) & 0xFFFFFFFFFFFFFFFF, L.ptr(out, L._u64p), len(out)))
+        return out.reshape(-1, 2)
+
     def free(self):
         if self.handle:
             L.lib().s2h_mesh_free(self.ctx.handle, self.handle)

@@ -93,3 +93,41 @@ def test_packing_against_voxel_exact(ctx):
     for r in range(1, 9):
         sig = math.sqrt(max(ex[r] * (1 - ex[r]), 1e-12) / n)
         assert abs(mc[r] - ex[r]) <= 4 * sig + 0.1 * ex[r], (r, mc[r], ex[r])
+
+
+def test_resident_mesh_counts_equal_host_entry_and_oracle(ctx):
+    from rustmspt_amd.devmesh import DeviceMesh
+    mesh = spheres([(10, 10, 10), (13, 11, 10), (4, 16, 15)], [5.0, 4.0, 3.0])   # overlap and a separate one
+    bbox = BoundingBox(np.zeros(3), np.full(3, 22.0))
+    dm = DeviceMesh.from_mesh(ctx, mesh)
+    got = dm.mesh_mc_counts(bbox, 6, 4000, seed=13)
+    assert np.array_equal(got, s2.mesh_mc_counts(ctx, mesh, bbox, 6, 4000, seed=13))
+    assert np.array_equal(got, o.mesh_mc_counts(mesh.vertices, mesh.faces, bbox.min, bbox.max, 6, 4000, 13))
+    dm.free()
+
+
+def test_rigid_moves_through_set_vertices(ctx):
+    # an optimize-style sequence: move one particle of a resident packing at a
+    # time and re-evaluate; each result equals the host-array entry on the moved mesh
+    from rustmspt_amd.devmesh import DeviceMesh
+    parts = [icosphere_mesh(np.asarray(c, np.float64), r, 2) for c, r in
+             [((5, 5, 5), 3.0), ((13, 6, 9), 2.5), ((8, 14, 12), 3.5), ((15, 15, 4), 2.0)]]
+    mesh = merge_meshes(parts)
+    starts = np.cumsum([0] + [len(p.vertices) for p in parts])
+    bbox = BoundingBox(np.zeros(3), np.full(3, 20.0))
+    dm = DeviceMesh.from_mesh(ctx, mesh)
+    rng = np.random.default_rng(2)
+    verts = mesh.vertices.copy()
+    for step in range(4):
+        i = int(rng.integers(0, len(parts)))
+        shift = rng.uniform(-2.0, 2.0, 3)
+        sl = slice(starts[i], starts[i + 1])
+        verts[sl] = verts[sl] + shift
+        dm.set_vertices(int(starts[i]), verts[sl])
+        moved = type(mesh)(verts.copy(), mesh.faces)
+        assert np.array_equal(dm.mesh_mc_counts(bbox, 5, 3000, seed=step),
+                              s2.mesh_mc_counts(ctx, moved, bbox, 5, 3000, seed=step))
+        assert dm.bbox.min == tuple(np.fmin.reduce(verts, axis=0).tolist())
+    with pytest.raises(Exception, match="vertex range outside"):
+        dm.set_vertices(len(verts) - 2, np.zeros((3, 3)))
+    dm.free()
