@@ -210,6 +210,14 @@ s2h_status s2h_shell_count(const uint64_t dims[3], uint32_t r_max, double pitch,
 s2h_status s2h_shell_offsets(const uint64_t dims[3], uint32_t r_max, double pitch, int64_t *offs,
                              uint64_t n_offsets, uint64_t *radius_start, uint64_t n_radius_start);
 
+/* The reference's ordered f64 fold, sum = sum + t_k left to right
+ * (src/geometry/s2.rs:1187-1201), of each segment [starts[g], starts[g+1]) of
+ * terms, computed on the device by the same code as the exact curve's fold
+ * (64 terms per step on the exact binade grid; bit-identical to the sequential
+ * sum).  starts has nseg+1 entries. */
+s2h_status s2h_ordered_sum(s2h_ctx *ctx, const double *terms, uint64_t n, const uint64_t *starts, uint32_t nseg,
+                           double *sums);
+
 /* ------------------------------------------------------------------ exact */
 
 /* Exact pair counts ("hits") of every in-domain offset in shell order (replaces

@@ -47,6 +47,7 @@ SIGNATURES = {
     "s2h_mesh_set_vertices": (C.c_int32, [_vp, _vp, C.c_uint64, _f64p, C.c_uint64]),
     "s2h_mesh_mc_counts_resident": (C.c_int32, [_vp, _vp, _f64p, _f64p, C.c_uint32, C.c_uint64, C.c_uint64, _u64p,
                                                 C.c_uint64]),
+    "s2h_ordered_sum": (C.c_int32, [_vp, _f64p, C.c_uint64, _u64p, C.c_uint32, _f64p]),
     "s2h_volume_probe": (C.c_int32, [_vp, _vp]),
     "s2h_volume_read": (C.c_int32, [_vp, _vp, C.c_uint64, _vp]),
     "s2h_vol_from_samples": (C.c_int32, [_vp, _vp, C.c_uint64, _vp, _vp, C.POINTER(_vp), _i64p, _u64p]),

@@ -144,35 +144,88 @@ __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32
     terms[i] = h / (double)valid;
 }
 
-// One workgroup per radius: stage the radius' terms through LDS and fold them
-// in shell order on lane 0 (the reference's sequential left fold).
-static constexpr int kFoldChunk = 2048;
+// One wave per radius folds the radius' terms in shell order: the reference's
+// sequential left fold, bit for bit, 64 terms per step (fold_chunk).  Terms
+// stream from HBM in groups of kFoldGroup chunks, the next group's loads in
+// flight while the current group folds.  After a negative or non-finite term
+// the rest of the radius folds one term at a time.
+static constexpr int kFoldK = 8;     // terms per lane per block (512 per block)
+static constexpr int kFoldAhead = 2; // blocks loaded per step: the next step's loads overlap this step
 
-__global__ __launch_bounds__(256) void fold_kernel(const double *__restrict__ terms,
-                                                   const uint64_t *__restrict__ rstart, uint32_t r_max,
-                                                   double *__restrict__ values, uint8_t *__restrict__ support) {
-    __shared__ double buf[kFoldChunk];
+// The ordered f64 sum of t[0 .. n) on one wave (all 64 lanes call it; the
+// result is wave-uniform).  Loads run kFoldAhead blocks ahead of the fold.
+__device__ __forceinline__ double fold_segment(const double *__restrict__ t, uint64_t n, uint32_t lane) {
+    constexpr uint64_t B = 64 * kFoldK, SB = B * kFoldAhead;
+    FoldState st;
+    if (n == 0) return 0.0;
+    bool seq = false;
+    double cur[kFoldAhead][kFoldK], nxt[kFoldAhead][kFoldK];
+    // loads past the end read t[0] (always valid) and are replaced by 0
+#pragma unroll
+    for (int b = 0; b < kFoldAhead; ++b)
+#pragma unroll
+        for (int i = 0; i < kFoldK; ++i) {
+            const uint64_t j = b * B + (uint64_t)lane * kFoldK + i;
+            const double v = t[j < n ? j : 0];
+            cur[b][i] = j < n ? v : 0.0;
+        }
+    for (uint64_t s0 = 0; s0 < n; s0 += SB) {
+#pragma unroll
+        for (int b = 0; b < kFoldAhead; ++b)
+#pragma unroll
+            for (int i = 0; i < kFoldK; ++i) {
+                const uint64_t j = s0 + SB + b * B + (uint64_t)lane * kFoldK + i;
+                const double v = t[j < n ? j : 0];
+                nxt[b][i] = j < n ? v : 0.0;
+            }
+#pragma unroll
+        for (int b = 0; b < kFoldAhead; ++b) {
+            const uint64_t b0 = s0 + b * B;
+            if (b0 >= n) break;
+            const uint32_t cnt = (n - b0) < B ? (uint32_t)(n - b0) : (uint32_t)B;
+            bool bad = false;
+#pragma unroll
+            for (int i = 0; i < kFoldK; ++i)
+                bad = bad || (lane * kFoldK + i < cnt && !(cur[b][i] >= 0.0 && cur[b][i] <= 1.7976931348623157e308));
+            if (!seq && __ballot(bad)) seq = true;
+            if (seq) {
+                // one term at a time, in order (lane l's terms are l*K .. l*K+K-1)
+                for (uint32_t l = 0; l * kFoldK < cnt; ++l)
+#pragma unroll
+                    for (int i = 0; i < kFoldK; ++i)
+                        if (l * kFoldK + i < cnt) st.S = st.S + read_lane_f64(cur[b][i], (int)l);
+            } else {
+                fold_block<kFoldK>(st, cur[b], cnt, lane);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kFoldAhead; ++b)
+#pragma unroll
+            for (int i = 0; i < kFoldK; ++i) cur[b][i] = nxt[b][i];
+    }
+    return st.S;
+}
+
+__global__ __launch_bounds__(64) void fold_kernel(const double *__restrict__ terms,
+                                                  const uint64_t *__restrict__ rstart, uint32_t r_max,
+                                                  double *__restrict__ values, uint8_t *__restrict__ support) {
     const uint32_t r = blockIdx.x + 1;
     if (r > r_max) return;
-    const uint64_t s = rstart[r], e = rstart[r + 1];
-    double sum = 0.0;
-    for (uint64_t c0 = s; c0 < e; c0 += kFoldChunk) {
-        const uint64_t len = (e - c0) < (uint64_t)kFoldChunk ? (e - c0) : (uint64_t)kFoldChunk;
-        __syncthreads();
-        for (uint64_t k = threadIdx.x; k < len; k += blockDim.x) buf[k] = terms[c0 + k];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (uint64_t k = 0; k < len; ++k) sum = sum + buf[k];
-        }
-    }
+    const uint64_t s = rstart[r], n = rstart[r + 1] - s;
+    const double sum = fold_segment(terms + s, n, threadIdx.x);
     if (threadIdx.x == 0) {
-        if (e > s) {
-            values[r] = sum / (double)(e - s);
-            support[r] = 1;
-        } else {
-            values[r] = 0.0;
-            support[r] = 0;
-        }
+        values[r] = n > 0 ? sum / (double)n : 0.0;
+        support[r] = n > 0 ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(64) void segment_sum_kernel(const double *__restrict__ terms,
+                                                         const uint64_t *__restrict__ starts, uint32_t nseg,
+                                                         double *__restrict__ sums) {
+    for (uint32_t g = blockIdx.x; g < nseg; g += gridDim.x) {
+        const uint64_t s = starts[g];
+        const double sum = fold_segment(terms + s, starts[g + 1] - s, threadIdx.x);
+        if (threadIdx.x == 0) sums[g] = sum;
     }
 }
 
@@ -714,7 +767,7 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, cons
     double *values = (double *)p->d_curve.ptr;
     uint8_t *support = (uint8_t *)(values + p->r_max + 1);
     if (p->r_max > 0) {
-        fold_kernel<<<p->r_max, 256, 0, ctx->stream>>>((const double *)p->d_terms.ptr,
+        fold_kernel<<<p->r_max, 64, 0, ctx->stream>>>((const double *)p->d_terms.ptr,
                                                        (const uint64_t *)p->d_rstart.ptr, p->r_max,
                                                        values, support);
         S2H_HIP(hipGetLastError());
@@ -823,3 +876,30 @@ s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
 }
 
 }  // extern "C"
+
+extern "C" s2h_status s2h_ordered_sum(s2h_ctx *ctx, const double *terms, uint64_t n, const uint64_t *starts,
+                                      uint32_t nseg, double *sums) {
+    if (!ctx || !starts || !sums || (n && !terms)) return fail(S2H_ERR_ARG, "null argument");
+    for (uint32_t g = 0; g < nseg; ++g)
+        if (starts[g] > starts[g + 1] || starts[g + 1] > n) return fail(S2H_ERR_ARG, "segment bounds out of order");
+    if (nseg == 0) return S2H_OK;
+    S2H_HIP(hipSetDevice(ctx->device));
+    DevBuf d_t, d_s, d_o;
+    s2h_status st = upload(ctx, d_t, terms, std::max<uint64_t>(n, 1) * 8);
+    if (st == S2H_OK) st = upload(ctx, d_s, starts, ((uint64_t)nseg + 1) * 8);
+    if (st == S2H_OK) st = dev_alloc(ctx, d_o, (uint64_t)nseg * 8);
+    hipError_t e = hipSuccess;
+    if (st == S2H_OK) {
+        segment_sum_kernel<<<std::min<uint32_t>(nseg, 4096), 64, 0, ctx->stream>>>(
+            (const double *)d_t.ptr, (const uint64_t *)d_s.ptr, nseg, (double *)d_o.ptr);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(sums, d_o.ptr, (uint64_t)nseg * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    }
+    dev_free(ctx, d_t);
+    dev_free(ctx, d_s);
+    dev_free(ctx, d_o);
+    if (st != S2H_OK) return st;
+    if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("ordered sum: ") + hipGetErrorString(e));
+    return S2H_OK;
+}

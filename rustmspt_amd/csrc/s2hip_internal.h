@@ -63,6 +63,149 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
     out[3] = c3;
 }
 
+// ---------------------------------------------------------------- ordered fold
+//
+// The reference's curve is a sequential left fold of f64 terms, sum = sum + t_k
+// in shell order (src/geometry/s2.rs:1187-1201): a chain of dependent adds that
+// one lane runs at the add latency.  For non-negative terms the same bits come
+// from a wave 64 terms at a time.  While the running sum S lies in one binade
+// [2^m, 2^(m+1)), S = a*u with u = 2^(m-52) and an integer a in [2^52, 2^53],
+// and for t >= 0 with S + t < 2^(m+1) round-to-nearest gives
+//   RN(S + t) = (a + n + [f > 1/2]) * u,   t/u = n + f, n integer, 0 <= f < 1,
+// independent of S except on exact ties (f = 1/2, where the even neighbour
+// depends on a).  So the grid increments q = n + [f > 1/2] of a block of terms
+// are summed with a wave prefix sum, exactly, in integers.  The first term whose
+// exact sum reaches the next binade (a + n >= 2^53), and any tie, is added
+// with the hardware add from the exact S before it, and the grid restarts at
+// the new binade.  Every step therefore rounds exactly as the sequential
+// fold.  A segment with a negative or non-finite term is folded sequentially.
+
+// DPP move of a double (both halves move alike); disabled rows and lanes
+// without a source read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xf, true);
+    return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// Exclusive prefix sum over the 64 lanes: a wave shift right by one lane,
+// then row shifts 1, 2, 4, 8 within rows of 16 and the row_bcast:15 and
+// row_bcast:31 carries into rows 1, 3 and 2, 3.  Lane l sums only lanes below
+// l, so it is exact for non-negative integer values while those sums stay
+// below 2^53.
+__device__ __forceinline__ double wave_exclusive_sum_f64(double x) {
+    x = dpp_f64<0x138, 0xf>(x);
+    x += dpp_f64<0x111, 0xf>(x);
+    x += dpp_f64<0x112, 0xf>(x);
+    x += dpp_f64<0x114, 0xf>(x);
+    x += dpp_f64<0x118, 0xf>(x);
+    x += dpp_f64<0x142, 0xa>(x);
+    x += dpp_f64<0x143, 0xc>(x);
+    return x;
+}
+
+// A wave-uniform read of lane l's double.
+__device__ __forceinline__ double read_lane_f64(double x, int l) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+struct FoldState {
+    double S = 0.0;   // the exact running sum (wave-uniform)
+    double a = 0.0;   // S / u while S > 0: an integer in [2^52, 2^53]
+    double u = 0.0, inv_u = 0.0;
+    __device__ void rebase() {
+        if (S > 0.0) {
+            const int m = ilogb(S);
+            u = ldexp(1.0, m - 52);
+            inv_u = ldexp(1.0, 52 - m);
+            a = S * inv_u;
+        }
+    }
+};
+
+// Fold a block of 64*K terms into st: lane l holds terms l*K .. l*K+K-1 of the
+// block in t[], and the first cnt terms of the block are active.  Each lane sums
+// its run's grid increments; one wave prefix sum places the runs.  The
+// increments and their sums are integers held in doubles, exact below 2^53
+// (every sum used is below it; once a sum reaches it, rounding is monotone, so
+// the comparison with 2^53 is still decided right).  All 64 lanes call this
+// with the same cnt.
+template <int K>
+__device__ __forceinline__ void fold_block(FoldState &st, const double (&t)[K], uint32_t cnt, uint32_t lane) {
+    constexpr double kTwo53 = 9007199254740992.0;
+    uint32_t pos = 0;
+    const uint32_t j0 = lane * K;  // block index of t[0]
+    while (pos < cnt) {
+        if (st.S == 0.0) {
+            // 0 + t is t exactly; zero terms leave the sum at zero
+            int first = -1;
+#pragma unroll
+            for (int i = K - 1; i >= 0; --i)
+                if (j0 + i >= pos && j0 + i < cnt && t[i] != 0.0) first = i;
+            const uint64_t m = __ballot(first >= 0);
+            if (!m) return;
+            const int l = __ffsll((long long)m) - 1;
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+                if (i == first) v = t[i];
+            const int fi = __builtin_amdgcn_readlane(first, l);
+            st.S = read_lane_f64(v, l);
+            st.rebase();
+            pos = (uint32_t)l * K + (uint32_t)fi + 1;
+            continue;
+        }
+        double q[K], nfl[K];
+        bool flag[K], act[K];
+        double T = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            act[i] = j0 + i >= pos && j0 + i < cnt;
+            const double y = act[i] ? t[i] * st.inv_u : 0.0;  // exact: scaling by a power of two
+            const bool big = !(y < kTwo53);                     // y >= 2^53: certainly the next binade
+            nfl[i] = big ? 0.0 : floor(y);
+            const double f = y - nfl[i];
+            q[i] = nfl[i] + (f > 0.5 ? 1.0 : 0.0);
+            flag[i] = act[i] && (big || f == 0.5);
+            T += q[i];
+        }
+        double run = wave_exclusive_sum_f64(T);  // increments of the lanes before
+        int stop = -1;
+        double spex = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            if (stop < 0 && act[i] && (flag[i] || (st.a + run) + nfl[i] >= kTwo53)) {
+                stop = i;
+                spex = run;
+            }
+            run += q[i];
+        }
+        const uint64_t sm = __ballot(stop >= 0);
+        if (!sm) {
+            st.a += read_lane_f64(run, 63);  // run = all increments of the block at lane 63
+            st.S = st.a * st.u;
+            if (st.a >= kTwo53) st.rebase();  // rounded up to exactly 2^(m+1)
+            return;
+        }
+        const int l = __ffsll((long long)sm) - 1;
+        double v = 0.0;
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+            if (i == stop) v = t[i];
+        const int si = __builtin_amdgcn_readlane(stop, l);
+        st.a += read_lane_f64(spex, l);  // the terms before the stopping one, on the grid
+        st.S = st.a * st.u;
+        st.S = st.S + read_lane_f64(v, l);  // the stopping term: one hardware add
+        st.rebase();
+        pos = (uint32_t)l * K + (uint32_t)si + 1;
+    }
+}
+
 // ---------------------------------------------------------------- shells (host)
 struct Offset {
     int32_t dx, dy, dz;

@@ -307,6 +307,17 @@ def geometric_volume_fraction(mesh: Mesh, bbox: BoundingBox) -> Optional[float]:
     return min(max(total / box_vol, 0.0), 1.0)
 
 
+def ordered_sum(ctx: Context, terms, starts) -> np.ndarray:
+    """Left-to-right f64 sums of the segments [starts[g], starts[g+1]) of terms,
+    folded on the device by the exact curve's fold (s2h_ordered_sum)."""
+    t = np.ascontiguousarray(terms, np.float64)
+    st = np.ascontiguousarray(starts, np.uint64)
+    out = np.zeros(max(len(st) - 1, 0), np.float64)
+    L.check(L.lib().s2h_ordered_sum(ctx.handle, L.ptr(t, L._f64p), len(t), L.ptr(st, L._u64p), len(out),
+                                    L.ptr(out, L._f64p)))
+    return out
+
+
 def mesh_mc_counts(ctx: Context, mesh: Mesh, bbox: BoundingBox, r_max: int, samples: int, seed: int) -> np.ndarray:
     """Continuous mesh Monte Carlo (hits, valid) per radius (s2h_mesh_mc_counts)."""
     out = np.zeros(2 * (int(r_max) + 1), np.uint64)
