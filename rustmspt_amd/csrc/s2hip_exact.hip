@@ -145,10 +145,9 @@ __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32
 }
 
 // One wave per radius folds the radius' terms in shell order: the reference's
-// sequential left fold, bit for bit, 64 terms per step (fold_chunk).  Terms
-// stream from HBM in groups of kFoldGroup chunks, the next group's loads in
-// flight while the current group folds.  After a negative or non-finite term
-// the rest of the radius folds one term at a time.
+// sequential left fold, bit for bit, 512 terms per step (fold_block).  Terms
+// stream from HBM kFoldAhead blocks ahead of the fold.  After a negative or
+// non-finite term the rest of the radius folds one term at a time.
 static constexpr int kFoldK = 8;     // terms per lane per block (512 per block)
 static constexpr int kFoldAhead = 2; // blocks loaded per step: the next step's loads overlap this step
 
