@@ -14,13 +14,16 @@
 //  2. for every row crossing B, the added bits A = new & ~old and the removed
 //     bits D = old & ~new (new bit = coverage + delta > 0), and each row's
 //     z-span of changes;
-//  3. exact delta counts.  With n = o + A - D,
-//        C'(d) - C(d) = X_A(d) + X_A(-d) - X_D(d) - X_D(-d) + Y(d),
-//        X_M(d) = sum_u o(u) M(u + d),   Y(d) = sum_u (A - D)(u) (A - D)(u + d).
-//     A wave takes one changed row q (wave-uniform: its A and D words shifted
-//     by dz on the scalar unit) and 64 half-ball columns (dx, dy) as lanes,
-//     which read the unchanged seed rows q -/+ (dx, dy) -- so only the words
-//     inside q's own change span are visited, and unchanged rows are skipped;
+//  3. exact delta counts.  With n = o + A - D and delta = A - D,
+//        n(u) n(u+d) - o(u) o(u+d) = n(u) delta(u+d) + delta(u) o(u+d),
+//     so C'(d) - C(d) = sum_u n(u) (A - D)(u + d) + sum_u (A - D)(u) o(u + d):
+//     one pass over each changed row's partners on each side, with no
+//     separate changed-changed term.  A wave takes one changed row q
+//     (wave-uniform: its A and D words shifted by dz on the scalar unit) and 64
+//     half-ball columns (dx, dy) as lanes, which read the rows q - (dx, dy) of
+//     the new occupancy n (the seed row, with A and D applied inside the box)
+//     and q + (dx, dy) of the seed -- so only the words inside q's own change
+//     span are visited, and unchanged rows are skipped;
 //  4. C' = C + dC, then the reference's ordered f64 fold per radius -> curves.
 // All integer arithmetic, so every curve is bit-identical to a full evaluation
 // of the candidate packing.
@@ -249,33 +252,29 @@ __device__ void delta_item(const SeedDev &s, const CandDev &c, const GB gb, int 
         const int zlo = s.nz - (int)span.x, zhi = (int)span.y;
         const int xq = c.bx0 + r / sy, yq = c.by0 + r % sy;
         const uint2 *m = ad + c.ad_off + (uint64_t)r * nwd;  // uniform (A, D) row of q
-        // ---- X(+d) and Y: lanes u = q - (dx, dy); M(q) >> dz meets u at z in [zlo - dz, zhi - dz)
+        // ---- X(+d) on the new occupancy: lanes u = q - (dx, dy) read n(u) = (o & ~D) | A
+        // (n = o outside the box); M(q) >> dz meets u at z in [zlo - dz, zhi - dz)
         {
             const int xu = xq - col.dx, yu = yq - col.dy;
             const bool in = active && xu >= 0 && xu < s.nx && yu >= 0 && yu < s.ny;
             const int aoff = in ? (int)(((uint64_t)xu * s.ny + yu) * s.row_dw * 4) : oob;
             const bool inbox = in && xu >= c.bx0 && xu < c.bx1 && yu >= c.by0 && yu < c.by1;
             const int uoff = inbox ? (int)((c.ad_off + ((uint64_t)(xu - c.bx0) * sy + (yu - c.by0)) * nwd) * 8) : oob;
-            const bool any_y = __any(inbox);
+            const bool any_box = __any(inbox);
             const int w_lo = max(0, zlo - dz0 - DZB + 1) >> 5, w_hi = min(nwd, (zhi - dz0 + 31) >> 5);
             for (int w = w_lo; w < w_hi; ++w) {
-                const uint32_t a = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, aoff + 4 * w, 0, 0);
+                uint32_t a = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, aoff + 4 * w, 0, 0);
+                if (any_box) {  // lanes outside the box read (0, 0): n = o
+                    const uint32_t ua = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w, 0, 0);
+                    const uint32_t ud = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w + 4, 0, 0);
+                    a = (a & ~ud) | ua;
+                }
                 const uint2 m0 = adw(m, w + q0, nwd), m1 = adw(m, w + q0 + 1, nwd);
                 const uint64_t pa = ((uint64_t)m1.x << 32) | m0.x, pd = ((uint64_t)m1.y << 32) | m0.y;
 #pragma unroll
                 for (int j = 0; j < DZB; ++j) {
                     cp[j] = popc_acc(a & (uint32_t)(pa >> j), cp[j]);
                     cm[j] = popc_acc(a & (uint32_t)(pd >> j), cm[j]);
-                }
-                if (any_y) {
-                    const uint32_t ua = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w, 0, 0);
-                    const uint32_t ud = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w + 4, 0, 0);
-#pragma unroll
-                    for (int j = 0; j < DZB; ++j) {
-                        const uint32_t sa = (uint32_t)(pa >> j), sd = (uint32_t)(pd >> j);
-                        cp[j] = popc_acc(ua & sa, popc_acc(ud & sd, cp[j]));
-                        cm[j] = popc_acc(ua & sd, popc_acc(ud & sa, cm[j]));
-                    }
                 }
             }
         }
