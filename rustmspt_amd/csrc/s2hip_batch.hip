@@ -25,6 +25,10 @@
 //     and q + (dx, dy) of the seed -- so only the words inside q's own change
 //     span are visited, and unchanged rows are skipped;
 //  4. C' = C + dC, then the reference's ordered f64 fold per radius -> curves.
+//     Delta counts and the seed's counts are kept in canonical shell order
+//     (each radius' canonical offsets contiguous, in order of first
+//     appearance in the reference's shell order), so a radius' fold reads one
+//     contiguous slice of each instead of gathering across the whole table.
 // All integer arithmetic, so every curve is bit-identical to a full evaluation
 // of the candidate packing.
 //
@@ -52,8 +56,9 @@ struct GB {            // (column group, dz block) of a seed's exact plan
 struct SeedDev {
     const uint32_t *vol;        // packed seed rows (dwords)
     const uint32_t *cov;        // granule coverage count per voxel
-    const unsigned long long *table;  // seed counts, dense canonical table
-    const int32_t *gather;      // shell order -> table index (-1: zero offset)
+    const unsigned long long *shtable;  // seed counts in canonical shell order
+    const int32_t *sg;          // shell order -> canonical shell index (-1: zero offset)
+    const int32_t *smap;        // count-table index -> canonical shell index (ncanon: no shell)
     const Offset *offs;         // shell order offsets
     const uint64_t *rstart;     // r_max + 2
     const ColInfo *cols;
@@ -70,7 +75,7 @@ struct CandDev {
     uint64_t ad_off;                         // uint2 offset of the (A, D) words (box rows x nwd)
     uint64_t info_off;                       // uint2 offset of the row change spans (box rows)
     uint64_t box_off;                        // int32 offset of the coverage delta box
-    uint64_t delta_off;                      // int32 offset of the delta table
+    uint64_t delta_off;                      // int32 offset of the delta table (canonical shell order)
     uint64_t item_start;                     // first delta item of this candidate
     uint64_t curve_off;                      // double offset of the curve values
 };
@@ -301,7 +306,7 @@ __device__ void delta_item(const SeedDev &s, const CandDev &c, const GB gb, int 
     for (int j = 0; j < DZB; ++j) {
         const int dz = dz0 + j;
         const int d = (int)(cp[j] - cm[j]);
-        if (active && dz <= col.m && d) atomicAdd(&dt[col.base + dz], d);
+        if (active && dz <= col.m && d) atomicAdd(&dt[s.smap[col.base + dz]], d);
     }
 }
 
@@ -343,55 +348,53 @@ __global__ __launch_bounds__(64 * kDeltaWaves) void delta_count_kernel(const See
 
 // ---------------------------------------------------------------- 4. gather + ordered fold
 
-static constexpr int kFoldChunkB = 1024;
+static constexpr int kFoldWaves = 4;
 
-// One workgroup per (candidate, radius): terms are produced in parallel into LDS
-// (hits = seed count + delta, over analytic valid) and folded in shell order by
-// lane 0, the reference's sequential left fold.
-__global__ __launch_bounds__(256) void batch_fold_kernel(const SeedDev *__restrict__ seeds,
-                                                         const CandDev *__restrict__ cands,
-                                                         const BatchHeader *__restrict__ hdr,
-                                                         const uint32_t *__restrict__ fold_prefix,
-                                                         const int32_t *__restrict__ delta,
-                                                         const int *__restrict__ occ_delta,
-                                                         double *__restrict__ curves, uint8_t *__restrict__ support) {
-    __shared__ double buf[kFoldChunkB];
-    const uint32_t ncand = hdr->ncand, nfold = hdr->nfold;
-    for (uint32_t task = blockIdx.x; task < nfold; task += gridDim.x) {
-        uint32_t lo = 0, hi = ncand;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (fold_prefix[mid] <= task) lo = mid;
-            else hi = mid;
-        }
-        const CandDev &c = cands[lo];
+struct FoldTask {
+    uint32_t cand, r;
+};
+
+// One wave per (candidate, radius) task: term j of the radius is
+// (seed count + delta) / valid of the radius' j-th offset in shell order, and
+// the terms are folded in that order -- bit for bit the reference's sequential
+// left fold (fold_terms).  The host orders the tasks by seed, then radius
+// (largest first), then candidate, so waves running together share a radius'
+// shell data in L2.
+__global__ __launch_bounds__(64 * kFoldWaves) void batch_fold_kernel(const SeedDev *__restrict__ seeds,
+                                                                     const CandDev *__restrict__ cands,
+                                                                     const BatchHeader *__restrict__ hdr,
+                                                                     const FoldTask *__restrict__ tasks,
+                                                                     const int32_t *__restrict__ delta,
+                                                                     const int *__restrict__ occ_delta,
+                                                                     double *__restrict__ curves,
+                                                                     uint8_t *__restrict__ support) {
+    const uint32_t nfold = hdr->nfold, lane = threadIdx.x & 63;
+    const uint32_t nwaves = gridDim.x * kFoldWaves;
+    for (uint32_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kFoldWaves + threadIdx.x / 64)); t < nfold;
+         t += nwaves) {
+        const FoldTask tk = tasks[t];
+        const CandDev &c = cands[tk.cand];
         const SeedDev &s = seeds[c.seed];
-        const uint32_t r = task - fold_prefix[lo] + 1;
-        const uint64_t b = s.rstart[r], e = s.rstart[r + 1];
-        const int32_t *dt = delta + c.delta_off;
-        const uint64_t occ = (uint64_t)((int64_t)s.occupied + occ_delta[lo]);
-        double sum = 0.0;
-        for (uint64_t c0 = b; c0 < e; c0 += kFoldChunkB) {
-            const uint64_t len = min((uint64_t)kFoldChunkB, e - c0);
-            __syncthreads();
-            for (uint64_t k = threadIdx.x; k < len; k += blockDim.x) {
-                const uint64_t i = c0 + k;
-                const Offset o = s.offs[i];
-                const int32_t g = s.gather[i];
-                const uint64_t h = g >= 0 ? (uint64_t)((int64_t)s.table[g] + dt[g]) : occ;
-                const uint64_t valid = (uint64_t)(s.nx - abs(o.dx)) * (uint64_t)(s.ny - abs(o.dy)) *
-                                       (uint64_t)(s.nz - abs(o.dz));
-                buf[k] = (double)h / (double)valid;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0)
-                for (uint64_t k = 0; k < len; ++k) sum = sum + buf[k];
+        const uint64_t b = s.rstart[tk.r], n = s.rstart[tk.r + 1] - b;
+        const int32_t *__restrict__ dt = delta + c.delta_off;
+        const unsigned long long *__restrict__ cnt = s.shtable;
+        const int32_t *__restrict__ sg = s.sg + b;
+        const Offset *__restrict__ offs = s.offs + b;
+        const uint64_t occ = (uint64_t)((int64_t)s.occupied + occ_delta[tk.cand]);
+        const uint64_t nx = (uint64_t)s.nx, ny = (uint64_t)s.ny, nz = (uint64_t)s.nz;
+        const double sum = fold_terms<8, 1>(n, lane, [&](uint64_t j) {
+            const int32_t k = sg[j];
+            const Offset o = offs[j];
+            const int32_t kk = k >= 0 ? k : 0;  // the zero offset counts every occupied voxel
+            const uint64_t hk = (uint64_t)((long long)cnt[kk] + dt[kk]);
+            const uint64_t h = k >= 0 ? hk : occ;
+            const uint64_t valid = (nx - (uint64_t)abs(o.dx)) * (ny - (uint64_t)abs(o.dy)) * (nz - (uint64_t)abs(o.dz));
+            return (double)h / (double)valid;
+        });
+        if (lane == 0) {
+            curves[c.curve_off + tk.r] = n > 0 ? sum / (double)n : 0.0;
+            support[c.curve_off + tk.r] = n > 0 ? 1 : 0;
         }
-        if (threadIdx.x == 0) {
-            curves[c.curve_off + r] = e > b ? sum / (double)(e - b) : 0.0;
-            support[c.curve_off + r] = e > b ? 1 : 0;
-        }
-        __syncthreads();
     }
 }
 
@@ -429,7 +432,7 @@ __global__ void apply_rows_kernel(const CandDev *__restrict__ cands, const uint2
     }
 }
 
-// seed table += delta table of candidate 0
+// seed counts += delta counts of candidate 0 (both in canonical shell order)
 __global__ void apply_table_kernel(const CandDev *__restrict__ cands, const int32_t *__restrict__ delta,
                                    unsigned long long *__restrict__ table, uint64_t n) {
     const uint64_t off = cands[0].delta_off;
@@ -470,14 +473,14 @@ __global__ void zero_u32_kernel(uint32_t *__restrict__ p, uint64_t n) {
 
 // Fixed launch shapes of the batch sequence (grid-stride kernels).
 constexpr unsigned kClearBlocks = 2048, kPrepBlocks = 1024, kVoxelBlocks = 8192, kOverflowBlocks = 64,
-                   kAdBlocks = 4096, kDeltaBlocks = 2048, kFoldBlocks = 8192;
+                   kAdBlocks = 4096, kDeltaBlocks = 2048, kFoldBlocks = 2048;
 
 }  // namespace batch
 
 // Persistent batch workspace of a context: grow-only device buffers, pinned
 // staging for the per-batch inputs and outputs, and the captured graph.
 struct BatchWS {
-    DevBuf seeds, cands, hdr, geo, gran, tiles, origin, adpre, foldpre, delta, box, info, occ, ad, curves, over,
+    DevBuf seeds, cands, hdr, geo, gran, tiles, origin, adpre, tasks, delta, box, info, occ, ad, curves, over,
         newxyz;
     void *pin = nullptr;
     uint64_t pin_bytes = 0;
@@ -492,7 +495,7 @@ void batch_ws_release(s2h_ctx *ctx) {
     if (!w) return;
     if (w->exec) (void)hipGraphExecDestroy(w->exec);
     if (w->graph) (void)hipGraphDestroy(w->graph);
-    DevBuf *bufs[] = {&w->seeds, &w->cands, &w->hdr, &w->geo, &w->gran, &w->tiles, &w->origin, &w->adpre, &w->foldpre,
+    DevBuf *bufs[] = {&w->seeds, &w->cands, &w->hdr, &w->geo, &w->gran, &w->tiles, &w->origin, &w->adpre, &w->tasks,
                       &w->delta, &w->box, &w->info, &w->occ, &w->ad, &w->curves, &w->over, &w->newxyz};
     for (DevBuf *b : bufs) dev_free(ctx, *b);
     if (w->pin) (void)hipHostFree(w->pin);
@@ -506,8 +509,9 @@ void batch_ws_release(s2h_ctx *ctx) {
 
 struct s2h_seed {
     s2h_vol *vol = nullptr;
-    s2h::DevBuf cov, table, gbs, d_xyz, d_tri;
+    s2h::DevBuf cov, shtable, sg, smap, gbs, d_xyz, d_tri;
     s2h::ExactPlan *plan = nullptr;
+    uint64_t ncanon = 0;               // canonical offsets of the shells (delta table entries, + 1 unused)
     double bmin[3];
     double pitch;
     uint32_t r_max;
@@ -525,6 +529,56 @@ struct s2h_seed {
 
 namespace s2h {
 namespace batch {
+
+__global__ void shell_table_kernel(const unsigned long long *__restrict__ table, const int32_t *__restrict__ tab_of,
+                                   uint64_t n, unsigned long long *__restrict__ out) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
+        out[k] = table[tab_of[k]];
+}
+
+// Canonical shell order of a seed's plan: each canonical offset numbered at its
+// first appearance in the reference's shell order (a radius' canonical offsets
+// are then contiguous, as d and -d share a radius).  Builds the shell-order ->
+// canonical map (sg), the count-table -> canonical map (smap; ncanon for table
+// entries no shell uses) and the seed's counts in canonical order.
+static s2h_status seed_shell_tables(s2h_ctx *ctx, s2h_seed *sd) {
+    const ExactPlan *p = sd->plan;
+    const std::vector<int32_t> &gather = p->shells->host->gather;
+    const uint64_t K = gather.size(), T = p->table_size;
+    std::vector<int32_t> smap(std::max<uint64_t>(T, 1), -1), sg(std::max<uint64_t>(K, 1), -1), tab_of;
+    for (uint64_t i = 0; i < K; ++i) {
+        const int32_t g = gather[i];
+        if (g < 0) continue;
+        if (smap[g] < 0) {
+            smap[g] = (int32_t)tab_of.size();
+            tab_of.push_back(g);
+        }
+        sg[i] = smap[g];
+    }
+    const uint64_t nc = tab_of.size();
+    if (nc >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "too many offsets for a seed");
+    for (int32_t &m : smap)
+        if (m < 0) m = (int32_t)nc;
+    sd->ncanon = nc;
+    S2H_TRY(upload(ctx, sd->sg, sg.data(), sg.size() * 4));
+    S2H_TRY(upload(ctx, sd->smap, smap.data(), smap.size() * 4));
+    S2H_TRY(dev_alloc(ctx, sd->shtable, std::max<uint64_t>(nc, 1) * 8));
+    DevBuf tab;
+    s2h_status st = upload(ctx, tab, tab_of.data(), std::max<uint64_t>(nc, 1) * 4);
+    if (st == S2H_OK && nc) {
+        shell_table_kernel<<<1024, 256, 0, ctx->stream>>>((const unsigned long long *)p->d_table.ptr,
+                                                          (const int32_t *)tab.ptr, nc,
+                                                          (unsigned long long *)sd->shtable.ptr);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("shell_table_kernel: ") + hipGetErrorString(e));
+    }
+    if (st == S2H_OK) {
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("seed tables: ") + hipGetErrorString(e));
+    }
+    dev_free(ctx, tab);
+    return st;
+}
 
 // Whole-mesh coverage counts of a seed (one count per granule covering a centre).
 static s2h_status seed_coverage(s2h_ctx *ctx, s2h_seed *sd, const double *xyz, const uint32_t *tri, uint64_t ntris,
@@ -616,7 +670,7 @@ struct HostBatch {
     std::vector<vox::Tile> tiles;
     std::vector<double> origin;
     std::vector<uint64_t> ad_prefix;
-    std::vector<uint32_t> fold_prefix;
+    std::vector<FoldTask> tasks;
     std::vector<uint64_t> xyz_off;  // per candidate: first new vertex (3 doubles each)
     BatchHeader hdr{};
     uint64_t geo_total = 0, newxyz_total = 0, curve_total = 0;
@@ -632,8 +686,9 @@ static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, u
         SeedDev &d = hb.sdev[i];
         d.vol = (const uint32_t *)sd->vol->d_words;
         d.cov = (const uint32_t *)sd->cov.ptr;
-        d.table = (const unsigned long long *)sd->table.ptr;
-        d.gather = (const int32_t *)p->d_gather.ptr;
+        d.shtable = (const unsigned long long *)sd->shtable.ptr;
+        d.sg = (const int32_t *)sd->sg.ptr;
+        d.smap = (const int32_t *)sd->smap.ptr;
         d.offs = (const Offset *)p->d_offs.ptr;
         d.rstart = (const uint64_t *)p->d_rstart.ptr;
         d.cols = (const ColInfo *)p->d_cols.ptr;
@@ -663,7 +718,7 @@ static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, u
     hb.cd.resize(ncand);
     hb.origin.resize(4 * (size_t)ncand);
     hb.ad_prefix.assign(ncand + 1, 0);
-    hb.fold_prefix.assign(ncand + 1, 0);
+    hb.tasks.clear();
     uint64_t ad_total = 0, info_total = 0, box_total = 0, delta_total = 0, items = 0;
     std::vector<double> mn, mx, amax;
     std::vector<uint8_t> have;
@@ -742,12 +797,12 @@ static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, u
         d.box_off = box_total;
         box_total += (uint64_t)(bhi[0] - blo[0]) * (bhi[1] - blo[1]) * (bhi[2] - blo[2]);
         d.delta_off = delta_total;
-        delta_total += sd->plan->table_size;
+        delta_total += sd->ncanon + 1;
         d.item_start = items;
         items += (uint64_t)d.nchunks * hb.sdev[k.seed].ngb;
         d.curve_off = hb.curve_total;
         hb.curve_total += sd->r_max + 1;
-        hb.fold_prefix[c + 1] = hb.fold_prefix[c] + sd->r_max;
+        for (uint32_t r = 1; r <= sd->r_max; ++r) hb.tasks.push_back(FoldTask{c, r});
         hb.origin[4 * c] = sd->bmin[0];
         hb.origin[4 * c + 1] = sd->bmin[1];
         hb.origin[4 * c + 2] = sd->bmin[2];
@@ -762,7 +817,15 @@ static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, u
     hb.hdr.box_words = box_total;
     hb.hdr.delta_words = delta_total;
     hb.hdr.ncand = ncand;
-    hb.hdr.nfold = hb.fold_prefix[ncand];
+    // fold tasks by seed, radius (largest first), candidate
+    std::sort(hb.tasks.begin(), hb.tasks.end(), [&](const FoldTask &a, const FoldTask &b) {
+        const uint32_t sa = hb.cd[a.cand].seed, sb = hb.cd[b.cand].seed;
+        if (sa != sb) return sa < sb;
+        if (a.r != b.r) return a.r > b.r;
+        return a.cand < b.cand;
+    });
+    if (hb.tasks.size() >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "batch too large");
+    hb.hdr.nfold = (uint32_t)hb.tasks.size();
     hb.hdr.ngran = (uint32_t)hb.gran.size();
     hb.hdr.ntiles = (uint32_t)hb.tiles.size();
     hb.hdr.row_chunk = row_chunk;
@@ -813,13 +876,13 @@ static void launch_sequence(hipStream_t s, BatchWS &w) {
     build_ad_kernel<<<kAdBlocks, 256, 0, s>>>(seeds, cands, hdr, (const uint64_t *)w.adpre.ptr, box, ad, info, occ);
     delta_count_kernel<<<kDeltaBlocks, 64 * kDeltaWaves, 0, s>>>(seeds, cands, hdr, ad, ad_records(w), info,
                                                                  delta);
-    batch_fold_kernel<<<kFoldBlocks, 256, 0, s>>>(seeds, cands, hdr, (const uint32_t *)w.foldpre.ptr, delta, occ,
-                                                  (double *)w.curves.ptr, curve_support(w));
+    batch_fold_kernel<<<kFoldBlocks, 64 * kFoldWaves, 0, s>>>(seeds, cands, hdr, (const FoldTask *)w.tasks.ptr, delta,
+                                                              occ, (double *)w.curves.ptr, curve_support(w));
 }
 
 static std::vector<void *> ws_ptrs(const BatchWS &w) {
     return {w.seeds.ptr, w.cands.ptr, w.hdr.ptr, w.geo.ptr, w.gran.ptr, w.tiles.ptr, w.origin.ptr, w.adpre.ptr,
-            w.foldpre.ptr, w.delta.ptr, w.box.ptr, w.info.ptr, w.occ.ptr, w.ad.ptr, w.curves.ptr, w.over.ptr,
+            w.tasks.ptr, w.delta.ptr, w.box.ptr, w.info.ptr, w.occ.ptr, w.ad.ptr, w.curves.ptr, w.over.ptr,
             (void *)(uintptr_t)w.ad.bytes, (void *)(uintptr_t)w.curves.bytes};
 }
 
@@ -885,7 +948,7 @@ static s2h_status upload_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, co
     S2H_TRY(ws_reserve(ctx, w.tiles, std::max<size_t>(hb.tiles.size(), 1) * sizeof(vox::Tile)));
     S2H_TRY(ws_reserve(ctx, w.origin, (size_t)nc * 32));
     S2H_TRY(ws_reserve(ctx, w.adpre, ((size_t)ncand + 1) * 8));
-    S2H_TRY(ws_reserve(ctx, w.foldpre, ((size_t)ncand + 1) * 4));
+    S2H_TRY(ws_reserve(ctx, w.tasks, std::max<size_t>(hb.tasks.size(), 1) * sizeof(FoldTask)));
     S2H_TRY(ws_reserve(ctx, w.delta, std::max<uint64_t>(hb.hdr.delta_words, 1) * 4));
     S2H_TRY(ws_reserve(ctx, w.box, std::max<uint64_t>(hb.hdr.box_words, 1) * 4));
     S2H_TRY(ws_reserve(ctx, w.info, std::max<uint64_t>(hb.hdr.info_total, 1) * 8));
@@ -900,7 +963,7 @@ static s2h_status upload_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, co
     const uint64_t in_bytes = (hb.sdev.size() * sizeof(SeedDev) + 16) + ((uint64_t)ncand * sizeof(CandDev) + 16) +
                               (sizeof(BatchHeader) + 16) + (hb.gran.size() * sizeof(vox::GranuleInfo) + 16) +
                               (hb.tiles.size() * sizeof(vox::Tile) + 16) + ((uint64_t)ncand * 32 + 16) +
-                              (((uint64_t)ncand + 1) * 8 + 16) + (((uint64_t)ncand + 1) * 4 + 16) +
+                              (((uint64_t)ncand + 1) * 8 + 16) + (hb.tasks.size() * sizeof(FoldTask) + 16) +
                               hb.newxyz_total * 24 + 64;
     const uint64_t need = in_bytes + out_region(hb, ncand) + 64;
     if (w.pin_bytes < need) {
@@ -926,7 +989,7 @@ static s2h_status upload_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, co
     S2H_TRY(put(w.tiles, hb.tiles.data(), hb.tiles.size() * sizeof(vox::Tile)));
     S2H_TRY(put(w.origin, hb.origin.data(), (uint64_t)ncand * 32));
     S2H_TRY(put(w.adpre, hb.ad_prefix.data(), ((uint64_t)ncand + 1) * 8));
-    S2H_TRY(put(w.foldpre, hb.fold_prefix.data(), ((uint64_t)ncand + 1) * 4));
+    S2H_TRY(put(w.tasks, hb.tasks.data(), hb.tasks.size() * sizeof(FoldTask)));
     // new vertices: one contiguous block in candidate order
     if (hb.newxyz_total) {
         for (uint32_t c = 0; c < ncand; ++c) {
@@ -1151,11 +1214,10 @@ s2h_status s2h_seed_create(s2h_ctx *ctx, const double *xyz, uint64_t nverts, con
     const double pm = pitch < 1e-9 ? 1e-9 : pitch;
     if ((st = exact_plan_get(ctx, dims, r_max, pm, &sd->plan)) != S2H_OK) return cleanup(st);
     if ((st = exact_run_counts(ctx, sd->vol, sd->plan)) != S2H_OK) return cleanup(st);
-    if ((st = dev_alloc(ctx, sd->table, std::max<uint64_t>(sd->plan->table_size, 1) * 8)) != S2H_OK)
-        return cleanup(st);
-    S2H_HIP(hipMemcpyAsync(sd->table.ptr, sd->plan->d_table.ptr, sd->plan->table_size * 8,
-                           hipMemcpyDeviceToDevice, ctx->stream));
-    for (const DzBlock &b : sd->plan->blocks) sd->gb_host.push_back(batch::GB{b.group, b.dz0, b.dzb});
+    if ((st = batch::seed_shell_tables(ctx, sd)) != S2H_OK) return cleanup(st);
+    // the delta kernel runs dz blocks in widths of 8 (lanes past a column's m do not count)
+    for (const DzBlock &b : sd->plan->blocks)
+        sd->gb_host.push_back(batch::GB{b.group, b.dz0, (uint16_t)((b.dzb + 7) / 8 * 8)});
     if ((st = upload(ctx, sd->gbs, sd->gb_host.data(), std::max<size_t>(sd->gb_host.size(), 1) * sizeof(batch::GB))) !=
         S2H_OK)
         return cleanup(st);
@@ -1169,7 +1231,9 @@ void s2h_seed_free(s2h_ctx *ctx, s2h_seed *sd) {
     (void)hipStreamSynchronize(ctx->stream);
     if (sd->vol) s2h_vol_free(ctx, sd->vol);
     dev_free(ctx, sd->cov);
-    dev_free(ctx, sd->table);
+    dev_free(ctx, sd->shtable);
+    dev_free(ctx, sd->sg);
+    dev_free(ctx, sd->smap);
     dev_free(ctx, sd->gbs);
     dev_free(ctx, sd->d_xyz);
     dev_free(ctx, sd->d_tri);
@@ -1213,7 +1277,7 @@ s2h_status s2h_seed_accept(s2h_ctx *ctx, s2h_seed *sd, uint32_t particle, const 
                                                      (uint32_t *)sd->vol->d_words, (int)dims[1], nwd,
                                                      (int)(sd->vol->row_words * 2));
     apply_table_kernel<<<1024, 256, 0, ctx->stream>>>((const CandDev *)w.cands.ptr, (const int32_t *)w.delta.ptr,
-                                                      (unsigned long long *)sd->table.ptr, sd->plan->table_size);
+                                                      (unsigned long long *)sd->shtable.ptr, sd->ncanon);
     S2H_HIP(hipGetLastError());
     const uint32_t v0 = sd->vstart[particle], nv = sd->vstart[particle + 1] - v0;
     std::memcpy(sd->xyz.data() + 3 * (uint64_t)v0, new_xyz, (size_t)nv * 24);

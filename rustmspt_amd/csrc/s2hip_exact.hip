@@ -7,7 +7,7 @@
 //
 // Kernel shape (one wave per work item):
 //   * lanes    = 64 columns (dx, dy) with nearly equal m (sorted by m);
-//   * uniform  = the partner ("b") row p = (xb, yb) and a block of 8/16/32 dz
+//   * uniform  = the partner ("b") row p = (xb, yb) and a block of 4..32 dz
 //                values starting at a multiple of 32, so the b row shifted right
 //                by dz is wave-uniform: it lives in SGPRs and is produced by the
 //                scalar unit (64-bit shift of a dword pair), free of VALU cost;
@@ -97,6 +97,23 @@ __device__ __forceinline__ void count_item(const uint32_t *__restrict__ vol, con
     }
 }
 
+// dz blocks are 4..32 wide in steps of 4 (the wave-uniform width of the item).
+template <int NW>
+__device__ __forceinline__ void count_dzb(const uint32_t *__restrict__ vol, const __amdgpu_buffer_rsrc_t rsrc, int nx,
+                                          int ny, int row_dw, const WorkItem &it, const ColInfo &c,
+                                          unsigned long long *__restrict__ table) {
+    switch (it.dzb) {
+        case 4: count_item<NW, 4>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 8: count_item<NW, 8>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 12: count_item<NW, 12>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 16: count_item<NW, 16>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 20: count_item<NW, 20>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 24: count_item<NW, 24>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 28: count_item<NW, 28>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        default: count_item<NW, 32>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+    }
+}
+
 // One wave per work item; every dz-block width runs in the same launch (the
 // branch is wave-uniform), items are ordered heaviest first.
 template <int NW>
@@ -112,12 +129,13 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
     // per-lane rows through a buffer resource: an out-of-range offset reads 0
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *)vol, 0, (int)vol_bytes, 0x00020000);
-    switch (it.dzb) {
-        case 8: count_item<NW, 8>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        case 16: count_item<NW, 16>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        case 24: count_item<NW, 24>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        default: count_item<NW, 32>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-    }
+    // Blocks whose partner words end one dword early (dz0 >= 32, or a row of
+    // fewer dwords than NW) run the NW - 1 word body: the last word would
+    // only meet zeros.
+    if (NW > 1 && it.nw < NW)
+        count_dzb<(NW > 1 ? NW - 1 : 1)>(vol, rsrc, nx, ny, row_dw, it, c, table);
+    else
+        count_dzb<NW>(vol, rsrc, nx, ny, row_dw, it, c, table);
 }
 
 // ---------------------------------------------------------------- gather / fold
@@ -154,55 +172,7 @@ static constexpr int kFoldAhead = 2; // blocks loaded per step: the next step's 
 // The ordered f64 sum of t[0 .. n) on one wave (all 64 lanes call it; the
 // result is wave-uniform).  Loads run kFoldAhead blocks ahead of the fold.
 __device__ __forceinline__ double fold_segment(const double *__restrict__ t, uint64_t n, uint32_t lane) {
-    constexpr uint64_t B = 64 * kFoldK, SB = B * kFoldAhead;
-    FoldState st;
-    if (n == 0) return 0.0;
-    bool seq = false;
-    double cur[kFoldAhead][kFoldK], nxt[kFoldAhead][kFoldK];
-    // loads past the end read t[0] (always valid) and are replaced by 0
-#pragma unroll
-    for (int b = 0; b < kFoldAhead; ++b)
-#pragma unroll
-        for (int i = 0; i < kFoldK; ++i) {
-            const uint64_t j = b * B + (uint64_t)lane * kFoldK + i;
-            const double v = t[j < n ? j : 0];
-            cur[b][i] = j < n ? v : 0.0;
-        }
-    for (uint64_t s0 = 0; s0 < n; s0 += SB) {
-#pragma unroll
-        for (int b = 0; b < kFoldAhead; ++b)
-#pragma unroll
-            for (int i = 0; i < kFoldK; ++i) {
-                const uint64_t j = s0 + SB + b * B + (uint64_t)lane * kFoldK + i;
-                const double v = t[j < n ? j : 0];
-                nxt[b][i] = j < n ? v : 0.0;
-            }
-#pragma unroll
-        for (int b = 0; b < kFoldAhead; ++b) {
-            const uint64_t b0 = s0 + b * B;
-            if (b0 >= n) break;
-            const uint32_t cnt = (n - b0) < B ? (uint32_t)(n - b0) : (uint32_t)B;
-            bool bad = false;
-#pragma unroll
-            for (int i = 0; i < kFoldK; ++i)
-                bad = bad || (lane * kFoldK + i < cnt && !(cur[b][i] >= 0.0 && cur[b][i] <= 1.7976931348623157e308));
-            if (!seq && __ballot(bad)) seq = true;
-            if (seq) {
-                // one term at a time, in order (lane l's terms are l*K .. l*K+K-1)
-                for (uint32_t l = 0; l * kFoldK < cnt; ++l)
-#pragma unroll
-                    for (int i = 0; i < kFoldK; ++i)
-                        if (l * kFoldK + i < cnt) st.S = st.S + read_lane_f64(cur[b][i], (int)l);
-            } else {
-                fold_block<kFoldK>(st, cur[b], cnt, lane);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < kFoldAhead; ++b)
-#pragma unroll
-            for (int i = 0; i < kFoldK; ++i) cur[b][i] = nxt[b][i];
-    }
-    return st.S;
+    return fold_terms<kFoldK, kFoldAhead>(n, lane, [&](uint64_t j) { return t[j]; });
 }
 
 __global__ __launch_bounds__(64) void fold_kernel(const double *__restrict__ terms,
@@ -470,11 +440,12 @@ static void group_columns(const ExactPlan *p, int band, Grouping &g) {
         if (x0 >= x1 || y0 >= y1) continue;
         for (int dz0 = 0; dz0 <= M; dz0 += 32) {
             const int rem = std::min(32, M + 1 - dz0);
-            const int dzb = (rem + 7) / 8 * 8;
+            const int dzb = (rem + 3) / 4 * 4;
             const int nwi = need_dw - dz0 / 32;
             if (nwi <= 0) continue;
+            const int words = (NW > 1 && nwi < NW) ? NW - 1 : NW;  // the kernel's body for this block
             const double rows = (double)(x1 - x0) * (y1 - y0);
-            const double cost = rows * (NW * dzb * 2.0 + kRowOverhead);
+            const double cost = rows * (words * dzb * 2.0 + kRowOverhead);
             g.blks.push_back(Grouping::Blk{DzBlock{gi, (uint16_t)dz0, (uint16_t)dzb}, x0, x1, y0, y1, nwi, cost});
             g.total += cost;
         }

@@ -206,6 +206,65 @@ __device__ __forceinline__ void fold_block(FoldState &st, const double (&t)[K], 
     }
 }
 
+// The ordered f64 sum of term(0), ..., term(n - 1) on one wave (all 64 lanes
+// call it; the result is wave-uniform): blocks of 64 * K terms, lane l holding
+// terms l*K .. l*K+K-1 of a block, folded by fold_block.  Terms are produced
+// AHEAD blocks before they are folded, so their loads overlap the fold.  term(j)
+// is only called with j < n (past the end it is called with 0 and the value
+// replaced by 0).  After a negative or non-finite term the rest of the sum
+// folds one term at a time.
+template <int K, int AHEAD, class Term>
+__device__ __forceinline__ double fold_terms(uint64_t n, uint32_t lane, Term term) {
+    constexpr uint64_t B = 64 * K, SB = B * AHEAD;
+    FoldState st;
+    if (n == 0) return 0.0;
+    bool seq = false;
+    double cur[AHEAD][K], nxt[AHEAD][K];
+#pragma unroll
+    for (int b = 0; b < AHEAD; ++b)
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            const uint64_t j = b * B + (uint64_t)lane * K + i;
+            const double v = term(j < n ? j : 0);
+            cur[b][i] = j < n ? v : 0.0;
+        }
+    for (uint64_t s0 = 0; s0 < n; s0 += SB) {
+#pragma unroll
+        for (int b = 0; b < AHEAD; ++b)
+#pragma unroll
+            for (int i = 0; i < K; ++i) {
+                const uint64_t j = s0 + SB + b * B + (uint64_t)lane * K + i;
+                const double v = term(j < n ? j : 0);
+                nxt[b][i] = j < n ? v : 0.0;
+            }
+#pragma unroll
+        for (int b = 0; b < AHEAD; ++b) {
+            const uint64_t b0 = s0 + b * B;
+            if (b0 >= n) break;
+            const uint32_t cnt = (n - b0) < B ? (uint32_t)(n - b0) : (uint32_t)B;
+            bool bad = false;
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+                bad = bad || (lane * K + i < cnt && !(cur[b][i] >= 0.0 && cur[b][i] <= 1.7976931348623157e308));
+            if (!seq && __ballot(bad)) seq = true;
+            if (seq) {
+                // one term at a time, in order (lane l's terms are l*K .. l*K+K-1)
+                for (uint32_t l = 0; l * K < cnt; ++l)
+#pragma unroll
+                    for (int i = 0; i < K; ++i)
+                        if (l * K + i < cnt) st.S = st.S + read_lane_f64(cur[b][i], (int)l);
+            } else {
+                fold_block<K>(st, cur[b], cnt, lane);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < AHEAD; ++b)
+#pragma unroll
+            for (int i = 0; i < K; ++i) cur[b][i] = nxt[b][i];
+    }
+    return st.S;
+}
+
 // ---------------------------------------------------------------- shells (host)
 struct Offset {
     int32_t dx, dy, dz;
