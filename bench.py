@@ -27,7 +27,7 @@ VALU_PEAK_OPS = 256 * 64 * 2.4e9  # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 G
 # packing, launch shape band 3 / 32768 waves), from rocprofv3 --pmc FETCH_SIZE
 # (x2.0) and WRITE_SIZE (x1.0) of the count dispatch; factors from a 512 MiB
 # device copy.  Command lines and figures: profiles/exact_count_kernel_256cube_r64.md.
-COUNT_PHASE_TRAFFIC = 12549 * 1024 * 2.0 + 1696499 * 1024 * 1.0
+COUNT_PHASE_TRAFFIC = 13060 * 1024 * 2.0 + 1677000 * 1024 * 1.0
 
 
 def _cpu_model() -> str:
