@@ -338,9 +338,13 @@ __global__ __launch_bounds__(64 * kDeltaWaves) void delta_count_kernel(const See
         const GB gb = s.gbs[local / c.nchunks];
         const int chunk = (int)(local % c.nchunks);
         switch (gb.dzb) {
+            case 4: delta_item<4>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
             case 8: delta_item<8>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
+            case 12: delta_item<12>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
             case 16: delta_item<16>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
+            case 20: delta_item<20>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
             case 24: delta_item<24>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
+            case 28: delta_item<28>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
             default: delta_item<32>(s, c, gb, chunk, row_chunk, ad, ad_rs, info, delta); break;
         }
     }
@@ -357,9 +361,9 @@ struct FoldTask {
 // One wave per (candidate, radius) task: term j of the radius is
 // (seed count + delta) / valid of the radius' j-th offset in shell order, and
 // the terms are folded in that order -- bit for bit the reference's sequential
-// left fold (fold_terms).  The host orders the tasks by seed, then radius
-// (largest first), then candidate, so waves running together share a radius'
-// shell data in L2.
+// left fold (fold_terms).  The host orders the tasks longest shell first, and
+// a shell's tasks (one per candidate of its seed) next to each other, so waves
+// running together share the shell's data in L2.
 __global__ __launch_bounds__(64 * kFoldWaves) void batch_fold_kernel(const SeedDev *__restrict__ seeds,
                                                                      const CandDev *__restrict__ cands,
                                                                      const BatchHeader *__restrict__ hdr,
@@ -817,11 +821,18 @@ static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, u
     hb.hdr.box_words = box_total;
     hb.hdr.delta_words = delta_total;
     hb.hdr.ncand = ncand;
-    // fold tasks by seed, radius (largest first), candidate
+    // fold tasks: longest shells first (the last waves to start are short), then
+    // by seed, radius and candidate, so a shell's tasks run side by side
+    auto shell_len = [&](const FoldTask &t) {
+        const std::vector<uint64_t> &rs = seeds[hb.cd[t.cand].seed]->plan->radius_start;
+        return rs[t.r + 1] - rs[t.r];
+    };
     std::sort(hb.tasks.begin(), hb.tasks.end(), [&](const FoldTask &a, const FoldTask &b) {
+        const uint64_t la = shell_len(a), lb = shell_len(b);
+        if (la != lb) return la > lb;
         const uint32_t sa = hb.cd[a.cand].seed, sb = hb.cd[b.cand].seed;
         if (sa != sb) return sa < sb;
-        if (a.r != b.r) return a.r > b.r;
+        if (a.r != b.r) return a.r < b.r;
         return a.cand < b.cand;
     });
     if (hb.tasks.size() >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "batch too large");
@@ -1215,9 +1226,7 @@ s2h_status s2h_seed_create(s2h_ctx *ctx, const double *xyz, uint64_t nverts, con
     if ((st = exact_plan_get(ctx, dims, r_max, pm, &sd->plan)) != S2H_OK) return cleanup(st);
     if ((st = exact_run_counts(ctx, sd->vol, sd->plan)) != S2H_OK) return cleanup(st);
     if ((st = batch::seed_shell_tables(ctx, sd)) != S2H_OK) return cleanup(st);
-    // the delta kernel runs dz blocks in widths of 8 (lanes past a column's m do not count)
-    for (const DzBlock &b : sd->plan->blocks)
-        sd->gb_host.push_back(batch::GB{b.group, b.dz0, (uint16_t)((b.dzb + 7) / 8 * 8)});
+    for (const DzBlock &b : sd->plan->blocks) sd->gb_host.push_back(batch::GB{b.group, b.dz0, b.dzb});
     if ((st = upload(ctx, sd->gbs, sd->gb_host.data(), std::max<size_t>(sd->gb_host.size(), 1) * sizeof(batch::GB))) !=
         S2H_OK)
         return cleanup(st);
