@@ -162,38 +162,200 @@ __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32
     terms[i] = h / (double)valid;
 }
 
-// One wave per radius folds the radius' terms in shell order: the reference's
-// sequential left fold, bit for bit, 512 terms per step (fold_block).  Terms
-// stream from HBM kFoldAhead blocks ahead of the fold.  After a negative or
-// non-finite term the rest of the radius folds one term at a time.
-static constexpr int kFoldK = 8;     // terms per lane per block (512 per block)
-static constexpr int kFoldAhead = 2; // blocks loaded per step: the next step's loads overlap this step
+// One workgroup per radius folds the radius' terms in shell order: the
+// reference's sequential left fold, bit for bit.  The terms are cut into blocks
+// of 512 (64 lanes x kFoldK).  Inside one binade of the running sum S (grid
+// u = 2^(ilogb(S) - 52)), a block's effect on S / u is the sum G of its terms'
+// rounded grid increments -- which depends on u and the terms only -- as long
+// as no term is a tie on the grid or too large and S stays in the binade
+// (fold_block's non-stopping case).  So:
+//   pass 0  all waves: each block's plain f64 sum (any order) and whether it
+//           holds a negative or non-finite term;
+//   pass 0b wave 0: a running sum of those from the exact S so far guesses
+//           the binade at each block's start;
+//   pass 1  all waves: G of each block on its guessed grid, and whether a tie
+//           or a large term (y >= 2^43, so every sum of 512 is exact) occurs;
+//   pass 2  wave 0, in order: a block whose guess matches the exact S's
+//           binade, with no flag and S / u + G < 2^53, adds G (the same
+//           integers fold_block would add); any other block runs fold_block
+//           itself, or one term at a time after a negative or non-finite term
+//           (as a one-wave fold would, block for block).
+// The guesses only pick which blocks take the short path; every value comes
+// from exact arithmetic.
+static constexpr int kFoldK = 8;          // terms per lane per block (512 per block)
+static constexpr int kWideWaves = 8;      // waves per radius
+static constexpr int kWideBlocks = 1024;  // blocks per pass (a longer radius takes several passes)
+static constexpr int kWideAhead = 4;      // blocks a wave loads at once in passes 0 and 1
+static constexpr int kNoBinade = -100000; // guessed start S == 0
 
-// The ordered f64 sum of t[0 .. n) on one wave (all 64 lanes call it; the
-// result is wave-uniform).  Loads run kFoldAhead blocks ahead of the fold.
-__device__ __forceinline__ double fold_segment(const double *__restrict__ t, uint64_t n, uint32_t lane) {
-    return fold_terms<kFoldK, kFoldAhead>(n, lane, [&](uint64_t j) { return t[j]; });
+struct WideLds {
+    double g[kWideBlocks];     // approximate block sums, then grid increments
+    int32_t m[kWideBlocks];    // guessed binade at the block's start
+    uint32_t flag[kWideBlocks];  // 2: negative or non-finite term; 1: tie or large term
+};
+
+__device__ __forceinline__ double wave_sum_any(double x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
 }
 
-__global__ __launch_bounds__(64) void fold_kernel(const double *__restrict__ terms,
-                                                  const uint64_t *__restrict__ rstart, uint32_t r_max,
-                                                  double *__restrict__ values, uint8_t *__restrict__ support) {
+__device__ __forceinline__ void load_block(const double *__restrict__ t, uint64_t n, uint64_t start, uint32_t lane,
+                                           double (&v)[kFoldK]) {
+#pragma unroll
+    for (int i = 0; i < kFoldK; ++i) {
+        const uint64_t j = start + (uint64_t)lane * kFoldK + i;
+        const double x = t[j < n ? j : 0];
+        v[i] = j < n ? x : 0.0;
+    }
+}
+
+// The ordered sum of t[0 .. n), valid in wave 0 of the workgroup (all threads call it).
+__device__ __forceinline__ double fold_wide(const double *__restrict__ t, uint64_t n, WideLds &L) {
+    constexpr double kTwo53 = 9007199254740992.0, kTwo43 = 8796093022208.0;
+    constexpr uint64_t B = 64 * kFoldK;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    FoldState st;
+    bool seq = false;
+    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kWideBlocks * B) {
+        const uint32_t nb = (uint32_t)min<uint64_t>((uint64_t)kWideBlocks, (n - c0 + B - 1) / B);
+        // kWideAhead blocks per wave at a time: their loads are all in flight together
+        for (uint32_t b0 = wave * kWideAhead; b0 < nb; b0 += kWideWaves * kWideAhead) {
+            double v[kWideAhead][kFoldK];
+#pragma unroll
+            for (int q = 0; q < kWideAhead; ++q)
+                load_block(t, n, c0 + (uint64_t)min(b0 + q, nb - 1) * B, lane, v[q]);
+#pragma unroll
+            for (int q = 0; q < kWideAhead; ++q) {
+                double sum = 0.0;
+                bool bad = false;
+#pragma unroll
+                for (int i = 0; i < kFoldK; ++i) {
+                    sum += v[q][i];
+                    bad = bad || !(v[q][i] >= 0.0 && v[q][i] <= 1.7976931348623157e308);
+                }
+                sum = wave_sum_any(sum);
+                const bool anybad = __ballot(bad) != 0;
+                if (lane == 0 && b0 + q < nb) {
+                    L.g[b0 + q] = sum;
+                    L.flag[b0 + q] = anybad ? 2u : 0u;
+                }
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const uint32_t per = (nb + 63) / 64;
+            const uint32_t b0 = min(nb, lane * per), b1 = min(nb, b0 + per);
+            double loc = 0.0;
+            for (uint32_t b = b0; b < b1; ++b) loc += L.g[b];
+            double pre = st.S + wave_exclusive_sum_f64(loc);
+            for (uint32_t b = b0; b < b1; ++b) {
+                L.m[b] = pre > 0.0 ? ilogb(pre) : kNoBinade;
+                pre += L.g[b];
+            }
+        }
+        __syncthreads();
+        for (uint32_t b0 = wave * kWideAhead; b0 < nb; b0 += kWideWaves * kWideAhead) {
+            double v[kWideAhead][kFoldK];
+#pragma unroll
+            for (int q = 0; q < kWideAhead; ++q)
+                load_block(t, n, c0 + (uint64_t)min(b0 + q, nb - 1) * B, lane, v[q]);
+#pragma unroll
+            for (int q = 0; q < kWideAhead; ++q) {
+                const uint32_t b = min(b0 + q, nb - 1);
+                const int m = L.m[b];
+                // blocks with a flag or no binade run the exact path (uniform per wave)
+                if (b0 + q >= nb || m == kNoBinade || L.flag[b] != 0) continue;
+                const double inv_u = ldexp(1.0, 52 - m);
+                double T = 0.0;
+                bool fl = false;
+#pragma unroll
+                for (int i = 0; i < kFoldK; ++i) {
+                    const double y = v[q][i] * inv_u;  // exact: scaling by a power of two (0 past the end)
+                    const bool big = !(y < kTwo43);
+                    const double nfl = big ? 0.0 : floor(y);
+                    const double f = y - nfl;
+                    T += nfl + (f > 0.5 ? 1.0 : 0.0);
+                    fl = fl || big || f == 0.5;
+                }
+                T = wave_sum_any(T);  // integers below 2^53: exact in any order
+                const bool anyfl = __ballot(fl) != 0;
+                if (lane == 0) {
+                    L.g[b] = T;
+                    L.flag[b] = anyfl ? 1u : 0u;
+                }
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            for (uint32_t g0 = 0; g0 < nb; g0 += 64) {
+                const uint32_t bl = g0 + lane;
+                double gl = 0.0;
+                int ml = kNoBinade;
+                uint32_t fl = 1;
+                if (bl < nb) {
+                    gl = L.g[bl];
+                    ml = L.m[bl];
+                    fl = L.flag[bl];
+                }
+                const uint32_t cntb = min(64u, nb - g0);
+                for (uint32_t k = 0; k < cntb; ++k) {
+                    const double G = read_lane_f64(gl, (int)k);
+                    const int m = __builtin_amdgcn_readlane(ml, (int)k);
+                    const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)fl, (int)k);
+                    if (!seq && f == 0 && m != kNoBinade && st.S > 0.0 && ldexp(1.0, m - 52) == st.u &&
+                        st.a + G < kTwo53) {
+                        st.a += G;
+                        st.S = st.a * st.u;
+                        continue;
+                    }
+                    const uint64_t start = c0 + (uint64_t)(g0 + k) * B;
+                    const uint32_t cnt = (uint32_t)min<uint64_t>(B, n - start);
+                    double v[kFoldK];
+                    load_block(t, n, start, lane, v);
+                    bool bad = false;
+#pragma unroll
+                    for (int i = 0; i < kFoldK; ++i)
+                        bad = bad || (lane * kFoldK + i < cnt && !(v[i] >= 0.0 && v[i] <= 1.7976931348623157e308));
+                    if (!seq && __ballot(bad)) seq = true;
+                    if (seq) {
+                        for (uint32_t l = 0; l * kFoldK < cnt; ++l)
+#pragma unroll
+                            for (int i = 0; i < kFoldK; ++i)
+                                if (l * kFoldK + i < cnt) st.S = st.S + read_lane_f64(v[i], (int)l);
+                    } else {
+                        fold_block<kFoldK>(st, v, cnt, lane);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    return st.S;
+}
+
+__global__ __launch_bounds__(64 * kWideWaves) void fold_kernel(const double *__restrict__ terms,
+                                                               const uint64_t *__restrict__ rstart, uint32_t r_max,
+                                                               double *__restrict__ values,
+                                                               uint8_t *__restrict__ support) {
+    __shared__ WideLds L;
     const uint32_t r = blockIdx.x + 1;
     if (r > r_max) return;
     const uint64_t s = rstart[r], n = rstart[r + 1] - s;
-    const double sum = fold_segment(terms + s, n, threadIdx.x);
+    const double sum = fold_wide(terms + s, n, L);
     if (threadIdx.x == 0) {
         values[r] = n > 0 ? sum / (double)n : 0.0;
         support[r] = n > 0 ? 1 : 0;
     }
 }
 
-__global__ __launch_bounds__(64) void segment_sum_kernel(const double *__restrict__ terms,
-                                                         const uint64_t *__restrict__ starts, uint32_t nseg,
-                                                         double *__restrict__ sums) {
+__global__ __launch_bounds__(64 * kWideWaves) void segment_sum_kernel(const double *__restrict__ terms,
+                                                                      const uint64_t *__restrict__ starts,
+                                                                      uint32_t nseg, double *__restrict__ sums) {
+    __shared__ WideLds L;
     for (uint32_t g = blockIdx.x; g < nseg; g += gridDim.x) {
         const uint64_t s = starts[g];
-        const double sum = fold_segment(terms + s, starts[g + 1] - s, threadIdx.x);
+        const double sum = fold_wide(terms + s, starts[g + 1] - s, L);
         if (threadIdx.x == 0) sums[g] = sum;
     }
 }
@@ -737,7 +899,7 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, cons
     double *values = (double *)p->d_curve.ptr;
     uint8_t *support = (uint8_t *)(values + p->r_max + 1);
     if (p->r_max > 0) {
-        fold_kernel<<<p->r_max, 64, 0, ctx->stream>>>((const double *)p->d_terms.ptr,
+        fold_kernel<<<p->r_max, 64 * kWideWaves, 0, ctx->stream>>>((const double *)p->d_terms.ptr,
                                                        (const uint64_t *)p->d_rstart.ptr, p->r_max,
                                                        values, support);
         S2H_HIP(hipGetLastError());
@@ -860,7 +1022,7 @@ extern "C" s2h_status s2h_ordered_sum(s2h_ctx *ctx, const double *terms, uint64_
     if (st == S2H_OK) st = dev_alloc(ctx, d_o, (uint64_t)nseg * 8);
     hipError_t e = hipSuccess;
     if (st == S2H_OK) {
-        segment_sum_kernel<<<std::min<uint32_t>(nseg, 4096), 64, 0, ctx->stream>>>(
+        segment_sum_kernel<<<std::min<uint32_t>(nseg, 4096), 64 * kWideWaves, 0, ctx->stream>>>(
             (const double *)d_t.ptr, (const uint64_t *)d_s.ptr, nseg, (double *)d_o.ptr);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(sums, d_o.ptr, (uint64_t)nseg * 8, hipMemcpyDeviceToHost, ctx->stream);
