@@ -78,22 +78,53 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
+// 4-point DFT in registers, natural order in and out.
+template <int SIGN>
+__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
+    float2 s0 = cadd(a0, a2), d0 = csub(a0, a2);
+    float2 s1 = cadd(a1, a3), d1 = csub(a1, a3);
+    // multiply d1 by -i (forward) or +i (inverse)
+    float2 jd1 = SIGN < 0 ? make_float2(d1.y, -d1.x) : make_float2(-d1.y, d1.x);
+    a0 = cadd(s0, s1);
+    a2 = csub(s0, s1);
+    a1 = cadd(d0, jd1);
+    a3 = csub(d0, jd1);
+}
+
 // R-point DFT in registers; sign = -1 forward, +1 inverse (unnormalized).
 template <int SIGN>
 __device__ __forceinline__ void small_dft(float2 *a, int R) {
-    if (R == 2) {
+    if (R == 8) {
+        // even and odd 4-point DFTs, then X[k] = E[k] + w^k O[k], X[k+4] = E[k] - w^k O[k]
+        float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+        float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+        dft4<SIGN>(e0, e1, e2, e3);
+        dft4<SIGN>(o0, o1, o2, o3);
+        const float h = 0.70710678118654752440f;
+        float2 t1, t2, t3;
+        if (SIGN < 0) {  // w = exp(-i pi / 4)
+            t1 = make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+            t2 = make_float2(o2.y, -o2.x);
+            t3 = make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+        } else {         // w = exp(+i pi / 4)
+            t1 = make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y));
+            t2 = make_float2(-o2.y, o2.x);
+            t3 = make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y));
+        }
+        a[0] = cadd(e0, o0);
+        a[4] = csub(e0, o0);
+        a[1] = cadd(e1, t1);
+        a[5] = csub(e1, t1);
+        a[2] = cadd(e2, t2);
+        a[6] = csub(e2, t2);
+        a[3] = cadd(e3, t3);
+        a[7] = csub(e3, t3);
+    } else if (R == 2) {
         float2 t = a[0];
         a[0] = cadd(t, a[1]);
         a[1] = csub(t, a[1]);
     } else if (R == 4) {
-        float2 s0 = cadd(a[0], a[2]), d0 = csub(a[0], a[2]);
-        float2 s1 = cadd(a[1], a[3]), d1 = csub(a[1], a[3]);
-        // multiply d1 by -i (forward) or +i (inverse)
-        float2 jd1 = SIGN < 0 ? make_float2(d1.y, -d1.x) : make_float2(-d1.y, d1.x);
-        a[0] = cadd(s0, s1);
-        a[2] = csub(s0, s1);
-        a[1] = cadd(d0, jd1);
-        a[3] = csub(d0, jd1);
+        dft4<SIGN>(a[0], a[1], a[2], a[3]);
     } else if (R == 3) {
         const float c = -0.5f, s = SIGN * 0.86602540378443864676f;
         float2 t1 = cadd(a[1], a[2]), t2 = csub(a[1], a[2]);
@@ -150,6 +181,7 @@ __device__ __forceinline__ void fft_stage(float2 *d, int L, int S, int lines, in
 template <bool INV>
 __device__ __forceinline__ void fft_stage_any(float2 *d, int L, int S, int lines, int R, int m, const float2 *tw) {
     switch (R) {
+        case 8: fft_stage<8, INV>(d, L, S, lines, m, tw); break;
         case 4: fft_stage<4, INV>(d, L, S, lines, m, tw); break;
         case 2: fft_stage<2, INV>(d, L, S, lines, m, tw); break;
         case 3: fft_stage<3, INV>(d, L, S, lines, m, tw); break;
@@ -367,6 +399,7 @@ AxisPlan axis_plan(int L) {
     a.L = L;
     a.nf = 0;
     int n = L;
+    while (n % 8 == 0 && a.nf < kMaxFactors - 1) a.fac[a.nf++] = 8, n /= 8;
     while (n % 4 == 0 && a.nf < kMaxFactors - 1) a.fac[a.nf++] = 4, n /= 4;
     while (n % 2 == 0 && a.nf < kMaxFactors - 1) a.fac[a.nf++] = 2, n /= 2;
     while (n % 3 == 0 && a.nf < kMaxFactors - 1) a.fac[a.nf++] = 3, n /= 3;
