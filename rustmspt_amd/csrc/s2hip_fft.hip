@@ -12,7 +12,7 @@
 //   z-fwd : two real z rows per complex FFT (a + ib) read straight from the
 //           packed bits; DIF in LDS; split into the two half spectra
 //                                                  -> G1[nx][fy][H]  (y < ny)
-//   y-fwd : per (x, tile of 16 kz) DIF along y, rows y >= ny are zero -> G1
+//   y-fwd : per (x, tile of kz) DIF along y, rows y >= ny are zero -> G1
 //   x     : per (ky, tile of kz) DIF along x (x >= nx zero), |F|^2, DIT back,
 //           keep the 2R_x+1 needed x           -> G2[2Rx+1][fy][H]
 //   y-inv : per (ix, tile of kz) DIT along y, keep 2R_y+1 y -> G3[2Rx+1][2Ry+1][H]
@@ -51,7 +51,8 @@ struct FftPlan {
     DevBuf rev_z;    // int32[fz]: natural frequency held at each DIF output position
     DevBuf pos_z;    // int32[fz]: DIF output position of each natural frequency
     DevBuf g1, g2, g3, cube, fcounts;
-    int tile = 0;    // kz columns per workgroup in the y and x passes (0: not chosen yet)
+    int tile_y = 0;  // kz columns per workgroup in the y passes (0: not chosen yet)
+    int tile_x = 0;  // kz columns per workgroup in the x pass
 };
 
 void fft_plan_release(s2h_ctx *ctx, FftPlan *p) {
@@ -505,8 +506,10 @@ s2h_status set_lds(const void *fn, size_t bytes) {
     return S2H_OK;
 }
 
-// Run every pass; leaves the correlation cube in p->cube.
-s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
+// Run every pass; leaves the correlation cube in p->cube.  With ev, records
+// ev[0] before the y forward pass and ev[1..3] after it, the x pass and the y
+// inverse pass.
+s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = nullptr) {
     const int nx = (int)vol->dims[0], ny = (int)vol->dims[1], nz = (int)vol->dims[2];
     const int fx = (int)p->f[0], fy = (int)p->f[1], fz = (int)p->f[2];
     const int H = fz / 2 + 1;
@@ -514,8 +517,8 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     const int nxo = 2 * Rx + 1, nyo = 2 * Ry + 1;
     const Fac fcx = to_fac(p->ax[0]), fcy = to_fac(p->ax[1]), fcz = to_fac(p->ax[2]);
     hipStream_t s = ctx->stream;
-    const int tile = p->tile;
-    const int tiles = (H + tile - 1) / tile;
+    const int ty = p->tile_y, tx = p->tile_x;
+    const int tiles_y = (H + ty - 1) / ty, tiles_x = (H + tx - 1) / tx;
     // z forward
     {
         const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * 8))));
@@ -528,30 +531,34 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
+    if (ev) S2H_HIP(hipEventRecord(ev[0], s));
     {
-        const size_t lds = lds_bytes(fy, tile, fy + 1);
+        const size_t lds = lds_bytes(fy, ty, fy + 1);
         S2H_TRY(set_lds((const void *)fft_y_fwd, lds));
-        fft_y_fwd<<<nx * tiles, kFftThreads, lds, s>>>(ny, fy, H, tiles, tile, fcy, (const float2 *)p->tw[1].ptr,
-                                                       (float2 *)p->g1.ptr);
+        fft_y_fwd<<<nx * tiles_y, kFftThreads, lds, s>>>(ny, fy, H, tiles_y, ty, fcy, (const float2 *)p->tw[1].ptr,
+                                                         (float2 *)p->g1.ptr);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
+    if (ev) S2H_HIP(hipEventRecord(ev[1], s));
     {
-        const size_t lds = lds_bytes(fx, tile, fx + 1);
+        const size_t lds = lds_bytes(fx, tx, fx + 1);
         S2H_TRY(set_lds((const void *)fft_x_power, lds));
-        fft_x_power<<<fy * tiles, kFftThreads, lds, s>>>(nx, fy, H, tiles, tile, fcx, (const float2 *)p->tw[0].ptr, Rx,
-                                                         (const float2 *)p->g1.ptr, (float2 *)p->g2.ptr);
+        fft_x_power<<<fy * tiles_x, kFftThreads, lds, s>>>(nx, fy, H, tiles_x, tx, fcx, (const float2 *)p->tw[0].ptr,
+                                                           Rx, (const float2 *)p->g1.ptr, (float2 *)p->g2.ptr);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
+    if (ev) S2H_HIP(hipEventRecord(ev[2], s));
     {
-        const size_t lds = lds_bytes(fy, tile, fy + 1);
+        const size_t lds = lds_bytes(fy, ty, fy + 1);
         S2H_TRY(set_lds((const void *)fft_y_inv, lds));
-        fft_y_inv<<<nxo * tiles, kFftThreads, lds, s>>>(fy, H, tiles, tile, fcy, (const float2 *)p->tw[1].ptr, Ry,
-                                                        (const float2 *)p->g2.ptr, (float2 *)p->g3.ptr);
+        fft_y_inv<<<nxo * tiles_y, kFftThreads, lds, s>>>(fy, H, tiles_y, ty, fcy, (const float2 *)p->tw[1].ptr, Ry,
+                                                          (const float2 *)p->g2.ptr, (float2 *)p->g3.ptr);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
+    if (ev) S2H_HIP(hipEventRecord(ev[3], s));
     {
         const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * 8))));
         const size_t lds = lds_bytes(fz, ppb, fz);
@@ -568,52 +575,60 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     return S2H_OK;
 }
 
-// kz columns per workgroup of the y and x passes: the largest power of two up
-// to kTileMax whose LDS tile fits, or -- on a context that autotunes -- the
-// fastest of those candidates timed on the device (HIP events around one run
-// each, after a warm-up).  Every run rewrites all of the plan's buffers, so
-// the trial runs leave nothing behind.
+// kz columns per workgroup of the y and x passes: powers of two up to kTileMax
+// whose LDS tile fits (the widest for y, the third widest for x), or -- on a
+// context that autotunes -- for
+// each pass kind, the fastest of those candidates timed on the device (HIP
+// events around each pass of one run per candidate, after a warm-up).  The y
+// passes favour wide tiles (longer contiguous kz runs per row), the x pass
+// narrow ones (more workgroups per CU).  Every run rewrites all of the plan's
+// buffers, so the trial runs leave nothing behind.
 s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     const int lmax = (int)std::max(p->f[0], p->f[1]);
     std::vector<int> cands;
     for (int t = kTileMax; t >= 1; t /= 2)
         if (lds_bytes(lmax, t, lmax + 1) <= 150 * 1024) cands.push_back(t);
     if (cands.empty()) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
-    p->tile = cands.front();
+    // untimed: the widest tile for y, a narrow one for x (the usual picks)
+    p->tile_y = cands.front();
+    p->tile_x = cands[std::min<size_t>(2, cands.size() - 1)];
     const char *env = std::getenv("S2HIP_AUTOTUNE");
     const double cells = (double)p->f[0] * (double)p->f[1] * (double)p->f[2];
     // tune only where a run is short (up to a 1024^3 padded grid)
     if (!ctx->autotune || (env && env[0] == '0') || cands.size() < 2 || cells > 1.1e9) return S2H_OK;
     if (cands.size() > 3) cands.resize(3);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    S2H_HIP(hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return fail(S2H_ERR_HIP, "hipEventCreate failed");
-    }
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     s2h_status st = S2H_OK;
-    float best = 1e30f;
-    int pick = cands.front();
+    for (hipEvent_t &e : ev)
+        if (st == S2H_OK && hipEventCreate(&e) != hipSuccess) st = fail(S2H_ERR_HIP, "hipEventCreate failed");
+    float best_y = 1e30f, best_x = 1e30f;
+    int pick_y = cands.front(), pick_x = cands.front();
     for (int t : cands) {
-        p->tile = t;
+        if (st != S2H_OK) break;
+        p->tile_y = p->tile_x = t;
         st = fft_run(ctx, vol, p);  // warm-up
+        if (st == S2H_OK) st = fft_run(ctx, vol, p, ev);
         hipError_t e = hipSuccess;
-        if (st == S2H_OK) e = hipEventRecord(e0, ctx->stream);
-        if (st == S2H_OK && e == hipSuccess) st = fft_run(ctx, vol, p);
-        if (st == S2H_OK && e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
-        if (st == S2H_OK && e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0;
-        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (st == S2H_OK) e = hipEventSynchronize(ev[3]);
+        float yf = 0, xm = 0, yi = 0;
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yf, ev[0], ev[1]);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&xm, ev[1], ev[2]);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yi, ev[2], ev[3]);
         if (st == S2H_OK && e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
         if (st != S2H_OK) break;
-        if (ms < best) {
-            best = ms;
-            pick = t;
+        if (yf + yi < best_y) {
+            best_y = yf + yi;
+            pick_y = t;
+        }
+        if (xm < best_x) {
+            best_x = xm;
+            pick_x = t;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    p->tile = pick;
+    for (hipEvent_t &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    p->tile_y = pick_y;
+    p->tile_x = pick_x;
     return st;
 }
 
@@ -628,7 +643,7 @@ s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double 
     S2H_TRY(fft_plan_get(ctx, vol->dims, R, &fp));
     const uint64_t K = ep->offs.size();
     S2H_TRY(dev_alloc(ctx, fp->fcounts, std::max<uint64_t>(K, 1) * 4));
-    if (fp->tile == 0) S2H_TRY(fft_pick_tile(ctx, vol, fp));
+    if (fp->tile_y == 0) S2H_TRY(fft_pick_tile(ctx, vol, fp));
     prof_mark(ctx, 0);
     S2H_TRY(fft_run(ctx, vol, fp));
     if (K) {
