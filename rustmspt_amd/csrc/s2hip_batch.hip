@@ -267,19 +267,36 @@ __device__ void delta_item(const SeedDev &s, const CandDev &c, const GB gb, int 
             const int uoff = inbox ? (int)((c.ad_off + ((uint64_t)(xu - c.bx0) * sy + (yu - c.by0)) * nwd) * 8) : oob;
             const bool any_box = __any(inbox);
             const int w_lo = max(0, zlo - dz0 - DZB + 1) >> 5, w_hi = min(nwd, (zhi - dz0 + 31) >> 5);
-            for (int w = w_lo; w < w_hi; ++w) {
-                uint32_t a = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, aoff + 4 * w, 0, 0);
-                if (any_box) {  // lanes outside the box read (0, 0): n = o
-                    const uint32_t ua = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w, 0, 0);
-                    const uint32_t ud = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w + 4, 0, 0);
-                    a = (a & ~ud) | ua;
+            // word w + 1's loads are issued while word w counts; the changed
+            // row's words m carry over (word w's upper word is w + 1's lower)
+            if (w_lo < w_hi) {
+                uint32_t an = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, aoff + 4 * w_lo, 0, 0);
+                uint32_t uan = 0, udn = 0;  // lanes outside the box read (0, 0): n = o
+                if (any_box) {
+                    uan = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w_lo, 0, 0);
+                    udn = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * w_lo + 4, 0, 0);
                 }
-                const uint2 m0 = adw(m, w + q0, nwd), m1 = adw(m, w + q0 + 1, nwd);
-                const uint64_t pa = ((uint64_t)m1.x << 32) | m0.x, pd = ((uint64_t)m1.y << 32) | m0.y;
+                uint2 m0 = adw(m, w_lo + q0, nwd), m1n = adw(m, w_lo + q0 + 1, nwd);
+                for (int w = w_lo; w < w_hi; ++w) {
+                    uint32_t a = an;
+                    const uint32_t ua = uan, ud = udn;
+                    const uint2 m1 = m1n;
+                    if (w + 1 < w_hi) {
+                        an = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, aoff + 4 * (w + 1), 0, 0);
+                        if (any_box) {
+                            uan = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * (w + 1), 0, 0);
+                            udn = __builtin_amdgcn_raw_buffer_load_b32(ad_rs, uoff + 8 * (w + 1) + 4, 0, 0);
+                        }
+                        m1n = adw(m, w + q0 + 2, nwd);
+                    }
+                    if (any_box) a = (a & ~ud) | ua;
+                    const uint64_t pa = ((uint64_t)m1.x << 32) | m0.x, pd = ((uint64_t)m1.y << 32) | m0.y;
 #pragma unroll
-                for (int j = 0; j < DZB; ++j) {
-                    cp[j] = popc_acc(a & (uint32_t)(pa >> j), cp[j]);
-                    cm[j] = popc_acc(a & (uint32_t)(pd >> j), cm[j]);
+                    for (int j = 0; j < DZB; ++j) {
+                        cp[j] = popc_acc(a & (uint32_t)(pa >> j), cp[j]);
+                        cm[j] = popc_acc(a & (uint32_t)(pd >> j), cm[j]);
+                    }
+                    m0 = m1;
                 }
             }
         }
@@ -289,14 +306,23 @@ __device__ void delta_item(const SeedDev &s, const CandDev &c, const GB gb, int 
             const bool in = active && xv >= 0 && xv < s.nx && yv >= 0 && yv < s.ny;
             const int voff = in ? (int)(((uint64_t)xv * s.ny + yv) * s.row_dw * 4) : oob;
             const int w_lo = min(nwd, (zlo + dz0) >> 5), w_hi = min(nwd, (zhi + dz0 + DZB - 1 + 31) >> 5);
-            for (int w = w_lo; w < w_hi; ++w) {
-                const uint32_t a = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, voff + 4 * w, 0, 0);
-                const uint2 m1 = adw(m, w - q0, nwd), m0 = adw(m, w - q0 - 1, nwd);
-                const uint64_t pa = ((uint64_t)m1.x << 32) | m0.x, pd = ((uint64_t)m1.y << 32) | m0.y;
+            if (w_lo < w_hi) {
+                uint32_t an = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, voff + 4 * w_lo, 0, 0);
+                uint2 m0 = adw(m, w_lo - q0 - 1, nwd), m1n = adw(m, w_lo - q0, nwd);
+                for (int w = w_lo; w < w_hi; ++w) {
+                    const uint32_t a = an;
+                    const uint2 m1 = m1n;
+                    if (w + 1 < w_hi) {
+                        an = __builtin_amdgcn_raw_buffer_load_b32(seed_rs, voff + 4 * (w + 1), 0, 0);
+                        m1n = adw(m, w + 1 - q0, nwd);
+                    }
+                    const uint64_t pa = ((uint64_t)m1.x << 32) | m0.x, pd = ((uint64_t)m1.y << 32) | m0.y;
 #pragma unroll
-                for (int j = 0; j < DZB; ++j) {
-                    cp[j] = popc_acc(a & (uint32_t)((pa << j) >> 32), cp[j]);
-                    cm[j] = popc_acc(a & (uint32_t)((pd << j) >> 32), cm[j]);
+                    for (int j = 0; j < DZB; ++j) {
+                        cp[j] = popc_acc(a & (uint32_t)((pa << j) >> 32), cp[j]);
+                        cm[j] = popc_acc(a & (uint32_t)((pd << j) >> 32), cm[j]);
+                    }
+                    m0 = m1;
                 }
             }
         }
