@@ -620,21 +620,36 @@ static int block_chunks(const Grouping &g, const Grouping::Blk &b, int waves) {
     return std::max(1, std::min(b.x1 - b.x0, n));
 }
 
+// Workgroups (kWavesPerBlock waves) resident on the whole chip at once: 256 CUs
+// x 5 (the count kernel's 5 waves per SIMD).
+static constexpr double kBlockSlots = 256.0 * 5.0;
+
+// Each item ends with one u64 atomic per (lane, dz), which reaches memory at
+// 32-byte granularity; measured at 256^3 r64, doubling the items from 32768 to
+// 65536 added 0.46 ms for 1.6 GB more of these writes.
+static constexpr double kFlushBytesPerS = 3.5e12;
+
 // Cost model: executed VALU wave-instructions at the issue rate of 1024 SIMDs
-// (4 cycles each), scaled down when there are too few waves to hide latency,
-// and bounded below by the longest item.
+// (4 cycles each), scaled down when there are too few waves to hide latency
+// and for the partly filled last round of workgroups (r rounds of resident
+// workgroups lose about half a round: r / (r + 1/2)), bounded below by the
+// longest item, plus the items' counter flushes.
 static double model_ms(const Grouping &g, int waves, double *ops_out) {
-    double items = 0, maxitem = 0;
+    double items = 0, maxitem = 0, flush_bytes = 0;
     for (const Grouping::Blk &b : g.blks) {
         const int n = block_chunks(g, b, waves);
         items += n;
+        flush_bytes += (double)n * 64.0 * b.b.dzb * 32.0;
         maxitem = std::max(maxitem, b.cost * std::ceil((double)(b.x1 - b.x0) / n) / (b.x1 - b.x0) + kItemOverhead);
     }
     const double ops = g.total + kItemOverhead * items;
     if (ops_out) *ops_out = ops;
     const double simds = 1024.0, clk = 2.4e9;
     const double fill = std::min(1.0, items / (simds * 6.0));
-    return 1e3 * std::max(ops * 4.0 / (simds * clk * std::max(fill, 0.05)), maxitem * 4.0 * 2.0 / clk);
+    const double rounds = items / kWavesPerBlock / kBlockSlots;
+    const double tail = rounds / (rounds + 0.5);
+    return 1e3 * (std::max(ops * 4.0 / (simds * clk * std::max(fill, 0.05) * tail), maxitem * 4.0 * 2.0 / clk) +
+                  flush_bytes / kFlushBytesPerS);
 }
 
 static s2h_status apply_shape(s2h_ctx *ctx, ExactPlan *p, const Grouping &g, int waves) {
@@ -695,7 +710,8 @@ static std::vector<Ranked> rank_tunes(const ExactPlan *p) {
     for (int band : {1, 2, 3, 4, 6}) {
         Grouping g;
         group_columns(p, band, g);
-        for (int waves : {8192, 16384, 32768}) out.push_back(Ranked{ExactTune{band, waves}, model_ms(g, waves, nullptr)});
+        for (int waves : {8192, 16384, 32768, 65536})
+            out.push_back(Ranked{ExactTune{band, waves}, model_ms(g, waves, nullptr)});
     }
     std::stable_sort(out.begin(), out.end(), [](const Ranked &a, const Ranked &b) { return a.ms < b.ms; });
     return out;
@@ -831,6 +847,7 @@ static s2h_status launch_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
 // are integers, so the choice never changes a result.
 static constexpr double kTuneMaxMs = 80.0;
 static constexpr int kTuneCandidates = 4;
+static constexpr int kTuneRounds = 3;
 
 static s2h_status autotune(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
     const std::vector<Ranked> ranked = rank_tunes(p);
@@ -842,26 +859,29 @@ static s2h_status autotune(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
         return fail(S2H_ERR_HIP, "hipEventCreate failed");
     }
     s2h_status st = S2H_OK;
-    double best_ms = 1e30;
     int best = -1, timed = 0;
     const int ncand = std::min<int>(kTuneCandidates, (int)ranked.size());
-    for (int k = 0; k < ncand && st == S2H_OK; ++k) {
-        st = apply_tune(ctx, p, ranked[k].tune);
-        if (st == S2H_OK) st = launch_counts(ctx, vol, p);  // warm-up
-        if (st != S2H_OK) break;
-        hipError_t e = hipEventRecord(e0, ctx->stream);
-        if (e == hipSuccess) st = launch_counts(ctx, vol, p);
-        if (e == hipSuccess && st == S2H_OK) e = hipEventRecord(e1, ctx->stream);
-        if (e == hipSuccess && st == S2H_OK) e = hipEventSynchronize(e1);
-        float ms = 0;
-        if (e == hipSuccess && st == S2H_OK) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
-        if (st == S2H_OK && ms < best_ms) {
-            best_ms = ms;
-            best = k;
+    // kTuneRounds interleaved rounds, the fastest run of each candidate counts:
+    // one run each is too noisy to separate shapes a few percent apart
+    std::vector<float> best_ms(ncand, 1e30f);
+    for (int round = 0; round < kTuneRounds && st == S2H_OK; ++round) {
+        for (int k = 0; k < ncand && st == S2H_OK; ++k) {
+            st = apply_tune(ctx, p, ranked[k].tune);
+            if (st == S2H_OK && round == 0) st = launch_counts(ctx, vol, p);  // warm-up
+            if (st != S2H_OK) break;
+            hipError_t e = hipEventRecord(e0, ctx->stream);
+            if (e == hipSuccess) st = launch_counts(ctx, vol, p);
+            if (e == hipSuccess && st == S2H_OK) e = hipEventRecord(e1, ctx->stream);
+            if (e == hipSuccess && st == S2H_OK) e = hipEventSynchronize(e1);
+            float ms = 0;
+            if (e == hipSuccess && st == S2H_OK) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
+            if (st == S2H_OK) best_ms[k] = std::min(best_ms[k], ms);
+            if (round == 0) ++timed;
         }
-        ++timed;
     }
+    for (int k = 0; k < ncand; ++k)
+        if (best < 0 || best_ms[k] < best_ms[best]) best = k;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st != S2H_OK) return st;

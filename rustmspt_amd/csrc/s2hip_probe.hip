@@ -3,10 +3,14 @@
 //
 // Every lane runs the counting loop's instruction pair -- v_and_b32 with a
 // wave-uniform (SGPR) operand, then v_bcnt_u32_b32 accumulating -- for many
-// iterations, with enough waves to fill every SIMD, in two register patterns
-// (fresh operands; the counting loop's reuse of one lane word across 32
-// shifts).  The faster is reported.  The only memory traffic is one store per
-// lane at the end.
+// iterations, with enough waves to fill every SIMD.  The pair is written as
+// inline asm so each and/popcount pair is exactly two VALU instructions (left
+// to itself the compiler adds VALU work around the shifts, which understates
+// the rate); the shifts run on the scalar unit, as in the counting loop.
+// Three register patterns: fresh operands per chain; one lane word against 32
+// successive shifts of a wave-uniform pair (the counting loop's reuse); two
+// lane words per shifted value.  The fastest is reported.  The only memory
+// traffic is one store per lane at the end.
 #include <algorithm>
 
 #include "s2hip_internal.h"
@@ -17,7 +21,13 @@ namespace {
 constexpr int kChains = 16;
 constexpr int kProbeThreads = 256;
 
-// Variant 0: every instruction reads fresh operands (kChains independent chains).
+__device__ __forceinline__ void and_popc(uint32_t &acc, uint32_t lane_word, uint32_t uniform) {
+    uint32_t t;
+    asm volatile("v_and_b32 %0, %1, %2" : "=v"(t) : "s"(uniform), "v"(lane_word));
+    asm volatile("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(t));
+}
+
+// Variant 0: kChains independent chains, a new uniform operand every iteration.
 __global__ __launch_bounds__(kProbeThreads) void valu_probe_kernel(uint32_t seed, int iters,
                                                                    uint32_t *__restrict__ out) {
     uint32_t x[kChains], acc[kChains];
@@ -30,7 +40,7 @@ __global__ __launch_bounds__(kProbeThreads) void valu_probe_kernel(uint32_t seed
     for (int i = 0; i < iters; ++i) {
         m = m * 1664525u + 1013904223u;  // scalar: a new uniform mask every iteration
 #pragma unroll
-        for (int k = 0; k < kChains; ++k) acc[k] = popc_acc(x[k] & (m >> k), acc[k]);
+        for (int k = 0; k < kChains; ++k) and_popc(acc[k], x[k], m >> k);
     }
     uint32_t s = 0;
 #pragma unroll
@@ -38,18 +48,23 @@ __global__ __launch_bounds__(kProbeThreads) void valu_probe_kernel(uint32_t seed
     out[(size_t)blockIdx.x * kProbeThreads + threadIdx.x] = s;
 }
 
-// Variant 1: the counting loop's register pattern -- kWords lane words, each
-// ANDed with kShifts successive shifts of a wave-uniform 64-bit pair (scalar
-// s_lshr_b64) into kShifts accumulators.
+// Variants 1 and 2: the counting loop's register pattern -- kWords lane words,
+// each ANDed with kShifts successive shifts of a wave-uniform 64-bit pair
+// (scalar s_lshr_b64) into kShifts accumulators; with TWO, a second set of
+// lane words meets every shifted value too.
 constexpr int kWords = 4, kShifts = 32;
 
+template <bool TWO>
 __global__ __launch_bounds__(kProbeThreads) void valu_probe_reuse_kernel(uint32_t seed, int iters,
                                                                          uint32_t *__restrict__ out) {
-    uint32_t x[kWords], acc[kShifts];
+    uint32_t x[kWords], x2[kWords], acc[kShifts], acc2[kShifts];
 #pragma unroll
-    for (int w = 0; w < kWords; ++w) x[w] = seed * (threadIdx.x + 1u) + 0x9e3779b9u * (uint32_t)(w + 1);
+    for (int w = 0; w < kWords; ++w) {
+        x[w] = seed * (threadIdx.x + 1u) + 0x9e3779b9u * (uint32_t)(w + 1);
+        x2[w] = x[w] * 747796405u;
+    }
 #pragma unroll
-    for (int j = 0; j < kShifts; ++j) acc[j] = (uint32_t)j;
+    for (int j = 0; j < kShifts; ++j) acc[j] = acc2[j] = (uint32_t)j;
     uint64_t m = __builtin_amdgcn_readfirstlane(seed ^ 0x5bd1e995u);
     for (int i = 0; i < iters; ++i) {
         m = m * 6364136223846793005ull + 1442695040888963407ull;  // scalar
@@ -57,12 +72,16 @@ __global__ __launch_bounds__(kProbeThreads) void valu_probe_reuse_kernel(uint32_
         for (int w = 0; w < kWords; ++w) {
             const uint64_t pair = m ^ ((uint64_t)w << 32);
 #pragma unroll
-            for (int j = 0; j < kShifts; ++j) acc[j] = popc_acc(x[w] & (uint32_t)(pair >> j), acc[j]);
+            for (int j = 0; j < kShifts; ++j) {
+                const uint32_t sh = (uint32_t)(pair >> j);
+                and_popc(acc[j], x[w], sh);
+                if (TWO) and_popc(acc2[j], x2[w], sh);
+            }
         }
     }
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < kShifts; ++j) s += acc[j];
+    for (int j = 0; j < kShifts; ++j) s += acc[j] + (TWO ? acc2[j] : 0u);
     out[(size_t)blockIdx.x * kProbeThreads + threadIdx.x] = s;
 }
 
@@ -87,10 +106,10 @@ s2h_status s2h_valu_probe(s2h_ctx *ctx, double *lane_ops_per_s) {
     s2h_status st = S2H_OK;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = fail(S2H_ERR_HIP, "hipEventCreate");
     double best_rate = 0;
-    for (int variant = 0; variant < 2 && st == S2H_OK; ++variant) {
-        // same number of and/popcount pairs per lane in both variants
-        const int pairs_per_iter = variant == 0 ? kChains : kWords * kShifts;
-        const int it = iters * kChains / pairs_per_iter;
+    for (int variant = 0; variant < 3 && st == S2H_OK; ++variant) {
+        // about the same number of and/popcount pairs per lane in every variant
+        const int pairs_per_iter = variant == 0 ? kChains : variant == 1 ? kWords * kShifts : 2 * kWords * kShifts;
+        const int it = std::max(1, iters * kChains / pairs_per_iter);
         const double ops_v = (double)blocks * kProbeThreads * (double)it * pairs_per_iter * 2.0;
         float best = 1e30f;
         for (int rep = 0; rep < 4 && st == S2H_OK; ++rep) {
@@ -99,9 +118,12 @@ s2h_status s2h_valu_probe(s2h_ctx *ctx, double *lane_ops_per_s) {
                 if (variant == 0)
                     valu_probe_kernel<<<blocks, kProbeThreads, 0, ctx->stream>>>(0x1234567u + rep, it,
                                                                                   (uint32_t *)out.ptr);
+                else if (variant == 1)
+                    valu_probe_reuse_kernel<false><<<blocks, kProbeThreads, 0, ctx->stream>>>(0x1234567u + rep, it,
+                                                                                               (uint32_t *)out.ptr);
                 else
-                    valu_probe_reuse_kernel<<<blocks, kProbeThreads, 0, ctx->stream>>>(0x1234567u + rep, it,
-                                                                                        (uint32_t *)out.ptr);
+                    valu_probe_reuse_kernel<true><<<blocks, kProbeThreads, 0, ctx->stream>>>(0x1234567u + rep, it,
+                                                                                              (uint32_t *)out.ptr);
                 e = hipGetLastError();
             }
             if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
