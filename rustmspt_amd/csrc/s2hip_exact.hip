@@ -39,10 +39,54 @@ static constexpr int kWavesPerBlock = 4;
 // Volume buffers carry kVolTailBytes of zero padding after the last row, so an
 // unguarded read of up to 64 dwords past any row start stays inside the
 // allocation; bits that do not belong to the row are masked to zero.
+// The workgroup's flush: items are ordered by cost, so the waves of one
+// workgroup usually hold x-chunks of the same (group, dz block).  Then wave 0
+// adds the four waves' counters (through LDS) and issues one atomic per
+// (lane, dz) instead of four; otherwise every wave adds its own.  Every wave
+// of the workgroup passes both barriers exactly once.
+struct FlushLds {
+    uint32_t (*red)[32][64];  // counters of waves 1 .. kWavesPerBlock-1: [wave - 1][dz][lane]
+    const uint32_t *key;      // per wave: (group, dz0) of its item, or a value no item has
+    int w;                    // this wave's index in the workgroup
+};
+
+template <int DZB>
+__device__ __forceinline__ void flush_counts(const uint32_t (&cnt)[DZB], const WorkItem &it, const ColInfo &c,
+                                             unsigned long long *__restrict__ table, const FlushLds &f) {
+    const int lane = threadIdx.x & 63;
+    if (f.w > 0) {
+#pragma unroll
+        for (int j = 0; j < DZB; ++j) f.red[f.w - 1][j][lane] = cnt[j];
+    }
+    __syncthreads();
+    bool same = true;
+#pragma unroll
+    for (int q = 1; q < kWavesPerBlock; ++q) same = same && f.key[q] == f.key[0];
+    if (same) {
+        if (f.w == 0) {
+#pragma unroll
+            for (int j = 0; j < DZB; ++j) {
+                unsigned long long sum = cnt[j];
+#pragma unroll
+                for (int q = 0; q < kWavesPerBlock - 1; ++q) sum += f.red[q][j][lane];
+                const int dz = it.dz0 + j;
+                if (c.m >= 0 && dz <= c.m && sum) atomicAdd(&table[c.base + dz], sum);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < DZB; ++j) {
+            const int dz = it.dz0 + j;
+            if (c.m >= 0 && dz <= c.m && cnt[j]) atomicAdd(&table[c.base + dz], (unsigned long long)cnt[j]);
+        }
+    }
+    __syncthreads();
+}
+
 template <int NW, int DZB>
 __device__ __forceinline__ void count_item(const uint32_t *__restrict__ vol, const __amdgpu_buffer_rsrc_t rsrc, int nx,
                                            int ny, int row_dw, const WorkItem &it, const ColInfo &c,
-                                           unsigned long long *__restrict__ table) {
+                                           unsigned long long *__restrict__ table, const FlushLds &f) {
     const int q0 = it.dz0 >> 5;
     const int nw = it.nw;  // partner dwords q0 .. q0+nw-1 hold voxels; later ones read as 0
     const int oob = 0x7fffff00;  // + up to 255 bytes: past any buffer, no int overflow
@@ -90,27 +134,23 @@ __device__ __forceinline__ void count_item(const uint32_t *__restrict__ vol, con
             }
         }
     }
-#pragma unroll
-    for (int j = 0; j < DZB; ++j) {
-        const int dz = it.dz0 + j;
-        if (c.m >= 0 && dz <= c.m && cnt[j]) atomicAdd(&table[c.base + dz], (unsigned long long)cnt[j]);
-    }
+    flush_counts<DZB>(cnt, it, c, table, f);
 }
 
 // dz blocks are 4..32 wide in steps of 4 (the wave-uniform width of the item).
 template <int NW>
 __device__ __forceinline__ void count_dzb(const uint32_t *__restrict__ vol, const __amdgpu_buffer_rsrc_t rsrc, int nx,
                                           int ny, int row_dw, const WorkItem &it, const ColInfo &c,
-                                          unsigned long long *__restrict__ table) {
+                                          unsigned long long *__restrict__ table, const FlushLds &f) {
     switch (it.dzb) {
-        case 4: count_item<NW, 4>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        case 8: count_item<NW, 8>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        case 12: count_item<NW, 12>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        case 16: count_item<NW, 16>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        case 20: count_item<NW, 20>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        case 24: count_item<NW, 24>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        case 28: count_item<NW, 28>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
-        default: count_item<NW, 32>(vol, rsrc, nx, ny, row_dw, it, c, table); break;
+        case 4: count_item<NW, 4>(vol, rsrc, nx, ny, row_dw, it, c, table, f); break;
+        case 8: count_item<NW, 8>(vol, rsrc, nx, ny, row_dw, it, c, table, f); break;
+        case 12: count_item<NW, 12>(vol, rsrc, nx, ny, row_dw, it, c, table, f); break;
+        case 16: count_item<NW, 16>(vol, rsrc, nx, ny, row_dw, it, c, table, f); break;
+        case 20: count_item<NW, 20>(vol, rsrc, nx, ny, row_dw, it, c, table, f); break;
+        case 24: count_item<NW, 24>(vol, rsrc, nx, ny, row_dw, it, c, table, f); break;
+        case 28: count_item<NW, 28>(vol, rsrc, nx, ny, row_dw, it, c, table, f); break;
+        default: count_item<NW, 32>(vol, rsrc, nx, ny, row_dw, it, c, table, f); break;
     }
 }
 
@@ -121,10 +161,19 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
     const uint32_t *__restrict__ vol, uint32_t vol_bytes, int nx, int ny, int row_dw,
     const WorkItem *__restrict__ items, int nitems, const ColInfo *__restrict__ cols,
     unsigned long long *__restrict__ table) {
-    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + threadIdx.x / 64));
-    if (wave >= nitems) return;
+    __shared__ uint32_t s_red[kWavesPerBlock - 1][32][64];
+    __shared__ uint32_t s_key[kWavesPerBlock];
+    const int w = (int)(threadIdx.x / 64);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + w));
     const int lane = threadIdx.x & 63;
+    if (wave >= nitems) {  // no item: a key no item has, then the flush's two barriers
+        if (lane == 0) s_key[w] = 0xffffffffu - (uint32_t)w;
+        __syncthreads();
+        __syncthreads();
+        return;
+    }
     const WorkItem it = items[wave];
+    if (lane == 0) s_key[w] = (it.group << 10) | it.dz0;  // dz0 < 1024
     const ColInfo c = cols[it.group * 64 + lane];
     // per-lane rows through a buffer resource: an out-of-range offset reads 0
     const __amdgpu_buffer_rsrc_t rsrc =
@@ -132,10 +181,11 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
     // Blocks whose partner words end one dword early (dz0 >= 32, or a row of
     // fewer dwords than NW) run the NW - 1 word body: the last word would
     // only meet zeros.
+    const FlushLds f{s_red, s_key, w};
     if (NW > 1 && it.nw < NW)
-        count_dzb<(NW > 1 ? NW - 1 : 1)>(vol, rsrc, nx, ny, row_dw, it, c, table);
+        count_dzb<(NW > 1 ? NW - 1 : 1)>(vol, rsrc, nx, ny, row_dw, it, c, table, f);
     else
-        count_dzb<NW>(vol, rsrc, nx, ny, row_dw, it, c, table);
+        count_dzb<NW>(vol, rsrc, nx, ny, row_dw, it, c, table, f);
 }
 
 // ---------------------------------------------------------------- gather / fold
@@ -625,8 +675,9 @@ static int block_chunks(const Grouping &g, const Grouping::Blk &b, int waves) {
 static constexpr double kBlockSlots = 256.0 * 5.0;
 
 // Each item ends with one u64 atomic per (lane, dz), which reaches memory at
-// 32-byte granularity; measured at 256^3 r64, doubling the items from 32768 to
-// 65536 added 0.46 ms for 1.6 GB more of these writes.
+// 32-byte granularity; measured at 256^3 r64 (one flush per wave), doubling the
+// items from 32768 to 65536 added 0.46 ms for 1.6 GB more of these writes.
+// The workgroup flush usually merges its waves' items into one.
 static constexpr double kFlushBytesPerS = 3.5e12;
 
 // Cost model: executed VALU wave-instructions at the issue rate of 1024 SIMDs
@@ -639,7 +690,7 @@ static double model_ms(const Grouping &g, int waves, double *ops_out) {
     for (const Grouping::Blk &b : g.blks) {
         const int n = block_chunks(g, b, waves);
         items += n;
-        flush_bytes += (double)n * 64.0 * b.b.dzb * 32.0;
+        flush_bytes += (double)n * 64.0 * b.b.dzb * 32.0 / kWavesPerBlock;
         maxitem = std::max(maxitem, b.cost * std::ceil((double)(b.x1 - b.x0) / n) / (b.x1 - b.x0) + kItemOverhead);
     }
     const double ops = g.total + kItemOverhead * items;
