@@ -28,10 +28,10 @@ sys.path.insert(0, str(ROOT))
 # figure and this one is reported beside it.
 VALU_4CYCLE_OPS = 256 * 64 * 2.4e9
 # HBM bytes of one evaluation's count phase at 256^3 r_max=64 (500-particle
-# packing, launch shape band 3 / 32768 waves), from rocprofv3 --pmc FETCH_SIZE
+# packing, launch shape band 3 / 65536 waves), from rocprofv3 --pmc FETCH_SIZE
 # (x2.0) and WRITE_SIZE (x1.0) of the count dispatch; factors from a 512 MiB
 # device copy.  Command lines and figures: profiles/exact_count_kernel_256cube_r64.md.
-COUNT_PHASE_TRAFFIC = 13060 * 1024 * 2.0 + 1677000 * 1024 * 1.0
+COUNT_PHASE_TRAFFIC = 14840 * 1024 * 2.0 + 862900 * 1024 * 1.0
 
 
 def _cpu_model() -> str:
@@ -494,7 +494,7 @@ def main():
     achieved = alg_ops / (count_avg * 1e-3)
     valu_measured = ctx.valu_probe()  # issue ceiling of the counting loop's and/popcount pair
     shape = ctx.exact_tune_info([grid] * 3, r_max, 1.0)
-    profiled = (grid, r_max, args.particles, shape["band"], shape["waves"]) == (256, 64, 500, 3, 32768)
+    profiled = (grid, r_max, args.particles, shape["band"], shape["waves"]) == (256, 64, 500, 3, 65536)
 
     out = {
         "metric": "exact-S2 evals/s at 256^3 r_max=64, batched vs plain; p50/p99 served latency",
