@@ -26,7 +26,18 @@
 // triangles are prepared and binned by that bound on a 2-D grid over the plane
 // on the device (count per cell, prefix sum, scatter); a query tests only its
 // cell's list, so the hit multiset equals a full scan.
+//
+// Schedule: the queries are answered in cell order.  A first kernel draws
+// every sample's p and q (and counts the valid ones), a radix sort orders the
+// p queries by grid cell, a second kernel answers them, a second sort orders
+// the q queries of the samples whose p is inside, and a third kernel answers
+// those and counts the hits.  Neighbouring lanes then read the same cell lists
+// and triangles, which stay in the caches; answered in sample order, every
+// query was a fresh gather from memory.  hits and valid are sums over samples,
+// so the order changes no count.
 // Compiled with -ffp-contract=off: every f64 operation keeps its order.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -42,6 +53,7 @@ constexpr uint32_t kTag = 0x4D455348u;
 constexpr int kDirAttempts = 64;
 constexpr int kThreads = 256;
 constexpr int kPerThread = 8;
+constexpr uint64_t kPassSamples = 16ull << 20;  // samples per pass of the sorted schedule
 
 struct Grid {
     const vox::TriGeo *tris;
@@ -81,18 +93,29 @@ __device__ int overflow_count(const Grid &g, uint32_t c0, uint32_t c1, double px
     return unique;
 }
 
-__device__ bool contains(const Grid &g, double px, double py, double pz) {
-    if (!g.have_bbox) return false;
+// The key past every cell (sorts after all of them): no query.
+__device__ __forceinline__ uint32_t no_cell(const Grid &g) { return (uint32_t)g.gu * (uint32_t)g.gv; }
+
+// The grid cell a query at p reads, or no_cell when p lies outside the mesh's
+// vertex bbox +-1e-9 (then it is outside the mesh).
+__device__ uint32_t query_cell(const Grid &g, double px, double py, double pz) {
+    if (!g.have_bbox) return no_cell(g);
     const double eps = 1e-9;
     if (px < g.mn[0] - eps || px > g.mx[0] + eps || py < g.mn[1] - eps || py > g.mx[1] + eps ||
         pz < g.mn[2] - eps || pz > g.mx[2] + eps)
-        return false;
+        return no_cell(g);
     const double pu = vox::kUx * px + vox::kUy * py + vox::kUz * pz;
     const double pv = vox::kVx * px + vox::kVy * py + vox::kVz * pz;
     int iu = (int)floor((pu - g.u0) * g.inv_du), iv = (int)floor((pv - g.v0) * g.inv_dv);
     iu = min(max(iu, 0), g.gu - 1);
     iv = min(max(iv, 0), g.gv - 1);
-    const uint32_t cell = (uint32_t)iu * g.gv + iv;
+    return (uint32_t)iu * g.gv + iv;
+}
+
+// The parity query of a point whose cell is `cell` (from query_cell).
+__device__ bool contains_in_cell(const Grid &g, uint32_t cell, double px, double py, double pz) {
+    const double pu = vox::kUx * px + vox::kUy * py + vox::kUz * pz;
+    const double pv = vox::kVx * px + vox::kVy * py + vox::kVz * pz;
     const uint32_t c0 = g.start[cell], c1 = g.start[cell + 1];
     double hits[vox::kMaxHits];
     int nh = 0;
@@ -136,18 +159,30 @@ __device__ bool contains(const Grid &g, double px, double py, double pz) {
     return unique & 1;
 }
 
-__global__ __launch_bounds__(kThreads) void meshmc_kernel(Grid g, double bx0, double by0, double bz0, double bx1,
-                                                          double by1, double bz1, uint64_t samples, uint32_t seed_lo,
-                                                          uint32_t seed_hi, uint32_t blocks_per_r,
-                                                          unsigned long long *__restrict__ hv) {
-    const uint32_t r = blockIdx.x / blocks_per_r + 1;
+struct McSample {
+    double p[3], q[3];
+};
+
+// 1. Every sample's p and q (the stream described above) for radii r0 ..; a
+// valid sample's key is the cell of its p query (no_cell: no query, or p
+// outside the mesh bbox), and every sample carries its index.  Counts the
+// valid samples per radius.
+__global__ __launch_bounds__(kThreads) void mc_points_kernel(Grid g, double bx0, double by0, double bz0, double bx1,
+                                                             double by1, double bz1, uint64_t samples, uint32_t r0,
+                                                             uint32_t seed_lo, uint32_t seed_hi, uint32_t blocks_per_r,
+                                                             McSample *__restrict__ smp, uint32_t *__restrict__ key,
+                                                             uint32_t *__restrict__ idx,
+                                                             unsigned long long *__restrict__ hv) {
+    const uint32_t rl = blockIdx.x / blocks_per_r;  // radius within this pass
+    const uint32_t r = r0 + rl;
     const uint32_t blk = blockIdx.x % blocks_per_r;
     const double rr = (double)r;
-    uint64_t hits = 0, valid = 0;
+    uint64_t valid = 0;
     const uint64_t base = (uint64_t)blk * kThreads * kPerThread + threadIdx.x;
     for (int k = 0; k < kPerThread; ++k) {
         const uint64_t i = base + (uint64_t)k * kThreads;
         if (i >= samples) break;
+        const uint64_t sidx = (uint64_t)rl * samples + i;
         const uint32_t ilo = (uint32_t)i, ihi = (uint32_t)(i >> 32);
         uint32_t a[4], b[4];
         philox4x32_10(ilo, ihi, r, kTag + 0u, seed_lo, seed_hi, a);
@@ -172,17 +207,53 @@ __global__ __launch_bounds__(kThreads) void meshmc_kernel(Grid g, double bx0, do
             }
         }
         const double qx = px + dx * rr, qy = py + dy * rr, qz = pz + dz * rr;
-        if (qx < bx0 || qx > bx1 || qy < by0 || qy > by1 || qz < bz0 || qz > bz1) continue;
-        ++valid;
-        if (contains(g, px, py, pz) && contains(g, qx, qy, qz)) ++hits;
+        const bool ok = !(qx < bx0 || qx > bx1 || qy < by0 || qy > by1 || qz < bz0 || qz > bz1);
+        if (ok) ++valid;
+        McSample m;
+        m.p[0] = px;
+        m.p[1] = py;
+        m.p[2] = pz;
+        m.q[0] = qx;
+        m.q[1] = qy;
+        m.q[2] = qz;
+        smp[sidx] = m;
+        key[sidx] = ok ? query_cell(g, px, py, pz) : no_cell(g);
+        idx[sidx] = (uint32_t)sidx;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        hits += __shfl_xor(hits, o, 64);
-        valid += __shfl_xor(valid, o, 64);
+    for (int o = 32; o > 0; o >>= 1) valid += __shfl_xor(valid, o, 64);
+    if ((threadIdx.x & 63) == 0 && valid) atomicAdd(&hv[2 * r + 1], (unsigned long long)valid);
+}
+
+// 2. The p queries in cell order; a sample whose p is inside gets the cell of
+// its q query as key for the next sort (no_cell otherwise).
+__global__ __launch_bounds__(kThreads) void mc_query_p_kernel(Grid g, uint64_t n, const uint32_t *__restrict__ key,
+                                                              const uint32_t *__restrict__ idx,
+                                                              const McSample *__restrict__ smp,
+                                                              uint32_t *__restrict__ qkey,
+                                                              uint32_t *__restrict__ qidx) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = key[j], sidx = idx[j];
+        uint32_t k2 = no_cell(g);
+        if (c != no_cell(g)) {
+            const McSample &m = smp[sidx];
+            if (contains_in_cell(g, c, m.p[0], m.p[1], m.p[2])) k2 = query_cell(g, m.q[0], m.q[1], m.q[2]);
+        }
+        qkey[j] = k2;
+        qidx[j] = sidx;
     }
-    if ((threadIdx.x & 63) == 0) {
-        if (hits) atomicAdd(&hv[2 * r], (unsigned long long)hits);
-        if (valid) atomicAdd(&hv[2 * r + 1], (unsigned long long)valid);
+}
+
+// 3. The q queries in cell order: a hit when q is inside too.
+__global__ __launch_bounds__(kThreads) void mc_query_q_kernel(Grid g, uint64_t n, const uint32_t *__restrict__ key,
+                                                              const uint32_t *__restrict__ idx,
+                                                              const McSample *__restrict__ smp, uint64_t samples,
+                                                              uint32_t r0, unsigned long long *__restrict__ hv) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t c = key[j];
+        if (c == no_cell(g)) continue;
+        const uint32_t sidx = idx[j];
+        const McSample &m = smp[sidx];
+        if (contains_in_cell(g, c, m.q[0], m.q[1], m.q[2])) atomicAdd(&hv[2 * (r0 + sidx / samples)], 1ull);
     }
 }
 
@@ -345,16 +416,68 @@ s2h_status bin_and_run(s2h_ctx *ctx, Grid g, double cmax, double smax, double am
     prof_mark(ctx, 1);
     if (e == hipSuccess) {
         clear_u64_kernel<<<1, 256, 0, ctx->stream>>>((unsigned long long *)d_hv.ptr, n);
-        meshmc_kernel<<<(unsigned)(bpr * r_max), kThreads, 0, ctx->stream>>>(
-            g, bbox_min[0], bbox_min[1], bbox_min[2], bbox_max[0], bbox_max[1], bbox_max[2], spr, (uint32_t)seed,
-            (uint32_t)(seed >> 32), (uint32_t)bpr, (unsigned long long *)d_hv.ptr);
         e = hipGetLastError();
     }
-    ctx->launches = 7;
+    // radii in passes of at most kPassSamples samples (64 bytes each)
+    const uint64_t rpp = std::max<uint64_t>(1, std::min<uint64_t>(r_max, kPassSamples / spr));
+    const uint64_t cap = rpp * spr;
+    if (cap >= (1ull << 31)) e = hipErrorInvalidValue;
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) <= cells) ++bits;  // keys 0 .. cells (cells: no query)
+    size_t sort_bytes = 0;
+    DevBuf d_smp, d_k0, d_i0, d_k1, d_i1, d_sort;
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                               (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)cap, 0, bits,
+                                               ctx->stream);
+    if (e == hipSuccess) {
+        st = dev_alloc(ctx, d_smp, cap * sizeof(McSample));
+        if (st == S2H_OK) st = dev_alloc(ctx, d_k0, cap * 4);
+        if (st == S2H_OK) st = dev_alloc(ctx, d_i0, cap * 4);
+        if (st == S2H_OK) st = dev_alloc(ctx, d_k1, cap * 4);
+        if (st == S2H_OK) st = dev_alloc(ctx, d_i1, cap * 4);
+        if (st == S2H_OK) st = dev_alloc(ctx, d_sort, std::max<size_t>(sort_bytes, 16));
+    }
+    auto release_pass = [&]() {
+        for (DevBuf *b : {&d_smp, &d_k0, &d_i0, &d_k1, &d_i1, &d_sort}) dev_free(ctx, *b);
+    };
+    if (st != S2H_OK) {
+        release_pass();
+        release();
+        return st;
+    }
+    auto *smp = (McSample *)d_smp.ptr;
+    auto *k0 = (uint32_t *)d_k0.ptr, *i0 = (uint32_t *)d_i0.ptr, *k1 = (uint32_t *)d_k1.ptr, *i1 = (uint32_t *)d_i1.ptr;
+    auto *hv = (unsigned long long *)d_hv.ptr;
+    int passes = 0;
+    for (uint64_t r0 = 1; e == hipSuccess && r0 <= r_max; r0 += rpp) {
+        const uint64_t nr = std::min<uint64_t>(rpp, r_max - r0 + 1), T = nr * spr;
+        mc_points_kernel<<<(unsigned)(bpr * nr), kThreads, 0, ctx->stream>>>(
+            g, bbox_min[0], bbox_min[1], bbox_min[2], bbox_max[0], bbox_max[1], bbox_max[2], spr, (uint32_t)r0,
+            (uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)bpr, smp, k0, i0, hv);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortPairs(d_sort.ptr, sort_bytes, k0, k1, i0, i1, (int)T, 0, bits,
+                                                   ctx->stream);
+        if (e == hipSuccess) {
+            mc_query_p_kernel<<<grid_for(T), kThreads, 0, ctx->stream>>>(g, T, k1, i1, smp, k0, i0);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortPairs(d_sort.ptr, sort_bytes, k0, k1, i0, i1, (int)T, 0, bits,
+                                                   ctx->stream);
+        if (e == hipSuccess) {
+            mc_query_q_kernel<<<grid_for(T), kThreads, 0, ctx->stream>>>(g, T, k1, i1, smp, spr, (uint32_t)r0, hv);
+            e = hipGetLastError();
+        }
+        ++passes;
+    }
+    ctx->launches = 7 + 5 * passes;
     prof_mark(ctx, 2);
     if (e == hipSuccess) e = hipMemcpyAsync(hits_valid, d_hv.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     prof_collect(ctx);
+    release_pass();
     release();
     if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("mesh Monte Carlo: ") + hipGetErrorString(e));
     return S2H_OK;
