@@ -120,9 +120,32 @@ __device__ bool contains_in_cell(const Grid &g, uint32_t cell, double px, double
     double hits[vox::kMaxHits];
     int nh = 0;
     bool over = false;
+    // the list walk is a chain of dependent loads (index, then bound): the
+    // index two entries ahead and the bound one entry ahead load while this
+    // entry is tested
+    uint32_t id1 = c0 < c1 ? g.ids[c0] : 0u, id2 = c0 + 1 < c1 ? g.ids[c0 + 1] : 0u;
+    double b1[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c0 < c1) {
+        const vox::TriGeo &t1 = g.tris[id1];
+        b1[0] = t1.umin;
+        b1[1] = t1.umax;
+        b1[2] = t1.vmin;
+        b1[3] = t1.vmax;
+    }
     for (uint32_t k = c0; k < c1; ++k) {
-        const vox::TriGeo &t = g.tris[g.ids[k]];
-        if (pu < t.umin || pu > t.umax || pv < t.vmin || pv > t.vmax) continue;
+        const uint32_t id = id1;
+        const double umin = b1[0], umax = b1[1], vmin = b1[2], vmax = b1[3];
+        id1 = id2;
+        if (k + 2 < c1) id2 = g.ids[k + 2];
+        if (k + 1 < c1) {
+            const vox::TriGeo &t1 = g.tris[id1];
+            b1[0] = t1.umin;
+            b1[1] = t1.umax;
+            b1[2] = t1.vmin;
+            b1[3] = t1.vmax;
+        }
+        if (pu < umin || pu > umax || pv < vmin || pv > vmax) continue;
+        const vox::TriGeo &t = g.tris[id];
         const double th = vox::mt_hit(t, px, py, pz);
         if (th < 0.0) continue;
         if (nh >= vox::kMaxHits) {
