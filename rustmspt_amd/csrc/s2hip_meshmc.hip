@@ -28,7 +28,8 @@
 // cell's list, so the hit multiset equals a full scan.
 //
 // Schedule: the queries are answered in cell order.  A first kernel draws
-// every sample's p and q (and counts the valid ones), a radix sort orders the
+// every sample's p and q (and counts the valid ones; the later kernels draw a
+// sample again from its index rather than storing it), a radix sort orders the
 // p queries by grid cell, a second kernel answers them, a second sort orders
 // the q queries of the samples whose p is inside, and a third kernel answers
 // those and counts the hits.  Neighbouring lanes then read the same cell lists
@@ -182,66 +183,81 @@ __device__ bool contains_in_cell(const Grid &g, uint32_t cell, double px, double
     return unique & 1;
 }
 
-struct McSample {
-    double p[3], q[3];
+// The sample stream of one pass: sample sidx (0 .. radii x samples) is sample
+// i = sidx % samples of radius r0 + sidx / samples.  p and q are functions of
+// (seed, r, i) alone, so every kernel of the schedule draws them again from the
+// sample's index instead of storing them.
+struct McStream {
+    double b0[3], b1[3];  // measurement box
+    uint32_t samples;     // per radius
+    uint32_t r0;
+    uint32_t seed_lo, seed_hi;
 };
 
-// 1. Every sample's p and q (the stream described above) for radii r0 ..; a
-// valid sample's key is the cell of its p query (no_cell: no query, or p
-// outside the mesh bbox), and every sample carries its index.  Counts the
-// valid samples per radius.
-__global__ __launch_bounds__(kThreads) void mc_points_kernel(Grid g, double bx0, double by0, double bz0, double bx1,
-                                                             double by1, double bz1, uint64_t samples, uint32_t r0,
-                                                             uint32_t seed_lo, uint32_t seed_hi, uint32_t blocks_per_r,
-                                                             McSample *__restrict__ smp, uint32_t *__restrict__ key,
-                                                             uint32_t *__restrict__ idx,
+__device__ __forceinline__ void draw_p(const McStream &s, uint32_t r, uint32_t i, double &px, double &py,
+                                       double &pz) {
+    uint32_t a[4], b[4];
+    philox4x32_10(i, 0u, r, kTag + 0u, s.seed_lo, s.seed_hi, a);
+    philox4x32_10(i, 0u, r, kTag + 1u, s.seed_lo, s.seed_hi, b);
+    px = s.b0[0] + (s.b1[0] - s.b0[0]) * u53(a[0], a[1]);
+    py = s.b0[1] + (s.b1[1] - s.b0[1]) * u53(a[2], a[3]);
+    pz = s.b0[2] + (s.b1[2] - s.b0[2]) * u53(b[0], b[1]);
+}
+
+// q = p + r d for the sample's direction d (rejection in the cube; +x after
+// kDirAttempts rejected attempts).
+__device__ __forceinline__ void draw_q(const McStream &s, uint32_t r, uint32_t i, double px, double py, double pz,
+                                       double &qx, double &qy, double &qz) {
+    double dx = 1.0, dy = 0.0, dz = 0.0;
+    for (int t = 0; t < kDirAttempts; ++t) {
+        uint32_t a[4], b[4];
+        philox4x32_10(i, 0u, r, kTag + 2u + 2u * (uint32_t)t, s.seed_lo, s.seed_hi, a);
+        philox4x32_10(i, 0u, r, kTag + 3u + 2u * (uint32_t)t, s.seed_lo, s.seed_hi, b);
+        const double x = -1.0 + 2.0 * u53(a[0], a[1]);
+        const double y = -1.0 + 2.0 * u53(a[2], a[3]);
+        const double z = -1.0 + 2.0 * u53(b[0], b[1]);
+        const double n2 = x * x + y * y + z * z;
+        if (n2 > 1e-12 && n2 <= 1.0) {
+            const double inv = 1.0 / sqrt(n2);
+            dx = x * inv;
+            dy = y * inv;
+            dz = z * inv;
+            break;
+        }
+    }
+    const double rr = (double)r;
+    qx = px + dx * rr;
+    qy = py + dy * rr;
+    qz = pz + dz * rr;
+}
+
+__device__ __forceinline__ bool in_box(const McStream &s, double x, double y, double z) {
+    return !(x < s.b0[0] || x > s.b1[0] || y < s.b0[1] || y > s.b1[1] || z < s.b0[2] || z > s.b1[2]);
+}
+
+// 1. Every sample of radii r0 .. of the pass: a valid sample (q inside the
+// box) gets the cell of its p query as key (no_cell: no query, or p outside
+// the mesh bbox), an invalid one no_cell; every sample carries its index.
+// Counts the valid samples per radius.
+__global__ __launch_bounds__(kThreads) void mc_points_kernel(Grid g, McStream s, uint32_t blocks_per_r,
+                                                             uint32_t *__restrict__ key, uint32_t *__restrict__ idx,
                                                              unsigned long long *__restrict__ hv) {
     const uint32_t rl = blockIdx.x / blocks_per_r;  // radius within this pass
-    const uint32_t r = r0 + rl;
+    const uint32_t r = s.r0 + rl;
     const uint32_t blk = blockIdx.x % blocks_per_r;
-    const double rr = (double)r;
     uint64_t valid = 0;
-    const uint64_t base = (uint64_t)blk * kThreads * kPerThread + threadIdx.x;
+    const uint32_t base = blk * kThreads * kPerThread + threadIdx.x;
     for (int k = 0; k < kPerThread; ++k) {
-        const uint64_t i = base + (uint64_t)k * kThreads;
-        if (i >= samples) break;
-        const uint64_t sidx = (uint64_t)rl * samples + i;
-        const uint32_t ilo = (uint32_t)i, ihi = (uint32_t)(i >> 32);
-        uint32_t a[4], b[4];
-        philox4x32_10(ilo, ihi, r, kTag + 0u, seed_lo, seed_hi, a);
-        philox4x32_10(ilo, ihi, r, kTag + 1u, seed_lo, seed_hi, b);
-        const double px = bx0 + (bx1 - bx0) * u53(a[0], a[1]);
-        const double py = by0 + (by1 - by0) * u53(a[2], a[3]);
-        const double pz = bz0 + (bz1 - bz0) * u53(b[0], b[1]);
-        double dx = 1.0, dy = 0.0, dz = 0.0;
-        for (int t = 0; t < kDirAttempts; ++t) {
-            philox4x32_10(ilo, ihi, r, kTag + 2u + 2u * (uint32_t)t, seed_lo, seed_hi, a);
-            philox4x32_10(ilo, ihi, r, kTag + 3u + 2u * (uint32_t)t, seed_lo, seed_hi, b);
-            const double x = -1.0 + 2.0 * u53(a[0], a[1]);
-            const double y = -1.0 + 2.0 * u53(a[2], a[3]);
-            const double z = -1.0 + 2.0 * u53(b[0], b[1]);
-            const double n2 = x * x + y * y + z * z;
-            if (n2 > 1e-12 && n2 <= 1.0) {
-                const double inv = 1.0 / sqrt(n2);
-                dx = x * inv;
-                dy = y * inv;
-                dz = z * inv;
-                break;
-            }
-        }
-        const double qx = px + dx * rr, qy = py + dy * rr, qz = pz + dz * rr;
-        const bool ok = !(qx < bx0 || qx > bx1 || qy < by0 || qy > by1 || qz < bz0 || qz > bz1);
+        const uint32_t i = base + (uint32_t)k * kThreads;
+        if (i >= s.samples) break;
+        const uint32_t sidx = rl * s.samples + i;
+        double px, py, pz, qx, qy, qz;
+        draw_p(s, r, i, px, py, pz);
+        draw_q(s, r, i, px, py, pz, qx, qy, qz);
+        const bool ok = in_box(s, qx, qy, qz);
         if (ok) ++valid;
-        McSample m;
-        m.p[0] = px;
-        m.p[1] = py;
-        m.p[2] = pz;
-        m.q[0] = qx;
-        m.q[1] = qy;
-        m.q[2] = qz;
-        smp[sidx] = m;
         key[sidx] = ok ? query_cell(g, px, py, pz) : no_cell(g);
-        idx[sidx] = (uint32_t)sidx;
+        idx[sidx] = sidx;
     }
     for (int o = 32; o > 0; o >>= 1) valid += __shfl_xor(valid, o, 64);
     if ((threadIdx.x & 63) == 0 && valid) atomicAdd(&hv[2 * r + 1], (unsigned long long)valid);
@@ -249,17 +265,23 @@ __global__ __launch_bounds__(kThreads) void mc_points_kernel(Grid g, double bx0,
 
 // 2. The p queries in cell order; a sample whose p is inside gets the cell of
 // its q query as key for the next sort (no_cell otherwise).
-__global__ __launch_bounds__(kThreads) void mc_query_p_kernel(Grid g, uint64_t n, const uint32_t *__restrict__ key,
+__global__ __launch_bounds__(kThreads) void mc_query_p_kernel(Grid g, McStream s, uint32_t n,
+                                                              const uint32_t *__restrict__ key,
                                                               const uint32_t *__restrict__ idx,
-                                                              const McSample *__restrict__ smp,
                                                               uint32_t *__restrict__ qkey,
                                                               uint32_t *__restrict__ qidx) {
-    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t c = key[j], sidx = idx[j];
         uint32_t k2 = no_cell(g);
         if (c != no_cell(g)) {
-            const McSample &m = smp[sidx];
-            if (contains_in_cell(g, c, m.p[0], m.p[1], m.p[2])) k2 = query_cell(g, m.q[0], m.q[1], m.q[2]);
+            const uint32_t rl = sidx / s.samples, i = sidx - rl * s.samples, r = s.r0 + rl;
+            double px, py, pz;
+            draw_p(s, r, i, px, py, pz);
+            if (contains_in_cell(g, c, px, py, pz)) {
+                double qx, qy, qz;
+                draw_q(s, r, i, px, py, pz, qx, qy, qz);
+                k2 = query_cell(g, qx, qy, qz);
+            }
         }
         qkey[j] = k2;
         qidx[j] = sidx;
@@ -267,16 +289,19 @@ __global__ __launch_bounds__(kThreads) void mc_query_p_kernel(Grid g, uint64_t n
 }
 
 // 3. The q queries in cell order: a hit when q is inside too.
-__global__ __launch_bounds__(kThreads) void mc_query_q_kernel(Grid g, uint64_t n, const uint32_t *__restrict__ key,
+__global__ __launch_bounds__(kThreads) void mc_query_q_kernel(Grid g, McStream s, uint32_t n,
+                                                              const uint32_t *__restrict__ key,
                                                               const uint32_t *__restrict__ idx,
-                                                              const McSample *__restrict__ smp, uint64_t samples,
-                                                              uint32_t r0, unsigned long long *__restrict__ hv) {
-    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+                                                              unsigned long long *__restrict__ hv) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t c = key[j];
         if (c == no_cell(g)) continue;
         const uint32_t sidx = idx[j];
-        const McSample &m = smp[sidx];
-        if (contains_in_cell(g, c, m.q[0], m.q[1], m.q[2])) atomicAdd(&hv[2 * (r0 + sidx / samples)], 1ull);
+        const uint32_t rl = sidx / s.samples, i = sidx - rl * s.samples, r = s.r0 + rl;
+        double px, py, pz, qx, qy, qz;
+        draw_p(s, r, i, px, py, pz);
+        draw_q(s, r, i, px, py, pz, qx, qy, qz);
+        if (contains_in_cell(g, c, qx, qy, qz)) atomicAdd(&hv[2 * r], 1ull);
     }
 }
 
@@ -441,56 +466,61 @@ s2h_status bin_and_run(s2h_ctx *ctx, Grid g, double cmax, double smax, double am
         clear_u64_kernel<<<1, 256, 0, ctx->stream>>>((unsigned long long *)d_hv.ptr, n);
         e = hipGetLastError();
     }
-    // radii in passes of at most kPassSamples samples (64 bytes each)
+    // radii in passes of at most kPassSamples samples (keys and indices: 16 bytes each)
     const uint64_t rpp = std::max<uint64_t>(1, std::min<uint64_t>(r_max, kPassSamples / spr));
     const uint64_t cap = rpp * spr;
     if (cap >= (1ull << 31)) e = hipErrorInvalidValue;
     int bits = 1;
     while (bits < 32 && (1ull << bits) <= cells) ++bits;  // keys 0 .. cells (cells: no query)
     size_t sort_bytes = 0;
-    DevBuf d_smp, d_k0, d_i0, d_k1, d_i1, d_sort;
+    DevBuf d_k0, d_i0, d_k1, d_i1, d_sort;
     if (e == hipSuccess)
         e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
                                                (const uint32_t *)nullptr, (uint32_t *)nullptr, (int)cap, 0, bits,
                                                ctx->stream);
     if (e == hipSuccess) {
-        st = dev_alloc(ctx, d_smp, cap * sizeof(McSample));
-        if (st == S2H_OK) st = dev_alloc(ctx, d_k0, cap * 4);
+        st = dev_alloc(ctx, d_k0, cap * 4);
         if (st == S2H_OK) st = dev_alloc(ctx, d_i0, cap * 4);
         if (st == S2H_OK) st = dev_alloc(ctx, d_k1, cap * 4);
         if (st == S2H_OK) st = dev_alloc(ctx, d_i1, cap * 4);
         if (st == S2H_OK) st = dev_alloc(ctx, d_sort, std::max<size_t>(sort_bytes, 16));
     }
     auto release_pass = [&]() {
-        for (DevBuf *b : {&d_smp, &d_k0, &d_i0, &d_k1, &d_i1, &d_sort}) dev_free(ctx, *b);
+        for (DevBuf *b : {&d_k0, &d_i0, &d_k1, &d_i1, &d_sort}) dev_free(ctx, *b);
     };
     if (st != S2H_OK) {
         release_pass();
         release();
         return st;
     }
-    auto *smp = (McSample *)d_smp.ptr;
     auto *k0 = (uint32_t *)d_k0.ptr, *i0 = (uint32_t *)d_i0.ptr, *k1 = (uint32_t *)d_k1.ptr, *i1 = (uint32_t *)d_i1.ptr;
     auto *hv = (unsigned long long *)d_hv.ptr;
     int passes = 0;
     for (uint64_t r0 = 1; e == hipSuccess && r0 <= r_max; r0 += rpp) {
         const uint64_t nr = std::min<uint64_t>(rpp, r_max - r0 + 1), T = nr * spr;
-        mc_points_kernel<<<(unsigned)(bpr * nr), kThreads, 0, ctx->stream>>>(
-            g, bbox_min[0], bbox_min[1], bbox_min[2], bbox_max[0], bbox_max[1], bbox_max[2], spr, (uint32_t)r0,
-            (uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)bpr, smp, k0, i0, hv);
+        McStream ms;
+        for (int a = 0; a < 3; ++a) {
+            ms.b0[a] = bbox_min[a];
+            ms.b1[a] = bbox_max[a];
+        }
+        ms.samples = (uint32_t)spr;
+        ms.r0 = (uint32_t)r0;
+        ms.seed_lo = (uint32_t)seed;
+        ms.seed_hi = (uint32_t)(seed >> 32);
+        mc_points_kernel<<<(unsigned)(bpr * nr), kThreads, 0, ctx->stream>>>(g, ms, (uint32_t)bpr, k0, i0, hv);
         e = hipGetLastError();
         if (e == hipSuccess)
             e = hipcub::DeviceRadixSort::SortPairs(d_sort.ptr, sort_bytes, k0, k1, i0, i1, (int)T, 0, bits,
                                                    ctx->stream);
         if (e == hipSuccess) {
-            mc_query_p_kernel<<<grid_for(T), kThreads, 0, ctx->stream>>>(g, T, k1, i1, smp, k0, i0);
+            mc_query_p_kernel<<<grid_for(T), kThreads, 0, ctx->stream>>>(g, ms, (uint32_t)T, k1, i1, k0, i0);
             e = hipGetLastError();
         }
         if (e == hipSuccess)
             e = hipcub::DeviceRadixSort::SortPairs(d_sort.ptr, sort_bytes, k0, k1, i0, i1, (int)T, 0, bits,
                                                    ctx->stream);
         if (e == hipSuccess) {
-            mc_query_q_kernel<<<grid_for(T), kThreads, 0, ctx->stream>>>(g, T, k1, i1, smp, spr, (uint32_t)r0, hv);
+            mc_query_q_kernel<<<grid_for(T), kThreads, 0, ctx->stream>>>(g, ms, (uint32_t)T, k1, i1, hv);
             e = hipGetLastError();
         }
         ++passes;
