@@ -32,6 +32,59 @@ VALU_4CYCLE_OPS = 256 * 64 * 2.4e9
 # (x2.0) and WRITE_SIZE (x1.0) of the count dispatch; factors from a 512 MiB
 # device copy.  Command lines and figures: profiles/exact_count_kernel_256cube_r64.md.
 COUNT_PHASE_TRAFFIC = 14840 * 1024 * 2.0 + 862900 * 1024 * 1.0
+# The same for the FP64 FFT kernel at 256^3 r_max=64: FETCH_SIZE + WRITE_SIZE
+# summed over its six dispatches (z fwd, y fwd, x, y inv, z inv, gather), raw
+# KiB -- for these LDS-tiled passes the raw FETCH_SIZE already counts the bytes
+# read (the y forward pass fetches at least the 263,168 KiB it must read).
+FFT64_PHASE_TRAFFIC = (1518873 + 1362399) * 1024.0
+
+
+def fft64_bytes(dims, offs):
+    """(algorithmic, compulsory) HBM bytes of one FP64 FFT evaluation.
+    Algorithmic: 12 x f_x f_y (f_z/2+1) x 16 B -- three axes, forward and
+    inverse, read and write of the complex128 half spectrum, as for an unpruned
+    transform.  Compulsory: what the pruned passes must move -- each pass reads
+    its input and writes its output once (z forward: packed rows in, ny rows of
+    spectra out; y forward; x forward + |F|^2 + inverse, keeping 2R_x+1 planes;
+    y inverse keeping 2R_y+1; z inverse keeping 2R_z+1 f64 values)."""
+    import numpy as np
+
+    from rustmspt_amd.config import smooth_fft_length
+
+    nx, ny, nz = dims
+    fx, fy, fz = (smooth_fft_length(2 * n - 1) for n in dims)
+    h = fz // 2 + 1
+    rx, ry, rz = (int(v) for v in np.abs(offs).max(axis=0))
+    c = 16.0
+    row_words = (nz + 63) // 64 + 1
+    z_fwd = nx * ny * row_words * 8 + nx * ny * h * c
+    y_fwd = nx * ny * h * c + nx * fy * h * c
+    x_pow = nx * fy * h * c + (2 * rx + 1) * fy * h * c
+    y_inv = (2 * rx + 1) * fy * h * c + (2 * rx + 1) * (2 * ry + 1) * h * c
+    z_inv = (2 * rx + 1) * (2 * ry + 1) * h * c + (2 * rx + 1) * (2 * ry + 1) * (2 * rz + 1) * 8.0
+    return 12.0 * fx * fy * h * c, float(z_fwd + y_fwd + x_pow + y_inv + z_inv)
+
+
+def exact_timed(ctx, vol, r_max, method, steps, warmup):
+    """Evaluations of the exact curve with one kernel choice ("auto": the
+    model's pick): evals/s, count-phase and device ms, the curve."""
+    import numpy as np
+
+    ctx.set_exact_method(method)
+    curve = None
+    for _ in range(warmup):
+        curve = vol.exact_s2(r_max, 1.0)
+    count_ms, dev_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        curve = vol.exact_s2(r_max, 1.0)
+        p = ctx.profile_read()
+        count_ms.append(p["count_ms"])
+        dev_ms.append(p["device_ms"])
+    el = time.perf_counter() - t0
+    ctx.set_exact_method("auto")
+    return {"evals_per_s": steps / el, "ms_per_step": 1e3 * el / steps, "count_ms": float(np.mean(count_ms)),
+            "device_ms": float(np.mean(dev_ms)), "launches": ctx.profile_read()["launches"], "curve": curve}
 
 
 def _cpu_model() -> str:
@@ -317,29 +370,34 @@ def ct_leg(ctx, vol, r_max, steps):
     valid = np.prod(np.array(dims, dtype=np.float64)[None, :] - np.abs(offs), axis=1)
     half_work = float(valid.sum()) / 2.0
     t0 = time.perf_counter()
-    curve = vol.exact_s2(r_max, 1.0)  # plan (offsets, columns, launch shape) + first run
+    vol.exact_s2(r_max, 1.0)  # plan (offsets, columns, launch shape) + first run
     first_s = time.perf_counter() - t0
-    count_ms, dev_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        curve = vol.exact_s2(r_max, 1.0)
-        p = ctx.profile_read()
-        count_ms.append(p["count_ms"])
-        dev_ms.append(p["device_ms"])
-    el = time.perf_counter() - t0
-    count_avg = float(np.mean(count_ms))
-    achieved = half_work / 32.0 * 2.0 / (count_avg * 1e-3)
+    auto = exact_timed(ctx, vol, r_max, "auto", max(steps, 3), 1)
+    kernel = ctx.exact_tune_info(dims, r_max, 1.0)["kernel"]
+    direct = exact_timed(ctx, vol, r_max, "direct", steps, 1)
+    achieved = half_work / 32.0 * 2.0 / (direct["count_ms"] * 1e-3)
     peak = ctx.valu_probe()
+    out = {"grid": dims, "r_max": r_max, "offsets": int(len(offs)), "pair_work": 2.0 * half_work,
+           "exact_evals_per_s": auto["evals_per_s"], "exact_s_per_eval": auto["ms_per_step"] * 1e-3,
+           "exact_kernel": kernel, "first_call_s": first_s, "count_phase_ms": auto["count_ms"],
+           "device_ms_per_eval": auto["device_ms"],
+           "curves_identical": auto["curve"].tobytes() == direct["curve"].tobytes(),
+           "direct": {"evals_per_s": direct["evals_per_s"], "count_kernel_ms": direct["count_ms"],
+                      "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                                   "unit": "TOP/s (int32 lane-ops)", "frac": achieved / peak,
+                                   "peak_4cycle": VALU_4CYCLE_OPS / 1e12,
+                                   "frac_of_4cycle": achieved / VALU_4CYCLE_OPS},
+                      "launch_shape": ctx.exact_tune_info(dims, r_max, 1.0)},
+           "s2_head": [round(float(v), 6) for v in auto["curve"][:5]]}
+    if kernel == "fft64":
+        alg, comp = fft64_bytes(dims, offs)
+        sec = auto["count_ms"] * 1e-3
+        out["roofline"] = {"bound": "hbm", "achieved": alg / sec / 1e9, "peak": 8000.0, "unit": "GB/s",
+                           "frac": alg / sec / 8e12, "algorithmic_bytes": alg, "compulsory_bytes": comp,
+                           "frac_of_compulsory": comp / sec / 8e12}
     hits = vol.exact_counts(r_max, 1.0)
-    return {"grid": dims, "r_max": r_max, "offsets": int(len(offs)), "pair_work": 2.0 * half_work,
-            "exact_evals_per_s": steps / el, "exact_s_per_eval": el / steps, "first_call_s": first_s,
-            "count_kernel_ms": count_avg, "device_ms_per_eval": float(np.mean(dev_ms)),
-            "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                         "unit": "TOP/s (int32 lane-ops)", "frac": achieved / peak,
-                         "peak_4cycle": VALU_4CYCLE_OPS / 1e12, "frac_of_4cycle": achieved / VALU_4CYCLE_OPS},
-            "launch_shape": ctx.exact_tune_info(dims, r_max, 1.0),
-            "s2_head": [round(float(v), 6) for v in curve[:5]],
-            "fft32": fft_leg(ctx, vol, r_max, hits, offs, rs, curve, steps=2)}
+    out["fft32"] = fft_leg(ctx, vol, r_max, hits, offs, rs, auto["curve"], steps=2)
+    return out
 
 
 def stl_ingest_leg(ctx, mesh, bbox, reps=3):
@@ -475,54 +533,65 @@ def main():
     if args.tune:
         band, waves = (int(v) for v in args.tune.split(","))
         ctx.exact_tune_set([grid] * 3, r_max, 1.0, band, waves)
-    curve = None
-    for _ in range(args.warmup):
-        curve = vol.exact_s2(r_max, 1.0)
-    count_ms, dev_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        curve = vol.exact_s2(r_max, 1.0)
-        p = ctx.profile_read()
-        count_ms.append(p["count_ms"])
-        dev_ms.append(p["device_ms"])
-    elapsed = time.perf_counter() - t0
-    ms_per_step = 1e3 * elapsed / args.steps
-    count_avg = float(np.mean(count_ms))
-    launches = ctx.profile_read()["launches"]
+    # the exact path as callers get it: the kernel the model picks (direct
+    # counting or the FP64 FFT rounded to the same integer counts) ...
+    auto = exact_timed(ctx, vol, r_max, "auto", args.steps, args.warmup)
+    kernel = ctx.exact_tune_info([grid] * 3, r_max, 1.0)["kernel"]
+    curve = auto["curve"]
+    # ... and the direct counting kernel forced, beside it
+    direct = exact_timed(ctx, vol, r_max, "direct", args.steps, max(args.warmup, 1))
+    count_avg = direct["count_ms"]
     # algorithmic VALU work: one v_and_b32 + one v_bcnt_u32_b32 per 32 unordered pair-tests
     alg_ops = half_work / 32.0 * 2.0
     achieved = alg_ops / (count_avg * 1e-3)
     valu_measured = ctx.valu_probe()  # issue ceiling of the counting loop's and/popcount pair
     shape = ctx.exact_tune_info([grid] * 3, r_max, 1.0)
     profiled = (grid, r_max, args.particles, shape["band"], shape["waves"]) == (256, 64, 500, 3, 65536)
+    direct_roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": valu_measured / 1e12,
+                       "unit": "TOP/s (int32 lane-ops)", "frac": achieved / valu_measured,
+                       "traffic": COUNT_PHASE_TRAFFIC if profiled else None,
+                       "peak_source": "on-device probe of the counting loop's v_and_b32 (SGPR operand) + "
+                                      "v_bcnt_u32_b32 pair, every SIMD occupied (no published int32 VALU figure)",
+                       "peak_4cycle": VALU_4CYCLE_OPS / 1e12, "frac_of_4cycle": achieved / VALU_4CYCLE_OPS,
+                       "kernel": "exact_count_kernel", "kernel_ms": count_avg,
+                       "launches_per_eval": direct["launches"], "device_ms_per_eval": direct["device_ms"],
+                       "algorithmic_ops_per_eval": alg_ops, "launch_shape": shape}
+    if kernel == "fft64":
+        alg_b, comp_b = fft64_bytes([grid] * 3, offs)
+        sec = auto["count_ms"] * 1e-3
+        roofline = {"bound": "hbm", "achieved": alg_b / sec / 1e9, "peak": 8000.0, "unit": "GB/s",
+                    "frac": alg_b / sec / 8e12,
+                    "traffic": FFT64_PHASE_TRAFFIC if (grid, r_max, args.particles) == (256, 64, 500) else None,
+                    "kernel": "FP64 FFT passes (z fwd, y fwd, x fwd+|F|^2+inv, y inv, z inv, gather)",
+                    "kernel_ms": auto["count_ms"], "launches_per_eval": auto["launches"],
+                    "device_ms_per_eval": auto["device_ms"], "algorithmic_bytes_per_eval": alg_b,
+                    "compulsory_bytes_per_eval": comp_b, "frac_of_compulsory": comp_b / sec / 8e12,
+                    "note": "algorithmic bytes count an unpruned transform (12 passes over the complex128 "
+                            "half spectrum); the pruned passes must move only the compulsory bytes"}
+    else:
+        roofline = direct_roofline
 
     out = {
         "metric": "exact-S2 evals/s at 256^3 r_max=64, batched vs plain; p50/p99 served latency",
-        "value": args.steps / elapsed,
+        "value": auto["evals_per_s"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": auto["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32/u64 integer counts, f64 curve",
+        "dtype": "u64 integer counts (direct: u32 popcounts; fft64: f64 spectra rounded), f64 curve",
         "data": "synthetic: seeded packing, voxelized on device (bit-exact f64 predicate)",
         "config": {"workload": f"single large measure: {args.particles} particles (icospheres + ellipsoids), "
                                f"{grid}^3 grid, r_max={r_max}, exact S2, plain per-call path",
+                   "exact_kernel": kernel,
                    "grid": grid, "r_max": r_max, "offsets": int(len(offs)), "pair_work": pair_work,
                    "volume_fraction": occupied / grid ** 3, "voxelize_s": vox_s},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": valu_measured / 1e12,
-                     "unit": "TOP/s (int32 lane-ops)", "frac": achieved / valu_measured,
-                     "traffic": COUNT_PHASE_TRAFFIC if profiled else None,
-                     "peak_source": "on-device probe of the counting loop's v_and_b32 (SGPR operand) + "
-                                    "v_bcnt_u32_b32 pair, every SIMD occupied (no published int32 VALU figure)",
-                     "peak_4cycle": VALU_4CYCLE_OPS / 1e12, "frac_of_4cycle": achieved / VALU_4CYCLE_OPS,
-                     "kernel": "exact_count_kernel", "kernel_ms": count_avg, "launches_per_eval": launches,
-                     "device_ms_per_eval": float(np.mean(dev_ms)),
-                     "algorithmic_ops_per_eval": alg_ops,
-                     "launch_shape": shape},
+        "roofline": roofline,
+        "direct": {"evals_per_s": direct["evals_per_s"], "ms_per_step": direct["ms_per_step"],
+                   "curves_identical": direct["curve"].tobytes() == curve.tobytes(), "roofline": direct_roofline},
         "s2_head": [round(float(v), 6) for v in curve[:5]],
     }
     if not (args.no_fft and args.no_mc):
@@ -547,6 +616,7 @@ def main():
         words = vol.read_packed()
         out["cpu_baseline"] = cpu_baseline(words, [grid] * 3, r_max, offs, pair_work, args.cpu_seconds)
         out["cpu_baseline"]["gpu_speedup"] = out["value"] / out["cpu_baseline"]["value"]
+        out["cpu_baseline"]["gpu_direct_speedup"] = direct["evals_per_s"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 
 

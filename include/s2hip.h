@@ -233,6 +233,17 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
  * Bit-identical to the reference CPU exact path. */
 s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, double *s2,
                         uint64_t n);
+/* Which exact kernel s2h_exact_counts / s2h_exact_s2 run (replaces the method
+ * choice of plan_exact_cpu and calculate_exact_with's forced method,
+ * src/geometry/s2.rs:733-766, 1482-1495).  Both kernels give the same integer
+ * counts, so the choice never changes a result: 1 direct counting; 2 the FP64
+ * FFT correlation rounded to the nearest integer, `.re.round().max(0.0)` of
+ * calculate_s2_exact_fft (src/geometry/s2.rs:1219-1261), allowed only when the
+ * worst-case round-off bound for the volume, (3 rho + 3u) K^(3/2) with K
+ * occupied voxels, is below 1/4 (else S2H_ERR_UNSUPPORTED); 0 (default) the
+ * faster modelled kernel among the eligible ones.  s2h_exact_tune_info's
+ * info[7] reports the kernel of the plan's last evaluation (1 or 2). */
+s2h_status s2h_exact_method_set(s2h_ctx *ctx, int32_t method);
 
 /* Shape of the exact count launch for (dims, r_max, pitch) -- no counterpart in
  * the reference, whose direct kernel has no launch shape.  The half-ball columns
@@ -243,7 +254,8 @@ s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
  * Counts never depend on the shape.
  * info (8 values): [0] band, [1] target waves, [2] 1 picked by device timing,
  * -1 by the cost model, 0 not run yet, [3] candidates timed, [4] work items,
- * [5] column groups, [6] modelled VALU wave-instructions per run, [7] 0. */
+ * [5] column groups, [6] modelled VALU wave-instructions per run, [7] kernel of
+ * the last evaluation (1 direct, 2 FP64 FFT, 0 none yet). */
 s2h_status s2h_exact_tune_info(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
                                int64_t *info, uint64_t n);
 /* Force a shape (band >= 1, waves >= 1) for this context; the autotuner is

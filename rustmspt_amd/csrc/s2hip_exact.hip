@@ -191,11 +191,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
 // ---------------------------------------------------------------- gather / fold
 
 // term[i] = hits / valid in f64 (division correctly rounded, as Rust's `/`).
+// With dvals (FP64 correlation values) the hits are the values rounded to the
+// nearest integer, at least 0: `.re.round().max(0.0)` of the reference's FFT
+// kernel (src/geometry/s2.rs:1245-1248).
 __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32_t *__restrict__ gather,
                                     uint64_t n, const unsigned long long *__restrict__ table,
-                                    const float *__restrict__ fcounts, uint64_t occupied, uint64_t nx,
-                                    uint64_t ny, uint64_t nz, unsigned long long *__restrict__ hits,
-                                    double *__restrict__ terms) {
+                                    const float *__restrict__ fcounts, const double *__restrict__ dvals,
+                                    uint64_t occupied, uint64_t nx, uint64_t ny, uint64_t nz,
+                                    unsigned long long *__restrict__ hits, double *__restrict__ terms) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Offset o = offs[i];
@@ -203,6 +206,9 @@ __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32
     double h;
     if (fcounts) {
         h = (double)fcounts[i];
+    } else if (dvals) {
+        h = fmax(round(dvals[i]), 0.0);
+        hits[i] = (unsigned long long)h;
     } else {
         const int32_t g = gather[i];
         const uint64_t hv = g >= 0 ? table[g] : occupied;
@@ -736,7 +742,7 @@ static s2h_status apply_shape(s2h_ctx *ctx, ExactPlan *p, const Grouping &g, int
     std::vector<WorkItem> sorted(p->items.size());
     for (size_t k = 0; k < io.size(); ++k) sorted[k] = p->items[io[k]];
     p->items.swap(sorted);
-    model_ms(g, waves, &p->model_ops);
+    p->model_ms = model_ms(g, waves, &p->model_ops);
     dev_free(ctx, p->d_cols);
     dev_free(ctx, p->d_items);
     S2H_TRY(upload(ctx, p->d_cols, g.lanes.data(), g.lanes.size() * sizeof(ColInfo)));
@@ -957,13 +963,14 @@ s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
     return launch_counts(ctx, vol, p);
 }
 
-s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const float *d_fcounts) {
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const float *d_fcounts,
+                             const double *d_dvals) {
     const uint64_t K = p->offs.size();
     prof_mark(ctx, 1);
     if (K) {
         gather_terms_kernel<<<(unsigned)((K + 255) / 256), 256, 0, ctx->stream>>>(
             (const Offset *)p->d_offs.ptr, (const int32_t *)p->d_gather.ptr, K,
-            (const unsigned long long *)p->d_table.ptr, d_fcounts, occupied, p->dims[0], p->dims[1],
+            (const unsigned long long *)p->d_table.ptr, d_fcounts, d_dvals, occupied, p->dims[0], p->dims[1],
             p->dims[2], (unsigned long long *)p->d_hits.ptr, (double *)p->d_terms.ptr);
         S2H_HIP(hipGetLastError());
     }
@@ -977,6 +984,39 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, cons
     }
     prof_mark(ctx, 2);
     return S2H_OK;
+}
+
+// FP64 passes modelled at this share of the HBM peak over their algorithmic
+// bytes, plus launch and gather time.
+static constexpr double kFft64BytesPerS = 5.0e12;
+static constexpr double kFft64FixedMs = 0.05;
+static constexpr double kFft64MaxBound = 0.25;  // required worst-case error (rounding is exact below 0.5)
+
+// The exact kernel of one evaluation: direct counting into the table, or the
+// FP64 FFT whose values round to the same integer counts (*d_dvals set).  The
+// reference's planner picks between its two exact kernels by modelled time
+// under a memory budget (src/geometry/s2.rs:733-766) because both return the
+// same counts; here the FFT is eligible when its worst-case round-off bound
+// for this volume is below 1/4 and (for the model's choice) its working set
+// fits the context's FFT budget, and the faster model wins.
+static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, uint64_t occupied,
+                                    const double **d_dvals) {
+    *d_dvals = nullptr;
+    const bool fft_ok = fft64_supported(p->dims) && fft64_error_bound(p->dims, occupied) <= kFft64MaxBound;
+    int m = ctx->exact_method;
+    if (m == 2 && !fft_ok)
+        return fail(S2H_ERR_UNSUPPORTED, "FP64 FFT counts are not provably exact for this volume (error bound " +
+                                             std::to_string(fft64_error_bound(p->dims, occupied)) + ")");
+    if (m == 0) {
+        const int *ext = p->shells->host->extent;
+        const int R[3] = {ext[0], ext[1], ext[2]};
+        const bool fits = !ctx->fft_budget || fft64_plan_bytes(p->dims, R) <= (double)ctx->fft_budget;
+        const double fft_ms = fft64_model_bytes(p->dims) / kFft64BytesPerS * 1e3 + kFft64FixedMs;
+        m = fft_ok && fits && fft_ms < p->model_ms ? 2 : 1;
+    }
+    p->method = m;
+    if (m == 2) return fft64_exact_device(ctx, vol, p, d_dvals);
+    return exact_run_counts(ctx, vol, p);
 }
 
 s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *p, double vf, double *s2) {
@@ -1012,8 +1052,9 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
                                       std::to_string(p->offs.size()));
     uint64_t occupied = 0;
     S2H_TRY(vol_popcount(ctx, vol, &occupied));
-    S2H_TRY(exact_run_counts(ctx, vol, p));
-    S2H_TRY(exact_gather_fold(ctx, p, occupied, nullptr));
+    const double *dvals = nullptr;
+    S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &dvals));
+    S2H_TRY(exact_gather_fold(ctx, p, occupied, nullptr, dvals));
     if (n_offsets)
         S2H_HIP(hipMemcpyAsync(hits, p->d_hits.ptr, n_offsets * 8, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
@@ -1035,7 +1076,7 @@ s2h_status s2h_exact_tune_info(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_
     info[4] = (int64_t)p->items.size();
     info[5] = p->ngroups;
     info[6] = (int64_t)p->model_ops;
-    info[7] = 0;
+    info[7] = p->method;
     return S2H_OK;
 }
 
@@ -1059,6 +1100,13 @@ s2h_status s2h_set_autotune(s2h_ctx *ctx, int32_t enable) {
     return S2H_OK;
 }
 
+s2h_status s2h_exact_method_set(s2h_ctx *ctx, int32_t method) {
+    if (!ctx) return fail(S2H_ERR_ARG, "null argument");
+    if (method < 0 || method > 2) return fail(S2H_ERR_ARG, "method must be 0 (model), 1 (direct) or 2 (fft64)");
+    ctx->exact_method = method;
+    return S2H_OK;
+}
+
 s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, double *s2, uint64_t n) {
     if (!ctx || !vol || !s2) return fail(S2H_ERR_ARG, "null argument");
     if (n != (uint64_t)r_max + 1) return fail(S2H_ERR_SIZE, "s2 buffer must hold r_max+1 values");
@@ -1073,8 +1121,9 @@ s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     ExactPlan *p = nullptr;
     S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pm, &p));
     const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
-    S2H_TRY(exact_run_counts(ctx, vol, p));
-    S2H_TRY(exact_gather_fold(ctx, p, occupied, nullptr));
+    const double *dvals = nullptr;
+    S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &dvals));
+    S2H_TRY(exact_gather_fold(ctx, p, occupied, nullptr, dvals));
     return finish_curve_from_device(ctx, p, vf, s2);
 }
 

@@ -47,7 +47,8 @@ struct FftPlan {
     uint64_t f[3];
     int R[3];
     AxisPlan ax[3];
-    DevBuf tw[3];    // float2[L] = exp(-2 pi i k / L), per axis
+    int real_bytes = 4;  // 4: FP32 path (float2 spectra), 8: FP64 rounded exact path (double2)
+    DevBuf tw[3];    // complex[L] = exp(-2 pi i k / L), per axis, in the plan's precision
     DevBuf rev_z;    // int32[fz]: natural frequency held at each DIF output position
     DevBuf pos_z;    // int32[fz]: DIF output position of each natural frequency
     DevBuf g1, g2, g3, cube, fcounts;
@@ -70,22 +71,40 @@ struct Fac {
     int f[kMaxFactors];
 };
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Complex values of the working precision T (float: the FP32 path; double:
+// the rounded exact path).
+template <class T> struct Cx;
+template <> struct Cx<float> { using type = float2; };
+template <> struct Cx<double> { using type = double2; };
+template <class T> using C2 = typename Cx<T>::type;
+
+template <class V>
+__device__ __forceinline__ V mk(decltype(V::x) re, decltype(V::x) im) {
+    V v;
+    v.x = re;
+    v.y = im;
+    return v;
 }
-__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
-    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+template <class V>
+__device__ __forceinline__ V cmul(V a, V b) {
+    return mk<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+template <class V>
+__device__ __forceinline__ V cmulc(V a, V b) {  // a * conj(b)
+    return mk<V>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <class V>
+__device__ __forceinline__ V cadd(V a, V b) { return mk<V>(a.x + b.x, a.y + b.y); }
+template <class V>
+__device__ __forceinline__ V csub(V a, V b) { return mk<V>(a.x - b.x, a.y - b.y); }
 
 // 4-point DFT in registers, natural order in and out.
-template <int SIGN>
-__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
-    float2 s0 = cadd(a0, a2), d0 = csub(a0, a2);
-    float2 s1 = cadd(a1, a3), d1 = csub(a1, a3);
+template <int SIGN, class V>
+__device__ __forceinline__ void dft4(V &a0, V &a1, V &a2, V &a3) {
+    V s0 = cadd(a0, a2), d0 = csub(a0, a2);
+    V s1 = cadd(a1, a3), d1 = csub(a1, a3);
     // multiply d1 by -i (forward) or +i (inverse)
-    float2 jd1 = SIGN < 0 ? make_float2(d1.y, -d1.x) : make_float2(-d1.y, d1.x);
+    V jd1 = SIGN < 0 ? mk<V>(d1.y, -d1.x) : mk<V>(-d1.y, d1.x);
     a0 = cadd(s0, s1);
     a2 = csub(s0, s1);
     a1 = cadd(d0, jd1);
@@ -93,24 +112,25 @@ __device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 
 }
 
 // R-point DFT in registers; sign = -1 forward, +1 inverse (unnormalized).
-template <int SIGN>
-__device__ __forceinline__ void small_dft(float2 *a, int R) {
+template <int SIGN, class V>
+__device__ __forceinline__ void small_dft(V *a, int R) {
+    using T = decltype(V::x);
     if (R == 8) {
         // even and odd 4-point DFTs, then X[k] = E[k] + w^k O[k], X[k+4] = E[k] - w^k O[k]
-        float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
-        float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+        V e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+        V o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
         dft4<SIGN>(e0, e1, e2, e3);
         dft4<SIGN>(o0, o1, o2, o3);
-        const float h = 0.70710678118654752440f;
-        float2 t1, t2, t3;
+        const T h = (T)0.70710678118654752440;
+        V t1, t2, t3;
         if (SIGN < 0) {  // w = exp(-i pi / 4)
-            t1 = make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
-            t2 = make_float2(o2.y, -o2.x);
-            t3 = make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+            t1 = mk<V>(h * (o1.x + o1.y), h * (o1.y - o1.x));
+            t2 = mk<V>(o2.y, -o2.x);
+            t3 = mk<V>(h * (o3.y - o3.x), -h * (o3.x + o3.y));
         } else {         // w = exp(+i pi / 4)
-            t1 = make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y));
-            t2 = make_float2(-o2.y, o2.x);
-            t3 = make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y));
+            t1 = mk<V>(h * (o1.x - o1.y), h * (o1.x + o1.y));
+            t2 = mk<V>(-o2.y, o2.x);
+            t3 = mk<V>(-h * (o3.x + o3.y), h * (o3.x - o3.y));
         }
         a[0] = cadd(e0, o0);
         a[4] = csub(e0, o0);
@@ -121,30 +141,30 @@ __device__ __forceinline__ void small_dft(float2 *a, int R) {
         a[3] = cadd(e3, t3);
         a[7] = csub(e3, t3);
     } else if (R == 2) {
-        float2 t = a[0];
+        V t = a[0];
         a[0] = cadd(t, a[1]);
         a[1] = csub(t, a[1]);
     } else if (R == 4) {
         dft4<SIGN>(a[0], a[1], a[2], a[3]);
     } else if (R == 3) {
-        const float c = -0.5f, s = SIGN * 0.86602540378443864676f;
-        float2 t1 = cadd(a[1], a[2]), t2 = csub(a[1], a[2]);
-        float2 m = make_float2(a[0].x + c * t1.x, a[0].y + c * t1.y);
-        float2 js = make_float2(-s * t2.y, s * t2.x);  // i*s*t2
+        const T c = (T)-0.5, s = (T)(SIGN * 0.86602540378443864676);
+        V t1 = cadd(a[1], a[2]), t2 = csub(a[1], a[2]);
+        V m = mk<V>(a[0].x + c * t1.x, a[0].y + c * t1.y);
+        V js = mk<V>(-s * t2.y, s * t2.x);  // i*s*t2
         a[0] = cadd(a[0], t1);
         a[1] = cadd(m, js);
         a[2] = csub(m, js);
     } else {  // R == 5
-        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
-        const float s1 = SIGN * 0.95105651629515357212f, s2 = SIGN * 0.58778525229247312917f;
-        float2 b1 = cadd(a[1], a[4]), b4 = csub(a[1], a[4]);
-        float2 b2 = cadd(a[2], a[3]), b3 = csub(a[2], a[3]);
-        float2 x0 = a[0];
-        float2 m1 = make_float2(x0.x + c1 * b1.x + c2 * b2.x, x0.y + c1 * b1.y + c2 * b2.y);
-        float2 m2 = make_float2(x0.x + c2 * b1.x + c1 * b2.x, x0.y + c2 * b1.y + c1 * b2.y);
+        const T c1 = (T)0.30901699437494742410, c2 = (T)-0.80901699437494742410;
+        const T s1 = (T)(SIGN * 0.95105651629515357212), s2 = (T)(SIGN * 0.58778525229247312917);
+        V b1 = cadd(a[1], a[4]), b4 = csub(a[1], a[4]);
+        V b2 = cadd(a[2], a[3]), b3 = csub(a[2], a[3]);
+        V x0 = a[0];
+        V m1 = mk<V>(x0.x + c1 * b1.x + c2 * b2.x, x0.y + c1 * b1.y + c2 * b2.y);
+        V m2 = mk<V>(x0.x + c2 * b1.x + c1 * b2.x, x0.y + c2 * b1.y + c1 * b2.y);
         // i * (s1 b4 + s2 b3) and i * (s2 b4 - s1 b3)
-        float2 n1 = make_float2(-(s1 * b4.y + s2 * b3.y), s1 * b4.x + s2 * b3.x);
-        float2 n2 = make_float2(-(s2 * b4.y - s1 * b3.y), s2 * b4.x - s1 * b3.x);
+        V n1 = mk<V>(-(s1 * b4.y + s2 * b3.y), s1 * b4.x + s2 * b3.x);
+        V n2 = mk<V>(-(s2 * b4.y - s1 * b3.y), s2 * b4.x - s1 * b3.x);
         a[0] = cadd(x0, cadd(b1, b2));
         a[1] = cadd(m1, n1);
         a[4] = csub(m1, n1);
@@ -154,14 +174,14 @@ __device__ __forceinline__ void small_dft(float2 *a, int R) {
 }
 
 // One radix-R stage over all lines: in-place butterflies on sub-length m.
-template <int R, bool INV>
-__device__ __forceinline__ void fft_stage(float2 *d, int L, int S, int lines, int m, const float2 *tw) {
+template <int R, bool INV, class V>
+__device__ __forceinline__ void fft_stage(V *d, int L, int S, int lines, int m, const V *tw) {
     const int mp = m / R, nb = L / R, step = L / m;
     for (int t = threadIdx.x; t < nb * lines; t += blockDim.x) {
         const int line = t / nb, b = t - line * nb;
         const int grp = b / mp, k = b - grp * mp;
-        float2 *base = d + line * S + grp * m + k;
-        float2 a[R];
+        V *base = d + line * S + grp * m + k;
+        V a[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = base[r * mp];
         if (!INV) {
@@ -179,8 +199,8 @@ __device__ __forceinline__ void fft_stage(float2 *d, int L, int S, int lines, in
     __syncthreads();
 }
 
-template <bool INV>
-__device__ __forceinline__ void fft_stage_any(float2 *d, int L, int S, int lines, int R, int m, const float2 *tw) {
+template <bool INV, class V>
+__device__ __forceinline__ void fft_stage_any(V *d, int L, int S, int lines, int R, int m, const V *tw) {
     switch (R) {
         case 8: fft_stage<8, INV>(d, L, S, lines, m, tw); break;
         case 4: fft_stage<4, INV>(d, L, S, lines, m, tw); break;
@@ -194,8 +214,8 @@ __device__ __forceinline__ void fft_stage_any(float2 *d, int L, int S, int lines
 // d[line*S + pos].  Forward: decimation in frequency, natural in, digit-reversed
 // out.  Inverse: decimation in time with conjugate twiddles, digit-reversed in,
 // natural out (unnormalized).  tw = exp(-2 pi i k / L); k*q*step < L always.
-template <bool INV>
-__device__ void lds_fft(float2 *d, int L, int S, int lines, const Fac &fc, const float2 *tw) {
+template <bool INV, class V>
+__device__ void lds_fft(V *d, int L, int S, int lines, const Fac &fc, const V *tw) {
     if (!INV) {
         int m = L;
         for (int s = 0; s < fc.n; ++s) {
@@ -211,18 +231,22 @@ __device__ void lds_fft(float2 *d, int L, int S, int lines, const Fac &fc, const
     }
 }
 
-__device__ __forceinline__ void load_tw(float2 *s_tw, const float2 *tw, int L) {
+template <class V>
+__device__ __forceinline__ void load_tw(V *s_tw, const V *tw, int L) {
     for (int i = threadIdx.x; i < L; i += blockDim.x) s_tw[i] = tw[i];
 }
 
 // ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
+template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restrict__ words, int nx, int ny, int nz,
                                                          int row_words, int fy, int H, Fac fc, int L,
-                                                         const float2 *__restrict__ tw, const int *__restrict__ pos,
-                                                         int pairs_per_block, float2 *__restrict__ g1) {
-    extern __shared__ float2 sm[];
-    float2 *s_tw = sm;
-    float2 *d = sm + L;
+                                                         const C2<T> *__restrict__ tw, const int *__restrict__ pos,
+                                                         int pairs_per_block, C2<T> *__restrict__ g1) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    V *sm = reinterpret_cast<V *>(sm_raw);
+    V *s_tw = sm;
+    V *d = sm + L;
     load_tw(s_tw, tw, L);
     const int npairs = nx * ((ny + 1) / 2);
     const int p0 = blockIdx.x * pairs_per_block;
@@ -231,43 +255,47 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
     for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
         const int pl = t / L, z = t - pl * L;
         const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
-        float re = 0.f, im = 0.f;
+        T re = 0, im = 0;
         if (z < nz) {
             const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
-            re = (float)((ra[z >> 6] >> (z & 63)) & 1ull);
-            if (y + 1 < ny) im = (float)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
+            re = (T)((ra[z >> 6] >> (z & 63)) & 1ull);
+            if (y + 1 < ny) im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
         }
-        d[pl * L + z] = make_float2(re, im);
+        d[pl * L + z] = mk<V>(re, im);
     }
     __syncthreads();
     lds_fft<false>(d, L, L, np, fc, s_tw);
     for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
         const int pl = t / H, k = t - pl * H;
         const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
-        const float2 zk = d[pl * L + pos[k]];
-        const float2 zn = d[pl * L + pos[(L - k) % L]];
+        const V zk = d[pl * L + pos[k]];
+        const V zn = d[pl * L + pos[(L - k) % L]];
         // A = (Z[k] + conj Z[-k]) / 2,  B = (Z[k] - conj Z[-k]) / (2i)
-        const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y - zn.y));
-        const float2 B = make_float2(0.5f * (zk.y + zn.y), -0.5f * (zk.x - zn.x));
+        const T half = (T)0.5;
+        const V A = mk<V>(half * (zk.x + zn.x), half * (zk.y - zn.y));
+        const V B = mk<V>(half * (zk.y + zn.y), -half * (zk.x - zn.x));
         g1[((uint64_t)x * fy + y) * H + k] = A;
         if (y + 1 < ny) g1[((uint64_t)x * fy + y + 1) * H + k] = B;
     }
 }
 
 // ---- y forward: per (x, kz tile), rows y >= ny are zero; in place in G1
+template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, int tiles_kz, int tile, Fac fc,
-                                                         const float2 *__restrict__ tw, float2 *__restrict__ g1) {
-    extern __shared__ float2 sm[];
+                                                         const C2<T> *__restrict__ tw, C2<T> *__restrict__ g1) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = fy, S = L + 1;
-    float2 *s_tw = sm;
-    float2 *d = sm + L;
+    V *s_tw = sm;
+    V *d = sm + L;
     load_tw(s_tw, tw, L);
     const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
-    float2 *base = g1 + (uint64_t)x * fy * H + kz0;
+    V *base = g1 + (uint64_t)x * fy * H + kz0;
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         const int y = t / tile, c = t - y * tile;
-        float2 v = make_float2(0.f, 0.f);
+        V v = mk<V>(0, 0);
         if (y < ny && c < nk) v = base[(uint64_t)y * H + c];
         d[c * S + y] = v;
     }
@@ -280,20 +308,23 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
 }
 
 // ---- x: forward, |F|^2, inverse; keep the needed x.  One block per (ky, kz tile).
+template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H, int tiles_kz, int tile, Fac fc,
-                                                           const float2 *__restrict__ tw, int Rx,
-                                                           const float2 *__restrict__ g1, float2 *__restrict__ g2) {
-    extern __shared__ float2 sm[];
+                                                           const C2<T> *__restrict__ tw, int Rx,
+                                                           const C2<T> *__restrict__ g1, C2<T> *__restrict__ g2) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = (int)fc.f[kMaxFactors - 1];  // unused slot carries L
     const int S = L + 1;
-    float2 *s_tw = sm;
-    float2 *d = sm + L;
+    V *s_tw = sm;
+    V *d = sm + L;
     load_tw(s_tw, tw, L);
     const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         const int x = t / tile, c = t - x * tile;
-        float2 v = make_float2(0.f, 0.f);
+        V v = mk<V>(0, 0);
         if (x < nx && c < nk) v = g1[((uint64_t)x * fy + ky) * H + kz0 + c];
         d[c * S + x] = v;
     }
@@ -301,8 +332,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     lds_fft<false>(d, L, S, nk, fc, s_tw);
     for (int t = threadIdx.x; t < L * nk; t += blockDim.x) {
         const int c = t / L, x = t - c * L;
-        const float2 v = d[c * S + x];
-        d[c * S + x] = make_float2(v.x * v.x + v.y * v.y, 0.f);
+        const V v = d[c * S + x];
+        d[c * S + x] = mk<V>(v.x * v.x + v.y * v.y, 0);
     }
     __syncthreads();
     lds_fft<true>(d, L, S, nk, fc, s_tw);
@@ -316,20 +347,23 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
 }
 
 // ---- y inverse: per (ix, kz tile), keep the needed y
+template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tiles_kz, int tile, Fac fc,
-                                                         const float2 *__restrict__ tw, int Ry,
-                                                         const float2 *__restrict__ g2, float2 *__restrict__ g3) {
-    extern __shared__ float2 sm[];
+                                                         const C2<T> *__restrict__ tw, int Ry,
+                                                         const C2<T> *__restrict__ g2, C2<T> *__restrict__ g3) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = fy, S = L + 1;
-    float2 *s_tw = sm;
-    float2 *d = sm + L;
+    V *s_tw = sm;
+    V *d = sm + L;
     load_tw(s_tw, tw, L);
     const int ix = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
-    const float2 *base = g2 + (uint64_t)ix * fy * H + kz0;
+    const V *base = g2 + (uint64_t)ix * fy * H + kz0;
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         const int y = t / tile, c = t - y * tile;
-        d[c * S + y] = c < nk ? base[(uint64_t)y * H + c] : make_float2(0.f, 0.f);
+        d[c * S + y] = c < nk ? base[(uint64_t)y * H + c] : mk<V>(0, 0);
     }
     __syncthreads();
     lds_fft<true>(d, L, S, nk, fc, s_tw);
@@ -343,13 +377,16 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tile
 }
 
 // ---- z inverse: two Hermitian half spectra per complex inverse; keep needed z
+template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac fc, int L,
-                                                         const float2 *__restrict__ tw, const int *__restrict__ rev,
-                                                         int pairs_per_block, int Rz, float scale,
-                                                         const float2 *__restrict__ g3, float *__restrict__ cube) {
-    extern __shared__ float2 sm[];
-    float2 *s_tw = sm;
-    float2 *d = sm + L;
+                                                         const C2<T> *__restrict__ tw, const int *__restrict__ rev,
+                                                         int pairs_per_block, int Rz, T scale,
+                                                         const C2<T> *__restrict__ g3, T *__restrict__ cube) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    V *sm = reinterpret_cast<V *>(sm_raw);
+    V *s_tw = sm;
+    V *d = sm + L;
     load_tw(s_tw, tw, L);
     const int p0 = blockIdx.x * pairs_per_block;
     const int npairs = (nlines + 1) / 2;
@@ -360,13 +397,13 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac 
         const int k = rev[pos];
         const bool upper = k >= H;
         const int kk = upper ? L - k : k;
-        float2 A = g3[(uint64_t)la * H + kk];
-        float2 B = lb < nlines ? g3[(uint64_t)lb * H + kk] : make_float2(0.f, 0.f);
+        V A = g3[(uint64_t)la * H + kk];
+        V B = lb < nlines ? g3[(uint64_t)lb * H + kk] : mk<V>(0, 0);
         if (upper) {
             A.y = -A.y;
             B.y = -B.y;
         }
-        d[pl * L + pos] = make_float2(A.x - B.y, A.y + B.x);  // A + iB
+        d[pl * L + pos] = mk<V>(A.x - B.y, A.y + B.x);  // A + iB
     }
     __syncthreads();
     lds_fft<true>(d, L, L, np, fc, s_tw);
@@ -375,14 +412,15 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac 
         const int pl = t / nzo, iz = t - pl * nzo;
         const int z = iz <= Rz ? iz : L - nzo + iz;
         const int la = 2 * (p0 + pl), lb = la + 1;
-        const float2 v = d[pl * L + z];
+        const V v = d[pl * L + z];
         cube[(uint64_t)la * nzo + iz] = v.x * scale;
         if (lb < nlines) cube[(uint64_t)lb * nzo + iz] = v.y * scale;
     }
 }
 
-__global__ void fft_gather(const Offset *__restrict__ offs, uint64_t n, const float *__restrict__ cube, int Rx,
-                           int Ry, int Rz, float *__restrict__ out) {
+template <class T>
+__global__ void fft_gather(const Offset *__restrict__ offs, uint64_t n, const T *__restrict__ cube, int Rx,
+                           int Ry, int Rz, T *__restrict__ out) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Offset o = offs[i];
@@ -432,15 +470,16 @@ uint64_t smooth_length(uint64_t min_len) {
     return best;
 }
 
-s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], FftPlan **out) {
-    auto key = std::make_tuple(dims[0] | ((uint64_t)R[0] << 32), dims[1] | ((uint64_t)R[1] << 32),
-                               dims[2] | ((uint64_t)R[2] << 32));
+s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], int real_bytes, FftPlan **out) {
+    auto key = std::make_tuple(dims[0] | ((uint64_t)R[0] << 32) | ((uint64_t)(real_bytes == 8) << 63),
+                               dims[1] | ((uint64_t)R[1] << 32), dims[2] | ((uint64_t)R[2] << 32));
     auto it = ctx->fft_plans.find(key);
     if (it != ctx->fft_plans.end()) {
         *out = it->second;
         return S2H_OK;
     }
     auto *p = new FftPlan();
+    p->real_bytes = real_bytes;
     for (int a = 0; a < 3; ++a) {
         p->dims[a] = dims[a];
         p->f[a] = smooth_length(2 * dims[a] - 1);
@@ -457,12 +496,15 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], Ff
     };
     for (int a = 0; a < 3; ++a) {
         const int L = (int)p->f[a];
-        std::vector<float2> tw(L);
+        std::vector<double2> tw(L);
+        std::vector<float2> tw32(L);
         for (int k = 0; k < L; ++k) {
             const double ang = -2.0 * M_PI * (double)k / (double)L;
-            tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+            tw[k] = make_double2(std::cos(ang), std::sin(ang));
+            tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
         }
-        s2h_status st = upload(ctx, p->tw[a], tw.data(), (size_t)L * sizeof(float2));
+        s2h_status st = real_bytes == 8 ? upload(ctx, p->tw[a], tw.data(), (size_t)L * sizeof(double2))
+                                        : upload(ctx, p->tw[a], tw32.data(), (size_t)L * sizeof(float2));
         if (st != S2H_OK) return cleanup(st);
     }
     {   // natural frequency held at each DIF output position (z axis)
@@ -488,17 +530,19 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], Ff
     const uint64_t H = p->f[2] / 2 + 1;
     const uint64_t nxo = 2 * (uint64_t)R[0] + 1, nyo = 2 * (uint64_t)R[1] + 1, nzo = 2 * (uint64_t)R[2] + 1;
     s2h_status st;
-    if ((st = dev_alloc(ctx, p->g1, dims[0] * p->f[1] * H * 8)) != S2H_OK ||
-        (st = dev_alloc(ctx, p->g2, nxo * p->f[1] * H * 8)) != S2H_OK ||
-        (st = dev_alloc(ctx, p->g3, nxo * nyo * H * 8)) != S2H_OK ||
-        (st = dev_alloc(ctx, p->cube, nxo * nyo * nzo * 4)) != S2H_OK)
+    const uint64_t cb = 2 * (uint64_t)real_bytes;  // bytes per complex value
+    if ((st = dev_alloc(ctx, p->g1, dims[0] * p->f[1] * H * cb)) != S2H_OK ||
+        (st = dev_alloc(ctx, p->g2, nxo * p->f[1] * H * cb)) != S2H_OK ||
+        (st = dev_alloc(ctx, p->g3, nxo * nyo * H * cb)) != S2H_OK ||
+        (st = dev_alloc(ctx, p->cube, nxo * nyo * nzo * (uint64_t)real_bytes)) != S2H_OK)
         return cleanup(st);
     ctx->fft_plans[key] = p;
     *out = p;
     return S2H_OK;
 }
 
-size_t lds_bytes(int L, int lines, int stride) { return ((size_t)L + (size_t)lines * stride) * sizeof(float2); }
+template <class T>
+size_t lds_bytes(int L, int lines, int stride) { return ((size_t)L + (size_t)lines * stride) * sizeof(C2<T>); }
 
 s2h_status set_lds(const void *fn, size_t bytes) {
     if (bytes > 160 * 1024) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
@@ -509,7 +553,9 @@ s2h_status set_lds(const void *fn, size_t bytes) {
 // Run every pass; leaves the correlation cube in p->cube.  With ev, records
 // ev[0] before the y forward pass and ev[1..3] after it, the x pass and the y
 // inverse pass.
+template <class T>
 s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = nullptr) {
+    using V = C2<T>;
     const int nx = (int)vol->dims[0], ny = (int)vol->dims[1], nz = (int)vol->dims[2];
     const int fx = (int)p->f[0], fy = (int)p->f[1], fz = (int)p->f[2];
     const int H = fz / 2 + 1;
@@ -521,54 +567,54 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     const int tiles_y = (H + ty - 1) / ty, tiles_x = (H + tx - 1) / tx;
     // z forward
     {
-        const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * 8))));
-        const size_t lds = lds_bytes(fz, ppb, fz);
-        S2H_TRY(set_lds((const void *)fft_z_fwd, lds));
+        const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * sizeof(V)))));
+        const size_t lds = lds_bytes<T>(fz, ppb, fz);
+        S2H_TRY(set_lds((const void *)fft_z_fwd<T>, lds));
         const int npairs = nx * ((ny + 1) / 2);
-        fft_z_fwd<<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
-            vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, fcz, fz, (const float2 *)p->tw[2].ptr,
-            (const int *)p->pos_z.ptr, ppb, (float2 *)p->g1.ptr);
+        fft_z_fwd<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
+            vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, fcz, fz, (const V *)p->tw[2].ptr,
+            (const int *)p->pos_z.ptr, ppb, (V *)p->g1.ptr);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[0], s));
     {
-        const size_t lds = lds_bytes(fy, ty, fy + 1);
-        S2H_TRY(set_lds((const void *)fft_y_fwd, lds));
-        fft_y_fwd<<<nx * tiles_y, kFftThreads, lds, s>>>(ny, fy, H, tiles_y, ty, fcy, (const float2 *)p->tw[1].ptr,
-                                                         (float2 *)p->g1.ptr);
+        const size_t lds = lds_bytes<T>(fy, ty, fy + 1);
+        S2H_TRY(set_lds((const void *)fft_y_fwd<T>, lds));
+        fft_y_fwd<T><<<nx * tiles_y, kFftThreads, lds, s>>>(ny, fy, H, tiles_y, ty, fcy, (const V *)p->tw[1].ptr,
+                                                            (V *)p->g1.ptr);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[1], s));
     {
-        const size_t lds = lds_bytes(fx, tx, fx + 1);
-        S2H_TRY(set_lds((const void *)fft_x_power, lds));
-        fft_x_power<<<fy * tiles_x, kFftThreads, lds, s>>>(nx, fy, H, tiles_x, tx, fcx, (const float2 *)p->tw[0].ptr,
-                                                           Rx, (const float2 *)p->g1.ptr, (float2 *)p->g2.ptr);
+        const size_t lds = lds_bytes<T>(fx, tx, fx + 1);
+        S2H_TRY(set_lds((const void *)fft_x_power<T>, lds));
+        fft_x_power<T><<<fy * tiles_x, kFftThreads, lds, s>>>(nx, fy, H, tiles_x, tx, fcx, (const V *)p->tw[0].ptr,
+                                                              Rx, (const V *)p->g1.ptr, (V *)p->g2.ptr);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[2], s));
     {
-        const size_t lds = lds_bytes(fy, ty, fy + 1);
-        S2H_TRY(set_lds((const void *)fft_y_inv, lds));
-        fft_y_inv<<<nxo * tiles_y, kFftThreads, lds, s>>>(fy, H, tiles_y, ty, fcy, (const float2 *)p->tw[1].ptr, Ry,
-                                                          (const float2 *)p->g2.ptr, (float2 *)p->g3.ptr);
+        const size_t lds = lds_bytes<T>(fy, ty, fy + 1);
+        S2H_TRY(set_lds((const void *)fft_y_inv<T>, lds));
+        fft_y_inv<T><<<nxo * tiles_y, kFftThreads, lds, s>>>(fy, H, tiles_y, ty, fcy, (const V *)p->tw[1].ptr, Ry,
+                                                             (const V *)p->g2.ptr, (V *)p->g3.ptr);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[3], s));
     {
-        const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * 8))));
-        const size_t lds = lds_bytes(fz, ppb, fz);
-        S2H_TRY(set_lds((const void *)fft_z_inv, lds));
+        const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * sizeof(V)))));
+        const size_t lds = lds_bytes<T>(fz, ppb, fz);
+        S2H_TRY(set_lds((const void *)fft_z_inv<T>, lds));
         const int nlines = nxo * nyo;
         const int npairs = (nlines + 1) / 2;
-        const float scale = (float)(1.0 / ((double)fx * (double)fy * (double)fz));
-        fft_z_inv<<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
-            nlines, H, fcz, fz, (const float2 *)p->tw[2].ptr, (const int *)p->rev_z.ptr, ppb, Rz, scale,
-            (const float2 *)p->g3.ptr, (float *)p->cube.ptr);
+        const T scale = (T)(1.0 / ((double)fx * (double)fy * (double)fz));
+        fft_z_inv<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
+            nlines, H, fcz, fz, (const V *)p->tw[2].ptr, (const int *)p->rev_z.ptr, ppb, Rz, scale,
+            (const V *)p->g3.ptr, (T *)p->cube.ptr);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
@@ -583,11 +629,12 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
 // passes favour wide tiles (longer contiguous kz runs per row), the x pass
 // narrow ones (more workgroups per CU).  Every run rewrites all of the plan's
 // buffers, so the trial runs leave nothing behind.
+template <class T>
 s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     const int lmax = (int)std::max(p->f[0], p->f[1]);
     std::vector<int> cands;
     for (int t = kTileMax; t >= 1; t /= 2)
-        if (lds_bytes(lmax, t, lmax + 1) <= 150 * 1024) cands.push_back(t);
+        if (lds_bytes<T>(lmax, t, lmax + 1) <= 150 * 1024) cands.push_back(t);
     if (cands.empty()) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
     // untimed: the widest tile for y, a narrow one for x (the usual picks)
     p->tile_y = cands.front();
@@ -606,8 +653,8 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     for (int t : cands) {
         if (st != S2H_OK) break;
         p->tile_y = p->tile_x = t;
-        st = fft_run(ctx, vol, p);  // warm-up
-        if (st == S2H_OK) st = fft_run(ctx, vol, p, ev);
+        st = fft_run<T>(ctx, vol, p);  // warm-up
+        if (st == S2H_OK) st = fft_run<T>(ctx, vol, p, ev);
         hipError_t e = hipSuccess;
         if (st == S2H_OK) e = hipEventSynchronize(ev[3]);
         float yf = 0, xm = 0, yi = 0;
@@ -632,32 +679,93 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     return st;
 }
 
-// FP32 counts of every shell offset into p->fcounts (device) using the exact plan's offset list.
-s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, ExactPlan **ep_out,
-                             FftPlan **fp_out) {
-    ExactPlan *ep = nullptr;
-    S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pitch, &ep));
+// Correlation values (precision T) of every shell offset of the exact plan
+// into fp->fcounts (device).
+template <class T>
+s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, FftPlan **fp_out) {
     const int *ext = ep->shells->host->extent;
     int R[3] = {ext[0], ext[1], ext[2]};
     FftPlan *fp = nullptr;
-    S2H_TRY(fft_plan_get(ctx, vol->dims, R, &fp));
+    S2H_TRY(fft_plan_get(ctx, vol->dims, R, (int)sizeof(T), &fp));
     const uint64_t K = ep->offs.size();
-    S2H_TRY(dev_alloc(ctx, fp->fcounts, std::max<uint64_t>(K, 1) * 4));
-    if (fp->tile_y == 0) S2H_TRY(fft_pick_tile(ctx, vol, fp));
+    S2H_TRY(dev_alloc(ctx, fp->fcounts, std::max<uint64_t>(K, 1) * sizeof(T)));
+    if (fp->tile_y == 0) S2H_TRY(fft_pick_tile<T>(ctx, vol, fp));
     prof_mark(ctx, 0);
-    S2H_TRY(fft_run(ctx, vol, fp));
+    S2H_TRY(fft_run<T>(ctx, vol, fp));
     if (K) {
-        fft_gather<<<(unsigned)((K + 255) / 256), 256, 0, ctx->stream>>>(
-            (const Offset *)ep->d_offs.ptr, K, (const float *)fp->cube.ptr, R[0], R[1], R[2],
-            (float *)fp->fcounts.ptr);
+        fft_gather<T><<<(unsigned)((K + 255) / 256), 256, 0, ctx->stream>>>(
+            (const Offset *)ep->d_offs.ptr, K, (const T *)fp->cube.ptr, R[0], R[1], R[2], (T *)fp->fcounts.ptr);
         S2H_HIP(hipGetLastError());
     }
-    *ep_out = ep;
     *fp_out = fp;
     return S2H_OK;
 }
 
 }  // namespace
+
+// Worst-case bound on |computed - exact| of every FP64 correlation value.
+// The input is 0/1 with K ones (||x||_2^2 = K, |Y_k| <= K for Y = DFT x).  A
+// transform over L radix-2-equivalent levels has ||Y^ - Y||_2 <= L eta ||Y||_2
+// with eta = mu + gamma_4 (sqrt 2 + mu) <= 8u (u = 2^-53, twiddles within
+// mu <= 2u; Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed.,
+// Thm 24.2).  Then ||Z^ - Z||_2 <= (2 rho + gamma_2) K ||Y||_2 for Z = |Y|^2,
+// and the scaled inverse adds rho again, so every output satisfies
+// |c^ - c| <= ||c^ - c||_2 <= (3 rho + 3u) K^{3/2}, rho = L * 8u.  L counts
+// each radix-R stage as ceil(log2 R) levels over the three axes, plus two for
+// the real-to-complex packing and unpacking along z.  With the bound below
+// 1/2 the rounded values are the exact integer counts; the path requires 1/4.
+double fft64_error_bound(const uint64_t dims[3], uint64_t occupied) {
+    double levels = 2.0;
+    for (int a = 0; a < 3; ++a) {
+        const AxisPlan ap = axis_plan((int)std::min<uint64_t>(smooth_length(2 * dims[a] - 1), 1u << 30));
+        if (ap.nf < 0) return INFINITY;
+        for (int s = 0; s < ap.nf; ++s) levels += ap.fac[s] == 8 ? 3 : ap.fac[s] == 5 ? 3 : ap.fac[s] == 2 ? 1 : 2;
+    }
+    const double u = std::ldexp(1.0, -53);
+    const double rho = levels * 8.0 * u;
+    const double k = (double)occupied;
+    return (3.0 * rho + 3.0 * u) * k * std::sqrt(k);
+}
+
+// FP64 correlation of every shell offset of `ep` for `vol` (device array of
+// ep->offs.size() doubles, in shell order).  The caller rounds them.
+s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, const double **d_values) {
+    FftPlan *fp = nullptr;
+    S2H_TRY(fft_counts_device<double>(ctx, vol, ep, &fp));
+    *d_values = (const double *)fp->fcounts.ptr;
+    return S2H_OK;
+}
+
+// Algorithmic bytes of one FP64 evaluation (3 axes x forward + inverse x read
+// + write of the complex128 half spectrum), for the method model.
+double fft64_model_bytes(const uint64_t dims[3]) {
+    const double fx = (double)smooth_length(2 * dims[0] - 1), fy = (double)smooth_length(2 * dims[1] - 1);
+    const double h = (double)(smooth_length(2 * dims[2] - 1) / 2 + 1);
+    return 12.0 * fx * fy * h * 16.0;
+}
+
+// Device bytes of an FP64 plan's spectra, cube and per-offset values (the
+// working set the method choice budgets, as fft_working_set_bytes does for the
+// reference's planner, src/geometry/s2.rs:695-714).
+double fft64_plan_bytes(const uint64_t dims[3], const int R[3]) {
+    const double fy = (double)smooth_length(2 * dims[1] - 1);
+    const double h = (double)(smooth_length(2 * dims[2] - 1) / 2 + 1);
+    const double nxo = 2.0 * R[0] + 1, nyo = 2.0 * R[1] + 1, nzo = 2.0 * R[2] + 1;
+    return ((double)dims[0] * fy * h + nxo * fy * h + nxo * nyo * h) * 16.0 + nxo * nyo * nzo * 8.0;
+}
+
+// Whether the FP64 passes can run this shape (lengths of 2, 3, 5 factors up
+// to 8192 whose LDS tiles fit).
+bool fft64_supported(const uint64_t dims[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const uint64_t f = smooth_length(2 * dims[a] - 1);
+        if (f > 8192 || axis_plan((int)f).nf < 0) return false;
+    }
+    const int lmax = (int)std::max(smooth_length(2 * dims[0] - 1), smooth_length(2 * dims[1] - 1));
+    const int lz = (int)smooth_length(2 * dims[2] - 1);
+    return lds_bytes<double>(lmax, 1, lmax + 1) <= 150 * 1024 && lds_bytes<double>(lz, 1, lz) <= 150 * 1024;
+}
+
 }  // namespace s2h
 
 using namespace s2h;
@@ -675,7 +783,7 @@ s2h_status s2h_fft32_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
         return fail(S2H_ERR_SIZE, "counts buffer holds " + std::to_string(n_offsets) + " entries, plan has " +
                                       std::to_string(ep->offs.size()));
     FftPlan *fp = nullptr;
-    S2H_TRY(fft_counts_device(ctx, vol, r_max, pitch, &ep, &fp));
+    S2H_TRY(fft_counts_device<float>(ctx, vol, ep, &fp));
     prof_mark(ctx, 1);
     prof_mark(ctx, 2);
     if (n_offsets)
@@ -697,8 +805,9 @@ s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     }
     const double pm = pitch < 1e-9 ? 1e-9 : pitch;
     ExactPlan *ep = nullptr;
+    S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pm, &ep));
     FftPlan *fp = nullptr;
-    S2H_TRY(fft_counts_device(ctx, vol, r_max, pm, &ep, &fp));
+    S2H_TRY(fft_counts_device<float>(ctx, vol, ep, &fp));
     S2H_TRY(exact_gather_fold(ctx, ep, occupied, (const float *)fp->fcounts.ptr));
     const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
     return finish_curve_from_device(ctx, ep, vf, s2);

@@ -363,6 +363,8 @@ struct ExactPlan {
     int tuned = 0;                        // 0 not yet, 1 timed on the device, -1 model pick
     int timed_candidates = 0;
     double model_ops = 0;                 // modelled executed VALU lane-ops per run
+    double model_ms = 0;                  // modelled count-kernel time of the applied shape
+    int method = 0;                       // kernel of the last evaluation: 1 direct, 2 FP64 FFT (rounded)
     uint32_t ngroups = 0;
     std::vector<DzBlock> blocks;
     std::vector<WorkItem> items;          // heaviest first
@@ -417,6 +419,8 @@ struct s2h_ctx {
     s2h::DevBuf d_scalar;   // small scratch (popcount result etc.)
     bool profiling = false;
     bool autotune = true;   // time launch-shape candidates on a new exact plan
+    int exact_method = 0;   // exact kernel: 0 by the cost model, 1 direct counting, 2 FP64 FFT rounded
+    uint64_t fft_budget = 0;  // largest FP64 FFT working set the model's choice may take (0: no limit)
     std::map<std::tuple<uint32_t, uint64_t, uint64_t, uint64_t, uint64_t>, std::shared_ptr<s2h::ShellDev>>
         shell_devs;   // device copies of shared shell data, by (r_max, pitch, clipped extent)
     s2h::BatchWS *batch_ws = nullptr;            // persistent batch workspace and graph
@@ -454,8 +458,19 @@ void shell_devs_release(s2h_ctx *ctx);  // device copies of shared shell data
 // Launch the count kernels for vol into plan->d_table (zeroed first) on ctx->stream.
 s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan);
 // Gather table -> per-offset hits and f64 terms, fold per radius into d_curve.
-s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied,
-                             const float *d_fcounts /* nullptr: integer table */);
+// d_fcounts: FP32 correlation values (not rounded); d_dvals: FP64 correlation
+// values, rounded to the integer counts; both nullptr: the direct count table.
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied, const float *d_fcounts,
+                             const double *d_dvals = nullptr);
+// FP64 FFT exact kernel (s2hip_fft.hip): worst-case |computed - exact| of the
+// correlation values for a volume with `occupied` voxels (rounding is exact
+// below 1/2), whether the shape's lengths are supported, the algorithmic bytes
+// of one evaluation, and the run itself (device array in shell order).
+double fft64_error_bound(const uint64_t dims[3], uint64_t occupied);
+bool fft64_supported(const uint64_t dims[3]);
+double fft64_model_bytes(const uint64_t dims[3]);
+double fft64_plan_bytes(const uint64_t dims[3], const int R[3]);
+s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan, const double **d_values);
 // Copy folded curve values and supported flags back and finish the curve.
 s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *plan, double vf, double *s2);
 // Profiling marks: k = 0 (before counts), 1 (after counts), 2 (after fold).

@@ -61,8 +61,9 @@ uint64_t smooth_len(uint64_t n) {
 }
 
 // Device bytes one request holds while admitted: packed volume, shell plan
-// (offsets, gather, per-offset hits and terms, dense table) and, for FFT32,
-// the pruned spectra.  Shell size bound: the ball of radius r_max/pitch + 1.
+// (offsets, gather, per-offset hits and terms, dense table) and the pruned
+// spectra -- FP32 for FFT32, FP64 for an exact request (whose kernel may be
+// the rounded FP64 FFT).  Shell size bound: the ball of radius r_max/pitch + 1.
 uint64_t footprint(const uint64_t dims[3], uint32_t r_max, double pitch, int32_t method) {
     const double R = (double)r_max / pitch + 0.5 / pitch + 1.0;
     double K = 4.18879 * R * R * R;
@@ -71,13 +72,14 @@ uint64_t footprint(const uint64_t dims[3], uint32_t r_max, double pitch, int32_t
     const uint64_t row_words = (dims[2] + 63) / 64 + 1;
     double bytes = (double)(dims[0] * dims[1] * row_words * 8 + kVolTailBytes);
     bytes += K * (12 + 4 + 8 + 8) + K * 8 / 2 + 65536.0 * 64;
-    if (method == 1) {
+    {
         const double fy = (double)smooth_len(2 * dims[1] - 1), fz = (double)smooth_len(2 * dims[2] - 1);
         const double H = std::floor(fz / 2) + 1;
         const double rx = std::min<double>(dims[0] - 1, R), ry = std::min<double>(dims[1] - 1, R),
                      rz = std::min<double>(dims[2] - 1, R);
-        bytes += (double)dims[0] * fy * H * 8 + (2 * rx + 1) * fy * H * 8 + (2 * rx + 1) * (2 * ry + 1) * H * 8 +
-                 (2 * rx + 1) * (2 * ry + 1) * (2 * rz + 1) * 4 + K * 4;
+        const double real = method == 1 ? 4.0 : 8.0;  // bytes per real value of the spectra
+        bytes += ((double)dims[0] * fy * H * 2 + (2 * rx + 1) * fy * H * 2 + (2 * rx + 1) * (2 * ry + 1) * H * 2 +
+                  (2 * rx + 1) * (2 * ry + 1) * (2 * rz + 1) + K) * real;
     }
     return (uint64_t)(bytes * 1.1) + (1ull << 20);
 }
@@ -240,6 +242,10 @@ s2h_status s2h_serve_create(int device, uint64_t hbm_budget_bytes, uint32_t nslo
         // a stream of mixed shapes sees most (grid, r_max) pairs once: take the
         // cost model's launch shape (or one timed earlier in the process)
         if (st == S2H_OK) st = s2h_set_autotune(slot->ctx, 0);
+        // an exact request takes the FP64 FFT kernel only where its spectra
+        // fit half the slot's plan allowance: larger plans would be dropped
+        // and reallocated between requests, and hipFree stalls every slot
+        if (st == S2H_OK) slot->ctx->fft_budget = sv->slot_plan_cap / 2;
         if (st != S2H_OK) {
             if (slot->ctx) s2h_destroy(slot->ctx);
             delete slot;

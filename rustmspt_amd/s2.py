@@ -93,10 +93,20 @@ class Context:
         return {"band": int(out[0]), "waves": int(out[1]),
                 "picked_by": {1: "device timing", -1: "cost model", 0: "not run"}[int(out[2])],
                 "timed_candidates": int(out[3]), "items": int(out[4]), "groups": int(out[5]),
-                "model_wave_instructions": int(out[6])}
+                "model_wave_instructions": int(out[6]),
+                "kernel": {0: "none", 1: "direct", 2: "fft64"}[int(out[7])]}
 
     def set_autotune(self, enable: bool) -> None:
         L.check(L.lib().s2h_set_autotune(self.handle, 1 if enable else 0))
+
+    EXACT_METHODS = {"auto": 0, "direct": 1, "fft64": 2}
+
+    def set_exact_method(self, method: str) -> None:
+        """Exact kernel of exact_counts / exact_s2: "direct", "fft64" (FP64 FFT
+        rounded to the integer counts; only where its round-off bound allows) or
+        "auto" (the faster modelled one).  Mirrors ExactCpuMethod and
+        plan_exact_cpu (src/geometry/s2.rs:565-569, 733-766)."""
+        L.check(L.lib().s2h_exact_method_set(self.handle, self.EXACT_METHODS[method]))
 
     def exact_tune_set(self, dims, r_max: int, pitch: float, band: int, waves: int) -> None:
         L.check(L.lib().s2h_exact_tune_set(self.handle, L.ptr(_dims(dims), L._u64p), int(r_max), float(pitch),

@@ -13,6 +13,15 @@ pytestmark = pytest.mark.gpu
 G = Path(__file__).resolve().parent / "golden"
 
 
+@pytest.fixture(autouse=True, scope="module")
+def direct_kernel(ctx):
+    # these tests cover the direct counting kernel; tests/test_gpu_fft64.py
+    # covers the FP64 FFT kernel and the choice between them
+    ctx.set_exact_method("direct")
+    yield
+    ctx.set_exact_method("auto")
+
+
 def hits_oracle(occ, r_max, pitch=1.0):
     offs, rs = o.domain_offsets(occ.shape, r_max, pitch)
     return offs, rs, o.hits_list(o.pack(occ), occ.shape, offs)
