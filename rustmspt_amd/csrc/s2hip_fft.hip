@@ -54,6 +54,7 @@ struct FftPlan {
     DevBuf g1, g2, g3, cube, fcounts;
     int tile_y = 0;  // kz columns per workgroup in the y passes (0: not chosen yet)
     int tile_x = 0;  // kz columns per workgroup in the x pass
+    int sh_z = 3, sh_y = 3, sh_x = 3;  // LDS padding shift per pass kind (3: a slot every 8; 30: none)
 };
 
 void fft_plan_release(s2h_ctx *ctx, FftPlan *p) {
@@ -70,6 +71,20 @@ struct Fac {
     int n;
     int f[kMaxFactors];
 };
+
+// LDS layout of a line: element j sits at pidx(j, sh) = j + (j >> sh), with
+// sh = kPad (one slot every 8) or kNoPad.  The radix-8 stages of a 512-point
+// line read with strides 64 and 8 (and 1): unpadded, the lanes of a wave hit
+// the same banks 8 ways; padded, 16 consecutive lanes reach 16 different
+// 16-byte bank groups.  The padding costs 1/8 more LDS, which for some tiles
+// leaves room for one workgroup fewer per CU, so each pass kind has its own
+// choice.  Lines are lstride(L, sh) apart (odd, so the load phases, which step
+// across lines, spread too); the twiddle table is padded the same way.
+static constexpr int kPad = 3;
+static constexpr int kNoPad = 30;
+__host__ __device__ constexpr int pidx(int j, int sh) { return j + (j >> sh); }
+__host__ __device__ constexpr int ltw(int L, int sh) { return pidx(L - 1, sh) + 1; }
+__host__ __device__ constexpr int lstride(int L, int sh) { return ltw(L, sh) | 1; }
 
 // Complex values of the working precision T (float: the FP32 path; double:
 // the rounded exact path).
@@ -175,65 +190,66 @@ __device__ __forceinline__ void small_dft(V *a, int R) {
 
 // One radix-R stage over all lines: in-place butterflies on sub-length m.
 template <int R, bool INV, class V>
-__device__ __forceinline__ void fft_stage(V *d, int L, int S, int lines, int m, const V *tw) {
+__device__ __forceinline__ void fft_stage(V *d, int L, int S, int sh, int lines, int m, const V *tw) {
     const int mp = m / R, nb = L / R, step = L / m;
     for (int t = threadIdx.x; t < nb * lines; t += blockDim.x) {
         const int line = t / nb, b = t - line * nb;
         const int grp = b / mp, k = b - grp * mp;
-        V *base = d + line * S + grp * m + k;
+        V *base = d + line * S;
+        const int j0 = grp * m + k;
         V a[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = base[r * mp];
+        for (int r = 0; r < R; ++r) a[r] = base[pidx(j0 + r * mp, sh)];
         if (!INV) {
             small_dft<-1>(a, R);
 #pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[k * q * step]);
+            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[pidx(k * q * step, sh)]);
         } else {
 #pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[k * q * step]);
+            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[pidx(k * q * step, sh)]);
             small_dft<1>(a, R);
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) base[r * mp] = a[r];
+        for (int r = 0; r < R; ++r) base[pidx(j0 + r * mp, sh)] = a[r];
     }
     __syncthreads();
 }
 
 template <bool INV, class V>
-__device__ __forceinline__ void fft_stage_any(V *d, int L, int S, int lines, int R, int m, const V *tw) {
+__device__ __forceinline__ void fft_stage_any(V *d, int L, int S, int sh, int lines, int R, int m, const V *tw) {
     switch (R) {
-        case 8: fft_stage<8, INV>(d, L, S, lines, m, tw); break;
-        case 4: fft_stage<4, INV>(d, L, S, lines, m, tw); break;
-        case 2: fft_stage<2, INV>(d, L, S, lines, m, tw); break;
-        case 3: fft_stage<3, INV>(d, L, S, lines, m, tw); break;
-        default: fft_stage<5, INV>(d, L, S, lines, m, tw); break;
+        case 8: fft_stage<8, INV>(d, L, S, sh, lines, m, tw); break;
+        case 4: fft_stage<4, INV>(d, L, S, sh, lines, m, tw); break;
+        case 2: fft_stage<2, INV>(d, L, S, sh, lines, m, tw); break;
+        case 3: fft_stage<3, INV>(d, L, S, sh, lines, m, tw); break;
+        default: fft_stage<5, INV>(d, L, S, sh, lines, m, tw); break;
     }
 }
 
 // In-place mixed-radix transform of `lines` lines of length L held in LDS at
-// d[line*S + pos].  Forward: decimation in frequency, natural in, digit-reversed
+// d[line*S + pidx(pos, sh)].  Forward: decimation in frequency, natural in, digit-reversed
 // out.  Inverse: decimation in time with conjugate twiddles, digit-reversed in,
 // natural out (unnormalized).  tw = exp(-2 pi i k / L); k*q*step < L always.
 template <bool INV, class V>
-__device__ void lds_fft(V *d, int L, int S, int lines, const Fac &fc, const V *tw) {
+__device__ void lds_fft(V *d, int L, int S, int sh, int lines, const Fac &fc, const V *tw) {
     if (!INV) {
         int m = L;
         for (int s = 0; s < fc.n; ++s) {
-            fft_stage_any<false>(d, L, S, lines, fc.f[s], m, tw);
+            fft_stage_any<false>(d, L, S, sh, lines, fc.f[s], m, tw);
             m /= fc.f[s];
         }
     } else {
         int m = 1;  // stage s works on sub-length m_s = f_s * m_{s+1}, m_n = 1
         for (int s = fc.n - 1; s >= 0; --s) {
             m *= fc.f[s];
-            fft_stage_any<true>(d, L, S, lines, fc.f[s], m, tw);
+            fft_stage_any<true>(d, L, S, sh, lines, fc.f[s], m, tw);
         }
     }
 }
 
 template <class V>
-__device__ __forceinline__ void load_tw(V *s_tw, const V *tw, int L) {
-    for (int i = threadIdx.x; i < L; i += blockDim.x) s_tw[i] = tw[i];
+__device__ __forceinline__ void load_tw(V *s_tw, const V *tw, int L, int sh) {
+    for (int i = threadIdx.x; i < L; i += blockDim.x) s_tw[pidx(i, sh)] = tw[i];
 }
 
 // ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
@@ -241,13 +257,14 @@ template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restrict__ words, int nx, int ny, int nz,
                                                          int row_words, int fy, int H, Fac fc, int L,
                                                          const C2<T> *__restrict__ tw, const int *__restrict__ pos,
-                                                         int pairs_per_block, C2<T> *__restrict__ g1) {
+                                                         int pairs_per_block, C2<T> *__restrict__ g1, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     V *s_tw = sm;
-    V *d = sm + L;
-    load_tw(s_tw, tw, L);
+    V *d = sm + ltw(L, sh);
+    load_tw(s_tw, tw, L, sh);
+    const int S = lstride(L, sh);
     const int npairs = nx * ((ny + 1) / 2);
     const int p0 = blockIdx.x * pairs_per_block;
     const int np = min(pairs_per_block, npairs - p0);
@@ -261,15 +278,15 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
             re = (T)((ra[z >> 6] >> (z & 63)) & 1ull);
             if (y + 1 < ny) im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
         }
-        d[pl * L + z] = mk<V>(re, im);
+        d[pl * S + pidx(z, sh)] = mk<V>(re, im);
     }
     __syncthreads();
-    lds_fft<false>(d, L, L, np, fc, s_tw);
+    lds_fft<false>(d, L, S, sh, np, fc, s_tw);
     for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
         const int pl = t / H, k = t - pl * H;
         const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
-        const V zk = d[pl * L + pos[k]];
-        const V zn = d[pl * L + pos[(L - k) % L]];
+        const V zk = d[pl * S + pidx(pos[k], sh)];
+        const V zn = d[pl * S + pidx(pos[(L - k) % L], sh)];
         // A = (Z[k] + conj Z[-k]) / 2,  B = (Z[k] - conj Z[-k]) / (2i)
         const T half = (T)0.5;
         const V A = mk<V>(half * (zk.x + zn.x), half * (zk.y - zn.y));
@@ -282,14 +299,14 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
 // ---- y forward: per (x, kz tile), rows y >= ny are zero; in place in G1
 template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, int tiles_kz, int tile, Fac fc,
-                                                         const C2<T> *__restrict__ tw, C2<T> *__restrict__ g1) {
+                                                         const C2<T> *__restrict__ tw, C2<T> *__restrict__ g1, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
-    const int L = fy, S = L + 1;
+    const int L = fy, S = lstride(L, sh);
     V *s_tw = sm;
-    V *d = sm + L;
-    load_tw(s_tw, tw, L);
+    V *d = sm + ltw(L, sh);
+    load_tw(s_tw, tw, L, sh);
     const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     V *base = g1 + (uint64_t)x * fy * H + kz0;
@@ -297,13 +314,13 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
         const int y = t / tile, c = t - y * tile;
         V v = mk<V>(0, 0);
         if (y < ny && c < nk) v = base[(uint64_t)y * H + c];
-        d[c * S + y] = v;
+        d[c * S + pidx(y, sh)] = v;
     }
     __syncthreads();
-    lds_fft<false>(d, L, S, nk, fc, s_tw);
+    lds_fft<false>(d, L, S, sh, nk, fc, s_tw);
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         const int y = t / tile, c = t - y * tile;
-        if (c < nk) base[(uint64_t)y * H + c] = d[c * S + y];
+        if (c < nk) base[(uint64_t)y * H + c] = d[c * S + pidx(y, sh)];
     }
 }
 
@@ -311,38 +328,38 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
 template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H, int tiles_kz, int tile, Fac fc,
                                                            const C2<T> *__restrict__ tw, int Rx,
-                                                           const C2<T> *__restrict__ g1, C2<T> *__restrict__ g2) {
+                                                           const C2<T> *__restrict__ g1, C2<T> *__restrict__ g2, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = (int)fc.f[kMaxFactors - 1];  // unused slot carries L
-    const int S = L + 1;
+    const int S = lstride(L, sh);
     V *s_tw = sm;
-    V *d = sm + L;
-    load_tw(s_tw, tw, L);
+    V *d = sm + ltw(L, sh);
+    load_tw(s_tw, tw, L, sh);
     const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         const int x = t / tile, c = t - x * tile;
         V v = mk<V>(0, 0);
         if (x < nx && c < nk) v = g1[((uint64_t)x * fy + ky) * H + kz0 + c];
-        d[c * S + x] = v;
+        d[c * S + pidx(x, sh)] = v;
     }
     __syncthreads();
-    lds_fft<false>(d, L, S, nk, fc, s_tw);
+    lds_fft<false>(d, L, S, sh, nk, fc, s_tw);
     for (int t = threadIdx.x; t < L * nk; t += blockDim.x) {
         const int c = t / L, x = t - c * L;
-        const V v = d[c * S + x];
-        d[c * S + x] = mk<V>(v.x * v.x + v.y * v.y, 0);
+        const V v = d[c * S + pidx(x, sh)];
+        d[c * S + pidx(x, sh)] = mk<V>(v.x * v.x + v.y * v.y, 0);
     }
     __syncthreads();
-    lds_fft<true>(d, L, S, nk, fc, s_tw);
+    lds_fft<true>(d, L, S, sh, nk, fc, s_tw);
     const int nxo = 2 * Rx + 1;
     for (int t = threadIdx.x; t < nxo * tile; t += blockDim.x) {
         const int ix = t / tile, c = t - ix * tile;
         if (c >= nk) continue;
         const int x = ix <= Rx ? ix : L - nxo + ix;
-        g2[((uint64_t)ix * fy + ky) * H + kz0 + c] = d[c * S + x];
+        g2[((uint64_t)ix * fy + ky) * H + kz0 + c] = d[c * S + pidx(x, sh)];
     }
 }
 
@@ -350,29 +367,29 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
 template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tiles_kz, int tile, Fac fc,
                                                          const C2<T> *__restrict__ tw, int Ry,
-                                                         const C2<T> *__restrict__ g2, C2<T> *__restrict__ g3) {
+                                                         const C2<T> *__restrict__ g2, C2<T> *__restrict__ g3, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
-    const int L = fy, S = L + 1;
+    const int L = fy, S = lstride(L, sh);
     V *s_tw = sm;
-    V *d = sm + L;
-    load_tw(s_tw, tw, L);
+    V *d = sm + ltw(L, sh);
+    load_tw(s_tw, tw, L, sh);
     const int ix = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     const V *base = g2 + (uint64_t)ix * fy * H + kz0;
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         const int y = t / tile, c = t - y * tile;
-        d[c * S + y] = c < nk ? base[(uint64_t)y * H + c] : mk<V>(0, 0);
+        d[c * S + pidx(y, sh)] = c < nk ? base[(uint64_t)y * H + c] : mk<V>(0, 0);
     }
     __syncthreads();
-    lds_fft<true>(d, L, S, nk, fc, s_tw);
+    lds_fft<true>(d, L, S, sh, nk, fc, s_tw);
     const int nyo = 2 * Ry + 1;
     for (int t = threadIdx.x; t < nyo * tile; t += blockDim.x) {
         const int iy = t / tile, c = t - iy * tile;
         if (c >= nk) continue;
         const int y = iy <= Ry ? iy : L - nyo + iy;
-        g3[((uint64_t)ix * nyo + iy) * H + kz0 + c] = d[c * S + y];
+        g3[((uint64_t)ix * nyo + iy) * H + kz0 + c] = d[c * S + pidx(y, sh)];
     }
 }
 
@@ -381,13 +398,14 @@ template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac fc, int L,
                                                          const C2<T> *__restrict__ tw, const int *__restrict__ rev,
                                                          int pairs_per_block, int Rz, T scale,
-                                                         const C2<T> *__restrict__ g3, T *__restrict__ cube) {
+                                                         const C2<T> *__restrict__ g3, T *__restrict__ cube, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     V *s_tw = sm;
-    V *d = sm + L;
-    load_tw(s_tw, tw, L);
+    V *d = sm + ltw(L, sh);
+    load_tw(s_tw, tw, L, sh);
+    const int S = lstride(L, sh);
     const int p0 = blockIdx.x * pairs_per_block;
     const int npairs = (nlines + 1) / 2;
     const int np = min(pairs_per_block, npairs - p0);
@@ -403,16 +421,16 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac 
             A.y = -A.y;
             B.y = -B.y;
         }
-        d[pl * L + pos] = mk<V>(A.x - B.y, A.y + B.x);  // A + iB
+        d[pl * S + pidx(pos, sh)] = mk<V>(A.x - B.y, A.y + B.x);  // A + iB
     }
     __syncthreads();
-    lds_fft<true>(d, L, L, np, fc, s_tw);
+    lds_fft<true>(d, L, S, sh, np, fc, s_tw);
     const int nzo = 2 * Rz + 1;
     for (int t = threadIdx.x; t < np * nzo; t += blockDim.x) {
         const int pl = t / nzo, iz = t - pl * nzo;
         const int z = iz <= Rz ? iz : L - nzo + iz;
         const int la = 2 * (p0 + pl), lb = la + 1;
-        const V v = d[pl * L + z];
+        const V v = d[pl * S + pidx(z, sh)];
         cube[(uint64_t)la * nzo + iz] = v.x * scale;
         if (lb < nlines) cube[(uint64_t)lb * nzo + iz] = v.y * scale;
     }
@@ -542,7 +560,9 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], in
 }
 
 template <class T>
-size_t lds_bytes(int L, int lines, int stride) { return ((size_t)L + (size_t)lines * stride) * sizeof(C2<T>); }
+size_t lds_bytes(int L, int lines, int sh) {
+    return ((size_t)ltw(L, sh) + (size_t)lines * lstride(L, sh)) * sizeof(C2<T>);
+}
 
 s2h_status set_lds(const void *fn, size_t bytes) {
     if (bytes > 160 * 1024) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
@@ -568,57 +588,67 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     // z forward
     {
         const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * sizeof(V)))));
-        const size_t lds = lds_bytes<T>(fz, ppb, fz);
+        const size_t lds = lds_bytes<T>(fz, ppb, p->sh_z);
         S2H_TRY(set_lds((const void *)fft_z_fwd<T>, lds));
         const int npairs = nx * ((ny + 1) / 2);
         fft_z_fwd<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
             vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, fcz, fz, (const V *)p->tw[2].ptr,
-            (const int *)p->pos_z.ptr, ppb, (V *)p->g1.ptr);
+            (const int *)p->pos_z.ptr, ppb, (V *)p->g1.ptr, p->sh_z);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[0], s));
     {
-        const size_t lds = lds_bytes<T>(fy, ty, fy + 1);
+        const size_t lds = lds_bytes<T>(fy, ty, p->sh_y);
         S2H_TRY(set_lds((const void *)fft_y_fwd<T>, lds));
         fft_y_fwd<T><<<nx * tiles_y, kFftThreads, lds, s>>>(ny, fy, H, tiles_y, ty, fcy, (const V *)p->tw[1].ptr,
-                                                            (V *)p->g1.ptr);
+                                                            (V *)p->g1.ptr, p->sh_y);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[1], s));
     {
-        const size_t lds = lds_bytes<T>(fx, tx, fx + 1);
+        const size_t lds = lds_bytes<T>(fx, tx, p->sh_x);
         S2H_TRY(set_lds((const void *)fft_x_power<T>, lds));
         fft_x_power<T><<<fy * tiles_x, kFftThreads, lds, s>>>(nx, fy, H, tiles_x, tx, fcx, (const V *)p->tw[0].ptr,
-                                                              Rx, (const V *)p->g1.ptr, (V *)p->g2.ptr);
+                                                              Rx, (const V *)p->g1.ptr, (V *)p->g2.ptr, p->sh_x);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[2], s));
     {
-        const size_t lds = lds_bytes<T>(fy, ty, fy + 1);
+        const size_t lds = lds_bytes<T>(fy, ty, p->sh_y);
         S2H_TRY(set_lds((const void *)fft_y_inv<T>, lds));
         fft_y_inv<T><<<nxo * tiles_y, kFftThreads, lds, s>>>(fy, H, tiles_y, ty, fcy, (const V *)p->tw[1].ptr, Ry,
-                                                             (const V *)p->g2.ptr, (V *)p->g3.ptr);
+                                                             (const V *)p->g2.ptr, (V *)p->g3.ptr, p->sh_y);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[3], s));
     {
         const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * sizeof(V)))));
-        const size_t lds = lds_bytes<T>(fz, ppb, fz);
+        const size_t lds = lds_bytes<T>(fz, ppb, p->sh_z);
         S2H_TRY(set_lds((const void *)fft_z_inv<T>, lds));
         const int nlines = nxo * nyo;
         const int npairs = (nlines + 1) / 2;
         const T scale = (T)(1.0 / ((double)fx * (double)fy * (double)fz));
         fft_z_inv<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
             nlines, H, fcz, fz, (const V *)p->tw[2].ptr, (const int *)p->rev_z.ptr, ppb, Rz, scale,
-            (const V *)p->g3.ptr, (T *)p->cube.ptr);
+            (const V *)p->g3.ptr, (T *)p->cube.ptr, p->sh_z);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     return S2H_OK;
+}
+
+// Workgroups of `lds` bytes that fit one CU's 160 KB of LDS.
+static int blocks_per_cu(size_t lds) { return (int)((160 * 1024) / std::max<size_t>(lds, 1)); }
+
+// The padding shift for a pass of `lines` lines of length L: padded unless
+// the padding leaves room for fewer workgroups per CU.
+template <class T>
+int pad_rule(int L, int lines) {
+    return blocks_per_cu(lds_bytes<T>(L, lines, kPad)) >= blocks_per_cu(lds_bytes<T>(L, lines, kNoPad)) ? kPad : kNoPad;
 }
 
 // kz columns per workgroup of the y and x passes: powers of two up to kTileMax
@@ -628,17 +658,24 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
 // events around each pass of one run per candidate, after a warm-up).  The y
 // passes favour wide tiles (longer contiguous kz runs per row), the x pass
 // narrow ones (more workgroups per CU).  Every run rewrites all of the plan's
-// buffers, so the trial runs leave nothing behind.
+// buffers, so the trial runs leave nothing behind.  The LDS padding of the y
+// and x passes is timed with the tiles (each tile padded and unpadded).
 template <class T>
 s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     const int lmax = (int)std::max(p->f[0], p->f[1]);
     std::vector<int> cands;
     for (int t = kTileMax; t >= 1; t /= 2)
-        if (lds_bytes<T>(lmax, t, lmax + 1) <= 150 * 1024) cands.push_back(t);
+        if (lds_bytes<T>(lmax, t, kNoPad) <= 150 * 1024) cands.push_back(t);
     if (cands.empty()) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
-    // untimed: the widest tile for y, a narrow one for x (the usual picks)
+    // untimed: the widest tile for y, a narrow one for x (the usual picks),
+    // padded where the padding costs no occupancy
     p->tile_y = cands.front();
     p->tile_x = cands[std::min<size_t>(2, cands.size() - 1)];
+    const int fz = (int)p->f[2];
+    const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * sizeof(C2<T>)))));
+    p->sh_z = pad_rule<T>(fz, ppb);
+    p->sh_y = lds_bytes<T>((int)p->f[1], p->tile_y, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[1], p->tile_y) : kNoPad;
+    p->sh_x = lds_bytes<T>((int)p->f[0], p->tile_x, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[0], p->tile_x) : kNoPad;
     const char *env = std::getenv("S2HIP_AUTOTUNE");
     const double cells = (double)p->f[0] * (double)p->f[1] * (double)p->f[2];
     // tune only where a run is short (up to a 1024^3 padded grid)
@@ -649,33 +686,42 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     for (hipEvent_t &e : ev)
         if (st == S2H_OK && hipEventCreate(&e) != hipSuccess) st = fail(S2H_ERR_HIP, "hipEventCreate failed");
     float best_y = 1e30f, best_x = 1e30f;
-    int pick_y = cands.front(), pick_x = cands.front();
+    int pick_y = p->tile_y, pick_x = p->tile_x, psh_y = p->sh_y, psh_x = p->sh_x;
     for (int t : cands) {
-        if (st != S2H_OK) break;
-        p->tile_y = p->tile_x = t;
-        st = fft_run<T>(ctx, vol, p);  // warm-up
-        if (st == S2H_OK) st = fft_run<T>(ctx, vol, p, ev);
-        hipError_t e = hipSuccess;
-        if (st == S2H_OK) e = hipEventSynchronize(ev[3]);
-        float yf = 0, xm = 0, yi = 0;
-        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yf, ev[0], ev[1]);
-        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&xm, ev[1], ev[2]);
-        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yi, ev[2], ev[3]);
-        if (st == S2H_OK && e != hipSuccess) st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
-        if (st != S2H_OK) break;
-        if (yf + yi < best_y) {
-            best_y = yf + yi;
-            pick_y = t;
-        }
-        if (xm < best_x) {
-            best_x = xm;
-            pick_x = t;
+        for (int sh : {kPad, kNoPad}) {
+            if (st != S2H_OK) break;
+            if (lds_bytes<T>(lmax, t, sh) > 150 * 1024) continue;
+            p->tile_y = p->tile_x = t;
+            p->sh_y = p->sh_x = sh;
+            st = fft_run<T>(ctx, vol, p);  // warm-up
+            if (st == S2H_OK) st = fft_run<T>(ctx, vol, p, ev);
+            hipError_t e = hipSuccess;
+            if (st == S2H_OK) e = hipEventSynchronize(ev[3]);
+            float yf = 0, xm = 0, yi = 0;
+            if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yf, ev[0], ev[1]);
+            if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&xm, ev[1], ev[2]);
+            if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yi, ev[2], ev[3]);
+            if (st == S2H_OK && e != hipSuccess)
+                st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
+            if (st != S2H_OK) break;
+            if (yf + yi < best_y) {
+                best_y = yf + yi;
+                pick_y = t;
+                psh_y = sh;
+            }
+            if (xm < best_x) {
+                best_x = xm;
+                pick_x = t;
+                psh_x = sh;
+            }
         }
     }
     for (hipEvent_t &e : ev)
         if (e) (void)hipEventDestroy(e);
     p->tile_y = pick_y;
     p->tile_x = pick_x;
+    p->sh_y = psh_y;
+    p->sh_x = psh_x;
     return st;
 }
 
@@ -763,7 +809,7 @@ bool fft64_supported(const uint64_t dims[3]) {
     }
     const int lmax = (int)std::max(smooth_length(2 * dims[0] - 1), smooth_length(2 * dims[1] - 1));
     const int lz = (int)smooth_length(2 * dims[2] - 1);
-    return lds_bytes<double>(lmax, 1, lmax + 1) <= 150 * 1024 && lds_bytes<double>(lz, 1, lz) <= 150 * 1024;
+    return lds_bytes<double>(lmax, 1, kNoPad) <= 150 * 1024 && lds_bytes<double>(lz, 1, kNoPad) <= 150 * 1024;
 }
 
 }  // namespace s2h
