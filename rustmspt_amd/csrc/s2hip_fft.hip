@@ -79,7 +79,7 @@ struct Fac {
 // 16-byte bank groups.  The padding costs 1/8 more LDS, which for some tiles
 // leaves room for one workgroup fewer per CU, so each pass kind has its own
 // choice.  Lines are lstride(L, sh) apart (odd, so the load phases, which step
-// across lines, spread too); the twiddle table is padded the same way.
+// across lines, spread too).
 static constexpr int kPad = 3;
 static constexpr int kNoPad = 30;
 __host__ __device__ constexpr int pidx(int j, int sh) { return j + (j >> sh); }
@@ -200,13 +200,19 @@ __device__ __forceinline__ void fft_stage(V *d, int L, int S, int sh, int lines,
         V a[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = base[pidx(j0 + r * mp, sh)];
+        // twiddles w^(k q step) from the global table (L1-resident); the
+        // stage on sub-length R (mp == 1, k == 0) has only w^0 = 1
         if (!INV) {
             small_dft<-1>(a, R);
+            if (mp > 1) {
 #pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[pidx(k * q * step, sh)]);
+                for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[k * q * step]);
+            }
         } else {
+            if (mp > 1) {
 #pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[pidx(k * q * step, sh)]);
+                for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[k * q * step]);
+            }
             small_dft<1>(a, R);
         }
 #pragma unroll
@@ -247,10 +253,6 @@ __device__ void lds_fft(V *d, int L, int S, int sh, int lines, const Fac &fc, co
     }
 }
 
-template <class V>
-__device__ __forceinline__ void load_tw(V *s_tw, const V *tw, int L, int sh) {
-    for (int i = threadIdx.x; i < L; i += blockDim.x) s_tw[pidx(i, sh)] = tw[i];
-}
 
 // ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
 template <class T>
@@ -261,9 +263,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
-    V *s_tw = sm;
-    V *d = sm + ltw(L, sh);
-    load_tw(s_tw, tw, L, sh);
+    V *d = sm;
     const int S = lstride(L, sh);
     const int npairs = nx * ((ny + 1) / 2);
     const int p0 = blockIdx.x * pairs_per_block;
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
         d[pl * S + pidx(z, sh)] = mk<V>(re, im);
     }
     __syncthreads();
-    lds_fft<false>(d, L, S, sh, np, fc, s_tw);
+    lds_fft<false>(d, L, S, sh, np, fc, tw);
     for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
         const int pl = t / H, k = t - pl * H;
         const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
@@ -304,9 +304,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = fy, S = lstride(L, sh);
-    V *s_tw = sm;
-    V *d = sm + ltw(L, sh);
-    load_tw(s_tw, tw, L, sh);
+    V *d = sm;
     const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     V *base = g1 + (uint64_t)x * fy * H + kz0;
@@ -317,7 +315,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
         d[c * S + pidx(y, sh)] = v;
     }
     __syncthreads();
-    lds_fft<false>(d, L, S, sh, nk, fc, s_tw);
+    lds_fft<false>(d, L, S, sh, nk, fc, tw);
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         const int y = t / tile, c = t - y * tile;
         if (c < nk) base[(uint64_t)y * H + c] = d[c * S + pidx(y, sh)];
@@ -334,9 +332,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = (int)fc.f[kMaxFactors - 1];  // unused slot carries L
     const int S = lstride(L, sh);
-    V *s_tw = sm;
-    V *d = sm + ltw(L, sh);
-    load_tw(s_tw, tw, L, sh);
+    V *d = sm;
     const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
@@ -346,14 +342,14 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
         d[c * S + pidx(x, sh)] = v;
     }
     __syncthreads();
-    lds_fft<false>(d, L, S, sh, nk, fc, s_tw);
+    lds_fft<false>(d, L, S, sh, nk, fc, tw);
     for (int t = threadIdx.x; t < L * nk; t += blockDim.x) {
         const int c = t / L, x = t - c * L;
         const V v = d[c * S + pidx(x, sh)];
         d[c * S + pidx(x, sh)] = mk<V>(v.x * v.x + v.y * v.y, 0);
     }
     __syncthreads();
-    lds_fft<true>(d, L, S, sh, nk, fc, s_tw);
+    lds_fft<true>(d, L, S, sh, nk, fc, tw);
     const int nxo = 2 * Rx + 1;
     for (int t = threadIdx.x; t < nxo * tile; t += blockDim.x) {
         const int ix = t / tile, c = t - ix * tile;
@@ -372,9 +368,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tile
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = fy, S = lstride(L, sh);
-    V *s_tw = sm;
-    V *d = sm + ltw(L, sh);
-    load_tw(s_tw, tw, L, sh);
+    V *d = sm;
     const int ix = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     const V *base = g2 + (uint64_t)ix * fy * H + kz0;
@@ -383,7 +377,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tile
         d[c * S + pidx(y, sh)] = c < nk ? base[(uint64_t)y * H + c] : mk<V>(0, 0);
     }
     __syncthreads();
-    lds_fft<true>(d, L, S, sh, nk, fc, s_tw);
+    lds_fft<true>(d, L, S, sh, nk, fc, tw);
     const int nyo = 2 * Ry + 1;
     for (int t = threadIdx.x; t < nyo * tile; t += blockDim.x) {
         const int iy = t / tile, c = t - iy * tile;
@@ -402,9 +396,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac 
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
-    V *s_tw = sm;
-    V *d = sm + ltw(L, sh);
-    load_tw(s_tw, tw, L, sh);
+    V *d = sm;
     const int S = lstride(L, sh);
     const int p0 = blockIdx.x * pairs_per_block;
     const int npairs = (nlines + 1) / 2;
@@ -424,7 +416,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac 
         d[pl * S + pidx(pos, sh)] = mk<V>(A.x - B.y, A.y + B.x);  // A + iB
     }
     __syncthreads();
-    lds_fft<true>(d, L, S, sh, np, fc, s_tw);
+    lds_fft<true>(d, L, S, sh, np, fc, tw);
     const int nzo = 2 * Rz + 1;
     for (int t = threadIdx.x; t < np * nzo; t += blockDim.x) {
         const int pl = t / nzo, iz = t - pl * nzo;
@@ -561,7 +553,7 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], in
 
 template <class T>
 size_t lds_bytes(int L, int lines, int sh) {
-    return ((size_t)ltw(L, sh) + (size_t)lines * lstride(L, sh)) * sizeof(C2<T>);
+    return (size_t)lines * lstride(L, sh) * sizeof(C2<T>);
 }
 
 s2h_status set_lds(const void *fn, size_t bytes) {
