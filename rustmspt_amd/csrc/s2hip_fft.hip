@@ -86,6 +86,20 @@ __host__ __device__ constexpr int pidx(int j, int sh) { return j + (j >> sh); }
 __host__ __device__ constexpr int ltw(int L, int sh) { return pidx(L - 1, sh) + 1; }
 __host__ __device__ constexpr int lstride(int L, int sh) { return ltw(L, sh) | 1; }
 
+// q = t / n and r = t - q n; by shift and mask when n is a power of two
+// (lg = log2 n, else -1; wave-uniform), which is the usual case for the
+// stage and tile arithmetic, saving an integer division per use.
+__device__ __forceinline__ int log2_if_pow2(int n) { return (n & (n - 1)) == 0 ? __builtin_ctz(n) : -1; }
+__device__ __forceinline__ void divmod(int t, int n, int lg, int &q, int &r) {
+    if (lg >= 0) {
+        q = t >> lg;
+        r = t & (n - 1);
+    } else {
+        q = t / n;
+        r = t - q * n;
+    }
+}
+
 // Complex values of the working precision T (float: the FP32 path; double:
 // the rounded exact path).
 template <class T> struct Cx;
@@ -192,9 +206,11 @@ __device__ __forceinline__ void small_dft(V *a, int R) {
 template <int R, bool INV, class V>
 __device__ __forceinline__ void fft_stage(V *d, int L, int S, int sh, int lines, int m, const V *tw) {
     const int mp = m / R, nb = L / R, step = L / m;
+    const int lnb = log2_if_pow2(nb), lmp = log2_if_pow2(mp);
     for (int t = threadIdx.x; t < nb * lines; t += blockDim.x) {
-        const int line = t / nb, b = t - line * nb;
-        const int grp = b / mp, k = b - grp * mp;
+        int line, b, grp, k;
+        divmod(t, nb, lnb, line, b);
+        divmod(b, mp, lmp, grp, k);
         V *base = d + line * S;
         const int j0 = grp * m + k;
         V a[R];
@@ -264,13 +280,14 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     V *d = sm;
-    const int S = lstride(L, sh);
+    const int S = lstride(L, sh), lgl = log2_if_pow2(L);
     const int npairs = nx * ((ny + 1) / 2);
     const int p0 = blockIdx.x * pairs_per_block;
     const int np = min(pairs_per_block, npairs - p0);
     const int ypairs = (ny + 1) / 2;
     for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
-        const int pl = t / L, z = t - pl * L;
+        int pl, z;
+        divmod(t, L, lgl, pl, z);
         const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
         T re = 0, im = 0;
         if (z < nz) {
@@ -303,13 +320,14 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
-    const int L = fy, S = lstride(L, sh);
+    const int L = fy, S = lstride(L, sh), lgt = log2_if_pow2(tile);
     V *d = sm;
     const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     V *base = g1 + (uint64_t)x * fy * H + kz0;
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
-        const int y = t / tile, c = t - y * tile;
+        int y, c;
+        divmod(t, tile, lgt, y, c);
         V v = mk<V>(0, 0);
         if (y < ny && c < nk) v = base[(uint64_t)y * H + c];
         d[c * S + pidx(y, sh)] = v;
@@ -317,7 +335,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     __syncthreads();
     lds_fft<false>(d, L, S, sh, nk, fc, tw);
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
-        const int y = t / tile, c = t - y * tile;
+        int y, c;
+        divmod(t, tile, lgt, y, c);
         if (c < nk) base[(uint64_t)y * H + c] = d[c * S + pidx(y, sh)];
     }
 }
@@ -331,12 +350,13 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = (int)fc.f[kMaxFactors - 1];  // unused slot carries L
-    const int S = lstride(L, sh);
+    const int S = lstride(L, sh), lgt = log2_if_pow2(tile), lgl = log2_if_pow2(L);
     V *d = sm;
     const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
-        const int x = t / tile, c = t - x * tile;
+        int x, c;
+        divmod(t, tile, lgt, x, c);
         V v = mk<V>(0, 0);
         if (x < nx && c < nk) v = g1[((uint64_t)x * fy + ky) * H + kz0 + c];
         d[c * S + pidx(x, sh)] = v;
@@ -344,7 +364,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     __syncthreads();
     lds_fft<false>(d, L, S, sh, nk, fc, tw);
     for (int t = threadIdx.x; t < L * nk; t += blockDim.x) {
-        const int c = t / L, x = t - c * L;
+        int c, x;
+        divmod(t, L, lgl, c, x);
         const V v = d[c * S + pidx(x, sh)];
         d[c * S + pidx(x, sh)] = mk<V>(v.x * v.x + v.y * v.y, 0);
     }
@@ -352,7 +373,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     lds_fft<true>(d, L, S, sh, nk, fc, tw);
     const int nxo = 2 * Rx + 1;
     for (int t = threadIdx.x; t < nxo * tile; t += blockDim.x) {
-        const int ix = t / tile, c = t - ix * tile;
+        int ix, c;
+        divmod(t, tile, lgt, ix, c);
         if (c >= nk) continue;
         const int x = ix <= Rx ? ix : L - nxo + ix;
         g2[((uint64_t)ix * fy + ky) * H + kz0 + c] = d[c * S + pidx(x, sh)];
@@ -367,20 +389,22 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tile
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
-    const int L = fy, S = lstride(L, sh);
+    const int L = fy, S = lstride(L, sh), lgt = log2_if_pow2(tile);
     V *d = sm;
     const int ix = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     const V *base = g2 + (uint64_t)ix * fy * H + kz0;
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
-        const int y = t / tile, c = t - y * tile;
+        int y, c;
+        divmod(t, tile, lgt, y, c);
         d[c * S + pidx(y, sh)] = c < nk ? base[(uint64_t)y * H + c] : mk<V>(0, 0);
     }
     __syncthreads();
     lds_fft<true>(d, L, S, sh, nk, fc, tw);
     const int nyo = 2 * Ry + 1;
     for (int t = threadIdx.x; t < nyo * tile; t += blockDim.x) {
-        const int iy = t / tile, c = t - iy * tile;
+        int iy, c;
+        divmod(t, tile, lgt, iy, c);
         if (c >= nk) continue;
         const int y = iy <= Ry ? iy : L - nyo + iy;
         g3[((uint64_t)ix * nyo + iy) * H + kz0 + c] = d[c * S + pidx(y, sh)];
@@ -397,12 +421,13 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac 
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     V *d = sm;
-    const int S = lstride(L, sh);
+    const int S = lstride(L, sh), lgl = log2_if_pow2(L);
     const int p0 = blockIdx.x * pairs_per_block;
     const int npairs = (nlines + 1) / 2;
     const int np = min(pairs_per_block, npairs - p0);
     for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
-        const int pl = t / L, pos = t - pl * L;
+        int pl, pos;
+        divmod(t, L, lgl, pl, pos);
         const int la = 2 * (p0 + pl), lb = la + 1;
         const int k = rev[pos];
         const bool upper = k >= H;
