@@ -269,6 +269,59 @@ __device__ void lds_fft(V *d, int L, int S, int sh, int lines, const Fac &fc, co
     }
 }
 
+// |F|^2 correlation of `lines` lines in LDS: forward transform, |F|^2,
+// inverse transform.  The last forward stage and the first inverse stage both
+// work on sub-length R = f_{n-1} (mp = 1, no twiddles) over the same R
+// elements, so one thread runs forward DFT, |F|^2 and inverse DFT on them in
+// registers: one LDS round trip and two barriers fewer than running
+// lds_fft<false>, a |F|^2 pass and lds_fft<true>.
+template <int R, class V>
+__device__ __forceinline__ void power_stage(V *d, int L, int S, int sh, int lines) {
+    const int nb = L / R, lnb = log2_if_pow2(nb);
+    for (int t = threadIdx.x; t < nb * lines; t += blockDim.x) {
+        int line, b;
+        divmod(t, nb, lnb, line, b);
+        V *base = d + line * S;
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = base[pidx(b * R + r, sh)];
+        small_dft<-1>(a, R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = mk<V>(a[r].x * a[r].x + a[r].y * a[r].y, 0);
+        small_dft<1>(a, R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) base[pidx(b * R + r, sh)] = a[r];
+    }
+    __syncthreads();
+}
+
+template <class V>
+__device__ void lds_power(V *d, int L, int S, int sh, int lines, const Fac &fc, const V *tw) {
+    if (fc.n == 0) {  // L == 1
+        for (int t = threadIdx.x; t < lines; t += blockDim.x) {
+            const V v = d[t * S];
+            d[t * S] = mk<V>(v.x * v.x + v.y * v.y, 0);
+        }
+        __syncthreads();
+        return;
+    }
+    int m = L;
+    for (int s = 0; s + 1 < fc.n; ++s) {
+        fft_stage_any<false>(d, L, S, sh, lines, fc.f[s], m, tw);
+        m /= fc.f[s];
+    }
+    switch (fc.f[fc.n - 1]) {  // m == f_{n-1} here
+        case 8: power_stage<8>(d, L, S, sh, lines); break;
+        case 4: power_stage<4>(d, L, S, sh, lines); break;
+        case 2: power_stage<2>(d, L, S, sh, lines); break;
+        case 3: power_stage<3>(d, L, S, sh, lines); break;
+        default: power_stage<5>(d, L, S, sh, lines); break;
+    }
+    for (int s = fc.n - 2; s >= 0; --s) {
+        m *= fc.f[s];
+        fft_stage_any<true>(d, L, S, sh, lines, fc.f[s], m, tw);
+    }
+}
 
 // ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
 template <class T>
@@ -350,7 +403,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = (int)fc.f[kMaxFactors - 1];  // unused slot carries L
-    const int S = lstride(L, sh), lgt = log2_if_pow2(tile), lgl = log2_if_pow2(L);
+    const int S = lstride(L, sh), lgt = log2_if_pow2(tile);
     V *d = sm;
     const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
@@ -362,15 +415,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
         d[c * S + pidx(x, sh)] = v;
     }
     __syncthreads();
-    lds_fft<false>(d, L, S, sh, nk, fc, tw);
-    for (int t = threadIdx.x; t < L * nk; t += blockDim.x) {
-        int c, x;
-        divmod(t, L, lgl, c, x);
-        const V v = d[c * S + pidx(x, sh)];
-        d[c * S + pidx(x, sh)] = mk<V>(v.x * v.x + v.y * v.y, 0);
-    }
-    __syncthreads();
-    lds_fft<true>(d, L, S, sh, nk, fc, tw);
+    lds_power(d, L, S, sh, nk, fc, tw);
     const int nxo = 2 * Rx + 1;
     for (int t = threadIdx.x; t < nxo * tile; t += blockDim.x) {
         int ix, c;
