@@ -36,17 +36,18 @@ COUNT_PHASE_TRAFFIC = 14840 * 1024 * 2.0 + 862900 * 1024 * 1.0
 # summed over its six dispatches (z fwd, y fwd, x, y inv, z inv, gather), raw
 # KiB -- for these LDS-tiled passes the raw FETCH_SIZE already counts the bytes
 # read (the y forward pass fetches at least the 263,168 KiB it must read).
-FFT64_PHASE_TRAFFIC = (1518873 + 1362399) * 1024.0
+FFT64_PHASE_TRAFFIC = (1518992 + 1362816) * 1024.0
 
 
 def fft64_bytes(dims, offs):
-    """(algorithmic, compulsory) HBM bytes of one FP64 FFT evaluation.
-    Algorithmic: 12 x f_x f_y (f_z/2+1) x 16 B -- three axes, forward and
-    inverse, read and write of the complex128 half spectrum, as for an unpruned
-    transform.  Compulsory: what the pruned passes must move -- each pass reads
-    its input and writes its output once (z forward: packed rows in, ny rows of
-    spectra out; y forward; x forward + |F|^2 + inverse, keeping 2R_x+1 planes;
-    y inverse keeping 2R_y+1; z inverse keeping 2R_z+1 f64 values)."""
+    """(unpruned, compulsory) HBM bytes of one FP64 FFT evaluation.
+    Unpruned: 12 x f_x f_y (f_z/2+1) x 16 B -- three axes, forward and
+    inverse, read and write of the complex128 half spectrum of a transform that
+    keeps every output.  Compulsory (the roofline's algorithmic bytes): what
+    the pruned passes must move -- each pass reads its input and writes its
+    output once (z forward: packed rows in, ny rows of spectra out; y forward;
+    x forward + |F|^2 + inverse, keeping 2R_x+1 planes; y inverse keeping
+    2R_y+1; z inverse keeping 2R_z+1 f64 values)."""
     import numpy as np
 
     from rustmspt_amd.config import smooth_fft_length
@@ -390,11 +391,11 @@ def ct_leg(ctx, vol, r_max, steps):
                       "launch_shape": ctx.exact_tune_info(dims, r_max, 1.0)},
            "s2_head": [round(float(v), 6) for v in auto["curve"][:5]]}
     if kernel == "fft64":
-        alg, comp = fft64_bytes(dims, offs)
+        unpruned, comp = fft64_bytes(dims, offs)
         sec = auto["count_ms"] * 1e-3
-        out["roofline"] = {"bound": "hbm", "achieved": alg / sec / 1e9, "peak": 8000.0, "unit": "GB/s",
-                           "frac": alg / sec / 8e12, "algorithmic_bytes": alg, "compulsory_bytes": comp,
-                           "frac_of_compulsory": comp / sec / 8e12}
+        out["roofline"] = {"bound": "hbm", "achieved": comp / sec / 1e9, "peak": 8000.0, "unit": "GB/s",
+                           "frac": comp / sec / 8e12, "algorithmic_bytes": comp, "unpruned_bytes": unpruned,
+                           "frac_unpruned": unpruned / sec / 8e12}
     hits = vol.exact_counts(r_max, 1.0)
     out["fft32"] = fft_leg(ctx, vol, r_max, hits, offs, rs, auto["curve"], steps=2)
     return out
@@ -557,17 +558,18 @@ def main():
                        "launches_per_eval": direct["launches"], "device_ms_per_eval": direct["device_ms"],
                        "algorithmic_ops_per_eval": alg_ops, "launch_shape": shape}
     if kernel == "fft64":
-        alg_b, comp_b = fft64_bytes([grid] * 3, offs)
+        unpruned_b, comp_b = fft64_bytes([grid] * 3, offs)
         sec = auto["count_ms"] * 1e-3
-        roofline = {"bound": "hbm", "achieved": alg_b / sec / 1e9, "peak": 8000.0, "unit": "GB/s",
-                    "frac": alg_b / sec / 8e12,
+        roofline = {"bound": "hbm", "achieved": comp_b / sec / 1e9, "peak": 8000.0, "unit": "GB/s",
+                    "frac": comp_b / sec / 8e12,
                     "traffic": FFT64_PHASE_TRAFFIC if (grid, r_max, args.particles) == (256, 64, 500) else None,
                     "kernel": "FP64 FFT passes (z fwd, y fwd, x fwd+|F|^2+inv, y inv, z inv, gather)",
                     "kernel_ms": auto["count_ms"], "launches_per_eval": auto["launches"],
-                    "device_ms_per_eval": auto["device_ms"], "algorithmic_bytes_per_eval": alg_b,
-                    "compulsory_bytes_per_eval": comp_b, "frac_of_compulsory": comp_b / sec / 8e12,
-                    "note": "algorithmic bytes count an unpruned transform (12 passes over the complex128 "
-                            "half spectrum); the pruned passes must move only the compulsory bytes"}
+                    "device_ms_per_eval": auto["device_ms"], "algorithmic_bytes_per_eval": comp_b,
+                    "unpruned_bytes_per_eval": unpruned_b, "frac_unpruned": unpruned_b / sec / 8e12,
+                    "note": "algorithmic bytes: what the pruned passes must move (each reads its input and "
+                            "writes its output once); unpruned: 12 passes over the complex128 half spectrum "
+                            "of a transform keeping every output, which the pruning beats"}
     else:
         roofline = direct_roofline
 
