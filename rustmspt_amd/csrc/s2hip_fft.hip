@@ -394,6 +394,88 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     }
 }
 
+// The x pass's outer stages straight between global memory and registers.
+// Stage 0 of the forward transform (sub-length L, R = f_0, mp = L / R): thread
+// (b, c) loads x = b + mp r of kz column c (x >= nx are zero; kz columns vary
+// fastest across lanes, so neighbouring lanes read one contiguous row
+// segment), runs the butterfly and its twiddles, and stores to LDS.  The last
+// inverse stage reads its butterfly from LDS and writes only the kept x
+// (|dx| <= Rx) to G2.  Each saves the LDS round trip and barrier of a plain
+// load or store loop.
+template <int R, class V>
+__device__ __forceinline__ void x_first_stage(V *d, int L, int S, int sh, int tile, int lgt, int nk, int nx, int fy,
+                                              int H, int ky, int kz0, const V *__restrict__ g1,
+                                              const V *__restrict__ tw) {
+    const int mp = L / R;
+    for (int t = threadIdx.x; t < mp * tile; t += blockDim.x) {
+        int b, c;
+        divmod(t, tile, lgt, b, c);
+        if (c >= nk) continue;
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int x = b + mp * r;
+            a[r] = x < nx ? g1[((uint64_t)x * fy + ky) * H + kz0 + c] : mk<V>(0, 0);
+        }
+        small_dft<-1>(a, R);
+        if (mp > 1) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[b * q]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[c * S + pidx(b + mp * r, sh)] = a[r];
+    }
+    __syncthreads();
+}
+
+template <int R, class V>
+__device__ __forceinline__ void x_last_stage(const V *d, int L, int S, int sh, int tile, int lgt, int nk, int fy,
+                                             int H, int ky, int kz0, int Rx, V *__restrict__ g2,
+                                             const V *__restrict__ tw) {
+    const int mp = L / R, nxo = 2 * Rx + 1;
+    for (int t = threadIdx.x; t < mp * tile; t += blockDim.x) {
+        int b, c;
+        divmod(t, tile, lgt, b, c);
+        if (c >= nk) continue;
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = d[c * S + pidx(b + mp * r, sh)];
+        if (mp > 1) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[b * q]);
+        }
+        small_dft<1>(a, R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int x = b + mp * r;
+            const int ix = x <= Rx ? x : (x >= L - Rx ? x - L + nxo : -1);
+            if (ix >= 0) g2[((uint64_t)ix * fy + ky) * H + kz0 + c] = a[r];
+        }
+    }
+}
+
+// Stages 1 .. n-1 of the forward transform, |F|^2 and stages n-1 .. 1 of the
+// inverse (lds_power without the outer stages; n >= 2).
+template <class V>
+__device__ void lds_power_inner(V *d, int L, int S, int sh, int lines, const Fac &fc, const V *tw) {
+    int m = L / fc.f[0];
+    for (int s = 1; s + 1 < fc.n; ++s) {
+        fft_stage_any<false>(d, L, S, sh, lines, fc.f[s], m, tw);
+        m /= fc.f[s];
+    }
+    switch (fc.f[fc.n - 1]) {  // m == f_{n-1} here
+        case 8: power_stage<8>(d, L, S, sh, lines); break;
+        case 4: power_stage<4>(d, L, S, sh, lines); break;
+        case 2: power_stage<2>(d, L, S, sh, lines); break;
+        case 3: power_stage<3>(d, L, S, sh, lines); break;
+        default: power_stage<5>(d, L, S, sh, lines); break;
+    }
+    for (int s = fc.n - 2; s >= 1; --s) {
+        m *= fc.f[s];
+        fft_stage_any<true>(d, L, S, sh, lines, fc.f[s], m, tw);
+    }
+}
+
 // ---- x: forward, |F|^2, inverse; keep the needed x.  One block per (ky, kz tile).
 template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H, int tiles_kz, int tile, Fac fc,
@@ -407,6 +489,24 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     V *d = sm;
     const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
+    if (fc.n >= 2) {
+        switch (fc.f[0]) {
+            case 8: x_first_stage<8>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
+            case 4: x_first_stage<4>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
+            case 2: x_first_stage<2>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
+            case 3: x_first_stage<3>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
+            default: x_first_stage<5>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
+        }
+        lds_power_inner(d, L, S, sh, nk, fc, tw);
+        switch (fc.f[0]) {
+            case 8: x_last_stage<8>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
+            case 4: x_last_stage<4>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
+            case 2: x_last_stage<2>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
+            case 3: x_last_stage<3>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
+            default: x_last_stage<5>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
+        }
+        return;
+    }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         int x, c;
         divmod(t, tile, lgt, x, c);
