@@ -323,6 +323,115 @@ __device__ void lds_power(V *d, int L, int S, int sh, int lines, const Fac &fc, 
     }
 }
 
+// Outer stages straight between global memory and registers.  Line c's
+// element j lives at g[j * rs + c] (rs: the row stride; kz columns c vary
+// fastest across lanes, so neighbouring lanes touch one contiguous row
+// segment).  Each saves the LDS round trip and barrier of a plain load or
+// store loop.
+//   load_first_fwd: stage 0 of the forward (DIF) transform (sub-length L,
+//     mp = L / R): thread (b, c) loads j = b + mp r (j >= nvalid are zero),
+//     runs the butterfly and its twiddles, stores to LDS.
+//   store_last_inv: stage 0 of the inverse (DIT) transform: butterfly from
+//     LDS, conjugate twiddles, inverse DFT; writes only the kept j
+//     (j <= Rk -> row j, j >= L - Rk -> row j - L + 2Rk + 1).
+//   store_last_fwd: the forward transform's last stage (sub-length R, mp = 1,
+//     no twiddles) from LDS, every output j = R b + r written to g.
+//   load_first_inv: the inverse's first stage (sub-length R, mp = 1) from g.
+template <int R, class V>
+__device__ __forceinline__ void load_first_fwd(V *d, int L, int S, int sh, int tile, int lgt, int nk,
+                                               const V *__restrict__ g, uint64_t rs, int nvalid,
+                                               const V *__restrict__ tw) {
+    const int mp = L / R;
+    for (int t = threadIdx.x; t < mp * tile; t += blockDim.x) {
+        int b, c;
+        divmod(t, tile, lgt, b, c);
+        if (c >= nk) continue;
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = b + mp * r;
+            a[r] = j < nvalid ? g[(uint64_t)j * rs + c] : mk<V>(0, 0);
+        }
+        small_dft<-1>(a, R);
+        if (mp > 1) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[b * q]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[c * S + pidx(b + mp * r, sh)] = a[r];
+    }
+    __syncthreads();
+}
+
+template <int R, class V>
+__device__ __forceinline__ void store_last_inv(const V *d, int L, int S, int sh, int tile, int lgt, int nk,
+                                               V *__restrict__ g, uint64_t rs, int Rk, const V *__restrict__ tw) {
+    const int mp = L / R, nko = 2 * Rk + 1;
+    for (int t = threadIdx.x; t < mp * tile; t += blockDim.x) {
+        int b, c;
+        divmod(t, tile, lgt, b, c);
+        if (c >= nk) continue;
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = d[c * S + pidx(b + mp * r, sh)];
+        if (mp > 1) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[b * q]);
+        }
+        small_dft<1>(a, R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = b + mp * r;
+            const int o = j <= Rk ? j : (j >= L - Rk ? j - L + nko : -1);
+            if (o >= 0) g[(uint64_t)o * rs + c] = a[r];
+        }
+    }
+}
+
+template <int R, class V>
+__device__ __forceinline__ void store_last_fwd(const V *d, int L, int S, int sh, int tile, int lgt, int nk,
+                                               V *__restrict__ g, uint64_t rs) {
+    const int nb = L / R;
+    for (int t = threadIdx.x; t < nb * tile; t += blockDim.x) {
+        int b, c;
+        divmod(t, tile, lgt, b, c);
+        if (c >= nk) continue;
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = d[c * S + pidx(b * R + r, sh)];
+        small_dft<-1>(a, R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) g[(uint64_t)(b * R + r) * rs + c] = a[r];
+    }
+}
+
+template <int R, class V>
+__device__ __forceinline__ void load_first_inv(V *d, int L, int S, int sh, int tile, int lgt, int nk,
+                                               const V *__restrict__ g, uint64_t rs) {
+    const int nb = L / R;
+    for (int t = threadIdx.x; t < nb * tile; t += blockDim.x) {
+        int b, c;
+        divmod(t, tile, lgt, b, c);
+        if (c >= nk) continue;
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = g[(uint64_t)(b * R + r) * rs + c];
+        small_dft<1>(a, R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[c * S + pidx(b * R + r, sh)] = a[r];
+    }
+    __syncthreads();
+}
+
+#define S2H_RADIX_SWITCH(R, CALL)                    \
+    switch (R) {                                     \
+        case 8: { constexpr int RR = 8; CALL; } break; \
+        case 4: { constexpr int RR = 4; CALL; } break; \
+        case 2: { constexpr int RR = 2; CALL; } break; \
+        case 3: { constexpr int RR = 3; CALL; } break; \
+        default: { constexpr int RR = 5; CALL; } break; \
+    }
+
 // ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
 template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restrict__ words, int nx, int ny, int nz,
@@ -378,6 +487,16 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     V *base = g1 + (uint64_t)x * fy * H + kz0;
+    if (fc.n >= 2) {
+        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)H, ny, tw)));
+        int m = L / fc.f[0];
+        for (int s = 1; s + 1 < fc.n; ++s) {
+            fft_stage_any<false>(d, L, S, sh, nk, fc.f[s], m, tw);
+            m /= fc.f[s];
+        }
+        S2H_RADIX_SWITCH(fc.f[fc.n - 1], (store_last_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)H)));
+        return;
+    }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         int y, c;
         divmod(t, tile, lgt, y, c);
@@ -391,66 +510,6 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
         int y, c;
         divmod(t, tile, lgt, y, c);
         if (c < nk) base[(uint64_t)y * H + c] = d[c * S + pidx(y, sh)];
-    }
-}
-
-// The x pass's outer stages straight between global memory and registers.
-// Stage 0 of the forward transform (sub-length L, R = f_0, mp = L / R): thread
-// (b, c) loads x = b + mp r of kz column c (x >= nx are zero; kz columns vary
-// fastest across lanes, so neighbouring lanes read one contiguous row
-// segment), runs the butterfly and its twiddles, and stores to LDS.  The last
-// inverse stage reads its butterfly from LDS and writes only the kept x
-// (|dx| <= Rx) to G2.  Each saves the LDS round trip and barrier of a plain
-// load or store loop.
-template <int R, class V>
-__device__ __forceinline__ void x_first_stage(V *d, int L, int S, int sh, int tile, int lgt, int nk, int nx, int fy,
-                                              int H, int ky, int kz0, const V *__restrict__ g1,
-                                              const V *__restrict__ tw) {
-    const int mp = L / R;
-    for (int t = threadIdx.x; t < mp * tile; t += blockDim.x) {
-        int b, c;
-        divmod(t, tile, lgt, b, c);
-        if (c >= nk) continue;
-        V a[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int x = b + mp * r;
-            a[r] = x < nx ? g1[((uint64_t)x * fy + ky) * H + kz0 + c] : mk<V>(0, 0);
-        }
-        small_dft<-1>(a, R);
-        if (mp > 1) {
-#pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[b * q]);
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) d[c * S + pidx(b + mp * r, sh)] = a[r];
-    }
-    __syncthreads();
-}
-
-template <int R, class V>
-__device__ __forceinline__ void x_last_stage(const V *d, int L, int S, int sh, int tile, int lgt, int nk, int fy,
-                                             int H, int ky, int kz0, int Rx, V *__restrict__ g2,
-                                             const V *__restrict__ tw) {
-    const int mp = L / R, nxo = 2 * Rx + 1;
-    for (int t = threadIdx.x; t < mp * tile; t += blockDim.x) {
-        int b, c;
-        divmod(t, tile, lgt, b, c);
-        if (c >= nk) continue;
-        V a[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = d[c * S + pidx(b + mp * r, sh)];
-        if (mp > 1) {
-#pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[b * q]);
-        }
-        small_dft<1>(a, R);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int x = b + mp * r;
-            const int ix = x <= Rx ? x : (x >= L - Rx ? x - L + nxo : -1);
-            if (ix >= 0) g2[((uint64_t)ix * fy + ky) * H + kz0 + c] = a[r];
-        }
     }
 }
 
@@ -490,21 +549,12 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     if (fc.n >= 2) {
-        switch (fc.f[0]) {
-            case 8: x_first_stage<8>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
-            case 4: x_first_stage<4>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
-            case 2: x_first_stage<2>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
-            case 3: x_first_stage<3>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
-            default: x_first_stage<5>(d, L, S, sh, tile, lgt, nk, nx, fy, H, ky, kz0, g1, tw); break;
-        }
+        const uint64_t rs = (uint64_t)fy * H;
+        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, g1 + (uint64_t)ky * H + kz0, rs,
+                                                       nx, tw)));
         lds_power_inner(d, L, S, sh, nk, fc, tw);
-        switch (fc.f[0]) {
-            case 8: x_last_stage<8>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
-            case 4: x_last_stage<4>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
-            case 2: x_last_stage<2>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
-            case 3: x_last_stage<3>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
-            default: x_last_stage<5>(d, L, S, sh, tile, lgt, nk, fy, H, ky, kz0, Rx, g2, tw); break;
-        }
+        S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk, g2 + (uint64_t)ky * H + kz0, rs,
+                                                       Rx, tw)));
         return;
     }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
@@ -539,6 +589,18 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tile
     const int ix = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     const V *base = g2 + (uint64_t)ix * fy * H + kz0;
+    if (fc.n >= 2) {
+        S2H_RADIX_SWITCH(fc.f[fc.n - 1], (load_first_inv<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)H)));
+        int m = fc.f[fc.n - 1];
+        for (int s = fc.n - 2; s >= 1; --s) {
+            m *= fc.f[s];
+            fft_stage_any<true>(d, L, S, sh, nk, fc.f[s], m, tw);
+        }
+        S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk,
+                                                       g3 + (uint64_t)ix * (2 * Ry + 1) * H + kz0, (uint64_t)H, Ry,
+                                                       tw)));
+        return;
+    }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         int y, c;
         divmod(t, tile, lgt, y, c);
