@@ -36,7 +36,7 @@ COUNT_PHASE_TRAFFIC = 14840 * 1024 * 2.0 + 862900 * 1024 * 1.0
 # summed over its six dispatches (z fwd, y fwd, x, y inv, z inv, gather), raw
 # KiB -- for these LDS-tiled passes the raw FETCH_SIZE already counts the bytes
 # read (the y forward pass fetches at least the 263,168 KiB it must read).
-FFT64_PHASE_TRAFFIC = (1518992 + 1362816) * 1024.0
+FFT64_PHASE_TRAFFIC = (1519355 + 1363687) * 1024.0
 
 
 def fft64_bytes(dims, offs):
