@@ -797,6 +797,15 @@ s2h_status set_lds(const void *fn, size_t bytes) {
 // Run every pass; leaves the correlation cube in p->cube.  With ev, records
 // ev[0] before the y forward pass and ev[1..3] after it, the x pass and the y
 // inverse pass.
+// Line pairs per workgroup of the z passes: one radix-8 butterfly per thread
+// per stage (2048 points over 256 threads), capped by 48 KB of LDS.
+template <class T>
+int z_lines_per_block(int fz) {
+    const int by_work = std::max(1, 2048 / std::max(fz, 1));
+    const int by_lds = std::max(1, (int)(48 * 1024 / ((size_t)fz * sizeof(C2<T>))));
+    return std::min(16, std::min(by_work, by_lds));
+}
+
 template <class T>
 s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = nullptr) {
     using V = C2<T>;
@@ -811,7 +820,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     const int tiles_y = (H + ty - 1) / ty, tiles_x = (H + tx - 1) / tx;
     // z forward
     {
-        const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * sizeof(V)))));
+        const int ppb = z_lines_per_block<T>(fz);
         const size_t lds = lds_bytes<T>(fz, ppb, p->sh_z);
         S2H_TRY(set_lds((const void *)fft_z_fwd<T>, lds));
         const int npairs = nx * ((ny + 1) / 2);
@@ -850,7 +859,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     }
     if (ev) S2H_HIP(hipEventRecord(ev[3], s));
     {
-        const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * sizeof(V)))));
+        const int ppb = z_lines_per_block<T>(fz);
         const size_t lds = lds_bytes<T>(fz, ppb, p->sh_z);
         S2H_TRY(set_lds((const void *)fft_z_inv<T>, lds));
         const int nlines = nxo * nyo;
@@ -896,8 +905,7 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     p->tile_y = cands.front();
     p->tile_x = cands[std::min<size_t>(2, cands.size() - 1)];
     const int fz = (int)p->f[2];
-    const int ppb = std::max(1, std::min(16, (int)(48 * 1024 / ((size_t)fz * sizeof(C2<T>)))));
-    p->sh_z = pad_rule<T>(fz, ppb);
+    p->sh_z = pad_rule<T>(fz, z_lines_per_block<T>(fz));
     p->sh_y = lds_bytes<T>((int)p->f[1], p->tile_y, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[1], p->tile_y) : kNoPad;
     p->sh_x = lds_bytes<T>((int)p->f[0], p->tile_x, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[0], p->tile_x) : kNoPad;
     const char *env = std::getenv("S2HIP_AUTOTUNE");
