@@ -432,6 +432,43 @@ __device__ __forceinline__ void load_first_inv(V *d, int L, int S, int sh, int t
         default: { constexpr int RR = 5; CALL; } break; \
     }
 
+// Stage 0 of the z forward transform (sub-length L, R = f_0, mp = L / R)
+// from the packed rows: line pl holds rows y (real part) and y + 1 (imaginary
+// part) of column x; z >= nz and a missing row y + 1 are zero.
+template <int R, class V>
+__device__ __forceinline__ void z_first_stage(V *d, int L, int S, int sh, int np, int p0, int ypairs, int ny, int nz,
+                                              int row_words, const uint64_t *__restrict__ words,
+                                              const V *__restrict__ tw) {
+    using T = decltype(V::x);
+    const int mp = L / R, lmp = log2_if_pow2(mp);
+    for (int t = threadIdx.x; t < mp * np; t += blockDim.x) {
+        int pl, b;
+        divmod(t, mp, lmp, pl, b);
+        const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
+        const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
+        const bool two = y + 1 < ny;
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int z = b + mp * r;
+            T re = 0, im = 0;
+            if (z < nz) {
+                re = (T)((ra[z >> 6] >> (z & 63)) & 1ull);
+                if (two) im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
+            }
+            a[r] = mk<V>(re, im);
+        }
+        small_dft<-1>(a, R);
+        if (mp > 1) {
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[b * q]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[pl * S + pidx(b + mp * r, sh)] = a[r];
+    }
+    __syncthreads();
+}
+
 // ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
 template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restrict__ words, int nx, int ny, int nz,
@@ -447,20 +484,31 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
     const int p0 = blockIdx.x * pairs_per_block;
     const int np = min(pairs_per_block, npairs - p0);
     const int ypairs = (ny + 1) / 2;
-    for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
-        int pl, z;
-        divmod(t, L, lgl, pl, z);
-        const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
-        T re = 0, im = 0;
-        if (z < nz) {
-            const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
-            re = (T)((ra[z >> 6] >> (z & 63)) & 1ull);
-            if (y + 1 < ny) im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
+    if (fc.n >= 1) {
+        // the first DIF stage straight from the packed bits: thread (line, b)
+        // reads z = b + mp r of the two rows (lanes share each 64-bit word)
+        S2H_RADIX_SWITCH(fc.f[0], (z_first_stage<RR>(d, L, S, sh, np, p0, ypairs, ny, nz, row_words, words, tw)));
+        int m = L / fc.f[0];
+        for (int s = 1; s < fc.n; ++s) {
+            fft_stage_any<false>(d, L, S, sh, np, fc.f[s], m, tw);
+            m /= fc.f[s];
         }
-        d[pl * S + pidx(z, sh)] = mk<V>(re, im);
+    } else {
+        for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
+            int pl, z;
+            divmod(t, L, lgl, pl, z);
+            const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
+            T re = 0, im = 0;
+            if (z < nz) {
+                const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
+                re = (T)((ra[z >> 6] >> (z & 63)) & 1ull);
+                if (y + 1 < ny) im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
+            }
+            d[pl * S + pidx(z, sh)] = mk<V>(re, im);
+        }
+        __syncthreads();
+        lds_fft<false>(d, L, S, sh, np, fc, tw);
     }
-    __syncthreads();
-    lds_fft<false>(d, L, S, sh, np, fc, tw);
     for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
         const int pl = t / H, k = t - pl * H;
         const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
