@@ -266,6 +266,20 @@ __device__ __forceinline__ void load_block(const double *__restrict__ t, uint64_
     }
 }
 
+// A block's terms in any order (lane-interleaved, so each load instruction
+// reads 64 consecutive terms): for the passes whose results do not depend on
+// the order -- the plain block sum that only guesses binades, the exact
+// integer increments and the flags.
+__device__ __forceinline__ void load_block_any(const double *__restrict__ t, uint64_t n, uint64_t start, uint32_t lane,
+                                               double (&v)[kFoldK]) {
+#pragma unroll
+    for (int i = 0; i < kFoldK; ++i) {
+        const uint64_t j = start + (uint64_t)i * 64 + lane;
+        const double x = t[j < n ? j : 0];
+        v[i] = j < n ? x : 0.0;
+    }
+}
+
 // The ordered sum of t[0 .. n), valid in wave 0 of the workgroup (all threads call it).
 __device__ __forceinline__ double fold_wide(const double *__restrict__ t, uint64_t n, WideLds &L) {
     constexpr double kTwo53 = 9007199254740992.0, kTwo43 = 8796093022208.0;
@@ -280,7 +294,7 @@ __device__ __forceinline__ double fold_wide(const double *__restrict__ t, uint64
             double v[kWideAhead][kFoldK];
 #pragma unroll
             for (int q = 0; q < kWideAhead; ++q)
-                load_block(t, n, c0 + (uint64_t)min(b0 + q, nb - 1) * B, lane, v[q]);
+                load_block_any(t, n, c0 + (uint64_t)min(b0 + q, nb - 1) * B, lane, v[q]);
 #pragma unroll
             for (int q = 0; q < kWideAhead; ++q) {
                 double sum = 0.0;
@@ -315,7 +329,7 @@ __device__ __forceinline__ double fold_wide(const double *__restrict__ t, uint64
             double v[kWideAhead][kFoldK];
 #pragma unroll
             for (int q = 0; q < kWideAhead; ++q)
-                load_block(t, n, c0 + (uint64_t)min(b0 + q, nb - 1) * B, lane, v[q]);
+                load_block_any(t, n, c0 + (uint64_t)min(b0 + q, nb - 1) * B, lane, v[q]);
 #pragma unroll
             for (int q = 0; q < kWideAhead; ++q) {
                 const uint32_t b = min(b0 + q, nb - 1);
