@@ -933,7 +933,7 @@ int pad_rule(int L, int lines) {
 }
 
 // kz columns per workgroup of the y and x passes: powers of two up to kTileMax
-// whose LDS tile fits (the widest for y, the third widest for x), or -- on a
+// whose LDS tile fits (the widest for y, the widest up to 4 for x), or -- on a
 // context that autotunes -- for
 // each pass kind, the fastest of those candidates timed on the device (HIP
 // events around each pass of one run per candidate, after a warm-up).  The y
@@ -948,18 +948,24 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     for (int t = kTileMax; t >= 1; t /= 2)
         if (lds_bytes<T>(lmax, t, kNoPad) <= 150 * 1024) cands.push_back(t);
     if (cands.empty()) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
-    // untimed: the widest tile for y, a narrow one for x (the usual picks),
-    // padded where the padding costs no occupancy
+    // untimed: the widest tile for y, at most 4 columns for x (the usual
+    // picks), padded where the padding costs no occupancy
     p->tile_y = cands.front();
-    p->tile_x = cands[std::min<size_t>(2, cands.size() - 1)];
+    p->tile_x = cands.back();
+    for (int t : cands)
+        if (t <= 4) {
+            p->tile_x = t;
+            break;
+        }
     const int fz = (int)p->f[2];
     p->sh_z = pad_rule<T>(fz, z_lines_per_block<T>(fz));
     p->sh_y = lds_bytes<T>((int)p->f[1], p->tile_y, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[1], p->tile_y) : kNoPad;
     p->sh_x = lds_bytes<T>((int)p->f[0], p->tile_x, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[0], p->tile_x) : kNoPad;
     const char *env = std::getenv("S2HIP_AUTOTUNE");
     const double cells = (double)p->f[0] * (double)p->f[1] * (double)p->f[2];
-    // tune only where a run is short (up to a 1024^3 padded grid)
-    if (!ctx->autotune || (env && env[0] == '0') || cands.size() < 2 || cells > 1.1e9) return S2H_OK;
+    // tune only where a run is short (up to a 1536^3 padded grid: about 50 ms
+    // per FP64 run there, 12 runs once per plan)
+    if (!ctx->autotune || (env && env[0] == '0') || cands.size() < 2 || cells > 3.7e9) return S2H_OK;
     if (cands.size() > 3) cands.resize(3);
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     s2h_status st = S2H_OK;
