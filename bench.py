@@ -39,8 +39,9 @@ COUNT_PHASE_TRAFFIC = 14840 * 1024 * 2.0 + 862900 * 1024 * 1.0
 FFT64_PHASE_TRAFFIC = (1519355 + 1363687) * 1024.0
 
 
-def fft64_bytes(dims, offs):
-    """(unpruned, compulsory) HBM bytes of one FP64 FFT evaluation.
+def fft64_bytes(dims, offs, complex_bytes=16.0):
+    """(unpruned, compulsory) HBM bytes of one FFT evaluation (complex128
+    spectra by default; 8.0 for the FP32 path's complex64).
     Unpruned: 12 x f_x f_y (f_z/2+1) x 16 B -- three axes, forward and
     inverse, read and write of the complex128 half spectrum of a transform that
     keeps every output.  Compulsory (the roofline's algorithmic bytes): what
@@ -56,13 +57,13 @@ def fft64_bytes(dims, offs):
     fx, fy, fz = (smooth_fft_length(2 * n - 1) for n in dims)
     h = fz // 2 + 1
     rx, ry, rz = (int(v) for v in np.abs(offs).max(axis=0))
-    c = 16.0
+    c = float(complex_bytes)
     row_words = (nz + 63) // 64 + 1
     z_fwd = nx * ny * row_words * 8 + nx * ny * h * c
     y_fwd = nx * ny * h * c + nx * fy * h * c
     x_pow = nx * fy * h * c + (2 * rx + 1) * fy * h * c
     y_inv = (2 * rx + 1) * fy * h * c + (2 * rx + 1) * (2 * ry + 1) * h * c
-    z_inv = (2 * rx + 1) * (2 * ry + 1) * h * c + (2 * rx + 1) * (2 * ry + 1) * (2 * rz + 1) * 8.0
+    z_inv = (2 * rx + 1) * (2 * ry + 1) * h * c + (2 * rx + 1) * (2 * ry + 1) * (2 * rz + 1) * c / 2.0
     return 12.0 * fx * fy * h * c, float(z_fwd + y_fwd + x_pow + y_inv + z_inv)
 
 
@@ -232,11 +233,12 @@ def fft_leg(ctx, vol, r_max, hits, offs, rs, curve_exact, steps):
     from rustmspt_amd.config import smooth_fft_length
 
     f = [smooth_fft_length(2 * n - 1) for n in vol.dims]
-    alg_bytes = 12.0 * f[0] * f[1] * (f[2] // 2 + 1) * 8.0
+    unpruned, comp = fft64_bytes(vol.dims, offs, 8.0)
     dev_s = float(np.mean(dev)) * 1e-3
     return {"evals_per_s": steps / el, "device_ms_per_eval": float(np.mean(dev)),
-            "roofline": {"bound": "hbm", "achieved": alg_bytes / dev_s / 1e9, "peak": 8000.0, "unit": "GB/s",
-                         "frac": alg_bytes / dev_s / 8e12, "algorithmic_bytes_per_eval": alg_bytes,
+            "roofline": {"bound": "hbm", "achieved": comp / dev_s / 1e9, "peak": 8000.0, "unit": "GB/s",
+                         "frac": comp / dev_s / 8e12, "algorithmic_bytes_per_eval": comp,
+                         "unpruned_bytes_per_eval": unpruned, "frac_unpruned": unpruned / dev_s / 8e12,
                          "padded": f},
             "max_abs_count_error": float(err.max()), "max_rel_count_error": float(max(per_bin_rel)),
             "rounds_to_exact": bool(np.all(np.rint(fc) == hits)),
