@@ -51,7 +51,7 @@ struct FftPlan {
     DevBuf tw[3];    // complex[L] = exp(-2 pi i k / L), per axis, in the plan's precision
     DevBuf rev_z;    // int32[fz]: natural frequency held at each DIF output position
     DevBuf pos_z;    // int32[fz]: DIF output position of each natural frequency
-    DevBuf g1, g1t, g2, g3, cube, fcounts;  // g1t: the y pass's output in the x pass's tile layout
+    DevBuf g1, g2, g3, cube, fcounts;
     int tile_y = 0;  // kz columns per workgroup in the y passes (0: not chosen yet)
     int tile_x = 0;  // kz columns per workgroup in the x pass
     int sh_z = 3, sh_y = 3, sh_x = 3;  // LDS padding shift per pass kind (3: a slot every 8; 30: none)
@@ -60,7 +60,7 @@ struct FftPlan {
 void fft_plan_release(s2h_ctx *ctx, FftPlan *p) {
     if (!p) return;
     for (auto &b : p->tw) dev_free(ctx, b);
-    DevBuf *bufs[] = {&p->rev_z, &p->pos_z, &p->g1, &p->g1t, &p->g2, &p->g3, &p->cube, &p->fcounts};
+    DevBuf *bufs[] = {&p->rev_z, &p->pos_z, &p->g1, &p->g2, &p->g3, &p->cube, &p->fcounts};
     for (DevBuf *b : bufs) dev_free(ctx, *b);
     delete p;
 }
@@ -432,34 +432,6 @@ __device__ __forceinline__ void load_first_inv(V *d, int L, int S, int sh, int t
         default: { constexpr int RR = 5; CALL; } break; \
     }
 
-// The y forward pass writes its spectra for the x pass in tiles of T kz
-// columns: element (ky, kz) of column x at ((ky * nt + kz / T) * nx + x) * T
-// + kz % T (nt = tiles of T over the H kz).  An x-pass workgroup, which takes
-// one tile of one ky, then reads nx * T contiguous values instead of nx row
-// segments of T values 2 MB apart; the strided accesses move to the y pass's
-// writes.  T is the x pass's tile width (a power of two).
-__device__ __forceinline__ uint64_t xt_index(int ky, int kz, int x, int nx, int T, int lgT, int nt) {
-    return (((uint64_t)ky * nt + (kz >> lgT)) * nx + x) * T + (kz & (T - 1));
-}
-
-template <int R, class V>
-__device__ __forceinline__ void store_last_fwd_xt(const V *d, int L, int S, int sh, int tile, int lgt, int nk,
-                                                  V *__restrict__ g1t, int x, int kz0, int nx, int T, int lgT,
-                                                  int nt) {
-    const int nb = L / R;
-    for (int t = threadIdx.x; t < nb * tile; t += blockDim.x) {
-        int b, c;
-        divmod(t, tile, lgt, b, c);
-        if (c >= nk) continue;
-        V a[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = d[c * S + pidx(b * R + r, sh)];
-        small_dft<-1>(a, R);
-#pragma unroll
-        for (int r = 0; r < R; ++r) g1t[xt_index(b * R + r, kz0 + c, x, nx, T, lgT, nt)] = a[r];
-    }
-}
-
 // Stage 0 of the z forward transform (sub-length L, R = f_0, mp = L / R)
 // from the packed rows: line pl holds rows y (real part) and y + 1 (imaginary
 // part) of column x; z >= nz and a missing row y + 1 are zero.
@@ -554,8 +526,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
 // ---- y forward: per (x, kz tile), rows y >= ny are zero; in place in G1
 template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, int tiles_kz, int tile, Fac fc,
-                                                         const C2<T> *__restrict__ tw, const C2<T> *__restrict__ g1,
-                                                         C2<T> *__restrict__ g1t, int nx, int TX, int ntx, int sh) {
+                                                         const C2<T> *__restrict__ tw, C2<T> *__restrict__ g1, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
@@ -563,8 +534,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     V *d = sm;
     const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
-    const V *base = g1 + (uint64_t)x * fy * H + kz0;
-    const int lgT = log2_if_pow2(TX);
+    V *base = g1 + (uint64_t)x * fy * H + kz0;
     if (fc.n >= 2) {
         S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)H, ny, tw)));
         int m = L / fc.f[0];
@@ -572,8 +542,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
             fft_stage_any<false>(d, L, S, sh, nk, fc.f[s], m, tw);
             m /= fc.f[s];
         }
-        S2H_RADIX_SWITCH(fc.f[fc.n - 1],
-                         (store_last_fwd_xt<RR>(d, L, S, sh, tile, lgt, nk, g1t, x, kz0, nx, TX, lgT, ntx)));
+        S2H_RADIX_SWITCH(fc.f[fc.n - 1], (store_last_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)H)));
         return;
     }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
@@ -588,7 +557,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         int y, c;
         divmod(t, tile, lgt, y, c);
-        if (c < nk) g1t[xt_index(y, kz0 + c, x, nx, TX, lgT, ntx)] = d[c * S + pidx(y, sh)];
+        if (c < nk) base[(uint64_t)y * H + c] = d[c * S + pidx(y, sh)];
     }
 }
 
@@ -629,9 +598,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     const int nk = min(tile, H - kz0);
     if (fc.n >= 2) {
         const uint64_t rs = (uint64_t)fy * H;
-        // g1 holds the y pass's output in tiles of `tile` kz columns (xt_index)
-        const V *src = g1 + ((uint64_t)ky * tiles_kz + kz0 / tile) * nx * tile;
-        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, src, (uint64_t)tile, nx, tw)));
+        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, g1 + (uint64_t)ky * H + kz0, rs,
+                                                       nx, tw)));
         lds_power_inner(d, L, S, sh, nk, fc, tw);
         S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk, g2 + (uint64_t)ky * H + kz0, rs,
                                                        Rx, tw)));
@@ -641,7 +609,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
         int x, c;
         divmod(t, tile, lgt, x, c);
         V v = mk<V>(0, 0);
-        if (x < nx && c < nk) v = g1[xt_index(ky, kz0 + c, x, nx, tile, lgt, tiles_kz)];
+        if (x < nx && c < nk) v = g1[((uint64_t)x * fy + ky) * H + kz0 + c];
         d[c * S + pidx(x, sh)] = v;
     }
     __syncthreads();
@@ -854,7 +822,6 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], in
     s2h_status st;
     const uint64_t cb = 2 * (uint64_t)real_bytes;  // bytes per complex value
     if ((st = dev_alloc(ctx, p->g1, dims[0] * p->f[1] * H * cb)) != S2H_OK ||
-        (st = dev_alloc(ctx, p->g1t, dims[0] * p->f[1] * (H + kTileMax) * cb)) != S2H_OK ||
         (st = dev_alloc(ctx, p->g2, nxo * p->f[1] * H * cb)) != S2H_OK ||
         (st = dev_alloc(ctx, p->g3, nxo * nyo * H * cb)) != S2H_OK ||
         (st = dev_alloc(ctx, p->cube, nxo * nyo * nzo * (uint64_t)real_bytes)) != S2H_OK)
@@ -916,8 +883,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         const size_t lds = lds_bytes<T>(fy, ty, p->sh_y);
         S2H_TRY(set_lds((const void *)fft_y_fwd<T>, lds));
         fft_y_fwd<T><<<nx * tiles_y, kFftThreads, lds, s>>>(ny, fy, H, tiles_y, ty, fcy, (const V *)p->tw[1].ptr,
-                                                            (const V *)p->g1.ptr, (V *)p->g1t.ptr, nx, tx, tiles_x,
-                                                            p->sh_y);
+                                                            (V *)p->g1.ptr, p->sh_y);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
@@ -926,7 +892,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         const size_t lds = lds_bytes<T>(fx, tx, p->sh_x);
         S2H_TRY(set_lds((const void *)fft_x_power<T>, lds));
         fft_x_power<T><<<fy * tiles_x, kFftThreads, lds, s>>>(nx, fy, H, tiles_x, tx, fcx, (const V *)p->tw[0].ptr,
-                                                              Rx, (const V *)p->g1t.ptr, (V *)p->g2.ptr, p->sh_x);
+                                                              Rx, (const V *)p->g1.ptr, (V *)p->g2.ptr, p->sh_x);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
@@ -1117,8 +1083,7 @@ double fft64_plan_bytes(const uint64_t dims[3], const int R[3]) {
     const double fy = (double)smooth_length(2 * dims[1] - 1);
     const double h = (double)(smooth_length(2 * dims[2] - 1) / 2 + 1);
     const double nxo = 2.0 * R[0] + 1, nyo = 2.0 * R[1] + 1, nzo = 2.0 * R[2] + 1;
-    return ((double)dims[0] * fy * h + (double)dims[0] * fy * (h + kTileMax) + nxo * fy * h + nxo * nyo * h) * 16.0 +
-           nxo * nyo * nzo * 8.0;
+    return ((double)dims[0] * fy * h + nxo * fy * h + nxo * nyo * h) * 16.0 + nxo * nyo * nzo * 8.0;
 }
 
 // Whether the FP64 passes can run this shape (lengths of 2, 3, 5 factors up

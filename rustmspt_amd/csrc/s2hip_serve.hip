@@ -78,8 +78,7 @@ uint64_t footprint(const uint64_t dims[3], uint32_t r_max, double pitch, int32_t
         const double rx = std::min<double>(dims[0] - 1, R), ry = std::min<double>(dims[1] - 1, R),
                      rz = std::min<double>(dims[2] - 1, R);
         const double real = method == 1 ? 4.0 : 8.0;  // bytes per real value of the spectra
-        bytes += ((double)dims[0] * fy * H * 2 + (double)dims[0] * fy * (H + 16) * 2 + (2 * rx + 1) * fy * H * 2 +
-                  (2 * rx + 1) * (2 * ry + 1) * H * 2 +
+        bytes += ((double)dims[0] * fy * H * 2 + (2 * rx + 1) * fy * H * 2 + (2 * rx + 1) * (2 * ry + 1) * H * 2 +
                   (2 * rx + 1) * (2 * ry + 1) * (2 * rz + 1) + K) * real;
     }
     return (uint64_t)(bytes * 1.1) + (1ull << 20);
