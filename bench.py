@@ -154,19 +154,10 @@ def batched_leg(ctx, ncand: int, seed: int, plain_sample: int):
 
     from rustmspt_amd import batch, s2, synth
 
-    rng = synth.rng_for(seed, 77)
-    seeds, parts = [], []
-    for g in (96, 128, 192, 256):
-        n = int(rng.integers(150, 401))
-        r_eq = (0.2 * g ** 3 / n * 3 / (4 * np.pi)) ** (1 / 3)  # VF ~ 0.2
-        ps = synth.place_particles(n, g, rng, synth.ellipsoid_maker(0.8 * r_eq, 1.2 * r_eq), overlap=True)
-        seeds.append(batch.Seed(ctx, [p.mesh() for p in ps], s2.BoundingBox.from_size((g, g, g)), 1.0, g // 4))
-        parts.append(ps)
-    moves = []
-    for _ in range(ncand):
-        si = int(rng.integers(0, 4))
-        pi = int(rng.integers(0, len(parts[si])))
-        moves.append((si, pi, parts[si][pi].moved(rng.uniform(-2, 2, 3), synth.random_rotation(rng)).mesh().vertices))
+    packings, moves = synth.config_batched(seed, ncand)
+    seeds = [batch.Seed(ctx, [p.mesh() for p in ps], s2.BoundingBox.from_size((g, g, g)), 1.0, g // 4)
+             for g, ps in packings]
+    parts = [ps for _, ps in packings]
     # first call: code objects, graph capture and the one-time row-chunk tuning
     t0 = time.perf_counter()
     batch.batch_eval(ctx, seeds, moves)

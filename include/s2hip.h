@@ -398,7 +398,12 @@ s2h_status s2h_serve_submit(s2h_server *server, const s2h_request *req, uint64_t
  * result has been returned its ticket is released. */
 s2h_status s2h_serve_poll(s2h_server *server, uint64_t ticket, int32_t wait, s2h_result *res,
                           double *s2, uint64_t n);
-/* [admitted, deferred, completed, failed, waiting, bytes_in_use, peak_bytes, budget] */
+/* n = 8: [admitted, deferred, completed, failed, waiting, bytes_in_use, peak_bytes, budget], where
+ * bytes_in_use / peak_bytes sum the admitted footprints; n = 10 adds [8] the device bytes the slot
+ * contexts hold between requests (in use + cached blocks) and [9] the number of requests during
+ * which a slot held more device memory than its share (the request's footprint plus the slot's
+ * plan allowance; 0 unless an estimate was exceeded).  Under a budget each slot's device
+ * memory, cached blocks included, is held to that share. */
 s2h_status s2h_serve_stats(s2h_server *server, uint64_t *stats, uint64_t n);
 
 /* --------------------------------------------------------------- host helpers */

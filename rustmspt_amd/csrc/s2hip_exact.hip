@@ -685,9 +685,14 @@ static void group_columns(const ExactPlan *p, int band, Grouping &g) {
 }
 
 // x-chunks of a block for a wave target: proportional to its cost.
+// At least enough chunks that no lane's u32 counter can wrap: an item's
+// counter adds at most 32 per partner dword of each of its rows.
 static int block_chunks(const Grouping &g, const Grouping::Blk &b, int waves) {
-    const int n = (int)std::lround(b.cost / std::max(g.total, 1.0) * waves);
-    return std::max(1, std::min(b.x1 - b.x0, n));
+    const int span = b.x1 - b.x0;
+    const uint64_t per_x = (uint64_t)(b.y1 - b.y0) * (uint64_t)b.nw * 32u;
+    const int min_n = (int)(((uint64_t)span * per_x + 0xFFFFFFFEull) / 0xFFFFFFFFull);
+    const int n = std::max(min_n, (int)std::lround(b.cost / std::max(g.total, 1.0) * waves));
+    return std::max(1, std::min(span, n));
 }
 
 // Workgroups (kWavesPerBlock waves) resident on the whole chip at once: 256 CUs
@@ -1024,9 +1029,27 @@ static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, ui
     if (m == 0) {
         const int *ext = p->shells->host->extent;
         const int R[3] = {ext[0], ext[1], ext[2]};
-        const bool fits = !ctx->fft_budget || fft64_plan_bytes(p->dims, R) <= (double)ctx->fft_budget;
+        const double plan_bytes = fft64_plan_bytes(p->dims, R);
+        bool fits = !ctx->fft_budget || plan_bytes <= (double)ctx->fft_budget;
+        if (ctx->budget) {
+            // the spectra (unless this context already holds them) and the
+            // per-offset values must fit what the context's budget leaves
+            const double need = (fft64_plan_cached(ctx, p->dims, R) ? 0.0 : plan_bytes) + 8.0 * p->offs.size();
+            fits = fits && (double)ctx->in_use + need <= (double)ctx->budget;
+        }
         const double fft_ms = fft64_model_bytes(p->dims) / kFft64BytesPerS * 1e3 + kFft64FixedMs;
         m = fft_ok && fits && fft_ms < p->model_ms ? 2 : 1;
+        if (m == 2) {
+            p->method = 2;
+            const s2h_status st = fft64_exact_device(ctx, vol, p, d_dvals);
+            if (st != S2H_ERR_BUDGET) return st;
+            // the model's choice only: give the spectra back and count directly,
+            // as the reference's planner keeps to its budget (src/geometry/s2.rs:733-766)
+            (void)hipStreamSynchronize(ctx->stream);
+            fft_plans_release(ctx);
+            *d_dvals = nullptr;
+            m = 1;
+        }
     }
     p->method = m;
     if (m == 2) return fft64_exact_device(ctx, vol, p, d_dvals);

@@ -760,9 +760,13 @@ uint64_t smooth_length(uint64_t min_len) {
     return best;
 }
 
+std::tuple<uint64_t, uint64_t, uint64_t> fft_plan_key(const uint64_t dims[3], const int R[3], int real_bytes) {
+    return std::make_tuple(dims[0] | ((uint64_t)R[0] << 32) | ((uint64_t)(real_bytes == 8) << 63),
+                           dims[1] | ((uint64_t)R[1] << 32), dims[2] | ((uint64_t)R[2] << 32));
+}
+
 s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], int real_bytes, FftPlan **out) {
-    auto key = std::make_tuple(dims[0] | ((uint64_t)R[0] << 32) | ((uint64_t)(real_bytes == 8) << 63),
-                               dims[1] | ((uint64_t)R[1] << 32), dims[2] | ((uint64_t)R[2] << 32));
+    auto key = fft_plan_key(dims, R, real_bytes);
     auto it = ctx->fft_plans.find(key);
     if (it != ctx->fft_plans.end()) {
         *out = it->second;
@@ -1084,6 +1088,15 @@ double fft64_plan_bytes(const uint64_t dims[3], const int R[3]) {
     const double h = (double)(smooth_length(2 * dims[2] - 1) / 2 + 1);
     const double nxo = 2.0 * R[0] + 1, nyo = 2.0 * R[1] + 1, nzo = 2.0 * R[2] + 1;
     return ((double)dims[0] * fy * h + nxo * fy * h + nxo * nyo * h) * 16.0 + nxo * nyo * nzo * 8.0;
+}
+
+bool fft64_plan_cached(s2h_ctx *ctx, const uint64_t dims[3], const int R[3]) {
+    return ctx->fft_plans.count(fft_plan_key(dims, R, 8)) != 0;
+}
+
+void fft_plans_release(s2h_ctx *ctx) {
+    for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
+    ctx->fft_plans.clear();
 }
 
 // Whether the FP64 passes can run this shape (lengths of 2, 3, 5 factors up

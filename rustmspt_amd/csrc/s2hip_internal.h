@@ -426,6 +426,7 @@ struct s2h_ctx {
     s2h::BatchWS *batch_ws = nullptr;            // persistent batch workspace and graph
     std::multimap<uint64_t, void *> block_cache;  // released device blocks by size
     uint64_t cached_bytes = 0;
+    uint64_t held_peak = 0;  // largest in_use + cached_bytes since it was last reset
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double prof_ms[4] = {0, 0, 0, 0};
     int launches = 0;
@@ -444,6 +445,7 @@ namespace s2h {
 s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes);
 void dev_free(s2h_ctx *ctx, DevBuf &b);
 void dev_cache_release(s2h_ctx *ctx);  // hipFree every cached block
+void dev_cache_trim(s2h_ctx *ctx, uint64_t keep);  // hipFree cached blocks, largest first, down to keep bytes
 void batch_ws_release(s2h_ctx *ctx);
 s2h_status upload(s2h_ctx *ctx, DevBuf &b, const void *src, uint64_t bytes);
 s2h_status vol_alloc(s2h_ctx *ctx, const uint64_t dims[3], s2h_vol **out);
@@ -470,6 +472,11 @@ double fft64_error_bound(const uint64_t dims[3], uint64_t occupied);
 bool fft64_supported(const uint64_t dims[3]);
 double fft64_model_bytes(const uint64_t dims[3]);
 double fft64_plan_bytes(const uint64_t dims[3], const int R[3]);
+// Whether the context already holds the FP64 FFT plan of this shape (its
+// buffers are then part of ctx->in_use).
+bool fft64_plan_cached(s2h_ctx *ctx, const uint64_t dims[3], const int R[3]);
+// Release every FFT plan of the context (their blocks go to the block cache).
+void fft_plans_release(s2h_ctx *ctx);
 s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan, const double **d_values);
 // Copy folded curve values and supported flags back and finish the curve.
 s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *plan, double vf, double *s2);

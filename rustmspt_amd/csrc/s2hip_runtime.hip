@@ -55,6 +55,7 @@ s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes) {
         ctx->cached_bytes -= it->first;
         ctx->block_cache.erase(it);
         ctx->in_use += b.bytes;
+        ctx->held_peak = std::max(ctx->held_peak, ctx->in_use + ctx->cached_bytes);
         return S2H_OK;
     }
     if (ctx->budget && ctx->in_use + want > ctx->budget)
@@ -75,6 +76,7 @@ s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes) {
     }
     b.bytes = want;
     ctx->in_use += want;
+    ctx->held_peak = std::max(ctx->held_peak, ctx->in_use + ctx->cached_bytes);
     return S2H_OK;
 }
 
@@ -90,6 +92,8 @@ void dev_free(s2h_ctx *ctx, DevBuf &b) {
 }
 
 void dev_cache_release(s2h_ctx *ctx) { cache_trim(ctx, 0); }
+
+void dev_cache_trim(s2h_ctx *ctx, uint64_t keep) { cache_trim(ctx, keep); }
 
 s2h_status upload(s2h_ctx *ctx, DevBuf &b, const void *src, uint64_t bytes) {
     S2H_TRY(dev_alloc(ctx, b, bytes));

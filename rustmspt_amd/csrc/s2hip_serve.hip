@@ -105,11 +105,13 @@ struct s2h_server {
         s2h::Req *job = nullptr;
         std::thread th;
         std::condition_variable cv;
+        uint64_t held = 0;  // device bytes the slot's context holds (in use + cached) after its last job
     };
     std::vector<Slot *> slots;
     std::thread dispatcher;
     uint64_t slot_plan_cap = 8ull << 30;
     uint64_t admit_budget = 0;  // budget minus the slots' plan-cache allowance
+    uint64_t over_budget = 0;   // jobs whose context held more device memory than its share
 };
 
 namespace s2h {
@@ -141,13 +143,36 @@ void worker_loop(s2h_server *sv, s2h_server::Slot *slot) {
         if (!slot->job) return;
         Req *r = slot->job;
         r->t_start = now_s(sv);
+        // the context may hold the request's admitted footprint on top of its
+        // cached plans (at most the slot's plan allowance): its device memory,
+        // cached blocks included, stays within that, so the slots together
+        // stay within the server's budget
+        const uint64_t job_budget = sv->budget ? r->footprint + sv->slot_plan_cap : 0;
         lk.unlock();
+        s2h_ctx *ctx = slot->ctx;
+        ctx->budget = job_budget;
+        ctx->held_peak = ctx->in_use + ctx->cached_bytes;
         std::string err;
-        const s2h_status st = run_job(slot->ctx, r, err);
+        s2h_status st = run_job(ctx, r, err);
+        if (st == S2H_ERR_BUDGET) {
+            // the plans cached from earlier requests left too little room: drop
+            // them, give their blocks back to the device and run once more
+            exact_plans_clear(ctx);
+            dev_cache_release(ctx);
+            err.clear();
+            st = run_job(ctx, r, err);
+        }
         const double t_done = now_s(sv);
-        // cached plans beyond the per-slot cap are dropped between requests
-        if (slot->ctx->in_use > sv->slot_plan_cap) exact_plans_clear(slot->ctx);
+        // cached plans beyond the per-slot cap are dropped between requests,
+        // and under a budget the idle context keeps at most the allowance
+        if (ctx->in_use > sv->slot_plan_cap) exact_plans_clear(ctx);
+        if (sv->budget && ctx->in_use + ctx->cached_bytes > sv->slot_plan_cap)
+            dev_cache_trim(ctx, sv->slot_plan_cap > ctx->in_use ? sv->slot_plan_cap - ctx->in_use : 0);
+        const uint64_t held = ctx->in_use + ctx->cached_bytes;
+        const bool over = job_budget && ctx->held_peak > job_budget;
         lk.lock();
+        slot->held = held;
+        if (over) ++sv->over_budget;
         // publishing the status is the worker's last access to the request: a
         // poller may release it as soon as the lock is dropped
         sv->in_use -= r->footprint;
@@ -341,7 +366,7 @@ s2h_status s2h_serve_poll(s2h_server *sv, uint64_t ticket, int32_t wait, s2h_res
 
 s2h_status s2h_serve_stats(s2h_server *sv, uint64_t *stats, uint64_t n) {
     if (!sv || !stats) return fail(S2H_ERR_ARG, "null argument");
-    if (n != 8) return fail(S2H_ERR_SIZE, "stats buffer must hold 8 values");
+    if (n != 8 && n != 10) return fail(S2H_ERR_SIZE, "stats buffer must hold 8 or 10 values");
     std::lock_guard<std::mutex> lk(sv->mu);
     stats[0] = sv->admitted;
     stats[1] = sv->deferred;
@@ -351,6 +376,12 @@ s2h_status s2h_serve_stats(s2h_server *sv, uint64_t *stats, uint64_t n) {
     stats[5] = sv->in_use;
     stats[6] = sv->peak;
     stats[7] = sv->budget;
+    if (n == 10) {
+        uint64_t held = 0;
+        for (auto *s : sv->slots) held += s->held;
+        stats[8] = held;
+        stats[9] = sv->over_budget;
+    }
     return S2H_OK;
 }
 

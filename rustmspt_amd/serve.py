@@ -60,9 +60,10 @@ class Server:
         return info, None
 
     def stats(self) -> dict:
-        s = np.zeros(8, np.uint64)
-        L.check(L.lib().s2h_serve_stats(self.handle, L.ptr(s, L._u64p), 8))
-        keys = ["admitted", "deferred", "completed", "failed", "waiting", "bytes_in_use", "peak_bytes", "budget"]
+        s = np.zeros(10, np.uint64)
+        L.check(L.lib().s2h_serve_stats(self.handle, L.ptr(s, L._u64p), 10))
+        keys = ["admitted", "deferred", "completed", "failed", "waiting", "bytes_in_use", "peak_bytes", "budget",
+                "device_bytes_held", "over_budget_requests"]
         return {k: int(v) for k, v in zip(keys, s)}
 
     def close(self):

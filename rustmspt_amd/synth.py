@@ -133,6 +133,29 @@ def config_large(seed: int = 2, n: int = 500, grid: int = 256, r_eq=(0.040, 0.06
     return packing_mesh(parts), BoundingBox.from_size((grid, grid, grid)), parts
 
 
+def config_batched(seed: int = 3, ncand: int = 256, grids=(96, 128, 192, 256)):
+    """Batched optimize step: one seed packing per grid (U{150..400} ellipsoids,
+    axis ratios U[1,3], equivalent radius for VF ~ 0.2, overlap allowed: the
+    occupancy is a union) and ncand one-move candidates, each moving one particle
+    of a uniformly drawn seed by a translation U[-2, 2] voxels per axis and a
+    uniform random rotation.  Returns (seeds, moves): seeds is a list of
+    (grid, particles) with r_max = grid / 4, moves a list of
+    (seed index, particle index, new vertices)."""
+    rng = rng_for(seed, 77)
+    seeds = []
+    for g in grids:
+        n = int(rng.integers(150, 401))
+        r_eq = (0.2 * g ** 3 / n * 3 / (4 * np.pi)) ** (1 / 3)
+        seeds.append((g, place_particles(n, g, rng, ellipsoid_maker(0.8 * r_eq, 1.2 * r_eq), overlap=True)))
+    moves = []
+    for _ in range(ncand):
+        si = int(rng.integers(0, len(seeds)))
+        pi = int(rng.integers(0, len(seeds[si][1])))
+        p = seeds[si][1][pi]
+        moves.append((si, pi, p.moved(rng.uniform(-2, 2, 3), random_rotation(rng)).mesh().vertices))
+    return seeds, moves
+
+
 def grf_volume(n: int, corr_len: float, vf: float, seed: int) -> np.ndarray:
     """Gaussian random field (white noise filtered by a Gaussian kernel of the
     given correlation length, periodic FFT filter) thresholded at the (1-vf)

@@ -1,29 +1,51 @@
-"""The measured configurations at their full sizes, checked through properties
-that do not need a full CPU recount: sampled offsets against the oracle, the
-identity C(d) = C(-d), hits <= valid, the curve as the oracle's ordered fold of
-the device counts, FP32 FFT against the FP64 FFT per radius bin, Monte Carlo
-within 3 sigma.  Each case prints its per-bin figures (pytest -s).
+"""The five measured configurations at their stated workloads, against the
+oracle.  Each case prints its figures (pytest -s).
 
-  single large measure: 500 particles (icospheres + ellipsoids), 256^3, r_max=64
-  CT-like volume:       768^3 Gaussian random field (correlation length 12,
-                        volume fraction 0.35), r_max=128
+  plain measure (C1):   64 level-3 icospheres, 128^3, r_max=32: device grid ==
+                        the oracle's voxelization, curve == the oracle's exact
+                        curve, for the model's kernel and direct counting
+  single large (C2):    500 particles (icospheres + ellipsoids), 256^3, r_max=64:
+                        ALL 1,123,772 shell-offset counts of both exact kernels
+                        (FP64 FFT and direct counting) == the oracle's packed
+                        direct counts; FP32 FFT per bin; Monte Carlo
+  batched step (C3):    256 one-move candidates over 96^3-256^3 seeds (150-400
+                        ellipsoids, r_max = grid/4): every batched curve == the
+                        plain per-call path; one candidate per grid == the
+                        oracle's voxelization + exact curve
+  served stream (C4):   Poisson arrivals, grids 64^3-384^3 log-uniform, r_max
+                        16..96, 70 % exact / 30 % FP32 FFT: every result == the
+                        plain entry point; exact results on grids <= 192^3 ==
+                        the oracle's exact curve
+  CT-like volume (C5):  768^3 Gaussian random field (correlation length 12,
+                        VF 0.35), r_max=128: every 997th shell offset of both
+                        exact kernels == the oracle; FP32 per bin
 
-FP32 tolerances (stated, as for the small cases): per-offset count error
-<= 4e-7 * M * log2(P) + 1e-3 with M the occupied count and P the padded cell
-count; per-bin |S2_fft32 - S2_exact| <= 1e-5.  Requires the GPU (marker gpu)."""
+FP32 tolerances (stated): per-offset count error <= 2.5e-8 * M * log2(P) + 0.05
+with M the occupied count and P the padded cell count (observed: 0.75 counts
+at 256^3 r64 where the bound is 2.5, 46-53 at 768^3 r128 where it is 123);
+per-bin |S2_fft32 - S2_exact| <= 1e-5.
+
+Monte Carlo rule (family-wise over the r_max radii, fixed seeds): every radius
+within 3 sigma of the exact estimand sum(hits)/sum(valid), and the number of
+radii beyond 2 sigma at most the 99.9 % quantile of Binomial(r_max, 0.0455).
+Requires the GPU (marker gpu)."""
 import math
 import os
 
 import numpy as np
 import pytest
 import scipy.fft as sfft
-
 from oracle import oracle as o
-from rustmspt_amd import s2, synth
+from rustmspt_amd import batch, s2, serve, synth
+from stats_rules import mc_family_rule
 
 pytestmark = pytest.mark.gpu
 
 WORKERS = max(1, min(16, os.cpu_count() or 1))
+
+
+def fp32_tolerance(M, pad):
+    return 2.5e-8 * M * math.log2(float(np.prod(pad))) + 0.05
 
 
 def fp64_counts(occ, offs):
@@ -48,22 +70,16 @@ def index_of(offs):
     return lut[neg]
 
 
-def check_exact(vol, occ, r_max, nsample, seed):
-    got = vol.exact_counts(r_max)
-    offs, rs = s2.shell_order_offsets(occ.shape, r_max, 1.0)
-    assert len(got) == len(offs)
-    rng = np.random.default_rng(seed)
-    pick = np.sort(rng.choice(len(offs), nsample, replace=False))
-    words = o.pack(occ)
-    assert np.array_equal(got[pick], o.hits_list(words, occ.shape, offs[pick]))
-    twin = index_of(offs)
-    assert np.all(twin >= 0) and np.array_equal(got, got[twin])
-    valid = np.prod(np.array(occ.shape, np.int64)[None, :] - np.abs(offs), axis=1)
-    assert np.all(got <= valid.astype(np.uint64))
-    vf = occ.sum() / occ.size
-    curve = vol.exact_s2(r_max)
-    assert np.array_equal(curve, o.curve_from_hits(occ.shape, r_max, offs, rs, got, vf))
-    return got, offs, rs, curve
+def kernel_counts(ctx, vol, r_max, method):
+    """Exact counts and curve with one exact kernel forced; the kernel that ran."""
+    ctx.set_exact_method(method)
+    try:
+        counts = vol.exact_counts(r_max)
+        curve = vol.exact_s2(r_max)
+        ran = ctx.exact_tune_info(vol.dims, r_max, 1.0)["kernel"]
+    finally:
+        ctx.set_exact_method("auto")
+    return counts, curve, ran
 
 
 def check_fft32(vol, occ, r_max, offs, rs, exact, curve, with_fp64):
@@ -71,24 +87,27 @@ def check_fft32(vol, occ, r_max, offs, rs, exact, curve, with_fp64):
     ex = exact.astype(np.float64)
     M = float(occ.sum())
     pad = [int(v) for v in o.padded_fft_dims(occ.shape)]
-    tol = 4e-7 * M * math.log2(float(np.prod(pad))) + 1e-3
+    tol = fp32_tolerance(M, pad)
     f = vol.fft32_s2(r_max)
     d_s2 = np.abs(f - curve)
     if with_fp64:
         c64, _ = fp64_counts(occ, offs)
         assert np.array_equal(np.rint(c64), ex)  # the FP64 oracle rounds to the exact counts
-    print(f"\n{occ.shape} r_max={r_max} M={M:.0f} tolerance={tol:.1f} counts")
+    print(f"\n{occ.shape} r_max={r_max} M={M:.0f} tolerance={tol:.2f} counts")
     print(" r   max|c32-exact|" + ("   max|c32-c64|" if with_fp64 else "") + "   |S2_32-S2_exact|")
+    worst = 0.0
     for r in range(1, r_max + 1):
         sl = slice(int(rs[r]), int(rs[r + 1]))
         e_ex = float(np.max(np.abs(c32[sl] - ex[sl])))
+        worst = max(worst, e_ex)
         line = f"{r:3d} {e_ex:16.3f}"
         if with_fp64:
             e64 = float(np.max(np.abs(c32[sl] - c64[sl])))
             assert e64 <= tol, (r, e64, tol)
             line += f" {e64:14.3f}"
-        assert e_ex <= tol + 0.5, (r, e_ex, tol)
+        assert e_ex <= tol, (r, e_ex, tol)
         print(line + f" {d_s2[r]:18.3e}")
+    print(f"largest count error {worst:.3f} against tolerance {tol:.2f}")
     assert f[0] == curve[0] and float(d_s2.max()) <= 1e-5, d_s2
 
 
@@ -103,45 +122,170 @@ def check_mc(vol, occ, r_max, offs, rs, exact, total, seed):
         n = hv[r, 1]
         z.append((hv[r, 0] / n - p) / math.sqrt(p * (1 - p) / n))
     z = np.abs(np.array(z))
-    print(f"MC {per_r} samples per radius: max |z| = {z.max():.2f}, beyond 2 sigma: {np.mean(z > 2):.3f}")
-    # family-wise rule over r_max radii: all within 3.5 sigma, at most a quarter beyond 2 sigma
-    assert np.all(z < 3.5), z
-    assert np.mean(z > 2.0) <= 0.25, z
+    print(f"MC {per_r} samples per radius: max |z| = {z.max():.2f}, beyond 2 sigma: {int(np.sum(z > 2))}")
+    mc_family_rule(z)
 
+
+# ------------------------------------------------------------------ C1: plain measure
+
+def test_plain_measure_128_icospheres(ctx):
+    mesh, bbox, parts = synth.config_plain(seed=1, n=64, grid=128)
+    assert len(parts) == 64 and len(mesh.faces) == 64 * 1280
+    vol = s2.voxelize(ctx, mesh, bbox, 1.0)
+    occ = vol.to_bytes()
+    want = o.voxelize(mesh.vertices, mesh.faces, [0.0, 0.0, 0.0], 1.0, [128, 128, 128])
+    assert np.array_equal(occ, want), int(np.sum(occ != want))
+    ref = o.exact_s2(want, 32, 1.0)
+    auto = vol.exact_s2(32)
+    kernel = ctx.exact_tune_info([128] * 3, 32, 1.0)["kernel"]
+    assert np.array_equal(auto, ref), kernel
+    _, direct, ran = kernel_counts(ctx, vol, 32, "direct")
+    assert ran == "direct" and np.array_equal(direct, ref)
+    print(f"\nC1: {int(want.sum())} occupied (VF {want.mean():.4f}), model's kernel {kernel}; curves equal the oracle")
+    vol.free()
+
+
+# ------------------------------------------------------------------ C2: single large measure
 
 @pytest.fixture(scope="module")
 def large(ctx):
     mesh, bbox, _ = synth.config_large(seed=2, n=500, grid=256)
     vol = s2.voxelize(ctx, mesh, bbox, 1.0)
     occ = vol.to_bytes()
-    yield vol, occ
+    offs, rs = s2.shell_order_offsets(occ.shape, 64, 1.0)
+    want = o.hits_list(o.pack(occ), occ.shape, offs)   # every offset, the oracle's packed direct kernel
+    yield vol, occ, offs, rs, want
     vol.free()
 
 
-def test_single_large_measure_exact(ctx, large):
-    vol, occ = large
-    check_exact(vol, occ, 64, 2000, seed=1)
+def test_single_large_measure_all_offsets_both_kernels(ctx, large):
+    vol, occ, offs, rs, want = large
+    assert len(offs) == 1123772
+    vf = occ.sum() / occ.size
+    ref_curve = o.curve_from_hits(occ.shape, 64, offs, rs, want, vf)
+    curves = {}
+    for method in ("fft64", "direct"):
+        got, curve, ran = kernel_counts(ctx, vol, 64, method)
+        assert ran == method
+        bad = np.flatnonzero(got != want)
+        assert len(bad) == 0, (method, len(bad), offs[bad[:5]].tolist())
+        assert np.array_equal(curve, ref_curve), method
+        curves[method] = curve
+    auto = vol.exact_s2(64)
+    assert ctx.exact_tune_info([256] * 3, 64, 1.0)["kernel"] == "fft64"
+    assert np.array_equal(auto, ref_curve)
+    twin = index_of(offs)
+    assert np.all(twin >= 0) and np.array_equal(want, want[twin])
+    valid = np.prod(np.array(occ.shape, np.int64)[None, :] - np.abs(offs), axis=1)
+    assert np.all(want <= valid.astype(np.uint64))
+    print(f"\nC2: all {len(offs)} offsets equal the oracle for fft64 and direct; curves equal")
 
 
 def test_single_large_measure_fft32_per_bin(ctx, large):
-    vol, occ = large
-    exact = vol.exact_counts(64)
-    offs, rs = s2.shell_order_offsets(occ.shape, 64, 1.0)
-    check_fft32(vol, occ, 64, offs, rs, exact, vol.exact_s2(64), with_fp64=True)
+    vol, occ, offs, rs, want = large
+    check_fft32(vol, occ, 64, offs, rs, want, vol.exact_s2(64), with_fp64=True)
 
 
 def test_single_large_measure_monte_carlo(ctx, large):
-    vol, occ = large
-    exact = vol.exact_counts(64)
-    offs, rs = s2.shell_order_offsets(occ.shape, 64, 1.0)
-    check_mc(vol, occ, 64, offs, rs, exact, 10_000_000, seed=7)
+    vol, occ, offs, rs, want = large
+    check_mc(vol, occ, 64, offs, rs, want, 10_000_000, seed=7)
 
+
+# ------------------------------------------------------------------ C3: batched optimize step
+
+def test_batched_step_256_candidates(ctx):
+    packings, moves = synth.config_batched(seed=3, ncand=256)
+    seeds = [batch.Seed(ctx, [p.mesh() for p in ps], s2.BoundingBox.from_size((g, g, g)), 1.0, g // 4)
+             for g, ps in packings]
+    try:
+        curves = batch.batch_eval(ctx, seeds, moves)
+        assert len(curves) == 256
+        checked = set()
+        for k, ((si, pi, newv), got) in enumerate(zip(moves, curves)):
+            sd = seeds[si]
+            mesh = sd.moved_mesh(pi, newv)
+            vol = s2.voxelize(ctx, mesh, sd.bbox, 1.0)
+            plain = vol.exact_s2(sd.r_max)
+            assert np.array_equal(got, plain), (k, si, pi)
+            if si not in checked:   # first candidate of each grid: against the oracle
+                occ = vol.to_bytes()
+                g = sd.dims[0]
+                want = o.voxelize(mesh.vertices, mesh.faces, [0.0, 0.0, 0.0], 1.0, [g, g, g])
+                assert np.array_equal(occ, want), (k, int(np.sum(occ != want)))
+                assert np.array_equal(got, o.exact_s2(want, sd.r_max, 1.0)), k
+                checked.add(si)
+            vol.free()
+        assert checked == {0, 1, 2, 3}
+        print(f"\nC3: 256 batched curves equal the plain path; grids {[g for g, _ in packings]} "
+              f"with {[len(p) for _, p in packings]} particles; one per grid equals the oracle")
+    finally:
+        for sd in seeds:
+            sd.free()
+
+
+# ------------------------------------------------------------------ C4: served stream
+
+def test_served_stream_mix(ctx):
+    pool = serve.volume_pool(8, seed=9)
+    sizes = sorted(v.dims[0] for v in pool)
+    assert sizes[0] >= 64 and sizes[-1] <= 384
+    sched = serve.poisson_schedule(200.0, 0.25, pool, seed=9)
+    assert len(sched) >= 30
+    with serve.Server(slots=4, budget_bytes=16 << 30) as sv:
+        tickets = []
+        for a in sched:
+            v = pool[a.volume]
+            tickets.append(sv.submit(v.words, v.dims, a.r_max, a.method))
+        results = [sv.poll(t, wait=True)[1] for t in tickets]
+        st = sv.stats()
+    assert st["failed"] == 0 and st["completed"] == len(sched)
+    vols = {}
+    oracle_checked = 0
+    done = {}
+    for a, got in zip(sched, results):
+        key = (a.volume, a.r_max, a.method)
+        if key not in done:
+            v = pool[a.volume]
+            if a.volume not in vols:
+                vols[a.volume] = s2.Volume.from_packed(ctx, v.words, v.dims)
+            vol = vols[a.volume]
+            done[key] = vol.exact_s2(a.r_max) if a.method == serve.EXACT else vol.fft32_s2(a.r_max)
+            if a.method == serve.EXACT and v.dims[0] <= 192:
+                ref = o.exact_s2_packed(v.words, v.dims, int(np.sum(np.unpackbits(v.words.view(np.uint8)))),
+                                        a.r_max, 1.0)
+                assert np.array_equal(done[key], ref), key
+                oracle_checked += 1
+        assert np.array_equal(got, done[key]), key
+    for vol in vols.values():
+        vol.free()
+    assert oracle_checked >= 1
+    print(f"\nC4: {len(sched)} requests over grids {sizes}; every result equals the plain entry point; "
+          f"{oracle_checked} exact (volume, r_max) pairs on grids <= 192^3 equal the oracle")
+
+
+# ------------------------------------------------------------------ C5: CT-like volume
 
 def test_ct_like_volume_768(ctx):
     occ = synth.grf_volume(768, 12.0, 0.35, seed=5)
     assert abs(occ.mean() - 0.35) < 1e-4
     vol = s2.Volume.from_bytes(ctx, occ)
-    exact, offs, rs, curve = check_exact(vol, occ, 128, 300, seed=2)
+    offs, rs = s2.shell_order_offsets(occ.shape, 128, 1.0)
     assert len(offs) == 8887792
+    pick = np.arange(0, len(offs), 997)
+    want = o.hits_list(o.pack(occ), occ.shape, offs[pick])
+    counts = {}
+    for method in ("fft64", "direct"):
+        got, curve, ran = kernel_counts(ctx, vol, 128, method)
+        assert ran == method
+        assert np.array_equal(got[pick], want), method
+        counts[method] = (got, curve)
+    # both kernels agree on every offset and give the same curve bits
+    assert np.array_equal(counts["fft64"][0], counts["direct"][0])
+    assert counts["fft64"][1].tobytes() == counts["direct"][1].tobytes()
+    exact, curve = counts["fft64"]
+    vf = occ.sum() / occ.size
+    assert np.array_equal(curve, o.curve_from_hits(occ.shape, 128, offs, rs, exact, vf))
+    print(f"\nC5: {len(pick)} offsets (every 997th) equal the oracle for fft64 and direct; "
+          f"all {len(offs)} agree between the kernels")
     check_fft32(vol, occ, 128, offs, rs, exact, curve, with_fp64=False)
     vol.free()

@@ -156,3 +156,26 @@ def test_valu_probe_is_plausible(ctx):
     # 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s at one 64-lane VALU op per 4 cycles per SIMD
     rate = ctx.valu_probe()
     assert 5e12 < rate < 200e12, rate
+
+
+def test_u32_lane_counters_cannot_wrap():
+    # a solid 4096 x 2048 x 1024 volume at r_max = 1 with one x-chunk per
+    # column block asked for (waves = 1): a single item would add
+    # 4096 * 2047 * 1023 > 2^32 pairs into one lane's u32 counter, so the
+    # plan splits x until every item's bound fits; counts are the analytic
+    # valid-pair numbers of a solid grid
+    dims = (4096, 2048, 1024)
+    rw = s2.row_words(dims[2])
+    row = np.zeros(rw, np.uint64)
+    row[: dims[2] // 64] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    with s2.Context(0) as c:
+        c.set_exact_method("direct")
+        c.set_autotune(False)
+        vol = s2.Volume.from_packed(c, np.tile(row, dims[0] * dims[1]), dims)
+        c.exact_tune_set(dims, 1, 1.0, 1, 1)
+        got = vol.exact_counts(1)
+        offs, _ = s2.shell_order_offsets(dims, 1, 1.0)
+        want = np.prod(np.array(dims, np.int64)[None, :] - np.abs(offs), axis=1)
+        assert np.array_equal(got, want.astype(np.uint64)), (got, want)
+        assert c.exact_tune_info(dims, 1, 1.0)["items"] >= 2
+        vol.free()

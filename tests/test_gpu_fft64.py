@@ -101,3 +101,23 @@ def test_fft64_refused_where_rounding_is_not_provably_exact(ctx, method):
     offs, _ = s2.shell_order_offsets((n, n, n), 1, 1.0)
     assert np.array_equal(c, np.prod(n - np.abs(offs), axis=1))  # solid: every pair hits
     vol.free()
+
+
+def test_budgeted_context_counts_directly(ctx, method):
+    # the model's FFT choice keeps to the context's device budget: at 256^3
+    # r40 the FP64 spectra (about 0.9 GB) do not fit 256 MiB, so a budgeted
+    # context counts directly, as the reference's planner keeps to its budget
+    # (src/geometry/s2.rs:733-766); without a budget the same shape takes the FFT
+    occ = synth.grf_volume(256, 6.0, 0.3, seed=31)
+    ref = o.exact_s2(occ, 40)
+    method("auto")
+    vol = s2.Volume.from_bytes(ctx, occ)
+    assert np.array_equal(vol.exact_s2(40), ref)
+    assert ctx.exact_tune_info(occ.shape, 40, 1.0)["kernel"] == "fft64"
+    vol.free()
+    with s2.Context(0, 256 << 20) as small:
+        v2 = s2.Volume.from_bytes(small, occ)
+        assert np.array_equal(v2.exact_s2(40), ref)
+        assert small.exact_tune_info(occ.shape, 40, 1.0)["kernel"] == "direct"
+        assert small.bytes_in_use <= 256 << 20
+        v2.free()

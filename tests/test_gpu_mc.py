@@ -1,11 +1,13 @@
 """Voxel Monte Carlo on the GPU: integer counts equal the oracle's Philox stream
-exactly, and estimates fall within 3 sigma of the exact estimand
+exactly, and estimates meet the family-wise 3-sigma rule (stats_rules) against
+the exact estimand
 sum_d C(d) / sum_d valid(d) over each full shell.  Requires the GPU."""
 import numpy as np
 import pytest
 
 from oracle import oracle as o
 from rustmspt_amd import s2, synth
+from stats_rules import mc_family_rule
 
 pytestmark = pytest.mark.gpu
 
@@ -38,10 +40,8 @@ def test_within_three_sigma_of_exact_estimand(ctx):
         est = hv[r, 0] / n_valid
         sigma = np.sqrt(p * (1 - p) / n_valid)
         z.append((est - p) / sigma)
-    z = np.abs(np.array(z))
-    # family-wise rule: every radius within 3.5 sigma, and at most a quarter beyond 2 sigma
-    assert np.all(z < 3.5), z
-    assert np.mean(z > 2.0) <= 0.25, z
+    print(f"max |z| = {np.max(np.abs(z)):.2f}")
+    mc_family_rule(z)
 
 
 def test_curve_shape(ctx):
