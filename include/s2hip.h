@@ -417,6 +417,20 @@ s2h_status s2h_fill_missing(double *values, const uint8_t *has_support, uint64_t
 s2h_status s2h_stl_load(const char *path, double **xyz, uint64_t *nverts, uint32_t **tri,
                         uint64_t *ntris);
 void s2h_free(void *p);
+/* Volume of a closed mesh inside a box (particle_volume_in_bbox,
+ * src/geometry/volume.rs:424-430): the mesh volume (mesh_volume, 6-15) when every
+ * vertex lies in the box, else the volume of the mesh clipped by the six box
+ * planes with each cut capped (clip_mesh_by_bbox, 352-421).  Host only. */
+s2h_status s2h_mesh_volume_in_bbox(const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,
+                                   const double bbox_min[3], const double bbox_max[3], double *volume);
+/* Geometric volume fraction in the box (volume_fraction_in_bbox,
+ * src/geometry/volume.rs:437-469): the granules (reference BFS order) each
+ * measured by s2h_mesh_volume_in_bbox, summed in granule order, over the box
+ * volume (at least 1e-12), clamped to [0, 1].  Bit-identical steps, so the
+ * measure report's "Volume Fraction:" and S2(0)-VF lines print the same.
+ * Host only. */
+s2h_status s2h_volume_fraction_in_bbox(const double *xyz, uint64_t nverts, const uint32_t *tri, uint64_t ntris,
+                                       const double bbox_min[3], const double bbox_max[3], double *vf);
 /* Connected components over shared vertices (split_mesh_into_granules,
  * src/geometry/mesh_ops.rs:48-~120): granule id per triangle, in the reference's
  * component order. */

@@ -87,6 +87,8 @@ SIGNATURES = {
     "s2h_stl_load": (C.c_int32, [C.c_char_p, C.POINTER(_f64p), _u64p, C.POINTER(_u32p), _u64p]),
     "s2h_free": (None, [_vp]),
     "s2h_mesh_granules": (C.c_int32, [_u32p, C.c_uint64, C.c_uint64, _u32p, _u32p]),
+    "s2h_mesh_volume_in_bbox": (C.c_int32, [_f64p, C.c_uint64, _u32p, C.c_uint64, _f64p, _f64p, _f64p]),
+    "s2h_volume_fraction_in_bbox": (C.c_int32, [_f64p, C.c_uint64, _u32p, C.c_uint64, _f64p, _f64p, _f64p]),
 }
 
 

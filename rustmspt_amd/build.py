@@ -38,7 +38,8 @@ UNITS = {
     "s2hip_meshin.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = {"s2hip_host.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"],
-              "s2hip_volio.cpp": ["-O2", "-std=c++17", "-fPIC"]}
+              "s2hip_volio.cpp": ["-O2", "-std=c++17", "-fPIC"],
+              "s2hip_volfrac.cpp": ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off"]}
 
 
 def _run(cmd):

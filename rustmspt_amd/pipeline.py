@@ -34,18 +34,6 @@ StageTimer = report.StageTimer
 l2_error = report.l2_error
 
 
-def volume_fraction_in_bbox(mesh, bbox: BoundingBox, occupancy_vf: float):
-    """src/geometry/volume.rs:437-469 for granules wholly inside the box
-    (s2.geometric_volume_fraction); a granule crossing the box would need the
-    reference's clipping, which is outside this engine -- the occupancy VF is
-    reported instead, with a warning."""
-    vf = s2.geometric_volume_fraction(mesh, bbox)
-    if vf is None:
-        print("[Warning] particles cross the measurement box; Volume Fraction reports the occupancy fraction")
-        return occupancy_vf
-    return vf
-
-
 class MeasurePipeline:
     def __init__(self, params: cfg.MeasurementParams, env=None, out=sys.stdout):
         self.p = params
@@ -87,7 +75,7 @@ class MeasurePipeline:
             particle_count = dmesh.ngranules
         else:
             mesh = load_stl_or_merge_folder(p.stl_path)
-            _, particle_count = granules(mesh)
+            particle_count = None
         timer.stage("load")
         bbox = None
         for box in (p.bounding_box, p.stl_bounding_box):
@@ -99,6 +87,12 @@ class MeasurePipeline:
         self.log(f"[Info] STL file(s) loaded from: {p.stl_path}")
         self.log("[Info] Measurement bbox: min=({:.4f},{:.4f},{:.4f}), max=({:.4f},{:.4f},{:.4f})".format(
             *bbox.min, *bbox.max))
+        # particle count and the geometric volume fraction in the box (granules
+        # crossing it clipped), before any S2 work, as the reference's split_vf stage
+        if particle_count is None:
+            _, particle_count = granules(mesh)
+        vf = s2.geometric_volume_fraction(mesh, bbox)
+        timer.stage("split_vf")
         raw = p.mc_method.strip().lower()
         method = "exact" if raw == "exact" else ("both" if raw == "both" else ("fft32" if raw == "fft32" else
                                                                               "monte_carlo"))
@@ -116,7 +110,6 @@ class MeasurePipeline:
             voxel_count *= int(n)
         methods = ["exact", "monte_carlo"] if method == "both" else [method]
         curves, backends, grid = [], [], None
-        vf_geo = s2.geometric_volume_fraction(mesh, bbox)
         timer.restart()
         for m in methods:
             exactish = m in ("exact", "fft32")
@@ -139,12 +132,9 @@ class MeasurePipeline:
                 ctx = res["ctx"] = s2.Context(0, budget)
             seed = int(self.env.get("RUSTMSPT_S2_SEED", "0"))
             if mesh_mc:
-                if vf_geo is None:
-                    raise cfg.ConfigError("continuous mesh MC needs the geometric volume fraction, which is "
-                                          "defined here only for particles wholly inside the measurement box")
                 hv = s2.mesh_mc_counts(ctx, mesh, bbox, p.r_max, samples, seed).astype(np.float64)
-                curve = np.array([vf_geo] + [hv[r, 0] / hv[r, 1] if hv[r, 1] else 0.0
-                                             for r in range(1, p.r_max + 1)])
+                curve = np.array([vf] + [hv[r, 0] / hv[r, 1] if hv[r, 1] else 0.0
+                                         for r in range(1, p.r_max + 1)])
             else:
                 if grid is None:
                     self.log("[Info] Preparing shared HIP S2 occupancy grid")
@@ -158,11 +148,6 @@ class MeasurePipeline:
             self.log(f"[Info] Measure execution: method={m}, backend=hip, device={ctx.device_name}")
             curves.append(curve)
             timer.stage(f"s2_{m}")
-        if vf_geo is not None:
-            vf = vf_geo
-        else:
-            occ_vf = grid.volume.occupied / float(np.prod(grid.dims)) if grid else 0.0
-            vf = volume_fraction_in_bbox(mesh, bbox, occ_vf)
         out = Path(p.output_path)
         out.parent.mkdir(parents=True, exist_ok=True)
         out.write_text(report.measure_report(vf, backends, method, methods, curves))

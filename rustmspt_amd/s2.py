@@ -292,29 +292,35 @@ class VoxelS2Hip:
         return self.volume.mc_s2(r_max, self.pitch, max(int(samples), 200), 0 if seed is None else int(seed))
 
 
-def geometric_volume_fraction(mesh: Mesh, bbox: BoundingBox) -> Optional[float]:
-    """volume_fraction_in_bbox (src/geometry/volume.rs:437-469) when every granule
-    lies wholly inside the box: per granule |sum of a . (b x c) / 6| (mesh_volume,
-    src/geometry/volume.rs:6-15, summed in face order), granules summed in order,
-    over the box volume, clamped to [0, 1].  A granule crossing the box needs the
-    reference's mesh clipping, which this engine does not restate: None."""
-    v = np.asarray(mesh.vertices, np.float64)
-    lo, hi = np.asarray(bbox.min, np.float64), np.asarray(bbox.max, np.float64)
-    box_vol = max(float(np.prod(hi - lo)), 1e-12)
-    if len(mesh.faces) == 0:
-        return 0.0
-    if not np.all((v[mesh.faces.reshape(-1)] >= lo) & (v[mesh.faces.reshape(-1)] <= hi)):
-        return None
-    gid, n = granules(mesh)
-    a, b, c = (v[mesh.faces[:, k]] for k in range(3))
-    cross = np.stack([b[:, 1] * c[:, 2] - b[:, 2] * c[:, 1], b[:, 2] * c[:, 0] - b[:, 0] * c[:, 2],
-                      b[:, 0] * c[:, 1] - b[:, 1] * c[:, 0]], 1)
-    tet = ((a[:, 0] * cross[:, 0] + a[:, 1] * cross[:, 1]) + a[:, 2] * cross[:, 2]) / 6.0
-    total = 0.0
-    for g in range(n):
-        part = tet[gid == g]
-        total += abs(float(np.cumsum(part)[-1])) if len(part) else 0.0  # sequential, in face order
-    return min(max(total / box_vol, 0.0), 1.0)
+def _mesh_args(mesh: Mesh, bbox: BoundingBox):
+    v = np.ascontiguousarray(mesh.vertices, np.float64).reshape(-1, 3)
+    f = np.ascontiguousarray(mesh.faces, np.uint32).reshape(-1, 3)
+    lo = np.ascontiguousarray(bbox.min, np.float64)
+    hi = np.ascontiguousarray(bbox.max, np.float64)
+    return v, f, lo, hi
+
+
+def geometric_volume_fraction(mesh: Mesh, bbox: BoundingBox) -> float:
+    """volume_fraction_in_bbox (src/geometry/volume.rs:437-469): every granule's
+    in-box volume (its closed-mesh volume when it lies wholly inside the box,
+    else the volume of the granule clipped by the box planes with capped cuts),
+    summed in granule order, over the box volume, clamped to [0, 1]
+    (s2h_volume_fraction_in_bbox, host C++)."""
+    v, f, lo, hi = _mesh_args(mesh, bbox)
+    out = C.c_double()
+    L.check(L.lib().s2h_volume_fraction_in_bbox(L.ptr(v.reshape(-1), L._f64p), len(v), L.ptr(f.reshape(-1), L._u32p),
+                                                len(f), L.ptr(lo, L._f64p), L.ptr(hi, L._f64p), C.byref(out)))
+    return float(out.value)
+
+
+def mesh_volume_in_bbox(mesh: Mesh, bbox: BoundingBox) -> float:
+    """particle_volume_in_bbox (src/geometry/volume.rs:424-430) of one closed
+    mesh (s2h_mesh_volume_in_bbox)."""
+    v, f, lo, hi = _mesh_args(mesh, bbox)
+    out = C.c_double()
+    L.check(L.lib().s2h_mesh_volume_in_bbox(L.ptr(v.reshape(-1), L._f64p), len(v), L.ptr(f.reshape(-1), L._u32p),
+                                            len(f), L.ptr(lo, L._f64p), L.ptr(hi, L._f64p), C.byref(out)))
+    return float(out.value)
 
 
 def ordered_sum(ctx: Context, terms, starts) -> np.ndarray:
@@ -344,9 +350,6 @@ def mesh_mc_s2(ctx: Context, mesh: Mesh, bbox: BoundingBox, r_max: int, samples:
     """calculate_s2_mesh_mc_seeded (src/geometry/s2.rs:1273-1320): S2(0) is the
     geometric volume fraction in the box, S2(r) = hits / valid."""
     vf = geometric_volume_fraction(mesh, bbox)
-    if vf is None:
-        raise L.S2HipError(5, "mesh Monte Carlo: a granule crosses the measurement box (its clipped volume "
-                              "is not computed by this engine)")
     out = np.zeros(int(r_max) + 1, np.float64)
     v = np.ascontiguousarray(mesh.vertices, np.float64)
     f = np.ascontiguousarray(mesh.faces, np.uint32)

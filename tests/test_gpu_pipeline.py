@@ -88,3 +88,20 @@ def test_monte_carlo_with_pitch_zero_is_mesh_mc(tmp_path):
     assert f"Volume Fraction: {want[0]:.6f}" in text
     for r, v in enumerate(want):
         assert f"{r}: {v:.6f}" in text
+
+
+def test_boxes_crossing_the_region_report_clipped_volume_fraction(tmp_path):
+    # two outward-wound boxes cut by the 4^3 measurement box: in-box volumes
+    # 3.2 x 2.1 x 1.3 = 8.736 and 1.4 x 0.9 x 1.7 = 2.142, so the geometric
+    # volume fraction is 10.878 / 64 = 0.16996875 (volume_fraction_in_bbox,
+    # src/geometry/volume.rs:437-469); the voxel centres they cover at pitch 1
+    # are 3 x 2 x 1 and 1 x 1 x 2, so S2(0) = 8 / 64 = 0.125 and the
+    # S2(0)-VF difference is 0.04496875
+    def outward(lo, hi):
+        m = M.box_mesh(lo, hi)
+        return M.Mesh(m.vertices, m.faces[:, [0, 2, 1]].copy())
+    mesh = M.merge_meshes([outward([-1.0, 0.6, 0.9], [3.2, 2.7, 2.2]), outward([2.6, 3.1, -0.5], [5.0, 4.7, 1.7])])
+    text = run(tmp_path, mesh, r_max=1, voxel_pitch=1.0, bounding_box=[4, 4, 4])
+    assert text.startswith("Volume Fraction: 0.169969\n"), text
+    assert "S2(0)-VF diff [exact]: 0.044969\n" in text
+    assert "0: 0.125000\n" in text
