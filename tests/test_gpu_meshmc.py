@@ -74,9 +74,13 @@ def test_calculate_s2_routes_pitch_zero_to_mesh_mc(ctx):
     assert curve[0] == s2.geometric_volume_fraction(mesh, bbox)
     for r in range(1, 7):
         assert curve[r] == hv[r, 0] / hv[r, 1]
+    # a granule crossing the box: S2(0) is its clipped in-box volume fraction
+    # (volume_fraction_in_bbox clips it, src/geometry/volume.rs:424-430)
     crossing = spheres([(1, 10, 10)], [4.0])
-    with pytest.raises(Exception):
-        s2.mesh_mc_s2(ctx, crossing, bbox, 4, 1000, 1)
+    c = s2.mesh_mc_s2(ctx, crossing, bbox, 4, 1000, 1)
+    whole = s2.mesh_volume_in_bbox(crossing, BoundingBox(np.full(3, -50.0), np.full(3, 50.0)))
+    assert c[0] == s2.geometric_volume_fraction(crossing, bbox)
+    assert 0.5 * whole / 24 ** 3 < c[0] < 0.75 * whole / 24 ** 3
 
 
 def test_packing_against_voxel_exact(ctx):
