@@ -286,6 +286,16 @@ s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
  * 2*(r_max+1) entries, (hits, valid) per radius, radius 0 zero. */
 s2h_status s2h_mc_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch,
                          uint64_t samples, uint64_t seed, uint64_t *hits_valid, uint64_t n);
+/* s2h_mc_counts without any device allocation -- no counterpart in the reference; for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and the same counts bit for bit.  Everything the kernels need beyond
+ * the caller's volume (the shell table, built on the device, the radius starts and the counters)
+ * lives in one 32 MiB file-scope device array per GPU, held until the process ends.  The table
+ * holds about 4.19 million offsets: r_max / pitch up to about 96 voxels; larger shells (and
+ * r_max > 4096) are refused with S2H_ERR_BUDGET before any launch.  Calls on one GPU share the
+ * array and keep the last call's contents: calls that overlap on one GPU race. */
+s2h_status s2h_mc_counts_noalloc(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch,
+                                 uint64_t samples, uint64_t seed, uint64_t *hits_valid, uint64_t n);
 /* Finished MC curve (VoxelS2::calculate_seeded "monte_carlo", src/geometry/s2.rs:1521-1537):
  * hits/valid per radius, unsupported radii filled, S2(0)=vf. samples is per radius
  * and is raised to 200 as in the reference. */

@@ -71,6 +71,8 @@ SIGNATURES = {
     "s2h_fft32_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, _f32p, C.c_uint64]),
     "s2h_fft32_s2": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, _f64p, C.c_uint64]),
     "s2h_mc_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _u64p, C.c_uint64]),
+    "s2h_mc_counts_noalloc": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _u64p,
+                                          C.c_uint64]),
     "s2h_mc_s2": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _f64p, C.c_uint64]),
     "s2h_fill_missing": (C.c_int32, [_f64p, _u8p, C.c_uint64, C.c_double]),
     "s2h_serve_create": (C.c_int32, [C.c_int, C.c_uint64, C.c_uint32, C.POINTER(_vp)]),

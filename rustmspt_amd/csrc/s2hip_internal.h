@@ -480,6 +480,9 @@ void fft_plans_release(s2h_ctx *ctx);
 s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan, const double **d_values);
 // Copy folded curve values and supported flags back and finish the curve.
 s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *plan, double vf, double *s2);
+// Per-device address of the no-allocation entry point's scratch array
+// (s2hip_mc.hip); the caller has made `device` current.
+s2h_status noalloc_scratch_init(int device);
 // Profiling marks: k = 0 (before counts), 1 (after counts), 2 (after fold).
 void prof_mark(s2h_ctx *ctx, int k);
 void prof_collect(s2h_ctx *ctx);

@@ -396,6 +396,12 @@ s2h_status s2h_create(int device, uint64_t hbm_budget_bytes, s2h_ctx **out) {
         delete ctx;
         return fail(S2H_ERR_HIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
     }
+    const s2h_status st = noalloc_scratch_init(device);
+    if (st != S2H_OK) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return st;
+    }
     *out = ctx;
     return S2H_OK;
 }
