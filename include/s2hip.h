@@ -270,6 +270,14 @@ s2h_status s2h_set_autotune(s2h_ctx *ctx, int32_t enable);
 
 /* ------------------------------------------------------------------ FFT (FP32) */
 
+/* Padded transform lengths of the FFT kernels (FP64 exact and FP32) for this
+ * shape -- no counterpart in the reference, which pads each axis to the
+ * smallest 2,3,5-smooth length >= 2n - 1 (padded_fft_dims, src/geometry/s2.rs:554-562).
+ * Only offsets |d_a| <= R_a (the largest of the shells) are read, and a length
+ * f >= n + R leaves those exact, so each axis takes the 2,3,5-smooth length in
+ * [n + R, the reference's] with the least modelled work.  Host only. */
+s2h_status s2h_fft_padded_dims(const uint64_t dims[3], uint32_t r_max, double pitch, uint64_t padded[3]);
+
 /* FP32 real-FFT autocorrelation counts of every in-domain offset in shell order
  * (the FFT exact kernel's correlation, src/geometry/s2.rs:965-1012, 1219-1261, in
  * FP32 and NOT rounded: values carry FP32 round-off). */

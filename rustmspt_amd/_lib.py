@@ -68,6 +68,7 @@ SIGNATURES = {
     "s2h_exact_tune_set": (C.c_int32, [_vp, _u64p, C.c_uint32, C.c_double, C.c_int32, C.c_int32]),
     "s2h_set_autotune": (C.c_int32, [_vp, C.c_int32]),
     "s2h_exact_method_set": (C.c_int32, [_vp, C.c_int32]),
+    "s2h_fft_padded_dims": (C.c_int32, [_u64p, C.c_uint32, C.c_double, _u64p]),
     "s2h_fft32_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, _f32p, C.c_uint64]),
     "s2h_fft32_s2": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, _f64p, C.c_uint64]),
     "s2h_mc_counts": (C.c_int32, [_vp, _vp, C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, _u64p, C.c_uint64]),

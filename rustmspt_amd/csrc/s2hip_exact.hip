@@ -707,9 +707,10 @@ static constexpr double kFlushBytesPerS = 3.5e12;
 
 // Cost model: executed VALU wave-instructions at the issue rate of 1024 SIMDs
 // (4 cycles each), scaled down when there are too few waves to hide latency
-// and for the partly filled last round of workgroups (r rounds of resident
-// workgroups lose about half a round: r / (r + 1/2)), bounded below by the
-// longest item, plus the items' counter flushes.
+// and for the partly filled last round of workgroups (r >= 1 rounds of
+// resident workgroups lose about half a round: r / (r + 1/2); less than one
+// round is covered by the fill term), bounded below by the longest item, plus
+// the items' counter flushes.
 static double model_ms(const Grouping &g, int waves, double *ops_out) {
     double items = 0, maxitem = 0, flush_bytes = 0;
     for (const Grouping::Blk &b : g.blks) {
@@ -723,7 +724,7 @@ static double model_ms(const Grouping &g, int waves, double *ops_out) {
     const double simds = 1024.0, clk = 2.4e9;
     const double fill = std::min(1.0, items / (simds * 6.0));
     const double rounds = items / kWavesPerBlock / kBlockSlots;
-    const double tail = rounds / (rounds + 0.5);
+    const double tail = rounds >= 1.0 ? rounds / (rounds + 0.5) : 1.0;  // a single partial round loses nothing
     return 1e3 * (std::max(ops * 4.0 / (simds * clk * std::max(fill, 0.05) * tail), maxitem * 4.0 * 2.0 / clk) +
                   flush_bytes / kFlushBytesPerS);
 }
@@ -1005,6 +1006,11 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, cons
     return S2H_OK;
 }
 
+void shell_extent(const uint64_t dims[3], uint32_t r_max, double pitch, int R[3]) {
+    const std::shared_ptr<const ShellBase> b = shell_base(dims, r_max, pitch < 1e-9 ? 1e-9 : pitch);
+    for (int a = 0; a < 3; ++a) R[a] = b->extent[a];
+}
+
 // FP64 passes modelled at this share of the HBM peak over their algorithmic
 // bytes, plus launch and gather time.
 static constexpr double kFft64BytesPerS = 5.0e12;
@@ -1021,14 +1027,14 @@ static constexpr double kFft64MaxBound = 0.25;  // required worst-case error (ro
 static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, uint64_t occupied,
                                     const double **d_dvals) {
     *d_dvals = nullptr;
-    const bool fft_ok = fft64_supported(p->dims) && fft64_error_bound(p->dims, occupied) <= kFft64MaxBound;
+    const int *ext = p->shells->host->extent;
+    const int R[3] = {ext[0], ext[1], ext[2]};
+    const bool fft_ok = fft64_supported(p->dims, R) && fft64_error_bound(p->dims, R, occupied) <= kFft64MaxBound;
     int m = ctx->exact_method;
     if (m == 2 && !fft_ok)
         return fail(S2H_ERR_UNSUPPORTED, "FP64 FFT counts are not provably exact for this volume (error bound " +
-                                             std::to_string(fft64_error_bound(p->dims, occupied)) + ")");
+                                             std::to_string(fft64_error_bound(p->dims, R, occupied)) + ")");
     if (m == 0) {
-        const int *ext = p->shells->host->extent;
-        const int R[3] = {ext[0], ext[1], ext[2]};
         const double plan_bytes = fft64_plan_bytes(p->dims, R);
         bool fits = !ctx->fft_budget || plan_bytes <= (double)ctx->fft_budget;
         if (ctx->budget) {
@@ -1037,7 +1043,7 @@ static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, ui
             const double need = (fft64_plan_cached(ctx, p->dims, R) ? 0.0 : plan_bytes) + 8.0 * p->offs.size();
             fits = fits && (double)ctx->in_use + need <= (double)ctx->budget;
         }
-        const double fft_ms = fft64_model_bytes(p->dims) / kFft64BytesPerS * 1e3 + kFft64FixedMs;
+        const double fft_ms = fft64_model_bytes(p->dims, R) / kFft64BytesPerS * 1e3 + kFft64FixedMs;
         m = fft_ok && fits && fft_ms < p->model_ms ? 2 : 1;
         if (m == 2) {
             p->method = 2;

@@ -8,7 +8,10 @@
 // values are NOT rounded: this path reports its error against an FP64 FFT and
 // the exact integer counts instead of claiming exactness.
 //
-// Layout and passes (H = fz/2 + 1, R_a = largest |d_a| of any shell offset):
+// Layout and passes (H = fz/2 + 1, R_a = largest |d_a| of any shell offset;
+// spectrum rows are P >= H values apart, P a multiple of 128 bytes, so every
+// row starts on a 128-byte line and a tile of 128 bytes of kz columns reads and
+// writes whole lines):
 //   z-fwd : two real z rows per complex FFT (a + ib) read straight from the
 //           packed bits; DIF in LDS; split into the two half spectra
 //                                                  -> G1[nx][fy][H]  (y < ny)
@@ -34,7 +37,15 @@ namespace s2h {
 
 static constexpr int kMaxFactors = 24;
 static constexpr int kTileMax = 16;   // kz columns per workgroup in the y and x passes (max)
-static constexpr int kFftThreads = 256;
+static constexpr int kFftThreads = 256;      // z passes
+static constexpr int kFftMaxThreads = 1024;  // y and x passes: one first-stage butterfly per thread, up to this
+
+// Values between spectrum rows: H = f/2 + 1 rounded up to a whole number of
+// 128-byte lines.
+__host__ __device__ constexpr uint64_t spectrum_pitch(uint64_t f, int real_bytes) {
+    const uint64_t per_line = 128 / (2 * (uint64_t)real_bytes);
+    return (f / 2 + 1 + per_line - 1) / per_line * per_line;
+}
 
 struct AxisPlan {
     int L = 1;
@@ -472,7 +483,7 @@ __device__ __forceinline__ void z_first_stage(V *d, int L, int S, int sh, int np
 // ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
 template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restrict__ words, int nx, int ny, int nz,
-                                                         int row_words, int fy, int H, Fac fc, int L,
+                                                         int row_words, int fy, int H, int P, Fac fc, int L,
                                                          const C2<T> *__restrict__ tw, const int *__restrict__ pos,
                                                          int pairs_per_block, C2<T> *__restrict__ g1, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
@@ -518,14 +529,14 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
         const T half = (T)0.5;
         const V A = mk<V>(half * (zk.x + zn.x), half * (zk.y - zn.y));
         const V B = mk<V>(half * (zk.y + zn.y), -half * (zk.x - zn.x));
-        g1[((uint64_t)x * fy + y) * H + k] = A;
-        if (y + 1 < ny) g1[((uint64_t)x * fy + y + 1) * H + k] = B;
+        g1[((uint64_t)x * fy + y) * P + k] = A;
+        if (y + 1 < ny) g1[((uint64_t)x * fy + y + 1) * P + k] = B;
     }
 }
 
 // ---- y forward: per (x, kz tile), rows y >= ny are zero; in place in G1
 template <class T>
-__global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, int tiles_kz, int tile, Fac fc,
+__global__ __launch_bounds__(kFftMaxThreads) void fft_y_fwd(int ny, int fy, int H, int P, int tiles_kz, int tile, Fac fc,
                                                          const C2<T> *__restrict__ tw, C2<T> *__restrict__ g1, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
@@ -534,22 +545,22 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     V *d = sm;
     const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
-    V *base = g1 + (uint64_t)x * fy * H + kz0;
+    V *base = g1 + (uint64_t)x * fy * P + kz0;
     if (fc.n >= 2) {
-        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)H, ny, tw)));
+        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P, ny, tw)));
         int m = L / fc.f[0];
         for (int s = 1; s + 1 < fc.n; ++s) {
             fft_stage_any<false>(d, L, S, sh, nk, fc.f[s], m, tw);
             m /= fc.f[s];
         }
-        S2H_RADIX_SWITCH(fc.f[fc.n - 1], (store_last_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)H)));
+        S2H_RADIX_SWITCH(fc.f[fc.n - 1], (store_last_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P)));
         return;
     }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         int y, c;
         divmod(t, tile, lgt, y, c);
         V v = mk<V>(0, 0);
-        if (y < ny && c < nk) v = base[(uint64_t)y * H + c];
+        if (y < ny && c < nk) v = base[(uint64_t)y * P + c];
         d[c * S + pidx(y, sh)] = v;
     }
     __syncthreads();
@@ -557,7 +568,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_fwd(int ny, int fy, int H, 
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         int y, c;
         divmod(t, tile, lgt, y, c);
-        if (c < nk) base[(uint64_t)y * H + c] = d[c * S + pidx(y, sh)];
+        if (c < nk) base[(uint64_t)y * P + c] = d[c * S + pidx(y, sh)];
     }
 }
 
@@ -585,7 +596,7 @@ __device__ void lds_power_inner(V *d, int L, int S, int sh, int lines, const Fac
 
 // ---- x: forward, |F|^2, inverse; keep the needed x.  One block per (ky, kz tile).
 template <class T>
-__global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H, int tiles_kz, int tile, Fac fc,
+__global__ __launch_bounds__(kFftMaxThreads) void fft_x_power(int nx, int fy, int H, int P, int tiles_kz, int tile, Fac fc,
                                                            const C2<T> *__restrict__ tw, int Rx,
                                                            const C2<T> *__restrict__ g1, C2<T> *__restrict__ g2, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
@@ -597,11 +608,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
     const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
     if (fc.n >= 2) {
-        const uint64_t rs = (uint64_t)fy * H;
-        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, g1 + (uint64_t)ky * H + kz0, rs,
+        const uint64_t rs = (uint64_t)fy * P;
+        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, g1 + (uint64_t)ky * P + kz0, rs,
                                                        nx, tw)));
         lds_power_inner(d, L, S, sh, nk, fc, tw);
-        S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk, g2 + (uint64_t)ky * H + kz0, rs,
+        S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk, g2 + (uint64_t)ky * P + kz0, rs,
                                                        Rx, tw)));
         return;
     }
@@ -609,7 +620,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
         int x, c;
         divmod(t, tile, lgt, x, c);
         V v = mk<V>(0, 0);
-        if (x < nx && c < nk) v = g1[((uint64_t)x * fy + ky) * H + kz0 + c];
+        if (x < nx && c < nk) v = g1[((uint64_t)x * fy + ky) * P + kz0 + c];
         d[c * S + pidx(x, sh)] = v;
     }
     __syncthreads();
@@ -620,13 +631,13 @@ __global__ __launch_bounds__(kFftThreads) void fft_x_power(int nx, int fy, int H
         divmod(t, tile, lgt, ix, c);
         if (c >= nk) continue;
         const int x = ix <= Rx ? ix : L - nxo + ix;
-        g2[((uint64_t)ix * fy + ky) * H + kz0 + c] = d[c * S + pidx(x, sh)];
+        g2[((uint64_t)ix * fy + ky) * P + kz0 + c] = d[c * S + pidx(x, sh)];
     }
 }
 
 // ---- y inverse: per (ix, kz tile), keep the needed y
 template <class T>
-__global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tiles_kz, int tile, Fac fc,
+__global__ __launch_bounds__(kFftMaxThreads) void fft_y_inv(int fy, int H, int P, int tiles_kz, int tile, Fac fc,
                                                          const C2<T> *__restrict__ tw, int Ry,
                                                          const C2<T> *__restrict__ g2, C2<T> *__restrict__ g3, int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
@@ -636,23 +647,23 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tile
     V *d = sm;
     const int ix = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
     const int nk = min(tile, H - kz0);
-    const V *base = g2 + (uint64_t)ix * fy * H + kz0;
+    const V *base = g2 + (uint64_t)ix * fy * P + kz0;
     if (fc.n >= 2) {
-        S2H_RADIX_SWITCH(fc.f[fc.n - 1], (load_first_inv<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)H)));
+        S2H_RADIX_SWITCH(fc.f[fc.n - 1], (load_first_inv<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P)));
         int m = fc.f[fc.n - 1];
         for (int s = fc.n - 2; s >= 1; --s) {
             m *= fc.f[s];
             fft_stage_any<true>(d, L, S, sh, nk, fc.f[s], m, tw);
         }
         S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk,
-                                                       g3 + (uint64_t)ix * (2 * Ry + 1) * H + kz0, (uint64_t)H, Ry,
+                                                       g3 + (uint64_t)ix * (2 * Ry + 1) * P + kz0, (uint64_t)P, Ry,
                                                        tw)));
         return;
     }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         int y, c;
         divmod(t, tile, lgt, y, c);
-        d[c * S + pidx(y, sh)] = c < nk ? base[(uint64_t)y * H + c] : mk<V>(0, 0);
+        d[c * S + pidx(y, sh)] = c < nk ? base[(uint64_t)y * P + c] : mk<V>(0, 0);
     }
     __syncthreads();
     lds_fft<true>(d, L, S, sh, nk, fc, tw);
@@ -662,13 +673,13 @@ __global__ __launch_bounds__(kFftThreads) void fft_y_inv(int fy, int H, int tile
         divmod(t, tile, lgt, iy, c);
         if (c >= nk) continue;
         const int y = iy <= Ry ? iy : L - nyo + iy;
-        g3[((uint64_t)ix * nyo + iy) * H + kz0 + c] = d[c * S + pidx(y, sh)];
+        g3[((uint64_t)ix * nyo + iy) * P + kz0 + c] = d[c * S + pidx(y, sh)];
     }
 }
 
 // ---- z inverse: two Hermitian half spectra per complex inverse; keep needed z
 template <class T>
-__global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac fc, int L,
+__global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, int P, Fac fc, int L,
                                                          const C2<T> *__restrict__ tw, const int *__restrict__ rev,
                                                          int pairs_per_block, int Rz, T scale,
                                                          const C2<T> *__restrict__ g3, T *__restrict__ cube, int sh) {
@@ -687,8 +698,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, Fac 
         const int k = rev[pos];
         const bool upper = k >= H;
         const int kk = upper ? L - k : k;
-        V A = g3[(uint64_t)la * H + kk];
-        V B = lb < nlines ? g3[(uint64_t)lb * H + kk] : mk<V>(0, 0);
+        V A = g3[(uint64_t)la * P + kk];
+        V B = lb < nlines ? g3[(uint64_t)lb * P + kk] : mk<V>(0, 0);
         if (upper) {
             A.y = -A.y;
             B.y = -B.y;
@@ -760,6 +771,36 @@ uint64_t smooth_length(uint64_t min_len) {
     return best;
 }
 
+// Padded transform length of an axis of n voxels whose needed offsets reach
+// |d| <= R.  The circular correlation of length f is the linear one folded by
+// f: c_f(d) = sum_k c(d + k f), with c(m) = 0 for |m| >= n.  For |d| <= R every
+// folded term has |d + k f| >= f - R, so f >= n + R leaves the needed offsets
+// exact (the reference pads to 2n - 1, src/geometry/s2.rs:554-562, which keeps
+// every offset).  Among the 2,3,5-smooth lengths from n + R up to the
+// reference's, the one with the least f x (stages + 2) -- element passes of
+// the LDS stages plus the global load and store -- is taken: 320 for
+// n = 256, R = 64 (512 in the reference's padding), 960 = 8 8 3 5 for
+// n = 768, R = 128 (1536).
+uint64_t fft_length(uint64_t n, uint64_t R) {
+    const uint64_t full = smooth_length(2 * n - 1);
+    const uint64_t lo = n + std::min<uint64_t>(R, n - 1);
+    uint64_t best = full;
+    double best_cost = INFINITY;
+    for (uint64_t p2 = 1; p2 <= full; p2 *= 2)
+        for (uint64_t p23 = p2; p23 <= full; p23 *= 3)
+            for (uint64_t v = p23; v <= full; v *= 5) {
+                if (v < lo || v > 8192) continue;
+                const AxisPlan ap = axis_plan((int)v);
+                if (ap.nf < 0) continue;
+                const double cost = (double)v * (ap.nf + 2);
+                if (cost < best_cost || (cost == best_cost && v < best)) {
+                    best_cost = cost;
+                    best = v;
+                }
+            }
+    return best;
+}
+
 std::tuple<uint64_t, uint64_t, uint64_t> fft_plan_key(const uint64_t dims[3], const int R[3], int real_bytes) {
     return std::make_tuple(dims[0] | ((uint64_t)R[0] << 32) | ((uint64_t)(real_bytes == 8) << 63),
                            dims[1] | ((uint64_t)R[1] << 32), dims[2] | ((uint64_t)R[2] << 32));
@@ -776,7 +817,7 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], in
     p->real_bytes = real_bytes;
     for (int a = 0; a < 3; ++a) {
         p->dims[a] = dims[a];
-        p->f[a] = smooth_length(2 * dims[a] - 1);
+        p->f[a] = fft_length(dims[a], (uint64_t)R[a]);
         p->R[a] = R[a];
         p->ax[a] = axis_plan((int)p->f[a]);
         if (p->ax[a].nf < 0 || p->f[a] > 8192) {
@@ -821,7 +862,7 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], in
         if (st == S2H_OK) st = upload(ctx, p->pos_z, posv.data(), posv.size() * sizeof(int));
         if (st != S2H_OK) return cleanup(st);
     }
-    const uint64_t H = p->f[2] / 2 + 1;
+    const uint64_t H = spectrum_pitch(p->f[2], real_bytes);
     const uint64_t nxo = 2 * (uint64_t)R[0] + 1, nyo = 2 * (uint64_t)R[1] + 1, nzo = 2 * (uint64_t)R[2] + 1;
     s2h_status st;
     const uint64_t cb = 2 * (uint64_t)real_bytes;  // bytes per complex value
@@ -858,12 +899,20 @@ int z_lines_per_block(int fz) {
     return std::min(16, std::min(by_work, by_lds));
 }
 
+// Threads of a y or x pass workgroup: one butterfly of the first stage per
+// thread ((L / f_0) x tile), at least 256, at most kFftMaxThreads.
+static int pass_threads(int L, const Fac &fc, int tile) {
+    const int mp = fc.n >= 1 ? L / fc.f[0] : L;
+    const int t = (mp * tile + 63) / 64 * 64;
+    return std::min(kFftMaxThreads, std::max(kFftThreads, t));
+}
+
 template <class T>
 s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = nullptr) {
     using V = C2<T>;
     const int nx = (int)vol->dims[0], ny = (int)vol->dims[1], nz = (int)vol->dims[2];
     const int fx = (int)p->f[0], fy = (int)p->f[1], fz = (int)p->f[2];
-    const int H = fz / 2 + 1;
+    const int H = fz / 2 + 1, P = (int)spectrum_pitch((uint64_t)fz, (int)sizeof(T));
     const int Rx = p->R[0], Ry = p->R[1], Rz = p->R[2];
     const int nxo = 2 * Rx + 1, nyo = 2 * Ry + 1;
     const Fac fcx = to_fac(p->ax[0]), fcy = to_fac(p->ax[1]), fcz = to_fac(p->ax[2]);
@@ -877,7 +926,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         S2H_TRY(set_lds((const void *)fft_z_fwd<T>, lds));
         const int npairs = nx * ((ny + 1) / 2);
         fft_z_fwd<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
-            vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, fcz, fz, (const V *)p->tw[2].ptr,
+            vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, P, fcz, fz, (const V *)p->tw[2].ptr,
             (const int *)p->pos_z.ptr, ppb, (V *)p->g1.ptr, p->sh_z);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
@@ -886,7 +935,8 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     {
         const size_t lds = lds_bytes<T>(fy, ty, p->sh_y);
         S2H_TRY(set_lds((const void *)fft_y_fwd<T>, lds));
-        fft_y_fwd<T><<<nx * tiles_y, kFftThreads, lds, s>>>(ny, fy, H, tiles_y, ty, fcy, (const V *)p->tw[1].ptr,
+        fft_y_fwd<T><<<nx * tiles_y, pass_threads(fy, fcy, ty), lds, s>>>(ny, fy, H, P, tiles_y, ty, fcy,
+                                                                            (const V *)p->tw[1].ptr,
                                                             (V *)p->g1.ptr, p->sh_y);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
@@ -895,7 +945,8 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     {
         const size_t lds = lds_bytes<T>(fx, tx, p->sh_x);
         S2H_TRY(set_lds((const void *)fft_x_power<T>, lds));
-        fft_x_power<T><<<fy * tiles_x, kFftThreads, lds, s>>>(nx, fy, H, tiles_x, tx, fcx, (const V *)p->tw[0].ptr,
+        fft_x_power<T><<<fy * tiles_x, pass_threads(fx, fcx, tx), lds, s>>>(nx, fy, H, P, tiles_x, tx, fcx,
+                                                                             (const V *)p->tw[0].ptr,
                                                               Rx, (const V *)p->g1.ptr, (V *)p->g2.ptr, p->sh_x);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
@@ -904,7 +955,8 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     {
         const size_t lds = lds_bytes<T>(fy, ty, p->sh_y);
         S2H_TRY(set_lds((const void *)fft_y_inv<T>, lds));
-        fft_y_inv<T><<<nxo * tiles_y, kFftThreads, lds, s>>>(fy, H, tiles_y, ty, fcy, (const V *)p->tw[1].ptr, Ry,
+        fft_y_inv<T><<<nxo * tiles_y, pass_threads(fy, fcy, ty), lds, s>>>(fy, H, P, tiles_y, ty, fcy,
+                                                                             (const V *)p->tw[1].ptr, Ry,
                                                              (const V *)p->g2.ptr, (V *)p->g3.ptr, p->sh_y);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
@@ -918,7 +970,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         const int npairs = (nlines + 1) / 2;
         const T scale = (T)(1.0 / ((double)fx * (double)fy * (double)fz));
         fft_z_inv<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
-            nlines, H, fcz, fz, (const V *)p->tw[2].ptr, (const int *)p->rev_z.ptr, ppb, Rz, scale,
+            nlines, H, P, fcz, fz, (const V *)p->tw[2].ptr, (const int *)p->rev_z.ptr, ppb, Rz, scale,
             (const V *)p->g3.ptr, (T *)p->cube.ptr, p->sh_z);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
@@ -937,14 +989,17 @@ int pad_rule(int L, int lines) {
 }
 
 // kz columns per workgroup of the y and x passes: powers of two up to kTileMax
-// whose LDS tile fits (the widest for y, the widest up to 4 for x), or -- on a
-// context that autotunes -- for
-// each pass kind, the fastest of those candidates timed on the device (HIP
-// events around each pass of one run per candidate, after a warm-up).  The y
-// passes favour wide tiles (longer contiguous kz runs per row), the x pass
-// narrow ones (more workgroups per CU).  Every run rewrites all of the plan's
-// buffers, so the trial runs leave nothing behind.  The LDS padding of the y
-// and x passes is timed with the tiles (each tile padded and unpadded).
+// whose LDS tile fits -- untimed, the widest for y and for x the narrowest
+// whose row segment fills a whole 128-byte line (else the widest up to 4), or,
+// on a context that autotunes, for each pass kind the fastest of those
+// candidates timed on the device (HIP events around each pass of one run per
+// candidate, after a warm-up).  A segment narrower than a line still fetches
+// the whole line, and the neighbouring tile's workgroup seldom finds it in L2.
+// The y passes favour wide tiles (longer contiguous kz runs per row), the x
+// pass the narrowest whole-line tile (more workgroups per CU).  Every run
+// rewrites all of the plan's buffers, so the trial runs leave nothing behind.
+// The LDS padding of the y and x passes is timed with the tiles (each tile
+// padded and unpadded).
 template <class T>
 s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     const int lmax = (int)std::max(p->f[0], p->f[1]);
@@ -952,8 +1007,8 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     for (int t = kTileMax; t >= 1; t /= 2)
         if (lds_bytes<T>(lmax, t, kNoPad) <= 150 * 1024) cands.push_back(t);
     if (cands.empty()) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
-    // untimed: the widest tile for y, at most 4 columns for x (the usual
-    // picks), padded where the padding costs no occupancy
+    // untimed: the widest tile for y; for x the narrowest whole-line tile,
+    // else at most 4 columns; padded where the padding costs no occupancy
     p->tile_y = cands.front();
     p->tile_x = cands.back();
     for (int t : cands)
@@ -961,6 +1016,8 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
             p->tile_x = t;
             break;
         }
+    for (int t : cands)
+        if ((size_t)t * sizeof(C2<T>) >= 128) p->tile_x = t;
     const int fz = (int)p->f[2];
     p->sh_z = pad_rule<T>(fz, z_lines_per_block<T>(fz));
     p->sh_y = lds_bytes<T>((int)p->f[1], p->tile_y, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[1], p->tile_y) : kNoPad;
@@ -1047,15 +1104,22 @@ s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, FftPlan 
 // Thm 24.2).  Then ||Z^ - Z||_2 <= (2 rho + gamma_2) K ||Y||_2 for Z = |Y|^2,
 // and the scaled inverse adds rho again, so every output satisfies
 // |c^ - c| <= ||c^ - c||_2 <= (3 rho + 3u) K^{3/2}, rho = L * 8u.  L counts
-// each radix-R stage as ceil(log2 R) levels over the three axes, plus two for
-// the real-to-complex packing and unpacking along z.  With the bound below
-// 1/2 the rounded values are the exact integer counts; the path requires 1/4.
-double fft64_error_bound(const uint64_t dims[3], uint64_t occupied) {
+// each radix-8, -4 and -2 stage as the 3, 2 and 1 radix-2 levels its
+// butterfly is built of (its internal twiddles are +-1, +-i and (1 +- i)/sqrt 2),
+// a radix-3 stage as 3 levels and a radix-5 stage as 4 (their butterflies
+// satisfy ||y^ - y||_2 <= (sqrt 3 gamma_3 + mu) ||y||_2 < 3 * 8u and
+// (10 gamma_5 / sqrt 5 + mu) ||y||_2 < 4 * 8u: each output is at most three or
+// four roundings deep over terms of modulus at most 2|x_q|), over the three
+// axes, plus two for the real-to-complex packing and unpacking along z.  With
+// the bound below 1/2 the rounded values are the exact integer counts; the
+// path requires 1/4.
+double fft64_error_bound(const uint64_t dims[3], const int R[3], uint64_t occupied) {
     double levels = 2.0;
     for (int a = 0; a < 3; ++a) {
-        const AxisPlan ap = axis_plan((int)std::min<uint64_t>(smooth_length(2 * dims[a] - 1), 1u << 30));
+        const AxisPlan ap = axis_plan((int)std::min<uint64_t>(fft_length(dims[a], (uint64_t)R[a]), 1u << 30));
         if (ap.nf < 0) return INFINITY;
-        for (int s = 0; s < ap.nf; ++s) levels += ap.fac[s] == 8 ? 3 : ap.fac[s] == 5 ? 3 : ap.fac[s] == 2 ? 1 : 2;
+        for (int s = 0; s < ap.nf; ++s)
+            levels += ap.fac[s] == 8 ? 3 : ap.fac[s] == 4 ? 2 : ap.fac[s] == 2 ? 1 : ap.fac[s] == 3 ? 3 : 4;
     }
     const double u = std::ldexp(1.0, -53);
     const double rho = levels * 8.0 * u;
@@ -1072,20 +1136,24 @@ s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, const d
     return S2H_OK;
 }
 
-// Algorithmic bytes of one FP64 evaluation (3 axes x forward + inverse x read
-// + write of the complex128 half spectrum), for the method model.
-double fft64_model_bytes(const uint64_t dims[3]) {
-    const double fx = (double)smooth_length(2 * dims[0] - 1), fy = (double)smooth_length(2 * dims[1] - 1);
-    const double h = (double)(smooth_length(2 * dims[2] - 1) / 2 + 1);
-    return 12.0 * fx * fy * h * 16.0;
+// Bytes the pruned passes of one FP64 evaluation move (each reads its input
+// and writes its output once), for the method model.
+double fft64_model_bytes(const uint64_t dims[3], const int R[3]) {
+    const double fy = (double)fft_length(dims[1], (uint64_t)R[1]);
+    const double h = (double)(fft_length(dims[2], (uint64_t)R[2]) / 2 + 1);
+    const double nx = (double)dims[0], ny = (double)dims[1], nz = (double)dims[2];
+    const double xo = 2.0 * R[0] + 1, yo = 2.0 * R[1] + 1, zo = 2.0 * R[2] + 1;
+    const double c = 16.0;
+    return nx * ny * (std::ceil(nz / 64.0) + 1) * 8.0 + nx * ny * h * c + (nx * ny * h + nx * fy * h) * c +
+           (nx * fy * h + xo * fy * h) * c + (xo * fy * h + xo * yo * h) * c + xo * yo * h * c + xo * yo * zo * 8.0;
 }
 
 // Device bytes of an FP64 plan's spectra, cube and per-offset values (the
 // working set the method choice budgets, as fft_working_set_bytes does for the
 // reference's planner, src/geometry/s2.rs:695-714).
 double fft64_plan_bytes(const uint64_t dims[3], const int R[3]) {
-    const double fy = (double)smooth_length(2 * dims[1] - 1);
-    const double h = (double)(smooth_length(2 * dims[2] - 1) / 2 + 1);
+    const double fy = (double)fft_length(dims[1], (uint64_t)R[1]);
+    const double h = (double)spectrum_pitch(fft_length(dims[2], (uint64_t)R[2]), 8);
     const double nxo = 2.0 * R[0] + 1, nyo = 2.0 * R[1] + 1, nzo = 2.0 * R[2] + 1;
     return ((double)dims[0] * fy * h + nxo * fy * h + nxo * nyo * h) * 16.0 + nxo * nyo * nzo * 8.0;
 }
@@ -1101,13 +1169,14 @@ void fft_plans_release(s2h_ctx *ctx) {
 
 // Whether the FP64 passes can run this shape (lengths of 2, 3, 5 factors up
 // to 8192 whose LDS tiles fit).
-bool fft64_supported(const uint64_t dims[3]) {
+bool fft64_supported(const uint64_t dims[3], const int R[3]) {
+    uint64_t f[3];
     for (int a = 0; a < 3; ++a) {
-        const uint64_t f = smooth_length(2 * dims[a] - 1);
-        if (f > 8192 || axis_plan((int)f).nf < 0) return false;
+        f[a] = fft_length(dims[a], (uint64_t)R[a]);
+        if (f[a] > 8192 || axis_plan((int)f[a]).nf < 0) return false;
     }
-    const int lmax = (int)std::max(smooth_length(2 * dims[0] - 1), smooth_length(2 * dims[1] - 1));
-    const int lz = (int)smooth_length(2 * dims[2] - 1);
+    const int lmax = (int)std::max(f[0], f[1]);
+    const int lz = (int)f[2];
     return lds_bytes<double>(lmax, 1, kNoPad) <= 150 * 1024 && lds_bytes<double>(lz, 1, kNoPad) <= 150 * 1024;
 }
 
@@ -1116,6 +1185,17 @@ bool fft64_supported(const uint64_t dims[3]) {
 using namespace s2h;
 
 extern "C" {
+
+s2h_status s2h_fft_padded_dims(const uint64_t dims[3], uint32_t r_max, double pitch, uint64_t padded[3]) {
+    if (!dims || !padded) return fail(S2H_ERR_ARG, "null argument");
+    if (!(pitch > 0.0) || !std::isfinite(pitch)) return fail(S2H_ERR_ARG, "pitch must be finite and > 0");
+    for (int a = 0; a < 3; ++a)
+        if (dims[a] == 0 || dims[a] > 65535) return fail(S2H_ERR_ARG, "grid dimension outside 1..65535");
+    int R[3];
+    shell_extent(dims, r_max, pitch, R);
+    for (int a = 0; a < 3; ++a) padded[a] = fft_length(dims[a], (uint64_t)R[a]);
+    return S2H_OK;
+}
 
 s2h_status s2h_fft32_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, float *counts,
                             uint64_t n_offsets) {

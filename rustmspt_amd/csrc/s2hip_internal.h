@@ -468,10 +468,12 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied, c
 // correlation values for a volume with `occupied` voxels (rounding is exact
 // below 1/2), whether the shape's lengths are supported, the algorithmic bytes
 // of one evaluation, and the run itself (device array in shell order).
-double fft64_error_bound(const uint64_t dims[3], uint64_t occupied);
-bool fft64_supported(const uint64_t dims[3]);
-double fft64_model_bytes(const uint64_t dims[3]);
+double fft64_error_bound(const uint64_t dims[3], const int R[3], uint64_t occupied);
+bool fft64_supported(const uint64_t dims[3], const int R[3]);
+double fft64_model_bytes(const uint64_t dims[3], const int R[3]);
 double fft64_plan_bytes(const uint64_t dims[3], const int R[3]);
+// Largest |dx|, |dy|, |dz| of the in-domain shell offsets (host, cached).
+void shell_extent(const uint64_t dims[3], uint32_t r_max, double pitch, int R[3]);
 // Whether the context already holds the FP64 FFT plan of this shape (its
 // buffers are then part of ctx->in_use).
 bool fft64_plan_cached(s2h_ctx *ctx, const uint64_t dims[3], const int R[3]);

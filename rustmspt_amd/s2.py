@@ -236,6 +236,13 @@ def shell_order_offsets(dims, r_max: int, pitch: float):
     return offs[:n], rs
 
 
+def fft_padded_dims(dims, r_max: int, pitch: float = 1.0):
+    """Padded lengths of the FFT kernels for this shape (s2h_fft_padded_dims)."""
+    out = np.zeros(3, np.uint64)
+    L.check(L.lib().s2h_fft_padded_dims(L.ptr(_dims(dims), L._u64p), int(r_max), float(pitch), L.ptr(out, L._u64p)))
+    return [int(v) for v in out]
+
+
 def fill_missing(values, has_support, vf):
     v = np.ascontiguousarray(values, np.float64).copy()
     s = np.ascontiguousarray(has_support, np.uint8)
