@@ -59,7 +59,7 @@ struct FftPlan {
     int R[3];
     AxisPlan ax[3];
     int real_bytes = 4;  // 4: FP32 path (float2 spectra), 8: FP64 rounded exact path (double2)
-    DevBuf tw[3];    // complex[L] = exp(-2 pi i k / L), per axis, in the plan's precision
+    DevBuf tw[3];    // per axis: the stages' twiddle rows (Fac::tw) in the plan's precision
     DevBuf rev_z;    // int32[fz]: natural frequency held at each DIF output position
     DevBuf pos_z;    // int32[fz]: DIF output position of each natural frequency
     DevBuf g1, g2, g3, cube, fcounts;
@@ -78,9 +78,51 @@ void fft_plan_release(s2h_ctx *ctx, FftPlan *p) {
 
 namespace {
 
+// t / n for 0 <= t < 2^31 without a division: a shift when n is a power of
+// two, else the high word of t * m shifted by sh, with s = ceil(log2 n) and
+// m = floor(2^(31+s) / n) + 1 < 2^32.  t * m / 2^(31+s) exceeds t / n by at
+// most t / 2^(31+s) < 1 / n, too little to carry past the next multiple of n,
+// so the quotient is exact.
+struct FastDiv {
+    uint32_t n = 1, m = 0;
+    int32_t sh = 0;
+};
+
+FastDiv fast_div(uint32_t n) {
+    FastDiv d;
+    d.n = n;
+    if ((n & (n - 1)) == 0) {
+        d.m = 0;
+        d.sh = __builtin_ctz(n);
+        return d;
+    }
+    int s = 0;
+    while ((1u << s) < n) ++s;
+    d.m = (uint32_t)(((uint64_t)1 << (31 + s)) / n + 1);
+    d.sh = s - 1;
+    return d;
+}
+
+__device__ __forceinline__ void divmod(int t, const FastDiv &d, int &q, int &r) {
+    const uint32_t u = (uint32_t)t;
+    const uint32_t qq = d.m ? (__umulhi(u, d.m) >> d.sh) : (u >> d.sh);
+    q = (int)qq;
+    r = t - (int)(qq * d.n);
+}
+
+// Stages of one axis: radix f[s] of stage s (forward order; f[kMaxFactors - 1]
+// carries the length L), the offset of its twiddle rows in the axis table, the
+// butterflies per line nb[s] = L / f[s] and the sub-length quotient
+// mp[s] = m_s / f[s] (m_0 = L, m_{s+1} = mp[s]) as fast divisors, and L itself.
+// Stage s's row k (k < mp[s]) holds w^(k q L / m_s), q = 1 .. f[s] - 1, with
+// w = exp(-2 pi i / L): a butterfly reads its twiddles from one contiguous row.
 struct Fac {
     int n;
     int f[kMaxFactors];
+    int tw[kMaxFactors];
+    FastDiv nb[kMaxFactors];
+    FastDiv mp[kMaxFactors];
+    FastDiv len;
 };
 
 // LDS layout of a line: element j sits at pidx(j, sh) = j + (j >> sh), with
@@ -213,32 +255,34 @@ __device__ __forceinline__ void small_dft(V *a, int R) {
     }
 }
 
-// One radix-R stage over all lines: in-place butterflies on sub-length m.
+// One radix-R stage s over all lines: in-place butterflies on sub-length
+// m_s = R mp[s]; the butterfly (grp, k) takes its twiddles from row k of the
+// stage (none on sub-length R, where mp == 1 and k == 0).
 template <int R, bool INV, class V>
-__device__ __forceinline__ void fft_stage(V *d, int L, int S, int sh, int lines, int m, const V *tw) {
-    const int mp = m / R, nb = L / R, step = L / m;
-    const int lnb = log2_if_pow2(nb), lmp = log2_if_pow2(mp);
+__device__ __forceinline__ void fft_stage(V *d, int S, int sh, int lines, const Fac &fc, int s, const V *tw) {
+    const FastDiv dnb = fc.nb[s], dmp = fc.mp[s];
+    const int nb = (int)dnb.n, mp = (int)dmp.n, m = mp * R;
+    const V *trow = tw + fc.tw[s];
     for (int t = threadIdx.x; t < nb * lines; t += blockDim.x) {
         int line, b, grp, k;
-        divmod(t, nb, lnb, line, b);
-        divmod(b, mp, lmp, grp, k);
+        divmod(t, dnb, line, b);
+        divmod(b, dmp, grp, k);
         V *base = d + line * S;
         const int j0 = grp * m + k;
         V a[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = base[pidx(j0 + r * mp, sh)];
-        // twiddles w^(k q step) from the global table (L1-resident); the
-        // stage on sub-length R (mp == 1, k == 0) has only w^0 = 1
+        const V *w = trow + k * (R - 1);
         if (!INV) {
             small_dft<-1>(a, R);
             if (mp > 1) {
 #pragma unroll
-                for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[k * q * step]);
+                for (int q = 1; q < R; ++q) a[q] = cmul(a[q], w[q - 1]);
             }
         } else {
             if (mp > 1) {
 #pragma unroll
-                for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[k * q * step]);
+                for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], w[q - 1]);
             }
             small_dft<1>(a, R);
         }
@@ -249,34 +293,27 @@ __device__ __forceinline__ void fft_stage(V *d, int L, int S, int sh, int lines,
 }
 
 template <bool INV, class V>
-__device__ __forceinline__ void fft_stage_any(V *d, int L, int S, int sh, int lines, int R, int m, const V *tw) {
-    switch (R) {
-        case 8: fft_stage<8, INV>(d, L, S, sh, lines, m, tw); break;
-        case 4: fft_stage<4, INV>(d, L, S, sh, lines, m, tw); break;
-        case 2: fft_stage<2, INV>(d, L, S, sh, lines, m, tw); break;
-        case 3: fft_stage<3, INV>(d, L, S, sh, lines, m, tw); break;
-        default: fft_stage<5, INV>(d, L, S, sh, lines, m, tw); break;
+__device__ __forceinline__ void fft_stage_any(V *d, int S, int sh, int lines, const Fac &fc, int s, const V *tw) {
+    switch (fc.f[s]) {
+        case 8: fft_stage<8, INV>(d, S, sh, lines, fc, s, tw); break;
+        case 4: fft_stage<4, INV>(d, S, sh, lines, fc, s, tw); break;
+        case 2: fft_stage<2, INV>(d, S, sh, lines, fc, s, tw); break;
+        case 3: fft_stage<3, INV>(d, S, sh, lines, fc, s, tw); break;
+        default: fft_stage<5, INV>(d, S, sh, lines, fc, s, tw); break;
     }
 }
 
 // In-place mixed-radix transform of `lines` lines of length L held in LDS at
 // d[line*S + pidx(pos, sh)].  Forward: decimation in frequency, natural in, digit-reversed
 // out.  Inverse: decimation in time with conjugate twiddles, digit-reversed in,
-// natural out (unnormalized).  tw = exp(-2 pi i k / L); k*q*step < L always.
+// natural out (unnormalized); inverse stage s works on the same sub-length m_s
+// as forward stage s.
 template <bool INV, class V>
-__device__ void lds_fft(V *d, int L, int S, int sh, int lines, const Fac &fc, const V *tw) {
+__device__ void lds_fft(V *d, int S, int sh, int lines, const Fac &fc, const V *tw) {
     if (!INV) {
-        int m = L;
-        for (int s = 0; s < fc.n; ++s) {
-            fft_stage_any<false>(d, L, S, sh, lines, fc.f[s], m, tw);
-            m /= fc.f[s];
-        }
+        for (int s = 0; s < fc.n; ++s) fft_stage_any<false>(d, S, sh, lines, fc, s, tw);
     } else {
-        int m = 1;  // stage s works on sub-length m_s = f_s * m_{s+1}, m_n = 1
-        for (int s = fc.n - 1; s >= 0; --s) {
-            m *= fc.f[s];
-            fft_stage_any<true>(d, L, S, sh, lines, fc.f[s], m, tw);
-        }
+        for (int s = fc.n - 1; s >= 0; --s) fft_stage_any<true>(d, S, sh, lines, fc, s, tw);
     }
 }
 
@@ -287,11 +324,11 @@ __device__ void lds_fft(V *d, int L, int S, int sh, int lines, const Fac &fc, co
 // registers: one LDS round trip and two barriers fewer than running
 // lds_fft<false>, a |F|^2 pass and lds_fft<true>.
 template <int R, class V>
-__device__ __forceinline__ void power_stage(V *d, int L, int S, int sh, int lines) {
-    const int nb = L / R, lnb = log2_if_pow2(nb);
+__device__ __forceinline__ void power_stage(V *d, int S, int sh, int lines, const FastDiv dnb) {
+    const int nb = (int)dnb.n;
     for (int t = threadIdx.x; t < nb * lines; t += blockDim.x) {
         int line, b;
-        divmod(t, nb, lnb, line, b);
+        divmod(t, dnb, line, b);
         V *base = d + line * S;
         V a[R];
 #pragma unroll
@@ -307,7 +344,19 @@ __device__ __forceinline__ void power_stage(V *d, int L, int S, int sh, int line
 }
 
 template <class V>
-__device__ void lds_power(V *d, int L, int S, int sh, int lines, const Fac &fc, const V *tw) {
+__device__ __forceinline__ void power_stage_any(V *d, int S, int sh, int lines, const Fac &fc) {
+    const int last = fc.n - 1;  // sub-length f[last]: mp == 1
+    switch (fc.f[last]) {
+        case 8: power_stage<8>(d, S, sh, lines, fc.nb[last]); break;
+        case 4: power_stage<4>(d, S, sh, lines, fc.nb[last]); break;
+        case 2: power_stage<2>(d, S, sh, lines, fc.nb[last]); break;
+        case 3: power_stage<3>(d, S, sh, lines, fc.nb[last]); break;
+        default: power_stage<5>(d, S, sh, lines, fc.nb[last]); break;
+    }
+}
+
+template <class V>
+__device__ void lds_power(V *d, int S, int sh, int lines, const Fac &fc, const V *tw) {
     if (fc.n == 0) {  // L == 1
         for (int t = threadIdx.x; t < lines; t += blockDim.x) {
             const V v = d[t * S];
@@ -316,22 +365,9 @@ __device__ void lds_power(V *d, int L, int S, int sh, int lines, const Fac &fc, 
         __syncthreads();
         return;
     }
-    int m = L;
-    for (int s = 0; s + 1 < fc.n; ++s) {
-        fft_stage_any<false>(d, L, S, sh, lines, fc.f[s], m, tw);
-        m /= fc.f[s];
-    }
-    switch (fc.f[fc.n - 1]) {  // m == f_{n-1} here
-        case 8: power_stage<8>(d, L, S, sh, lines); break;
-        case 4: power_stage<4>(d, L, S, sh, lines); break;
-        case 2: power_stage<2>(d, L, S, sh, lines); break;
-        case 3: power_stage<3>(d, L, S, sh, lines); break;
-        default: power_stage<5>(d, L, S, sh, lines); break;
-    }
-    for (int s = fc.n - 2; s >= 0; --s) {
-        m *= fc.f[s];
-        fft_stage_any<true>(d, L, S, sh, lines, fc.f[s], m, tw);
-    }
+    for (int s = 0; s + 1 < fc.n; ++s) fft_stage_any<false>(d, S, sh, lines, fc, s, tw);
+    power_stage_any(d, S, sh, lines, fc);
+    for (int s = fc.n - 2; s >= 0; --s) fft_stage_any<true>(d, S, sh, lines, fc, s, tw);
 }
 
 // Outer stages straight between global memory and registers.  Line c's
@@ -348,6 +384,7 @@ __device__ void lds_power(V *d, int L, int S, int sh, int lines, const Fac &fc, 
 //   store_last_fwd: the forward transform's last stage (sub-length R, mp = 1,
 //     no twiddles) from LDS, every output j = R b + r written to g.
 //   load_first_inv: the inverse's first stage (sub-length R, mp = 1) from g.
+// tw0 is stage 0's twiddle rows (row b: w^(b q), q = 1 .. R - 1).
 template <int R, class V>
 __device__ __forceinline__ void load_first_fwd(V *d, int L, int S, int sh, int tile, int lgt, int nk,
                                                const V *__restrict__ g, uint64_t rs, int nvalid,
@@ -365,8 +402,9 @@ __device__ __forceinline__ void load_first_fwd(V *d, int L, int S, int sh, int t
         }
         small_dft<-1>(a, R);
         if (mp > 1) {
+            const V *w = tw + b * (R - 1);
 #pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[b * q]);
+            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], w[q - 1]);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) d[c * S + pidx(b + mp * r, sh)] = a[r];
@@ -386,8 +424,9 @@ __device__ __forceinline__ void store_last_inv(const V *d, int L, int S, int sh,
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = d[c * S + pidx(b + mp * r, sh)];
         if (mp > 1) {
+            const V *w = tw + b * (R - 1);
 #pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], tw[b * q]);
+            for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], w[q - 1]);
         }
         small_dft<1>(a, R);
 #pragma unroll
@@ -447,15 +486,17 @@ __device__ __forceinline__ void load_first_inv(V *d, int L, int S, int sh, int t
 // from the packed rows: line pl holds rows y (real part) and y + 1 (imaginary
 // part) of column x; z >= nz and a missing row y + 1 are zero.
 template <int R, class V>
-__device__ __forceinline__ void z_first_stage(V *d, int L, int S, int sh, int np, int p0, int ypairs, int ny, int nz,
-                                              int row_words, const uint64_t *__restrict__ words,
+__device__ __forceinline__ void z_first_stage(V *d, int S, int sh, int np, int p0, const FastDiv dyp, int ny, int nz,
+                                              int row_words, const uint64_t *__restrict__ words, const Fac &fc,
                                               const V *__restrict__ tw) {
     using T = decltype(V::x);
-    const int mp = L / R, lmp = log2_if_pow2(mp);
+    const FastDiv dmp = fc.mp[0];
+    const int mp = (int)dmp.n;
     for (int t = threadIdx.x; t < mp * np; t += blockDim.x) {
-        int pl, b;
-        divmod(t, mp, lmp, pl, b);
-        const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
+        int pl, b, x, yp;
+        divmod(t, dmp, pl, b);
+        divmod(p0 + pl, dyp, x, yp);
+        const int y = 2 * yp;
         const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
         const bool two = y + 1 < ny;
         V a[R];
@@ -471,8 +512,9 @@ __device__ __forceinline__ void z_first_stage(V *d, int L, int S, int sh, int np
         }
         small_dft<-1>(a, R);
         if (mp > 1) {
+            const V *w = tw + fc.tw[0] + b * (R - 1);
 #pragma unroll
-            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], tw[b * q]);
+            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], w[q - 1]);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) d[pl * S + pidx(b + mp * r, sh)] = a[r];
@@ -485,30 +527,27 @@ template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restrict__ words, int nx, int ny, int nz,
                                                          int row_words, int fy, int H, int P, Fac fc, int L,
                                                          const C2<T> *__restrict__ tw, const int *__restrict__ pos,
-                                                         int pairs_per_block, C2<T> *__restrict__ g1, int sh) {
+                                                         int pairs_per_block, C2<T> *__restrict__ g1, int sh,
+                                                         FastDiv dyp, FastDiv dh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     V *d = sm;
-    const int S = lstride(L, sh), lgl = log2_if_pow2(L);
+    const int S = lstride(L, sh);
     const int npairs = nx * ((ny + 1) / 2);
     const int p0 = blockIdx.x * pairs_per_block;
     const int np = min(pairs_per_block, npairs - p0);
-    const int ypairs = (ny + 1) / 2;
     if (fc.n >= 1) {
         // the first DIF stage straight from the packed bits: thread (line, b)
         // reads z = b + mp r of the two rows (lanes share each 64-bit word)
-        S2H_RADIX_SWITCH(fc.f[0], (z_first_stage<RR>(d, L, S, sh, np, p0, ypairs, ny, nz, row_words, words, tw)));
-        int m = L / fc.f[0];
-        for (int s = 1; s < fc.n; ++s) {
-            fft_stage_any<false>(d, L, S, sh, np, fc.f[s], m, tw);
-            m /= fc.f[s];
-        }
+        S2H_RADIX_SWITCH(fc.f[0], (z_first_stage<RR>(d, S, sh, np, p0, dyp, ny, nz, row_words, words, fc, tw)));
+        for (int s = 1; s < fc.n; ++s) fft_stage_any<false>(d, S, sh, np, fc, s, tw);
     } else {
         for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
-            int pl, z;
-            divmod(t, L, lgl, pl, z);
-            const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
+            int pl, z, x, yp;
+            divmod(t, fc.len, pl, z);
+            divmod(p0 + pl, dyp, x, yp);
+            const int y = 2 * yp;
             T re = 0, im = 0;
             if (z < nz) {
                 const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
@@ -518,13 +557,15 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
             d[pl * S + pidx(z, sh)] = mk<V>(re, im);
         }
         __syncthreads();
-        lds_fft<false>(d, L, S, sh, np, fc, tw);
+        lds_fft<false>(d, S, sh, np, fc, tw);
     }
     for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
-        const int pl = t / H, k = t - pl * H;
-        const int pi = p0 + pl, x = pi / ypairs, y = 2 * (pi - (pi / ypairs) * ypairs);
+        int pl, k, x, yp;
+        divmod(t, dh, pl, k);
+        divmod(p0 + pl, dyp, x, yp);
+        const int y = 2 * yp;
         const V zk = d[pl * S + pidx(pos[k], sh)];
-        const V zn = d[pl * S + pidx(pos[(L - k) % L], sh)];
+        const V zn = d[pl * S + pidx(pos[k == 0 ? 0 : L - k], sh)];
         // A = (Z[k] + conj Z[-k]) / 2,  B = (Z[k] - conj Z[-k]) / (2i)
         const T half = (T)0.5;
         const V A = mk<V>(half * (zk.x + zn.x), half * (zk.y - zn.y));
@@ -547,12 +588,9 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_y_fwd(int ny, int fy, int 
     const int nk = min(tile, H - kz0);
     V *base = g1 + (uint64_t)x * fy * P + kz0;
     if (fc.n >= 2) {
-        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P, ny, tw)));
-        int m = L / fc.f[0];
-        for (int s = 1; s + 1 < fc.n; ++s) {
-            fft_stage_any<false>(d, L, S, sh, nk, fc.f[s], m, tw);
-            m /= fc.f[s];
-        }
+        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P, ny,
+                                                       tw + fc.tw[0])));
+        for (int s = 1; s + 1 < fc.n; ++s) fft_stage_any<false>(d, S, sh, nk, fc, s, tw);
         S2H_RADIX_SWITCH(fc.f[fc.n - 1], (store_last_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P)));
         return;
     }
@@ -564,7 +602,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_y_fwd(int ny, int fy, int 
         d[c * S + pidx(y, sh)] = v;
     }
     __syncthreads();
-    lds_fft<false>(d, L, S, sh, nk, fc, tw);
+    lds_fft<false>(d, S, sh, nk, fc, tw);
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
         int y, c;
         divmod(t, tile, lgt, y, c);
@@ -575,23 +613,10 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_y_fwd(int ny, int fy, int 
 // Stages 1 .. n-1 of the forward transform, |F|^2 and stages n-1 .. 1 of the
 // inverse (lds_power without the outer stages; n >= 2).
 template <class V>
-__device__ void lds_power_inner(V *d, int L, int S, int sh, int lines, const Fac &fc, const V *tw) {
-    int m = L / fc.f[0];
-    for (int s = 1; s + 1 < fc.n; ++s) {
-        fft_stage_any<false>(d, L, S, sh, lines, fc.f[s], m, tw);
-        m /= fc.f[s];
-    }
-    switch (fc.f[fc.n - 1]) {  // m == f_{n-1} here
-        case 8: power_stage<8>(d, L, S, sh, lines); break;
-        case 4: power_stage<4>(d, L, S, sh, lines); break;
-        case 2: power_stage<2>(d, L, S, sh, lines); break;
-        case 3: power_stage<3>(d, L, S, sh, lines); break;
-        default: power_stage<5>(d, L, S, sh, lines); break;
-    }
-    for (int s = fc.n - 2; s >= 1; --s) {
-        m *= fc.f[s];
-        fft_stage_any<true>(d, L, S, sh, lines, fc.f[s], m, tw);
-    }
+__device__ void lds_power_inner(V *d, int S, int sh, int lines, const Fac &fc, const V *tw) {
+    for (int s = 1; s + 1 < fc.n; ++s) fft_stage_any<false>(d, S, sh, lines, fc, s, tw);
+    power_stage_any(d, S, sh, lines, fc);
+    for (int s = fc.n - 2; s >= 1; --s) fft_stage_any<true>(d, S, sh, lines, fc, s, tw);
 }
 
 // ---- x: forward, |F|^2, inverse; keep the needed x.  One block per (ky, kz tile).
@@ -610,10 +635,10 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_x_power(int nx, int fy, in
     if (fc.n >= 2) {
         const uint64_t rs = (uint64_t)fy * P;
         S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, g1 + (uint64_t)ky * P + kz0, rs,
-                                                       nx, tw)));
-        lds_power_inner(d, L, S, sh, nk, fc, tw);
+                                                       nx, tw + fc.tw[0])));
+        lds_power_inner(d, S, sh, nk, fc, tw);
         S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk, g2 + (uint64_t)ky * P + kz0, rs,
-                                                       Rx, tw)));
+                                                       Rx, tw + fc.tw[0])));
         return;
     }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
@@ -624,7 +649,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_x_power(int nx, int fy, in
         d[c * S + pidx(x, sh)] = v;
     }
     __syncthreads();
-    lds_power(d, L, S, sh, nk, fc, tw);
+    lds_power(d, S, sh, nk, fc, tw);
     const int nxo = 2 * Rx + 1;
     for (int t = threadIdx.x; t < nxo * tile; t += blockDim.x) {
         int ix, c;
@@ -650,14 +675,10 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_y_inv(int fy, int H, int P
     const V *base = g2 + (uint64_t)ix * fy * P + kz0;
     if (fc.n >= 2) {
         S2H_RADIX_SWITCH(fc.f[fc.n - 1], (load_first_inv<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P)));
-        int m = fc.f[fc.n - 1];
-        for (int s = fc.n - 2; s >= 1; --s) {
-            m *= fc.f[s];
-            fft_stage_any<true>(d, L, S, sh, nk, fc.f[s], m, tw);
-        }
+        for (int s = fc.n - 2; s >= 1; --s) fft_stage_any<true>(d, S, sh, nk, fc, s, tw);
         S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk,
                                                        g3 + (uint64_t)ix * (2 * Ry + 1) * P + kz0, (uint64_t)P, Ry,
-                                                       tw)));
+                                                       tw + fc.tw[0])));
         return;
     }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
@@ -666,7 +687,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_y_inv(int fy, int H, int P
         d[c * S + pidx(y, sh)] = c < nk ? base[(uint64_t)y * P + c] : mk<V>(0, 0);
     }
     __syncthreads();
-    lds_fft<true>(d, L, S, sh, nk, fc, tw);
+    lds_fft<true>(d, S, sh, nk, fc, tw);
     const int nyo = 2 * Ry + 1;
     for (int t = threadIdx.x; t < nyo * tile; t += blockDim.x) {
         int iy, c;
@@ -682,18 +703,19 @@ template <class T>
 __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, int P, Fac fc, int L,
                                                          const C2<T> *__restrict__ tw, const int *__restrict__ rev,
                                                          int pairs_per_block, int Rz, T scale,
-                                                         const C2<T> *__restrict__ g3, T *__restrict__ cube, int sh) {
+                                                         const C2<T> *__restrict__ g3, T *__restrict__ cube, int sh,
+                                                         FastDiv dzo) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     V *d = sm;
-    const int S = lstride(L, sh), lgl = log2_if_pow2(L);
+    const int S = lstride(L, sh);
     const int p0 = blockIdx.x * pairs_per_block;
     const int npairs = (nlines + 1) / 2;
     const int np = min(pairs_per_block, npairs - p0);
     for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
         int pl, pos;
-        divmod(t, L, lgl, pl, pos);
+        divmod(t, fc.len, pl, pos);
         const int la = 2 * (p0 + pl), lb = la + 1;
         const int k = rev[pos];
         const bool upper = k >= H;
@@ -707,10 +729,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, int 
         d[pl * S + pidx(pos, sh)] = mk<V>(A.x - B.y, A.y + B.x);  // A + iB
     }
     __syncthreads();
-    lds_fft<true>(d, L, S, sh, np, fc, tw);
+    lds_fft<true>(d, S, sh, np, fc, tw);
     const int nzo = 2 * Rz + 1;
     for (int t = threadIdx.x; t < np * nzo; t += blockDim.x) {
-        const int pl = t / nzo, iz = t - pl * nzo;
+        int pl, iz;
+        divmod(t, dzo, pl, iz);
         const int z = iz <= Rz ? iz : L - nzo + iz;
         const int la = 2 * (p0 + pl), lb = la + 1;
         const V v = d[pl * S + pidx(z, sh)];
@@ -753,7 +776,35 @@ Fac to_fac(const AxisPlan &a) {
     f.n = a.nf;
     for (int i = 0; i < kMaxFactors; ++i) f.f[i] = i < a.nf ? a.fac[i] : 0;
     f.f[kMaxFactors - 1] = a.L;
+    int m = a.L, off = 0;
+    for (int s = 0; s < kMaxFactors; ++s) {
+        f.tw[s] = off;
+        if (s < a.nf) {
+            const int mp = m / a.fac[s];
+            f.nb[s] = fast_div((uint32_t)(a.L / a.fac[s]));
+            f.mp[s] = fast_div((uint32_t)mp);
+            off += mp * (a.fac[s] - 1);
+            m = mp;
+        }
+    }
+    f.len = fast_div((uint32_t)std::max(a.L, 1));
     return f;
+}
+
+// The twiddle rows of an axis (Fac::tw): stage s, row k < m_s / f_s, entry q
+// = 1 .. f_s - 1 holds w^(k q L / m_s), taken from w[j] = exp(-2 pi i j / L).
+template <class W>
+std::vector<W> stage_twiddles(const AxisPlan &a, const std::vector<W> &w) {
+    std::vector<W> out;
+    int m = a.L;
+    for (int s = 0; s < a.nf; ++s) {
+        const int R = a.fac[s], mp = m / R, step = a.L / m;
+        for (int k = 0; k < mp; ++k)
+            for (int q = 1; q < R; ++q) out.push_back(w[(size_t)k * q * step]);
+        m = mp;
+    }
+    if (out.empty()) out.push_back(w[0]);
+    return out;
 }
 
 // src/geometry/s2.rs:519-551 smooth_fft_length
@@ -838,8 +889,10 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], in
             tw[k] = make_double2(std::cos(ang), std::sin(ang));
             tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
         }
-        s2h_status st = real_bytes == 8 ? upload(ctx, p->tw[a], tw.data(), (size_t)L * sizeof(double2))
-                                        : upload(ctx, p->tw[a], tw32.data(), (size_t)L * sizeof(float2));
+        const std::vector<double2> rows = stage_twiddles(p->ax[a], tw);
+        const std::vector<float2> rows32 = stage_twiddles(p->ax[a], tw32);
+        s2h_status st = real_bytes == 8 ? upload(ctx, p->tw[a], rows.data(), rows.size() * sizeof(double2))
+                                        : upload(ctx, p->tw[a], rows32.data(), rows32.size() * sizeof(float2));
         if (st != S2H_OK) return cleanup(st);
     }
     {   // natural frequency held at each DIF output position (z axis)
@@ -927,7 +980,8 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         const int npairs = nx * ((ny + 1) / 2);
         fft_z_fwd<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
             vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, P, fcz, fz, (const V *)p->tw[2].ptr,
-            (const int *)p->pos_z.ptr, ppb, (V *)p->g1.ptr, p->sh_z);
+            (const int *)p->pos_z.ptr, ppb, (V *)p->g1.ptr, p->sh_z, fast_div((uint32_t)((ny + 1) / 2)),
+            fast_div((uint32_t)H));
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
@@ -971,7 +1025,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         const T scale = (T)(1.0 / ((double)fx * (double)fy * (double)fz));
         fft_z_inv<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
             nlines, H, P, fcz, fz, (const V *)p->tw[2].ptr, (const int *)p->rev_z.ptr, ppb, Rz, scale,
-            (const V *)p->g3.ptr, (T *)p->cube.ptr, p->sh_z);
+            (const V *)p->g3.ptr, (T *)p->cube.ptr, p->sh_z, fast_div((uint32_t)(2 * Rz + 1)));
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
