@@ -122,6 +122,119 @@ def load_measurement_config(path) -> MeasurementParams:
     return MeasurementParams.parse(doc["measurement"])
 
 
+@dataclass
+class TargetConfig:
+    """src/config/optimization.rs:5-11."""
+    type: str
+    s2_array: Optional[list] = None
+    stl_path: Optional[str] = None
+    stl_bounding_box: Optional[list] = None
+
+
+@dataclass
+class OptimizationParams:
+    """src/config/optimization.rs:13-59: field names and optional fields as the
+    reference's; integers accept the "5_000" spelling."""
+    max_iterations: int
+    initial_temperature: float
+    cooling_rate: float
+    r_max: int
+    voxel_pitch: float
+    mc_method: str
+    mc_samples: int
+    max_translation: float
+    max_rotation_deg: float
+    seed: Optional[int] = None
+    adaptive_temp_window: Optional[int] = None
+    target_acceptance_low: Optional[float] = None
+    target_acceptance_high: Optional[float] = None
+    adaptive_heat_factor: Optional[float] = None
+    adaptive_cool_factor: Optional[float] = None
+    adaptive_temp_ceiling_factor: Optional[float] = None
+    min_neighbor_distance: Optional[float] = None
+    mode: Optional[int] = None
+    min_boundary_dist: Optional[float] = None
+    min_cross_boundary_depth: Optional[float] = None
+    prune_enabled: Optional[bool] = None
+    prune_tolerance: Optional[float] = None
+    prune_max_rounds: Optional[int] = None
+    prune_eval_samples: Optional[int] = None
+    cpu_max: Optional[int] = None
+    orient_to_positive_volume: Optional[bool] = None
+    rotation_mode: Optional[str] = None
+    rotation_axis_vector: Optional[list] = None
+    islands: Optional[int] = None
+    migration_interval: Optional[int] = None
+    acceleration: AccelerationConfig = field(default_factory=AccelerationConfig)
+
+    _REQUIRED = ("max_iterations", "initial_temperature", "cooling_rate", "r_max", "voxel_pitch", "mc_method",
+                 "mc_samples", "max_translation", "max_rotation_deg")
+    _INTS = ("max_iterations", "r_max", "mc_samples", "adaptive_temp_window", "prune_max_rounds",
+             "prune_eval_samples", "cpu_max", "islands", "migration_interval", "mode", "seed")
+    _FLOATS = ("initial_temperature", "cooling_rate", "voxel_pitch", "max_translation", "max_rotation_deg",
+               "target_acceptance_low", "target_acceptance_high", "adaptive_heat_factor", "adaptive_cool_factor",
+               "adaptive_temp_ceiling_factor", "min_neighbor_distance", "min_boundary_dist",
+               "min_cross_boundary_depth", "prune_tolerance")
+
+    @staticmethod
+    def parse(d) -> "OptimizationParams":
+        d = dict(d)
+        for req in OptimizationParams._REQUIRED:
+            if req not in d:
+                raise ConfigError(f"missing field `{req}`")
+        known = {f.name for f in fields(OptimizationParams)}
+        kw = {}
+        for k, v in d.items():
+            if k not in known:   # serde's default: unknown fields are ignored
+                continue
+            if k == "acceleration":
+                kw[k] = AccelerationConfig.parse(v)
+            elif v is None:
+                kw[k] = None
+            elif k in OptimizationParams._INTS:
+                kw[k] = _flex_int(v, k)
+            elif k in OptimizationParams._FLOATS:
+                kw[k] = float(v)
+            elif k in ("prune_enabled", "orient_to_positive_volume"):
+                kw[k] = bool(v)
+            elif k == "mc_method" or k == "rotation_mode":
+                kw[k] = str(v)
+            else:
+                kw[k] = v
+        if kw.get("mode") is not None and not 0 <= kw["mode"] <= 255:
+            raise ConfigError("optimization.mode must fit u8")
+        if kw.get("seed") is not None and kw["seed"] < 0:
+            raise ConfigError("optimization.seed must be a u64")
+        return OptimizationParams(**kw)
+
+
+@dataclass
+class OptimizationConfig:
+    """src/config/optimization.rs:61-68: input.stl_path, output.path, target,
+    box.dimensions, optimization."""
+    input_stl: str
+    output_path: str
+    target: TargetConfig
+    box_dimensions: list
+    optimization: OptimizationParams
+
+
+def load_optimization_config(path) -> OptimizationConfig:
+    with open(path) as fh:
+        doc = yaml.safe_load(fh)
+    if not isinstance(doc, dict):
+        raise ConfigError("optimization config must be a mapping")
+    for k in ("input", "output", "target", "box", "optimization"):
+        if k not in doc:
+            raise ConfigError(f"missing field `{k}`")
+    t = dict(doc["target"])
+    if "type" not in t:
+        raise ConfigError("missing field `type`")
+    target = TargetConfig(str(t["type"]), t.get("s2_array"), t.get("stl_path"), t.get("stl_bounding_box"))
+    return OptimizationConfig(str(doc["input"]["stl_path"]), str(doc["output"]["path"]), target,
+                              list(doc["box"]["dimensions"]), OptimizationParams.parse(doc["optimization"]))
+
+
 def parse_box_dimensions(v):
     """src/config/mod.rs:64-77: [sx, sy, sz] or [min.., max..]."""
     from .mesh import BoundingBox

@@ -17,7 +17,8 @@ from .mesh import BoundingBox, Mesh, merge_meshes, unit_icosphere
 
 
 def rng_for(seed: int, stream: int = 0) -> np.random.Generator:
-    return np.random.Generator(np.random.Philox(key=[int(seed) & (2 ** 64 - 1), int(stream) & (2 ** 64 - 1)]))
+    key = np.array([int(seed) & (2 ** 64 - 1), int(stream) & (2 ** 64 - 1)], np.uint64)
+    return np.random.Generator(np.random.Philox(key=key))
 
 
 def random_rotation(rng: np.random.Generator) -> np.ndarray:
