@@ -191,23 +191,31 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
 // ---------------------------------------------------------------- gather / fold
 
 // term[i] = hits / valid in f64 (division correctly rounded, as Rust's `/`).
-// With dvals (FP64 correlation values) the hits are the values rounded to the
-// nearest integer, at least 0: `.re.round().max(0.0)` of the reference's FFT
-// kernel (src/geometry/s2.rs:1245-1248).
+// With an FFT cube the hits are read at the offset's cube position: FP32
+// values as they are, FP64 values rounded to the nearest integer, at least 0
+// (`.re.round().max(0.0)` of the reference's FFT kernel, src/geometry/s2.rs:1245-1248).
 __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32_t *__restrict__ gather,
                                     uint64_t n, const unsigned long long *__restrict__ table,
-                                    const float *__restrict__ fcounts, const double *__restrict__ dvals,
-                                    uint64_t occupied, uint64_t nx, uint64_t ny, uint64_t nz,
+                                    const float *__restrict__ fcube, const double *__restrict__ dcube, int Rx,
+                                    int Ry, int Rz, uint64_t occupied, uint64_t nx, uint64_t ny, uint64_t nz,
                                     unsigned long long *__restrict__ hits, double *__restrict__ terms) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Offset o = offs[i];
     const uint64_t valid = (nx - (uint64_t)abs(o.dx)) * (ny - (uint64_t)abs(o.dy)) * (nz - (uint64_t)abs(o.dz));
+    uint64_t ci = 0;
+    if (fcube || dcube) {
+        const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
+        const int ix = o.dx >= 0 ? o.dx : 2 * Rx + 1 + o.dx;
+        const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
+        const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
+        ci = ((uint64_t)ix * nyo + iy) * nzo + iz;
+    }
     double h;
-    if (fcounts) {
-        h = (double)fcounts[i];
-    } else if (dvals) {
-        h = fmax(round(dvals[i]), 0.0);
+    if (fcube) {
+        h = (double)fcube[ci];
+    } else if (dcube) {
+        h = fmax(round(dcube[ci]), 0.0);
         hits[i] = (unsigned long long)h;
     } else {
         const int32_t g = gather[i];
@@ -239,7 +247,7 @@ __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32
 // The guesses only pick which blocks take the short path; every value comes
 // from exact arithmetic.
 static constexpr int kFoldK = 8;          // terms per lane per block (512 per block)
-static constexpr int kWideWaves = 8;      // waves per radius
+static constexpr int kWideWaves = 16;     // waves per radius
 static constexpr int kWideBlocks = 1024;  // blocks per pass (a longer radius takes several passes)
 static constexpr int kWideAhead = 4;      // blocks a wave loads at once in passes 0 and 1
 static constexpr int kNoBinade = -100000; // guessed start S == 0
@@ -370,15 +378,24 @@ __device__ __forceinline__ double fold_wide(const double *__restrict__ t, uint64
                 }
                 const uint32_t cntb = min(64u, nb - g0);
                 for (uint32_t k = 0; k < cntb; ++k) {
-                    const double G = read_lane_f64(gl, (int)k);
-                    const int m = __builtin_amdgcn_readlane(ml, (int)k);
-                    const uint32_t f = (uint32_t)__builtin_amdgcn_readlane((int)fl, (int)k);
-                    if (!seq && f == 0 && m != kNoBinade && st.S > 0.0 && ldexp(1.0, m - 52) == st.u &&
-                        st.a + G < kTwo53) {
-                        st.a += G;
+                    // The blocks from k on that take the short path, all at
+                    // once: lane j's block is short when its guess names the
+                    // exact binade (unchanged while the blocks before it are
+                    // short), it has no flag, and S / u plus the increments of
+                    // blocks k .. j stays below 2^53.  Their increments are
+                    // added in one step, up to the first block that is not.
+                    const bool live = lane >= k && lane < cntb;
+                    const double inc = wave_inclusive_sum_f64(live ? gl : 0.0);
+                    const bool shortp = live && !seq && fl == 0 && ml != kNoBinade && st.S > 0.0 &&
+                                        ldexp(1.0, ml - 52) == st.u && st.a + inc < kTwo53;
+                    const uint64_t stops = __ballot(!shortp) & (~0ull << k);
+                    const uint32_t j = stops ? (uint32_t)(__ffsll((long long)stops) - 1) : 64u;
+                    if (j > k) {
+                        st.a += read_lane_f64(inc, (int)(j - 1));
                         st.S = st.a * st.u;
-                        continue;
                     }
+                    if (j >= cntb) break;
+                    k = j;  // block j runs the exact path
                     const uint64_t start = c0 + (uint64_t)(g0 + k) * B;
                     const uint32_t cnt = (uint32_t)min<uint64_t>(B, n - start);
                     double v[kFoldK];
@@ -983,15 +1000,16 @@ s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
     return launch_counts(ctx, vol, p);
 }
 
-s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const float *d_fcounts,
-                             const double *d_dvals) {
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const FftCube *cube) {
     const uint64_t K = p->offs.size();
+    const FftCube none;
+    const FftCube &c = cube ? *cube : none;
     prof_mark(ctx, 1);
     if (K) {
         gather_terms_kernel<<<(unsigned)((K + 255) / 256), 256, 0, ctx->stream>>>(
             (const Offset *)p->d_offs.ptr, (const int32_t *)p->d_gather.ptr, K,
-            (const unsigned long long *)p->d_table.ptr, d_fcounts, d_dvals, occupied, p->dims[0], p->dims[1],
-            p->dims[2], (unsigned long long *)p->d_hits.ptr, (double *)p->d_terms.ptr);
+            (const unsigned long long *)p->d_table.ptr, c.f32, c.f64, c.R[0], c.R[1], c.R[2], occupied, p->dims[0],
+            p->dims[1], p->dims[2], (unsigned long long *)p->d_hits.ptr, (double *)p->d_terms.ptr);
         S2H_HIP(hipGetLastError());
     }
     double *values = (double *)p->d_curve.ptr;
@@ -1018,15 +1036,14 @@ static constexpr double kFft64FixedMs = 0.05;
 static constexpr double kFft64MaxBound = 0.25;  // required worst-case error (rounding is exact below 0.5)
 
 // The exact kernel of one evaluation: direct counting into the table, or the
-// FP64 FFT whose values round to the same integer counts (*d_dvals set).  The
+// FP64 FFT whose values round to the same integer counts (cube->f64 set).  The
 // reference's planner picks between its two exact kernels by modelled time
 // under a memory budget (src/geometry/s2.rs:733-766) because both return the
 // same counts; here the FFT is eligible when its worst-case round-off bound
 // for this volume is below 1/4 and (for the model's choice) its working set
 // fits the context's FFT budget, and the faster model wins.
-static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, uint64_t occupied,
-                                    const double **d_dvals) {
-    *d_dvals = nullptr;
+static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, uint64_t occupied, FftCube *cube) {
+    *cube = FftCube();
     const int *ext = p->shells->host->extent;
     const int R[3] = {ext[0], ext[1], ext[2]};
     const bool fft_ok = fft64_supported(p->dims, R) && fft64_error_bound(p->dims, R, occupied) <= kFft64MaxBound;
@@ -1038,39 +1055,40 @@ static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, ui
         const double plan_bytes = fft64_plan_bytes(p->dims, R);
         bool fits = !ctx->fft_budget || plan_bytes <= (double)ctx->fft_budget;
         if (ctx->budget) {
-            // the spectra (unless this context already holds them) and the
-            // per-offset values must fit what the context's budget leaves
-            const double need = (fft64_plan_cached(ctx, p->dims, R) ? 0.0 : plan_bytes) + 8.0 * p->offs.size();
+            // the spectra and the cube (unless this context already holds
+            // them) must fit what the context's budget leaves
+            const double need = fft64_plan_cached(ctx, p->dims, R) ? 0.0 : plan_bytes;
             fits = fits && (double)ctx->in_use + need <= (double)ctx->budget;
         }
         const double fft_ms = fft64_model_bytes(p->dims, R) / kFft64BytesPerS * 1e3 + kFft64FixedMs;
         m = fft_ok && fits && fft_ms < p->model_ms ? 2 : 1;
         if (m == 2) {
             p->method = 2;
-            const s2h_status st = fft64_exact_device(ctx, vol, p, d_dvals);
+            const s2h_status st = fft64_exact_device(ctx, vol, p, cube);
             if (st != S2H_ERR_BUDGET) return st;
             // the model's choice only: give the spectra back and count directly,
             // as the reference's planner keeps to its budget (src/geometry/s2.rs:733-766)
             (void)hipStreamSynchronize(ctx->stream);
             fft_plans_release(ctx);
-            *d_dvals = nullptr;
+            *cube = FftCube();
             m = 1;
         }
     }
     p->method = m;
-    if (m == 2) return fft64_exact_device(ctx, vol, p, d_dvals);
+    if (m == 2) return fft64_exact_device(ctx, vol, p, cube);
     return exact_run_counts(ctx, vol, p);
 }
 
 s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *p, double vf, double *s2) {
     const uint32_t n = p->r_max + 1;
-    std::vector<uint8_t> host((size_t)n * 9);
-    S2H_HIP(hipMemcpyAsync(host.data(), p->d_curve.ptr, (size_t)n * 9, hipMemcpyDeviceToHost, ctx->stream));
+    void *stage = nullptr;  // page-locked: the copy is one DMA the stream sync waits for
+    S2H_TRY(pinned_staging(ctx, (uint64_t)n * 9, &stage));
+    S2H_HIP(hipMemcpyAsync(stage, p->d_curve.ptr, (size_t)n * 9, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
     std::vector<uint8_t> sup(n, 0);
-    std::memcpy(s2, host.data(), (size_t)n * 8);
-    std::memcpy(sup.data(), host.data() + (size_t)n * 8, n);
+    std::memcpy(s2, stage, (size_t)n * 8);
+    std::memcpy(sup.data(), (const uint8_t *)stage + (size_t)n * 8, n);
     s2[0] = vf;
     sup[0] = 1;
     fill_missing(s2, sup.data(), n, vf);
@@ -1095,9 +1113,9 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
                                       std::to_string(p->offs.size()));
     uint64_t occupied = 0;
     S2H_TRY(vol_popcount(ctx, vol, &occupied));
-    const double *dvals = nullptr;
-    S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &dvals));
-    S2H_TRY(exact_gather_fold(ctx, p, occupied, nullptr, dvals));
+    FftCube cube;
+    S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &cube));
+    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube));
     if (n_offsets)
         S2H_HIP(hipMemcpyAsync(hits, p->d_hits.ptr, n_offsets * 8, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
@@ -1164,9 +1182,9 @@ s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     ExactPlan *p = nullptr;
     S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pm, &p));
     const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
-    const double *dvals = nullptr;
-    S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &dvals));
-    S2H_TRY(exact_gather_fold(ctx, p, occupied, nullptr, dvals));
+    FftCube cube;
+    S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &cube));
+    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube));
     return finish_curve_from_device(ctx, p, vf, s2);
 }
 

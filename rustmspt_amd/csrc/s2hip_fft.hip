@@ -1126,20 +1126,21 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     return st;
 }
 
-// Correlation values (precision T) of every shell offset of the exact plan
-// into fp->fcounts (device).
+// The pruned correlation cube (precision T) of the exact plan's shape in
+// fp->cube (device), and with per_offset also the value of every shell offset,
+// in shell order, in fp->fcounts.
 template <class T>
-s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, FftPlan **fp_out) {
+s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, FftPlan **fp_out, bool per_offset) {
     const int *ext = ep->shells->host->extent;
     int R[3] = {ext[0], ext[1], ext[2]};
     FftPlan *fp = nullptr;
     S2H_TRY(fft_plan_get(ctx, vol->dims, R, (int)sizeof(T), &fp));
     const uint64_t K = ep->offs.size();
-    S2H_TRY(dev_alloc(ctx, fp->fcounts, std::max<uint64_t>(K, 1) * sizeof(T)));
+    if (per_offset) S2H_TRY(dev_alloc(ctx, fp->fcounts, std::max<uint64_t>(K, 1) * sizeof(T)));
     if (fp->tile_y == 0) S2H_TRY(fft_pick_tile<T>(ctx, vol, fp));
     prof_mark(ctx, 0);
     S2H_TRY(fft_run<T>(ctx, vol, fp));
-    if (K) {
+    if (per_offset && K) {
         fft_gather<T><<<(unsigned)((K + 255) / 256), 256, 0, ctx->stream>>>(
             (const Offset *)ep->d_offs.ptr, K, (const T *)fp->cube.ptr, R[0], R[1], R[2], (T *)fp->fcounts.ptr);
         S2H_HIP(hipGetLastError());
@@ -1183,10 +1184,12 @@ double fft64_error_bound(const uint64_t dims[3], const int R[3], uint64_t occupi
 
 // FP64 correlation of every shell offset of `ep` for `vol` (device array of
 // ep->offs.size() doubles, in shell order).  The caller rounds them.
-s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, const double **d_values) {
+s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, FftCube *cube) {
     FftPlan *fp = nullptr;
-    S2H_TRY(fft_counts_device<double>(ctx, vol, ep, &fp));
-    *d_values = (const double *)fp->fcounts.ptr;
+    S2H_TRY(fft_counts_device<double>(ctx, vol, ep, &fp, false));
+    *cube = FftCube();
+    cube->f64 = (const double *)fp->cube.ptr;
+    for (int a = 0; a < 3; ++a) cube->R[a] = fp->R[a];
     return S2H_OK;
 }
 
@@ -1262,7 +1265,7 @@ s2h_status s2h_fft32_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
         return fail(S2H_ERR_SIZE, "counts buffer holds " + std::to_string(n_offsets) + " entries, plan has " +
                                       std::to_string(ep->offs.size()));
     FftPlan *fp = nullptr;
-    S2H_TRY(fft_counts_device<float>(ctx, vol, ep, &fp));
+    S2H_TRY(fft_counts_device<float>(ctx, vol, ep, &fp, true));
     prof_mark(ctx, 1);
     prof_mark(ctx, 2);
     if (n_offsets)
@@ -1286,8 +1289,11 @@ s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     ExactPlan *ep = nullptr;
     S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pm, &ep));
     FftPlan *fp = nullptr;
-    S2H_TRY(fft_counts_device<float>(ctx, vol, ep, &fp));
-    S2H_TRY(exact_gather_fold(ctx, ep, occupied, (const float *)fp->fcounts.ptr));
+    S2H_TRY(fft_counts_device<float>(ctx, vol, ep, &fp, false));
+    FftCube cube;
+    cube.f32 = (const float *)fp->cube.ptr;
+    for (int a = 0; a < 3; ++a) cube.R[a] = fp->R[a];
+    S2H_TRY(exact_gather_fold(ctx, ep, occupied, &cube));
     const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
     return finish_curve_from_device(ctx, ep, vf, s2);
 }

@@ -106,6 +106,21 @@ __device__ __forceinline__ double wave_exclusive_sum_f64(double x) {
     return x;
 }
 
+// Inclusive prefix sum over the wave's lanes (lane l gets x_0 + ... + x_l),
+// the same steps as above without the initial shift: exact for non-negative
+// integer values while those sums stay below 2^53, and never below an exact
+// sum that reaches it (every partial sum is of non-negative values, and
+// rounding is monotone).
+__device__ __forceinline__ double wave_inclusive_sum_f64(double x) {
+    x += dpp_f64<0x111, 0xf>(x);
+    x += dpp_f64<0x112, 0xf>(x);
+    x += dpp_f64<0x114, 0xf>(x);
+    x += dpp_f64<0x118, 0xf>(x);
+    x += dpp_f64<0x142, 0xa>(x);
+    x += dpp_f64<0x143, 0xc>(x);
+    return x;
+}
+
 // A wave-uniform read of lane l's double.
 __device__ __forceinline__ double read_lane_f64(double x, int l) {
     const uint64_t b = (uint64_t)__double_as_longlong(x);
@@ -430,6 +445,8 @@ struct s2h_ctx {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double prof_ms[4] = {0, 0, 0, 0};
     int launches = 0;
+    void *pinned = nullptr;     // page-locked staging for small read-backs (hipHostMalloc)
+    uint64_t pinned_bytes = 0;
 };
 
 // A welded, granule-split mesh resident on the device (s2hip_meshin.hip).
@@ -452,6 +469,8 @@ s2h_status vol_alloc(s2h_ctx *ctx, const uint64_t dims[3], s2h_vol **out);
 // Exclusive prefix sum of n (1..2^31-1) u32 on ctx->stream; waits and returns the total.
 s2h_status exclusive_sum(s2h_ctx *ctx, const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *total);
 s2h_status vol_popcount(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied);
+// The context's page-locked staging buffer, grown to at least `bytes`.
+s2h_status pinned_staging(s2h_ctx *ctx, uint64_t bytes, void **out);
 
 s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, double pitch,
                           ExactPlan **out);
@@ -459,11 +478,19 @@ void exact_plan_release(s2h_ctx *ctx, ExactPlan *plan);
 void shell_devs_release(s2h_ctx *ctx);  // device copies of shared shell data
 // Launch the count kernels for vol into plan->d_table (zeroed first) on ctx->stream.
 s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan);
-// Gather table -> per-offset hits and f64 terms, fold per radius into d_curve.
-// d_fcounts: FP32 correlation values (not rounded); d_dvals: FP64 correlation
-// values, rounded to the integer counts; both nullptr: the direct count table.
-s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied, const float *d_fcounts,
-                             const double *d_dvals = nullptr);
+// The pruned correlation cube an FFT evaluation leaves on the device: values
+// of offsets |dx| <= R[0], |dy| <= R[1], |dz| <= R[2], x-major, each axis
+// holding 0 .. R then the negative offsets -R .. -1.  FP32 values are used as
+// they are; FP64 values round to the integer counts.
+struct FftCube {
+    const float *f32 = nullptr;
+    const double *f64 = nullptr;
+    int R[3] = {0, 0, 0};
+};
+// Gather per-offset hits and f64 terms -- from the direct count table, or from
+// an FFT cube (cube != nullptr and one of its pointers set) -- and fold per
+// radius into d_curve.
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied, const FftCube *cube);
 // FP64 FFT exact kernel (s2hip_fft.hip): worst-case |computed - exact| of the
 // correlation values for a volume with `occupied` voxels (rounding is exact
 // below 1/2), whether the shape's lengths are supported, the algorithmic bytes
@@ -479,7 +506,7 @@ void shell_extent(const uint64_t dims[3], uint32_t r_max, double pitch, int R[3]
 bool fft64_plan_cached(s2h_ctx *ctx, const uint64_t dims[3], const int R[3]);
 // Release every FFT plan of the context (their blocks go to the block cache).
 void fft_plans_release(s2h_ctx *ctx);
-s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan, const double **d_values);
+s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan, FftCube *cube);
 // Copy folded curve values and supported flags back and finish the curve.
 s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *plan, double vf, double *s2);
 // Per-device address of the no-allocation entry point's scratch array

@@ -157,6 +157,22 @@ s2h_status vol_alloc(s2h_ctx *ctx, const uint64_t dims[3], s2h_vol **out) {
     return S2H_OK;
 }
 
+s2h_status pinned_staging(s2h_ctx *ctx, uint64_t bytes, void **out) {
+    if (bytes > ctx->pinned_bytes) {
+        if (ctx->pinned) {
+            S2H_HIP(hipStreamSynchronize(ctx->stream));
+            (void)hipHostFree(ctx->pinned);
+            ctx->pinned = nullptr;
+            ctx->pinned_bytes = 0;
+        }
+        const uint64_t want = std::max<uint64_t>(bytes, 4096);
+        S2H_HIP(hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault));
+        ctx->pinned_bytes = want;
+    }
+    *out = ctx->pinned;
+    return S2H_OK;
+}
+
 s2h_status vol_popcount(s2h_ctx *ctx, s2h_vol *vol, uint64_t *occupied) {
     if (vol->occupied == std::numeric_limits<uint64_t>::max()) {
         S2H_TRY(dev_alloc(ctx, ctx->d_scalar, 64));
@@ -424,6 +440,7 @@ void s2h_destroy(s2h_ctx *ctx) {
     dev_cache_release(ctx);
     for (hipEvent_t &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
