@@ -446,31 +446,79 @@ def stl_ingest_leg(ctx, mesh, bbox, reps=3):
             "grids_identical": same}
 
 
-def served_leg(rate: float, duration: float, pool_size: int, slots: int, budget_gb: float, seed: int):
+def served_leg(rate: float, duration: float, pool_size: int, slots: int, budget_gb: float, seed: int,
+               compare_slots: int = 4, compare_s: float = 20.0, sustained_s: float = 30.0, burst: int = 600,
+               plain_sample: int = 300):
     """Poisson stream of mixed requests (grid 64^3-384^3 log-uniform, VF
     U[0.1, 0.5], r_max U{16..96}, 70 % exact / 30 % FP32 FFT) through the serving
     loop; p50/p99 latency from submission to completion, evals/s, admission.
-    Run twice: at the offered rate (over one GPU's capacity: latency is queueing)
-    and at 70 % of the throughput that run measured (a stable queue)."""
-    from rustmspt_amd import serve
+    Runs: the offered rate for `duration` s on `slots` slots (the stated
+    workload) and for `compare_s` s on `compare_slots` slots; capacity: a burst
+    of `burst` requests submitted at once, on both slot counts; a stream at
+    70 % of the larger capacity for `sustained_s` s.  Beside them, the plain
+    per-call path (one context, one request at a time) on the first
+    `plain_sample` requests of the burst."""
+    from rustmspt_amd import s2, serve
 
     pool = serve.volume_pool(pool_size, seed)
     out = {}
-    with serve.Server(slots=slots, budget_bytes=int(budget_gb * (1 << 30))) as sv:
-        # warm-up: one small request per method
-        for m in (serve.EXACT, serve.FFT32):
-            t = sv.submit(pool[0].words, pool[0].dims, 16, m)
-            sv.poll(t, wait=True)
-        for name, lam in (("offered", rate), ("sustained", None)):
-            if lam is None:
-                lam = round(0.7 * out["offered"]["throughput_evals_per_s"], 1)
-            sched = serve.poisson_schedule(lam, duration, pool, seed + (0 if name == "offered" else 1))
+    bursts = serve.poisson_schedule(1e6, burst / 1e6, pool, seed + 7)
+
+    def run(label, nslots, sched, lam, secs):
+        with serve.Server(slots=nslots, budget_bytes=int(budget_gb * (1 << 30))) as sv:
+            for m in (serve.EXACT, serve.FFT32):  # warm-up: one small request per method
+                t = sv.submit(pool[0].words, pool[0].dims, 16, m)
+                sv.poll(t, wait=True)
             rep = serve.run_stream(sv, pool, sched, timeout=900)
-            rep.update({"rate": lam, "duration_s": duration})
-            out[name] = rep
-    out.update({"pool_volumes": pool_size, "slots": slots, "budget_gb": budget_gb,
-                "grids": sorted({v.dims[0] for v in pool})})
+        rep.update({"rate": lam, "duration_s": secs, "slots": nslots})
+        out[label] = rep
+        progress(f"served {label}: {rep['requests']} requests, {rep['throughput_evals_per_s']:.1f} evals/s, "
+                 f"p50 {rep['latency_p50_ms']:.1f} ms, p99 {rep['latency_p99_ms']:.1f} ms")
+
+    run("offered", slots, serve.poisson_schedule(rate, duration, pool, seed), rate, duration)
+    if compare_s > 0:
+        run(f"offered_{compare_slots}_slots", compare_slots, serve.poisson_schedule(rate, compare_s, pool, seed + 1),
+            rate, compare_s)
+    for n in sorted({slots, compare_slots}):
+        run(f"capacity_{n}_slots", n, bursts, None, 0.0)
+    cap = max(out[f"capacity_{n}_slots"]["throughput_evals_per_s"] for n in {slots, compare_slots})
+    best = max((n for n in {slots, compare_slots}), key=lambda n: out[f"capacity_{n}_slots"]["throughput_evals_per_s"])
+    if sustained_s > 0:
+        lam = round(0.7 * cap, 1)
+        run("sustained", best, serve.poisson_schedule(lam, sustained_s, pool, seed + 2), lam, sustained_s)
+    # the plain path on the burst's requests, one at a time
+    ctx = s2.Context(0)
+    ctx.set_autotune(False)
+    vols, n = {}, min(plain_sample, len(bursts))
+    try:
+        for a in bursts[:n]:  # volumes resident before the timing, as in the server
+            if a.volume not in vols:
+                vols[a.volume] = s2.Volume.from_packed(ctx, pool[a.volume].words, pool[a.volume].dims)
+        t0 = time.perf_counter()
+        for a in bursts[:n]:
+            vol = vols[a.volume]
+            if a.method == serve.EXACT:
+                vol.exact_s2(a.r_max, 1.0)
+            else:
+                vol.fft32_s2(a.r_max, 1.0)
+        plain_s = time.perf_counter() - t0
+    finally:
+        for vol in vols.values():
+            vol.free()
+        ctx.close()
+    out["plain"] = {"requests": n, "evals_per_s": n / plain_s,
+                    "note": "the burst's first requests in order through the plain per-call path on one context "
+                            "(volumes resident, plans built on first use as in the server)"}
+    out["capacity_evals_per_s"] = cap
+    out["capacity_slots"] = best
+    out["served_over_plain"] = cap / out["plain"]["evals_per_s"]
+    out.update({"pool_volumes": pool_size, "budget_gb": budget_gb, "grids": sorted({v.dims[0] for v in pool})})
     return out
+
+
+def progress(msg: str) -> None:
+    """A progress line on stderr (the result line is the only stdout output)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -494,7 +542,7 @@ def main():
     ap.add_argument("--ct-r-max", type=int, default=128,
                     help="r_max of the exact and FFT32 S2 of the ingested CT-like volume (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--tune", type=str, default="",
                     help="band,waves: force the exact launch shape instead of the autotuner's pick")
@@ -540,6 +588,7 @@ def main():
     # counting or the FP64 FFT rounded to the same integer counts) ...
     auto = exact_timed(ctx, vol, r_max, "auto", args.steps, args.warmup)
     kernel = ctx.exact_tune_info([grid] * 3, r_max, 1.0)["kernel"]
+    progress(f"exact ({kernel}): {auto['evals_per_s']:.1f} evals/s")
     curve = auto["curve"]
     # ... and the direct counting kernel forced, beside it
     direct = exact_timed(ctx, vol, r_max, "direct", args.steps, max(args.warmup, 1))
@@ -602,18 +651,23 @@ def main():
         hits = vol.exact_counts(r_max, 1.0)
         if not args.no_fft:
             out["fft32"] = fft_leg(ctx, vol, r_max, hits, offs, rs, curve, steps=5)
+            progress("fft32 done")
         if not args.no_mc:
             out["monte_carlo"] = mc_leg(ctx, vol, r_max, hits, offs, rs, args.mc_samples, seed=7)
+            progress("monte carlo done")
     if not args.no_mc:
         out["mesh_monte_carlo"] = mesh_mc_leg(ctx, mesh, bbox, r_max, args.mc_samples, seed=11)
+        progress("mesh monte carlo done")
     if not args.no_ingest:
         out["stl_ingest"] = stl_ingest_leg(ctx, mesh, bbox)
         ct_vol, out["ingest"] = ingest_leg(ctx, args.ingest_grid, seed=5, keep=args.ct_r_max > 0)
         if ct_vol is not None:
             out["ct_volume"] = ct_leg(ctx, ct_vol, args.ct_r_max, steps=2)
             ct_vol.free()
+        progress("ingest and CT volume done")
     if not args.no_batch:
         out["batched"] = batched_leg(ctx, args.batch_candidates, seed=3, plain_sample=32)
+        progress(f"batched: {out['batched'].get('batched_over_plain', 0):.2f}x plain")
     if not args.no_serve:
         out["served"] = served_leg(args.serve_rate, args.serve_seconds, 24, args.serve_slots, 64.0, seed=4)
     if not args.no_cpu:

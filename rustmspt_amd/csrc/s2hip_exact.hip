@@ -1054,7 +1054,8 @@ static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, ui
     if (m == 0) {
         const double plan_bytes = fft64_plan_bytes(p->dims, R);
         bool fits = !ctx->fft_budget || plan_bytes <= (double)ctx->fft_budget;
-        if (ctx->budget) {
+        const bool in_arena = ctx->fft_arena.ptr && plan_bytes <= (double)ctx->fft_arena.bytes;
+        if (ctx->budget && !in_arena) {
             // the spectra and the cube (unless this context already holds
             // them) must fit what the context's budget leaves
             const double need = fft64_plan_cached(ctx, p->dims, R) ? 0.0 : plan_bytes;

@@ -857,6 +857,19 @@ std::tuple<uint64_t, uint64_t, uint64_t> fft_plan_key(const uint64_t dims[3], co
                            dims[1] | ((uint64_t)R[1] << 32), dims[2] | ((uint64_t)R[2] << 32));
 }
 
+// Bytes of the plan's big buffers: g1 (n_x planes of f_y rows of the half
+// spectrum), g2 (the 2R_x+1 kept x planes), g3 (the kept (x, y) rows) and the
+// real cube of kept offsets; rows are spectrum_pitch complex values apart.
+void plan_buffer_sizes(const uint64_t dims[3], const int R[3], int real_bytes, uint64_t sizes[4]) {
+    const uint64_t fy = fft_length(dims[1], (uint64_t)R[1]);
+    const uint64_t H = spectrum_pitch(fft_length(dims[2], (uint64_t)R[2]), real_bytes);
+    const uint64_t nxo = 2 * (uint64_t)R[0] + 1, nyo = 2 * (uint64_t)R[1] + 1, nzo = 2 * (uint64_t)R[2] + 1;
+    const uint64_t cb = 2 * (uint64_t)real_bytes;  // bytes per complex value
+    const uint64_t raw[4] = {dims[0] * fy * H * cb, nxo * fy * H * cb, nxo * nyo * H * cb,
+                             nxo * nyo * nzo * (uint64_t)real_bytes};
+    for (int i = 0; i < 4; ++i) sizes[i] = (raw[i] + 255) / 256 * 256;
+}
+
 s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], int real_bytes, FftPlan **out) {
     auto key = fft_plan_key(dims, R, real_bytes);
     auto it = ctx->fft_plans.find(key);
@@ -915,15 +928,27 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], in
         if (st == S2H_OK) st = upload(ctx, p->pos_z, posv.data(), posv.size() * sizeof(int));
         if (st != S2H_OK) return cleanup(st);
     }
-    const uint64_t H = spectrum_pitch(p->f[2], real_bytes);
-    const uint64_t nxo = 2 * (uint64_t)R[0] + 1, nyo = 2 * (uint64_t)R[1] + 1, nzo = 2 * (uint64_t)R[2] + 1;
-    s2h_status st;
-    const uint64_t cb = 2 * (uint64_t)real_bytes;  // bytes per complex value
-    if ((st = dev_alloc(ctx, p->g1, dims[0] * p->f[1] * H * cb)) != S2H_OK ||
-        (st = dev_alloc(ctx, p->g2, nxo * p->f[1] * H * cb)) != S2H_OK ||
-        (st = dev_alloc(ctx, p->g3, nxo * nyo * H * cb)) != S2H_OK ||
-        (st = dev_alloc(ctx, p->cube, nxo * nyo * nzo * (uint64_t)real_bytes)) != S2H_OK)
-        return cleanup(st);
+    uint64_t sizes[4];
+    plan_buffer_sizes(dims, R, real_bytes, sizes);
+    DevBuf *big[4] = {&p->g1, &p->g2, &p->g3, &p->cube};
+    if (ctx->fft_arena.ptr && sizes[0] + sizes[1] + sizes[2] + sizes[3] <= ctx->fft_arena.bytes) {
+        // carve the spectra and cube from the arena; the plan they held before
+        // is released first (its kernels are earlier on this stream)
+        for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
+        ctx->fft_plans.clear();
+        uint64_t off = 0;
+        for (int i = 0; i < 4; ++i) {
+            big[i]->ptr = (unsigned char *)ctx->fft_arena.ptr + off;
+            big[i]->bytes = sizes[i];
+            big[i]->borrowed = true;
+            off += sizes[i];
+        }
+    } else {
+        for (int i = 0; i < 4; ++i) {
+            const s2h_status st = dev_alloc(ctx, *big[i], sizes[i]);
+            if (st != S2H_OK) return cleanup(st);
+        }
+    }
     ctx->fft_plans[key] = p;
     *out = p;
     return S2H_OK;
@@ -1209,11 +1234,16 @@ double fft64_model_bytes(const uint64_t dims[3], const int R[3]) {
 // working set the method choice budgets, as fft_working_set_bytes does for the
 // reference's planner, src/geometry/s2.rs:695-714).
 double fft64_plan_bytes(const uint64_t dims[3], const int R[3]) {
-    const double fy = (double)fft_length(dims[1], (uint64_t)R[1]);
-    const double h = (double)spectrum_pitch(fft_length(dims[2], (uint64_t)R[2]), 8);
-    const double nxo = 2.0 * R[0] + 1, nyo = 2.0 * R[1] + 1, nzo = 2.0 * R[2] + 1;
-    return ((double)dims[0] * fy * h + nxo * fy * h + nxo * nyo * h) * 16.0 + nxo * nyo * nzo * 8.0;
+    return (double)fft_plan_buffer_bytes(dims, R, 8);
 }
+
+uint64_t fft_plan_buffer_bytes(const uint64_t dims[3], const int R[3], int real_bytes) {
+    uint64_t sizes[4];
+    plan_buffer_sizes(dims, R, real_bytes, sizes);
+    return sizes[0] + sizes[1] + sizes[2] + sizes[3];
+}
+
+uint64_t fft_padded_length(uint64_t n, uint64_t R) { return fft_length(n, R); }
 
 bool fft64_plan_cached(s2h_ctx *ctx, const uint64_t dims[3], const int R[3]) {
     return ctx->fft_plans.count(fft_plan_key(dims, R, 8)) != 0;

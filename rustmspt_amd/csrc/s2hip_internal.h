@@ -298,6 +298,7 @@ void fill_missing(double *values, const uint8_t *has_support, uint64_t n, double
 struct DevBuf {
     void *ptr = nullptr;
     uint64_t bytes = 0;
+    bool borrowed = false;  // a slice of the context's FFT arena: dev_free only forgets it
 };
 
 // Plan for exact counting on one (dims, r_max, pitch): shell-order offsets, the
@@ -447,6 +448,10 @@ struct s2h_ctx {
     int launches = 0;
     void *pinned = nullptr;     // page-locked staging for small read-backs (hipHostMalloc)
     uint64_t pinned_bytes = 0;
+    // Pre-reserved device block that an FFT plan's spectra and cube are carved
+    // from when they fit (serving slots): no allocation on the request path.
+    // At most one plan lives in it; a new shape releases the old plan first.
+    s2h::DevBuf fft_arena;
 };
 
 // A welded, granule-split mesh resident on the device (s2hip_meshin.hip).
@@ -499,6 +504,11 @@ double fft64_error_bound(const uint64_t dims[3], const int R[3], uint64_t occupi
 bool fft64_supported(const uint64_t dims[3], const int R[3]);
 double fft64_model_bytes(const uint64_t dims[3], const int R[3]);
 double fft64_plan_bytes(const uint64_t dims[3], const int R[3]);
+// Padded transform length of an axis of n voxels whose offsets reach |d| <= R.
+uint64_t fft_padded_length(uint64_t n, uint64_t R);
+// Device bytes of an FFT plan's spectra and cube as allocated (real_bytes 4:
+// FP32 path, 8: FP64), each buffer rounded up to 256 bytes.
+uint64_t fft_plan_buffer_bytes(const uint64_t dims[3], const int R[3], int real_bytes);
 // Largest |dx|, |dy|, |dz| of the in-domain shell offsets (host, cached).
 void shell_extent(const uint64_t dims[3], uint32_t r_max, double pitch, int R[3]);
 // Whether the context already holds the FP64 FFT plan of this shape (its

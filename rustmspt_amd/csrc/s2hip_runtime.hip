@@ -40,7 +40,7 @@ static void cache_trim(s2h_ctx *ctx, uint64_t keep) {
 }
 
 s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes) {
-    if (b.ptr && b.bytes >= bytes) return S2H_OK;
+    if (b.ptr && !b.borrowed && b.bytes >= bytes) return S2H_OK;
     dev_free(ctx, b);
     if (bytes == 0) bytes = 8;
     const uint64_t want = round_block(bytes);
@@ -81,7 +81,7 @@ s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes) {
 }
 
 void dev_free(s2h_ctx *ctx, DevBuf &b) {
-    if (b.ptr) {
+    if (b.ptr && !b.borrowed) {
         ctx->in_use -= b.bytes;
         ctx->block_cache.emplace(b.bytes, b.ptr);
         ctx->cached_bytes += b.bytes;
@@ -89,6 +89,7 @@ void dev_free(s2h_ctx *ctx, DevBuf &b) {
     }
     b.ptr = nullptr;
     b.bytes = 0;
+    b.borrowed = false;
 }
 
 void dev_cache_release(s2h_ctx *ctx) { cache_trim(ctx, 0); }
@@ -435,6 +436,7 @@ void s2h_destroy(s2h_ctx *ctx) {
         dev_free(ctx, kv.second->d_hv);
     }
     for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
+    dev_free(ctx, ctx->fft_arena);
     dev_free(ctx, ctx->d_scalar);
     batch_ws_release(ctx);
     dev_cache_release(ctx);

@@ -46,40 +46,25 @@ struct Req {
     std::vector<double> s2;
 };
 
-uint64_t smooth_len(uint64_t n) {
-    uint64_t target = std::max<uint64_t>(n, 1), best = 0;
-    for (uint64_t p5 = 1;; p5 *= 5) {
-        for (uint64_t p35 = p5;; p35 *= 3) {
-            uint64_t v = p35;
-            while (v < target) v *= 2;
-            if (best == 0 || v < best) best = v;
-            if (p35 >= target) break;
-        }
-        if (p5 >= target) break;
-    }
-    return best;
-}
-
 // Device bytes one request holds while admitted: packed volume, shell plan
-// (offsets, gather, per-offset hits and terms, dense table) and the pruned
-// spectra -- FP32 for FFT32, FP64 for an exact request (whose kernel may be
-// the rounded FP64 FFT).  Shell size bound: the ball of radius r_max/pitch + 1.
-uint64_t footprint(const uint64_t dims[3], uint32_t r_max, double pitch, int32_t method) {
-    const double R = (double)r_max / pitch + 0.5 / pitch + 1.0;
-    double K = 4.18879 * R * R * R;
+// (offsets, gather, per-offset hits and terms, dense table) and, unless they
+// fit the slot's FFT arena (reserved apart from the admission budget), the
+// pruned spectra -- FP32 for FFT32; an exact request whose FP64 spectra do
+// not fit the arena counts directly and holds none.  Shell size bound: the
+// ball of radius r_max/pitch + 1.
+uint64_t footprint(const uint64_t dims[3], uint32_t r_max, double pitch, int32_t method, uint64_t arena) {
+    const double Rb = (double)r_max / pitch + 0.5 / pitch + 1.0;
+    double K = 4.18879 * Rb * Rb * Rb;
     const double box = (2.0 * dims[0] - 1) * (2.0 * dims[1] - 1) * (2.0 * dims[2] - 1);
     K = std::min(K, box);
     const uint64_t row_words = (dims[2] + 63) / 64 + 1;
     double bytes = (double)(dims[0] * dims[1] * row_words * 8 + kVolTailBytes);
     bytes += K * (12 + 4 + 8 + 8) + K * 8 / 2 + 65536.0 * 64;
-    {
-        const double fy = (double)smooth_len(2 * dims[1] - 1), fz = (double)smooth_len(2 * dims[2] - 1);
-        const double H = std::floor(fz / 2) + 1;
-        const double rx = std::min<double>(dims[0] - 1, R), ry = std::min<double>(dims[1] - 1, R),
-                     rz = std::min<double>(dims[2] - 1, R);
-        const double real = method == 1 ? 4.0 : 8.0;  // bytes per real value of the spectra
-        bytes += ((double)dims[0] * fy * H * 2 + (2 * rx + 1) * fy * H * 2 + (2 * rx + 1) * (2 * ry + 1) * H * 2 +
-                  (2 * rx + 1) * (2 * ry + 1) * (2 * rz + 1) + K) * real;
+    if (method == 1) {
+        int R[3];
+        for (int a = 0; a < 3; ++a) R[a] = (int)std::min<double>((double)dims[a] - 1, std::floor(Rb));
+        const uint64_t spectra = fft_plan_buffer_bytes(dims, R, 4);
+        if (!arena || spectra > arena) bytes += (double)spectra;
     }
     return (uint64_t)(bytes * 1.1) + (1ull << 20);
 }
@@ -110,7 +95,8 @@ struct s2h_server {
     std::vector<Slot *> slots;
     std::thread dispatcher;
     uint64_t slot_plan_cap = 8ull << 30;
-    uint64_t admit_budget = 0;  // budget minus the slots' plan-cache allowance
+    uint64_t arena_bytes = 2ull << 30;  // each slot's FFT arena
+    uint64_t admit_budget = 0;  // budget minus the slots' plan-cache allowances and arenas
     uint64_t over_budget = 0;   // jobs whose context held more device memory than its share
 };
 
@@ -147,7 +133,8 @@ void worker_loop(s2h_server *sv, s2h_server::Slot *slot) {
         // cached plans (at most the slot's plan allowance): its device memory,
         // cached blocks included, stays within that, so the slots together
         // stay within the server's budget
-        const uint64_t job_budget = sv->budget ? r->footprint + sv->slot_plan_cap : 0;
+        const uint64_t keep = sv->slot_plan_cap + sv->arena_bytes;  // what an idle slot may hold
+        const uint64_t job_budget = sv->budget ? r->footprint + keep : 0;
         lk.unlock();
         s2h_ctx *ctx = slot->ctx;
         ctx->budget = job_budget;
@@ -165,9 +152,10 @@ void worker_loop(s2h_server *sv, s2h_server::Slot *slot) {
         const double t_done = now_s(sv);
         // cached plans beyond the per-slot cap are dropped between requests,
         // and under a budget the idle context keeps at most the allowance
-        if (ctx->in_use > sv->slot_plan_cap) exact_plans_clear(ctx);
-        if (sv->budget && ctx->in_use + ctx->cached_bytes > sv->slot_plan_cap)
-            dev_cache_trim(ctx, sv->slot_plan_cap > ctx->in_use ? sv->slot_plan_cap - ctx->in_use : 0);
+        // (its FFT arena included)
+        if (ctx->in_use > keep) exact_plans_clear(ctx);
+        if (sv->budget && ctx->in_use + ctx->cached_bytes > keep)
+            dev_cache_trim(ctx, keep > ctx->in_use ? keep - ctx->in_use : 0);
         const uint64_t held = ctx->in_use + ctx->cached_bytes;
         const bool over = job_budget && ctx->held_peak > job_budget;
         lk.lock();
@@ -257,9 +245,12 @@ s2h_status s2h_serve_create(int device, uint64_t hbm_budget_bytes, uint32_t nslo
     sv->budget = hbm_budget_bytes;
     sv->t0 = Clock::now();
     if (hbm_budget_bytes) {
-        // a quarter of the budget is left to the slots' cached shell plans
+        // a quarter of the budget is left to the slots' cached shell plans and
+        // a quarter to their FFT arenas; requests are admitted against the rest
         sv->slot_plan_cap = std::max<uint64_t>(hbm_budget_bytes / (4 * nslots), 64ull << 20);
-        sv->admit_budget = hbm_budget_bytes - std::min(hbm_budget_bytes, sv->slot_plan_cap * nslots);
+        sv->arena_bytes = hbm_budget_bytes / (4 * nslots);
+        sv->admit_budget =
+            hbm_budget_bytes - std::min(hbm_budget_bytes, (sv->slot_plan_cap + sv->arena_bytes) * nslots);
     }
     for (uint32_t i = 0; i < nslots; ++i) {
         auto *slot = new s2h_server::Slot();
@@ -267,10 +258,12 @@ s2h_status s2h_serve_create(int device, uint64_t hbm_budget_bytes, uint32_t nslo
         // a stream of mixed shapes sees most (grid, r_max) pairs once: take the
         // cost model's launch shape (or one timed earlier in the process)
         if (st == S2H_OK) st = s2h_set_autotune(slot->ctx, 0);
-        // an exact request takes the FP64 FFT kernel only where its spectra
-        // fit half the slot's plan allowance: larger plans would be dropped
-        // and reallocated between requests, and hipFree stalls every slot
-        if (st == S2H_OK) slot->ctx->fft_budget = sv->slot_plan_cap / 2;
+        // the FFT arena, reserved once: spectra that fit it are never
+        // allocated or freed on the request path (a hipFree stalls every
+        // slot); an exact request takes the FP64 FFT kernel only where its
+        // spectra fit the arena
+        if (st == S2H_OK) st = dev_alloc(slot->ctx, slot->ctx->fft_arena, sv->arena_bytes);
+        if (st == S2H_OK) slot->ctx->fft_budget = sv->arena_bytes;
         if (st != S2H_OK) {
             if (slot->ctx) s2h_destroy(slot->ctx);
             delete slot;
@@ -318,7 +311,7 @@ s2h_status s2h_serve_submit(s2h_server *sv, const s2h_request *req, uint64_t *ti
     r->r_max = req->r_max;
     r->pitch = req->pitch;
     r->method = req->method;
-    r->footprint = footprint(r->dims, r->r_max, r->pitch, r->method);
+    r->footprint = footprint(r->dims, r->r_max, r->pitch, r->method, sv->arena_bytes);
     std::lock_guard<std::mutex> lk(sv->mu);
     if (sv->budget && r->footprint > sv->admit_budget) {
         delete r;

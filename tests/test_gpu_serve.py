@@ -54,31 +54,37 @@ def test_alone_equals_mix_equals_plain(ctx, pool):
 
 
 def test_budget_defers_and_drains(ctx):
-    # 160^3 r40 requests have footprints of about 250 MiB: three fit the 768 MiB
-    # that a 1 GiB budget leaves after four slots' 64 MiB plan allowances, so
-    # with four slots free the fourth waiting request must be deferred
+    # 160^3 r40 exact requests.  Under a budget B below 1 GiB four slots keep
+    # a 64 MiB plan allowance and a B/16 FFT arena each, and requests are
+    # admitted against the rest, 3B/4 - 256 MiB: B is set so that two
+    # footprints fit, so with four slots free the third waiting request must be
+    # deferred
     occ = serve.sphere_packing(160, 0.3, synth.rng_for(11, 2))
     words = serve.pack(occ)
-    budget = 1 << 30
+    with serve.Server(slots=1) as sv:
+        fp = sv.poll(sv.submit(words, occ.shape, 40, serve.EXACT), wait=True)[0]["footprint"]
+    budget = int(((256 << 20) + 2.5 * fp) / 0.75)
+    assert budget < 1 << 30
     with serve.Server(slots=4, budget_bytes=budget) as sv:
-        tickets = [sv.submit(words, occ.shape, 40, m % 2) for m in range(12)]
+        tickets = [sv.submit(words, occ.shape, 40, serve.EXACT) for _ in range(12)]
         res = [sv.poll(t, wait=True) for t in tickets]
         st = sv.stats()
     infos = [i for i, _ in res]
+    assert all(i["footprint"] == fp for i in infos)
     assert st["completed"] == 12 and st["failed"] == 0
     assert st["deferred"] > 0 and any(i["deferrals"] > 0 for i in infos), st
     assert st["peak_bytes"] <= st["budget"] == budget
     # device memory, not just the estimates: no slot held more than its share
-    # (footprint + plan allowance), and between requests the slots hold at most
-    # their allowances
+    # (footprint + plan allowance + arena), and between requests the slots hold
+    # at most their allowances
     assert st["over_budget_requests"] == 0, st
     assert st["device_bytes_held"] <= budget, st
     assert all(i["t_done"] >= i["t_start"] >= i["t_submit"] for i in infos)
     vol = s2.Volume.from_bytes(ctx, occ)
-    exact, fft = vol.exact_s2(40), vol.fft32_s2(40)
+    exact = vol.exact_s2(40)
     vol.free()
-    for m, (_, curve) in enumerate(res):
-        assert np.array_equal(curve, exact if m % 2 == 0 else fft), m
+    for _, curve in res:
+        assert np.array_equal(curve, exact)
     with serve.Server(slots=2, budget_bytes=64 << 20) as sv:
         big = serve.pack(np.ones((200, 200, 200), np.uint8))
         with pytest.raises(Exception):
