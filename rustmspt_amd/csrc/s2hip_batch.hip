@@ -511,7 +511,7 @@ constexpr unsigned kClearBlocks = 2048, kPrepBlocks = 1024, kVoxelBlocks = 8192,
 // staging for the per-batch inputs and outputs, and the captured graph.
 struct BatchWS {
     DevBuf seeds, cands, hdr, geo, gran, tiles, origin, adpre, tasks, delta, box, info, occ, ad, curves, over,
-        newxyz;
+        newxyz, fcands;
     void *pin = nullptr;
     uint64_t pin_bytes = 0;
     hipGraph_t graph = nullptr;
@@ -526,7 +526,7 @@ void batch_ws_release(s2h_ctx *ctx) {
     if (w->exec) (void)hipGraphExecDestroy(w->exec);
     if (w->graph) (void)hipGraphDestroy(w->graph);
     DevBuf *bufs[] = {&w->seeds, &w->cands, &w->hdr, &w->geo, &w->gran, &w->tiles, &w->origin, &w->adpre, &w->tasks,
-                      &w->delta, &w->box, &w->info, &w->occ, &w->ad, &w->curves, &w->over, &w->newxyz};
+                      &w->delta, &w->box, &w->info, &w->occ, &w->ad, &w->curves, &w->over, &w->newxyz, &w->fcands};
     for (DevBuf *b : bufs) dev_free(ctx, *b);
     if (w->pin) (void)hipHostFree(w->pin);
     delete w;
@@ -702,9 +702,86 @@ struct HostBatch {
     std::vector<uint64_t> ad_prefix;
     std::vector<FoldTask> tasks;
     std::vector<uint64_t> xyz_off;  // per candidate: first new vertex (3 doubles each)
+    // delta counts by region correlation: per candidate whether it takes them,
+    // the candidates' descriptors grouped by seed, and the groups
+    std::vector<uint8_t> cfftd;
+    std::vector<FftdCand> fcands;
+    struct FftdGroup {
+        uint32_t seed, start, count;
+        int F[3];
+    };
+    std::vector<FftdGroup> groups;
     BatchHeader hdr{};
     uint64_t geo_total = 0, newxyz_total = 0, curve_total = 0;
 };
+
+// The item split of the direct delta counts for a row chunk (candidates whose
+// deltas come from region correlation have none).
+static void set_row_chunk(HostBatch &hb, uint32_t row_chunk) {
+    uint64_t items = 0;
+    for (size_t c = 0; c < hb.cd.size(); ++c) {
+        CandDev &d = hb.cd[c];
+        const uint64_t nrows = (uint64_t)(d.bx1 - d.bx0) * (uint64_t)(d.by1 - d.by0);
+        d.nchunks = c < hb.cfftd.size() && hb.cfftd[c] ? 0u : (uint32_t)((nrows + row_chunk - 1) / row_chunk);
+        d.item_start = items;
+        items += (uint64_t)d.nchunks * hb.sdev[d.seed].ngb;
+    }
+    hb.hdr.nitems = items;
+    hb.hdr.row_chunk = row_chunk;
+}
+
+// Modelled device time of direct delta counting per (changed row x canonical
+// offset x span word), from the 256^3 r64 and 192^3 r48 seeds' timings.
+static constexpr double kDirectMsPerUnit = 2.0e-10;
+
+// Per seed, the delta counts of its candidates: direct counting, or region
+// correlation where its frames are supported, provably exact (error bound
+// below 1/4) and modelled faster.  S2HIP_BATCH_DELTA=direct or =fft forces
+// one (fft still only where supported and exact).
+static void choose_delta_modes(s2h_seed *const *seeds, uint32_t nseeds, HostBatch &hb) {
+    const size_t ncand = hb.cd.size();
+    hb.cfftd.assign(ncand, 0);
+    hb.fcands.clear();
+    hb.groups.clear();
+    const char *env = std::getenv("S2HIP_BATCH_DELTA");
+    const int force = !env ? 0 : std::strcmp(env, "fft") == 0 ? 2 : std::strcmp(env, "direct") == 0 ? 1 : 0;
+    if (force == 1) return;
+    std::vector<uint32_t> mine;
+    for (uint32_t si = 0; si < nseeds; ++si) {
+        const s2h_seed *sd = seeds[si];
+        mine.clear();
+        int bmax[3] = {1, 1, 1};
+        uint64_t changed = 1;
+        double direct_ms = 0.0;
+        for (uint32_t c = 0; c < ncand; ++c) {
+            const CandDev &d = hb.cd[c];
+            if (d.seed != si) continue;
+            mine.push_back(c);
+            const int b[3] = {d.bx1 - d.bx0, d.by1 - d.by0, d.bz1 - d.bz0};
+            for (int a = 0; a < 3; ++a) bmax[a] = std::max(bmax[a], b[a]);
+            changed = std::max<uint64_t>(changed, (uint64_t)b[0] * b[1] * b[2]);
+            direct_ms += kDirectMsPerUnit * (double)b[0] * b[1] * (double)sd->ncanon * (b[2] / 32.0 + 2.0);
+        }
+        if (mine.empty()) continue;
+        const int *ext = sd->plan->shells->host->extent;
+        const int R[3] = {ext[0], ext[1], ext[2]};
+        int F[3];
+        if (!fft_delta_frame(bmax, R, F)) continue;
+        if (fft_delta_error_bound(F, sd->occupied, changed) > 0.25) continue;
+        if (force != 2 && !(fft_delta_model_ms(F, R, (uint32_t)mine.size()) < direct_ms)) continue;
+        HostBatch::FftdGroup g;
+        g.seed = si;
+        g.start = (uint32_t)hb.fcands.size();
+        g.count = (uint32_t)mine.size();
+        for (int a = 0; a < 3; ++a) g.F[a] = F[a];
+        hb.groups.push_back(g);
+        for (uint32_t c : mine) {
+            const CandDev &d = hb.cd[c];
+            hb.cfftd[c] = 1;
+            hb.fcands.push_back(FftdCand{{d.bx0, d.by0, d.bz0}, {d.bx1, d.by1, d.bz1}, d.ad_off, d.delta_off});
+        }
+    }
+}
 
 static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, uint32_t nseeds,
                              const s2h_candidate *cands, uint32_t ncand, uint32_t row_chunk, HostBatch &hb) {
@@ -838,6 +915,9 @@ static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, u
         hb.origin[4 * c + 2] = sd->bmin[2];
         hb.origin[4 * c + 3] = sd->pitch;
     }
+    choose_delta_modes(seeds, nseeds, hb);
+    set_row_chunk(hb, row_chunk);
+    items = hb.hdr.nitems;
     if (hb.tiles.size() >= (1ull << 31) || hb.gran.size() >= (1ull << 31) || hb.geo_total >= (1ull << 32))
         return fail(S2H_ERR_UNSUPPORTED, "batch too large");
     if (ad_total * 8 + 64 >= 0x7ffffe00ull) return fail(S2H_ERR_UNSUPPORTED, "changed rows of the batch above 2 GiB");
@@ -870,19 +950,6 @@ static s2h_status build_host(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seeds, u
     return S2H_OK;
 }
 
-// Recompute the item split for another row chunk (autotuning).
-static void set_row_chunk(HostBatch &hb, uint32_t row_chunk) {
-    uint64_t items = 0;
-    for (CandDev &d : hb.cd) {
-        const uint64_t nrows = (uint64_t)(d.bx1 - d.bx0) * (uint64_t)(d.by1 - d.by0);
-        d.nchunks = (uint32_t)((nrows + row_chunk - 1) / row_chunk);
-        d.item_start = items;
-        items += (uint64_t)d.nchunks * hb.sdev[d.seed].ngb;
-    }
-    hb.hdr.nitems = items;
-    hb.hdr.row_chunk = row_chunk;
-}
-
 // Curve buffer: values, then one support byte per value, in one allocation.
 static uint8_t *curve_support(const BatchWS &w) {
     return (uint8_t *)w.curves.ptr + w.curves.bytes / 9 * 8;
@@ -892,7 +959,9 @@ static uint8_t *curve_support(const BatchWS &w) {
 // out-of-range offset the kernels use for lanes without a row).
 static uint32_t ad_records(const BatchWS &w) { return (uint32_t)std::min<uint64_t>(w.ad.bytes, 0x7ffffe00u); }
 
-static void launch_sequence(hipStream_t s, BatchWS &w) {
+// The launch sequence: clear through the direct delta counts (front), then
+// the fold.
+static void launch_front(hipStream_t s, BatchWS &w) {
     const BatchHeader *hdr = (const BatchHeader *)w.hdr.ptr;
     const SeedDev *seeds = (const SeedDev *)w.seeds.ptr;
     const CandDev *cands = (const CandDev *)w.cands.ptr;
@@ -913,8 +982,40 @@ static void launch_sequence(hipStream_t s, BatchWS &w) {
     build_ad_kernel<<<kAdBlocks, 256, 0, s>>>(seeds, cands, hdr, (const uint64_t *)w.adpre.ptr, box, ad, info, occ);
     delta_count_kernel<<<kDeltaBlocks, 64 * kDeltaWaves, 0, s>>>(seeds, cands, hdr, ad, ad_records(w), info,
                                                                  delta);
-    batch_fold_kernel<<<kFoldBlocks, 64 * kFoldWaves, 0, s>>>(seeds, cands, hdr, (const FoldTask *)w.tasks.ptr, delta,
-                                                              occ, (double *)w.curves.ptr, curve_support(w));
+}
+
+static void launch_fold(hipStream_t s, BatchWS &w) {
+    batch_fold_kernel<<<kFoldBlocks, 64 * kFoldWaves, 0, s>>>(
+        (const SeedDev *)w.seeds.ptr, (const CandDev *)w.cands.ptr, (const BatchHeader *)w.hdr.ptr,
+        (const FoldTask *)w.tasks.ptr, (int32_t *)w.delta.ptr, (int *)w.occ.ptr, (double *)w.curves.ptr,
+        curve_support(w));
+}
+
+static void launch_sequence(hipStream_t s, BatchWS &w) {
+    launch_front(s, w);
+    launch_fold(s, w);
+}
+
+// Region-correlation delta counts of each group's candidates (after the
+// change rows exist, before the fold).
+static s2h_status launch_fftd(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, s2h_seed *const *seeds) {
+    for (const HostBatch::FftdGroup &g : hb.groups) {
+        const s2h_seed *sd = seeds[g.seed];
+        FftdSeed fs;
+        fs.words = sd->vol->d_words;
+        fs.row_words = sd->vol->row_words;
+        for (int a = 0; a < 3; ++a) {
+            fs.dims[a] = (int)sd->vol->dims[a];
+            fs.R[a] = sd->plan->shells->host->extent[a];
+        }
+        fs.sg = (const int32_t *)sd->sg.ptr;
+        fs.offs = (const Offset *)sd->plan->d_offs.ptr;
+        fs.noffs = sd->plan->offs.size();
+        fs.occupied = sd->occupied;
+        S2H_TRY(fft_delta_run(ctx, fs, (const FftdCand *)w.fcands.ptr + g.start, g.count, g.F, (const uint2 *)w.ad.ptr,
+                              (int)((sd->vol->dims[2] + 31) / 32), (int32_t *)w.delta.ptr));
+    }
+    return S2H_OK;
 }
 
 static std::vector<void *> ws_ptrs(const BatchWS &w) {
@@ -929,8 +1030,18 @@ static bool use_graph() {
 }
 
 // Launch the batch sequence: replay the captured graph, capturing it first when
-// the workspace buffers moved since the last capture.
-static s2h_status run_sequence(s2h_ctx *ctx, BatchWS &w) {
+// the workspace buffers moved since the last capture.  A batch with region-
+// correlation groups launches directly (its frame passes depend on the batch).
+static s2h_status run_sequence(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, s2h_seed *const *seeds) {
+    if (!hb.groups.empty()) {
+        launch_front(ctx->stream, w);
+        S2H_HIP(hipGetLastError());
+        S2H_TRY(launch_fftd(ctx, w, hb, seeds));
+        launch_fold(ctx->stream, w);
+        S2H_HIP(hipGetLastError());
+        ctx->launches += 7;
+        return S2H_OK;
+    }
     if (!use_graph()) {
         launch_sequence(ctx->stream, w);
         S2H_HIP(hipGetLastError());
@@ -997,11 +1108,12 @@ static s2h_status upload_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, co
         S2H_TRY(dev_alloc(ctx, w.curves, entries * 9));
     }
     S2H_TRY(ws_reserve(ctx, w.over, (1 + 2 * (uint64_t)kOverflowCap) * 4));
+    S2H_TRY(ws_reserve(ctx, w.fcands, std::max<size_t>(hb.fcands.size(), 1) * sizeof(FftdCand)));
     const uint64_t in_bytes = (hb.sdev.size() * sizeof(SeedDev) + 16) + ((uint64_t)ncand * sizeof(CandDev) + 16) +
                               (sizeof(BatchHeader) + 16) + (hb.gran.size() * sizeof(vox::GranuleInfo) + 16) +
                               (hb.tiles.size() * sizeof(vox::Tile) + 16) + ((uint64_t)ncand * 32 + 16) +
                               (((uint64_t)ncand + 1) * 8 + 16) + (hb.tasks.size() * sizeof(FoldTask) + 16) +
-                              hb.newxyz_total * 24 + 64;
+                              hb.newxyz_total * 24 + (hb.fcands.size() * sizeof(FftdCand) + 16) + 64;
     const uint64_t need = in_bytes + out_region(hb, ncand) + 64;
     if (w.pin_bytes < need) {
         if (w.pin) (void)hipHostFree(w.pin);
@@ -1027,6 +1139,7 @@ static s2h_status upload_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, co
     S2H_TRY(put(w.origin, hb.origin.data(), (uint64_t)ncand * 32));
     S2H_TRY(put(w.adpre, hb.ad_prefix.data(), ((uint64_t)ncand + 1) * 8));
     S2H_TRY(put(w.tasks, hb.tasks.data(), hb.tasks.size() * sizeof(FoldTask)));
+    S2H_TRY(put(w.fcands, hb.fcands.data(), hb.fcands.size() * sizeof(FftdCand)));
     // new vertices: one contiguous block in candidate order
     if (hb.newxyz_total) {
         for (uint32_t c = 0; c < ncand; ++c) {
@@ -1083,6 +1196,7 @@ static s2h_status pick_row_chunk(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seed
         set_row_chunk(hb, it->second);
         return S2H_OK;
     }
+    if (hb.hdr.nitems == 0) return S2H_OK;  // no direct delta counts to tune
     const char *env = std::getenv("S2HIP_AUTOTUNE");
     if (!ctx->autotune || (env && env[0] == '0') || ncand < 16) return S2H_OK;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1097,10 +1211,10 @@ static s2h_status pick_row_chunk(s2h_ctx *ctx, BatchWS &w, s2h_seed *const *seed
     for (uint32_t rc : {32u, 64u, 128u, 256u, 512u}) {
         set_row_chunk(hb, rc);
         st = upload_batch(ctx, w, hb, cands, ncand, seeds);
-        if (st == S2H_OK) st = run_sequence(ctx, w);  // warm-up (and capture)
+        if (st == S2H_OK) st = run_sequence(ctx, w, hb, seeds);  // warm-up (and capture)
         hipError_t e = hipSuccess;
         if (st == S2H_OK) e = hipEventRecord(e0, ctx->stream);
-        if (st == S2H_OK && e == hipSuccess) st = run_sequence(ctx, w);
+        if (st == S2H_OK && e == hipSuccess) st = run_sequence(ctx, w, hb, seeds);
         if (st == S2H_OK && e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
         if (st == S2H_OK && e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0;
@@ -1132,7 +1246,7 @@ static s2h_status evaluate(s2h_ctx *ctx, s2h_seed *const *seeds, uint32_t nseeds
     S2H_TRY(upload_batch(ctx, w, hb, cands, ncand, seeds));
     ctx->launches = 0;
     prof_mark(ctx, 0);
-    S2H_TRY(run_sequence(ctx, w));
+    S2H_TRY(run_sequence(ctx, w, hb, seeds));
     prof_mark(ctx, 1);
     prof_mark(ctx, 2);
     S2H_TRY(download_batch(ctx, w, hb, ncand, out));

@@ -30,6 +30,8 @@
 #include <cstdlib>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <set>
 
 #include "s2hip_internal.h"
 
@@ -755,6 +757,191 @@ __global__ void fft_gather(const Offset *__restrict__ offs, uint64_t n, const T 
     out[i] = cube[((uint64_t)ix * nyo + iy) * nzo + iz];
 }
 
+// ---------------------------------------------------------------- delta counts by region correlation
+//
+// A candidate changes the seed occupancy o to n = o + delta inside its box B
+// (delta = A - D in {-1, 0, 1}).  Its pair-count change at offset d is
+//   dC(d) = sum_u [o(u) delta(u+d) + delta(u) o(u+d) + delta(u) delta(u+d)],
+// whose transform over a frame is 2 Re(conj(O) X) + |X|^2 (O, X: spectra of o
+// and delta).  The frame of candidate c starts at B's corner minus R (R: the
+// shell extent per axis) and has lengths F >= |B| + 2R: delta lives in
+// [R, R + |B|) and every |d| <= R term of the circular correlation is the
+// linear one (a wrapped u + d lands in [0, R) or [R + |B|, F), where delta is
+// zero), whatever o holds elsewhere in the frame.  The frame reads o from
+// the seed volume (zero outside it) and delta from the candidate's (A, D) rows.
+
+// z forward of frame rows: line (c, i, j) is the complex row o + i delta of
+// candidate c at frame x = i, y = j; its transform splits into the half
+// spectra of o (field 0) and delta (field 1), written to g1 planes
+// (2c + field) F_x + i.
+template <int R, class V>
+__device__ __forceinline__ void fftd_first_stage(V *d, int S, int sh, int np, int p0, const FftdCand *__restrict__ cands,
+                                                 const uint64_t *__restrict__ words, int nx, int ny, int nz,
+                                                 int row_words, const uint2 *__restrict__ ad, int nwd, int Rx, int Ry,
+                                                 int Rz, FastDiv dplane, FastDiv dFy, const Fac &fc,
+                                                 const V *__restrict__ tw) {
+    using T = decltype(V::x);
+    const FastDiv dmp = fc.mp[0];
+    const int mp = (int)dmp.n;
+    for (int t = threadIdx.x; t < mp * np; t += blockDim.x) {
+        int pl, b, c, rem, i, j;
+        divmod(t, dmp, pl, b);
+        divmod(p0 + pl, dplane, c, rem);
+        divmod(rem, dFy, i, j);
+        const FftdCand cd = cands[c];
+        const int x = cd.b0[0] - Rx + i, y = cd.b0[1] - Ry + j, z0 = cd.b0[2] - Rz;
+        const bool orow = x >= 0 && x < nx && y >= 0 && y < ny;
+        const bool drow = x >= cd.b0[0] && x < cd.b1[0] && y >= cd.b0[1] && y < cd.b1[1];
+        const uint64_t *ro = words + (orow ? ((uint64_t)x * ny + y) * row_words : 0);
+        const uint2 *rd = ad + (drow ? cd.ad_off + ((uint64_t)(x - cd.b0[0]) * (cd.b1[1] - cd.b0[1]) +
+                                                    (y - cd.b0[1])) * nwd
+                                     : 0);
+        V a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int z = z0 + b + mp * r;
+            T re = 0, im = 0;
+            if (z >= 0 && z < nz) {
+                if (orow) re = (T)((ro[z >> 6] >> (z & 63)) & 1ull);
+                if (drow) {
+                    const uint2 w = rd[z >> 5];
+                    im = (T)(int)(((w.x >> (z & 31)) & 1u)) - (T)(int)(((w.y >> (z & 31)) & 1u));
+                }
+            }
+            a[r] = mk<V>(re, im);
+        }
+        small_dft<-1>(a, R);
+        if (mp > 1) {
+            const V *w = tw + fc.tw[0] + b * (R - 1);
+#pragma unroll
+            for (int q = 1; q < R; ++q) a[q] = cmul(a[q], w[q - 1]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[pl * S + pidx(b + mp * r, sh)] = a[r];
+    }
+    __syncthreads();
+}
+
+template <class T>
+__global__ __launch_bounds__(kFftThreads) void fftd_z_fwd(const FftdCand *__restrict__ cands, int nlines,
+                                                          const uint64_t *__restrict__ words, int nx, int ny, int nz,
+                                                          int row_words, const uint2 *__restrict__ ad, int nwd, int Rx,
+                                                          int Ry, int Rz, int Fx, int Fy, int H, int P, Fac fc, int L,
+                                                          const C2<T> *__restrict__ tw, const int *__restrict__ pos,
+                                                          int lines_per_block, C2<T> *__restrict__ g1, int sh,
+                                                          FastDiv dplane, FastDiv dFy, FastDiv dh) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    V *d = reinterpret_cast<V *>(sm_raw);
+    const int S = lstride(L, sh);
+    const int p0 = blockIdx.x * lines_per_block;
+    const int np = min(lines_per_block, nlines - p0);
+    S2H_RADIX_SWITCH(fc.f[0], (fftd_first_stage<RR>(d, S, sh, np, p0, cands, words, nx, ny, nz, row_words, ad, nwd,
+                                                     Rx, Ry, Rz, dplane, dFy, fc, tw)));
+    for (int s = 1; s < fc.n; ++s) fft_stage_any<false>(d, S, sh, np, fc, s, tw);
+    for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
+        int pl, k, c, rem, i, j;
+        divmod(t, dh, pl, k);
+        divmod(p0 + pl, dplane, c, rem);
+        divmod(rem, dFy, i, j);
+        const V zk = d[pl * S + pidx(pos[k], sh)];
+        const V zn = d[pl * S + pidx(pos[k == 0 ? 0 : L - k], sh)];
+        const T half = (T)0.5;
+        const V A = mk<V>(half * (zk.x + zn.x), half * (zk.y - zn.y));   // spectrum of o
+        const V B = mk<V>(half * (zk.y + zn.y), -half * (zk.x - zn.x));  // spectrum of delta
+        const uint64_t row = ((uint64_t)(2 * c) * Fx + i) * Fy + j;
+        g1[row * P + k] = A;
+        g1[(row + (uint64_t)Fx * Fy) * P + k] = B;
+    }
+}
+
+// Last forward stage of both fields' lines (line l: o, line l + lines: delta),
+// 2 Re(conj(O) X) + |X|^2, and the first inverse stage, in registers; the
+// result replaces the o line.
+template <int R, class V>
+__device__ __forceinline__ void cross_stage(V *d, int S, int sh, int lines, const FastDiv dnb) {
+    const int nb = (int)dnb.n;
+    for (int t = threadIdx.x; t < nb * lines; t += blockDim.x) {
+        int line, b;
+        divmod(t, dnb, line, b);
+        V *bo = d + line * S, *bx = d + (line + lines) * S;
+        V a[R], x[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            a[r] = bo[pidx(b * R + r, sh)];
+            x[r] = bx[pidx(b * R + r, sh)];
+        }
+        small_dft<-1>(a, R);
+        small_dft<-1>(x, R);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            a[r] = mk<V>(2 * (a[r].x * x[r].x + a[r].y * x[r].y) + (x[r].x * x[r].x + x[r].y * x[r].y), 0);
+        small_dft<1>(a, R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) bo[pidx(b * R + r, sh)] = a[r];
+    }
+    __syncthreads();
+}
+
+// x pass of the frames: one block per (candidate, ky, kz tile); both fields'
+// lines forward, the cross spectrum, the inverse keeping the 2R_x + 1 needed
+// x (g2 planes c (2R_x + 1) + ix).
+template <class T>
+__global__ __launch_bounds__(kFftMaxThreads) void fftd_x_cross(int Fx, int Fy, int H, int P, int tiles_kz, int tile,
+                                                            Fac fc, const C2<T> *__restrict__ tw, int Rx,
+                                                            const C2<T> *__restrict__ g1, C2<T> *__restrict__ g2,
+                                                            int sh) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    V *d = reinterpret_cast<V *>(sm_raw);
+    const int L = Fx, S = lstride(L, sh), lgt = log2_if_pow2(tile);
+    const int per = Fy * tiles_kz;
+    const int c = blockIdx.x / per, rem = blockIdx.x % per;
+    const int ky = rem / tiles_kz, kz0 = (rem % tiles_kz) * tile;
+    const int nk = min(tile, H - kz0);
+    const uint64_t rs = (uint64_t)Fy * P, field = (uint64_t)Fx * Fy * P;
+    const V *go = g1 + (uint64_t)(2 * c) * field + (uint64_t)ky * P + kz0;
+    const int last = fc.n - 1;
+    S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, go, rs, L, tw + fc.tw[0])));
+    S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d + tile * S, L, S, sh, tile, lgt, nk, go + field, rs, L,
+                                                   tw + fc.tw[0])));
+    for (int s = 1; s < last; ++s) fft_stage_any<false>(d, S, sh, 2 * tile, fc, s, tw);
+    switch (fc.f[last]) {
+        case 8: cross_stage<8>(d, S, sh, tile, fc.nb[last]); break;
+        case 4: cross_stage<4>(d, S, sh, tile, fc.nb[last]); break;
+        case 2: cross_stage<2>(d, S, sh, tile, fc.nb[last]); break;
+        case 3: cross_stage<3>(d, S, sh, tile, fc.nb[last]); break;
+        default: cross_stage<5>(d, S, sh, tile, fc.nb[last]); break;
+    }
+    for (int s = last - 1; s >= 1; --s) fft_stage_any<true>(d, S, sh, tile, fc, s, tw);
+    const int nxo = 2 * Rx + 1;
+    S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk,
+                                                   g2 + (uint64_t)c * nxo * rs + (uint64_t)ky * P + kz0, rs, Rx,
+                                                   tw + fc.tw[0])));
+}
+
+// Delta counts of the candidates: the value at each shell offset rounded to
+// the nearest integer, stored at the offset's canonical index (d and -d share
+// one; the zero offset has none).
+__global__ void fftd_gather(const FftdCand *__restrict__ cands, uint32_t ncand, const Offset *__restrict__ offs,
+                            const int32_t *__restrict__ sg, uint64_t noffs, const double *__restrict__ cube, int Rx,
+                            int Ry, int Rz, int32_t *__restrict__ delta) {
+    const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
+    const uint64_t per = (uint64_t)(2 * Rx + 1) * nyo * nzo, total = (uint64_t)ncand * noffs;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = t / noffs, i = t - c * noffs;
+        const int32_t k = sg[i];
+        if (k < 0) continue;
+        const Offset o = offs[i];
+        const int ix = o.dx >= 0 ? o.dx : 2 * Rx + 1 + o.dx;
+        const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
+        const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
+        const double v = cube[c * per + ((uint64_t)ix * nyo + iy) * nzo + iz];
+        delta[cands[c].delta_off + k] = (int32_t)rint(v);
+    }
+}
+
 // ---------------------------------------------------------------- host plan
 
 AxisPlan axis_plan(int L) {
@@ -857,6 +1044,43 @@ std::tuple<uint64_t, uint64_t, uint64_t> fft_plan_key(const uint64_t dims[3], co
                            dims[1] | ((uint64_t)R[1] << 32), dims[2] | ((uint64_t)R[2] << 32));
 }
 
+// The per-length tables of a plan whose f[] and ax[] are set: each axis'
+// stage twiddle rows in the plan's precision, and the z axis' DIF output order.
+s2h_status fft_plan_tables(s2h_ctx *ctx, FftPlan *p) {
+    const int real_bytes = p->real_bytes;
+    for (int a = 0; a < 3; ++a) {
+        const int L = (int)p->f[a];
+        std::vector<double2> tw(L);
+        std::vector<float2> tw32(L);
+        for (int k = 0; k < L; ++k) {
+            const double ang = -2.0 * M_PI * (double)k / (double)L;
+            tw[k] = make_double2(std::cos(ang), std::sin(ang));
+            tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
+        }
+        const std::vector<double2> rows = stage_twiddles(p->ax[a], tw);
+        const std::vector<float2> rows32 = stage_twiddles(p->ax[a], tw32);
+        S2H_TRY(real_bytes == 8 ? upload(ctx, p->tw[a], rows.data(), rows.size() * sizeof(double2))
+                                : upload(ctx, p->tw[a], rows32.data(), rows32.size() * sizeof(float2)));
+    }
+    const AxisPlan &az = p->ax[2];  // natural frequency held at each DIF output position (z axis)
+    std::vector<int> rev(az.L);
+    for (int pos = 0; pos < az.L; ++pos) {
+        int k = 0, mult = 1, m = az.L, q = pos;
+        for (int s = 0; s < az.nf; ++s) {
+            const int mp = m / az.fac[s];
+            k += (q / mp) * mult;
+            q %= mp;
+            mult *= az.fac[s];
+            m = mp;
+        }
+        rev[pos] = k;
+    }
+    std::vector<int> posv(az.L);
+    for (int pos = 0; pos < az.L; ++pos) posv[rev[pos]] = pos;
+    S2H_TRY(upload(ctx, p->rev_z, rev.data(), rev.size() * sizeof(int)));
+    return upload(ctx, p->pos_z, posv.data(), posv.size() * sizeof(int));
+}
+
 // Bytes of the plan's big buffers: g1 (n_x planes of f_y rows of the half
 // spectrum), g2 (the 2R_x+1 kept x planes), g3 (the kept (x, y) rows) and the
 // real cube of kept offsets; rows are spectrum_pitch complex values apart.
@@ -893,39 +1117,8 @@ s2h_status fft_plan_get(s2h_ctx *ctx, const uint64_t dims[3], const int R[3], in
         fft_plan_release(ctx, p);
         return st;
     };
-    for (int a = 0; a < 3; ++a) {
-        const int L = (int)p->f[a];
-        std::vector<double2> tw(L);
-        std::vector<float2> tw32(L);
-        for (int k = 0; k < L; ++k) {
-            const double ang = -2.0 * M_PI * (double)k / (double)L;
-            tw[k] = make_double2(std::cos(ang), std::sin(ang));
-            tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
-        }
-        const std::vector<double2> rows = stage_twiddles(p->ax[a], tw);
-        const std::vector<float2> rows32 = stage_twiddles(p->ax[a], tw32);
-        s2h_status st = real_bytes == 8 ? upload(ctx, p->tw[a], rows.data(), rows.size() * sizeof(double2))
-                                        : upload(ctx, p->tw[a], rows32.data(), rows32.size() * sizeof(float2));
-        if (st != S2H_OK) return cleanup(st);
-    }
-    {   // natural frequency held at each DIF output position (z axis)
-        const AxisPlan &az = p->ax[2];
-        std::vector<int> rev(az.L);
-        for (int pos = 0; pos < az.L; ++pos) {
-            int k = 0, mult = 1, m = az.L, q = pos;
-            for (int s = 0; s < az.nf; ++s) {
-                const int mp = m / az.fac[s];
-                k += (q / mp) * mult;
-                q %= mp;
-                mult *= az.fac[s];
-                m = mp;
-            }
-            rev[pos] = k;
-        }
-        std::vector<int> posv(az.L);
-        for (int pos = 0; pos < az.L; ++pos) posv[rev[pos]] = pos;
-        s2h_status st = upload(ctx, p->rev_z, rev.data(), rev.size() * sizeof(int));
-        if (st == S2H_OK) st = upload(ctx, p->pos_z, posv.data(), posv.size() * sizeof(int));
+    {
+        const s2h_status st = fft_plan_tables(ctx, p);
         if (st != S2H_OK) return cleanup(st);
     }
     uint64_t sizes[4];
@@ -959,9 +1152,19 @@ size_t lds_bytes(int L, int lines, int sh) {
     return (size_t)lines * lstride(L, sh) * sizeof(C2<T>);
 }
 
+// Allow a kernel the whole 160 KB of LDS as dynamic shared memory, once per
+// kernel and process: the limit is per function, so setting each launch's own
+// size would let a context on another thread lower it under a launch in
+// flight.  Occupancy follows each launch's actual size.
 s2h_status set_lds(const void *fn, size_t bytes) {
-    if (bytes > 160 * 1024) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
-    S2H_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    constexpr size_t kMax = 160 * 1024;
+    if (bytes > kMax) return fail(S2H_ERR_UNSUPPORTED, "FFT tile exceeds LDS");
+    static std::mutex mu;
+    static std::set<const void *> done;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done.count(fn)) return S2H_OK;
+    S2H_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMax));
+    done.insert(fn);
     return S2H_OK;
 }
 
@@ -1215,6 +1418,196 @@ s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, FftCube
     *cube = FftCube();
     cube->f64 = (const double *)fp->cube.ptr;
     for (int a = 0; a < 3; ++a) cube->R[a] = fp->R[a];
+    return S2H_OK;
+}
+
+// ---------------------------------------------------------------- delta counts by region correlation (host)
+
+namespace {
+
+constexpr uint64_t kFftdChunkBytes = 2ull << 30;  // frame buffers of one launch sequence
+
+// Smooth length of least f x (stages + 2) from need (at least 16) to twice it.
+int fftd_length(int need) {
+    const uint64_t lo = (uint64_t)std::max(need, 16), hi = 2 * lo;
+    uint64_t best = 0;
+    double best_cost = INFINITY;
+    for (uint64_t p2 = 1; p2 <= hi; p2 *= 2)
+        for (uint64_t p23 = p2; p23 <= hi; p23 *= 3)
+            for (uint64_t v = p23; v <= hi; v *= 5) {
+                if (v < lo || v > 8192) continue;
+                const AxisPlan ap = axis_plan((int)v);
+                if (ap.nf < 2) continue;
+                const double cost = (double)v * (ap.nf + 2);
+                if (cost < best_cost || (cost == best_cost && v < best)) {
+                    best_cost = cost;
+                    best = v;
+                }
+            }
+    return (int)best;
+}
+
+// Radix-2-equivalent levels of a length (as in fft64_error_bound).
+double fft_levels(int L) {
+    const AxisPlan a = axis_plan(L);
+    double lv = 0;
+    for (int s = 0; s < a.nf; ++s)
+        lv += a.fac[s] == 8 ? 3 : a.fac[s] == 4 ? 2 : a.fac[s] == 2 ? 1 : a.fac[s] == 3 ? 3 : 4;
+    return lv;
+}
+
+// The frame plan: twiddles and z order for lengths F, tiles chosen untimed
+// (the y passes the widest tile whose LDS fits, the x pass -- two fields'
+// lines per column -- the narrowest whole-line tile), and grow-only frame
+// buffers.
+s2h_status fftd_plan_get(s2h_ctx *ctx, const int F[3], const int R[3], FftPlan **out) {
+    const auto key = std::make_tuple((uint64_t)F[0] | ((uint64_t)R[0] << 32), (uint64_t)F[1] | ((uint64_t)R[1] << 32),
+                                     (uint64_t)F[2] | ((uint64_t)R[2] << 32));
+    auto it = ctx->fftd_plans.find(key);
+    if (it != ctx->fftd_plans.end()) {
+        *out = it->second;
+        return S2H_OK;
+    }
+    auto *p = new FftPlan();
+    p->real_bytes = 8;
+    for (int a = 0; a < 3; ++a) {
+        p->dims[a] = (uint64_t)F[a];
+        p->f[a] = (uint64_t)F[a];
+        p->R[a] = R[a];
+        p->ax[a] = axis_plan(F[a]);
+        if (p->ax[a].nf < 2 || F[a] > 8192) {
+            delete p;
+            return fail(S2H_ERR_UNSUPPORTED, "frame length " + std::to_string(F[a]) + " unsupported");
+        }
+    }
+    const s2h_status st = fft_plan_tables(ctx, p);
+    if (st != S2H_OK) {
+        fft_plan_release(ctx, p);
+        return st;
+    }
+    p->tile_y = 1;
+    for (int t = kTileMax; t >= 1; t /= 2)
+        if (lds_bytes<double>(F[1], t, kNoPad) <= 150 * 1024) {
+            p->tile_y = t;
+            break;
+        }
+    p->tile_x = 1;
+    for (int t = 1; t <= kTileMax; t *= 2)
+        if (lds_bytes<double>(F[0], 2 * t, kNoPad) <= 150 * 1024) {
+            p->tile_x = t;
+            if ((size_t)t * sizeof(double2) >= 128) break;
+        }
+    p->sh_z = pad_rule<double>(F[2], z_lines_per_block<double>(F[2]));
+    p->sh_y = lds_bytes<double>(F[1], p->tile_y, kPad) <= 150 * 1024 ? pad_rule<double>(F[1], p->tile_y) : kNoPad;
+    p->sh_x = lds_bytes<double>(F[0], 2 * p->tile_x, kPad) <= 150 * 1024 ? pad_rule<double>(F[0], 2 * p->tile_x)
+                                                                       : kNoPad;
+    ctx->fftd_plans[key] = p;
+    *out = p;
+    return S2H_OK;
+}
+
+}  // namespace
+
+bool fft_delta_frame(const int bmax[3], const int R[3], int F[3]) {
+    for (int a = 0; a < 3; ++a) {
+        F[a] = fftd_length(std::max(bmax[a], 1) + 2 * R[a]);
+        if (F[a] <= 0) return false;
+    }
+    return lds_bytes<double>(std::max(F[0], F[1]), 2, kNoPad) <= 150 * 1024 &&
+           lds_bytes<double>(F[2], 1, kNoPad) <= 150 * 1024;
+}
+
+// Worst-case bound on |computed - exact| of every frame delta value.  With K_o
+// ones of o and K_x nonzeros of delta in the frame (N points), the spectra
+// satisfy |O_k| <= K_o, |X_k| <= K_x, ||O||_2 = sqrt(N K_o), ||X||_2 =
+// sqrt(N K_x), and the forward passes leave ||O^ - O||_2 <= rho ||O||_2 and
+// ||X^ - X||_2 <= rho ||X||_2, rho = 8u L with L the radix-2-equivalent levels
+// of the three axes plus two for the packing and splitting along z (as for
+// the autocorrelation).  The cross spectrum P = 2 Re(conj(O) X) + |X|^2,
+// formed with three roundings per element, then has ||P^ - P||_2 <=
+// (2 rho + 2 gamma_3)(K_o sqrt(N K_x)) + 2 rho K_x sqrt(N K_o) + (3 rho +
+// gamma_3) K_x sqrt(N K_x) up to second-order terms, and the scaled inverse
+// divides the 2-norm by sqrt(N) and adds rho ||P||_2 / sqrt(N) <= rho (2 K_o
+// sqrt(K_x) + K_x sqrt(K_x)).  Every value's error is at most the 2-norm:
+// (4 rho + 2 gamma_3) (K_o sqrt(K_x) + K_x sqrt(K_o) + K_x^{3/2}), taken here
+// with 5 rho + 8u and 2 % slack for the second-order terms, plus one rounding
+// of the scaling on a value of magnitude at most 2 K_x.
+double fft_delta_error_bound(const int F[3], uint64_t ones, uint64_t changed) {
+    constexpr double u = 1.1102230246251565e-16;
+    const double L = fft_levels(F[0]) + fft_levels(F[1]) + fft_levels(F[2]) + 2.0;
+    const double rho = 8.0 * u * L;
+    const double ko = (double)ones, kx = (double)changed;
+    return 1.02 * (5.0 * rho + 8.0 * u) * (ko * std::sqrt(kx) + kx * std::sqrt(ko) + kx * std::sqrt(kx)) +
+           2.0 * u * kx;
+}
+
+// Bytes the frame passes move per candidate (each reads its input and writes
+// its output once) at the FP64 passes' modelled rate, plus launch time.
+double fft_delta_model_ms(const int F[3], const int R[3], uint32_t ncand) {
+    const double fx = F[0], fy = F[1], h = F[2] / 2 + 1;
+    const double xo = 2.0 * R[0] + 1, yo = 2.0 * R[1] + 1, zo = 2.0 * R[2] + 1;
+    const double bytes = 2 * fx * fy * h * 16 * 3 + xo * fy * h * 16 * 2 + xo * yo * h * 16 * 2 + xo * yo * zo * 16;
+    return ncand * bytes / 4.0e12 * 1e3 + 0.04;
+}
+
+s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands, uint32_t ncand, const int F[3],
+                         const uint2 *ad, int nwd, int32_t *delta) {
+    using V = double2;
+    if (ncand == 0) return S2H_OK;
+    FftPlan *p = nullptr;
+    S2H_TRY(fftd_plan_get(ctx, F, sd.R, &p));
+    const int Fx = F[0], Fy = F[1], Fz = F[2];
+    const int H = Fz / 2 + 1, P = (int)spectrum_pitch((uint64_t)Fz, 8);
+    const int Rx = sd.R[0], Ry = sd.R[1], Rz = sd.R[2];
+    const int nxo = 2 * Rx + 1, nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
+    const uint64_t b1 = 2ull * Fx * Fy * P * 16, b2 = (uint64_t)nxo * Fy * P * 16, b3 = (uint64_t)nxo * nyo * P * 16,
+                   b4 = (uint64_t)nxo * nyo * nzo * 8;
+    uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(ncand, kFftdChunkBytes / (b1 + b2 + b3 + b4)));
+    // every line and block count of a launch stays below 2^31
+    const uint64_t per_cand_lines = std::max<uint64_t>({2ull * Fx * Fy, (uint64_t)nxo * nyo, (uint64_t)Fy * H});
+    chunk = std::min<uint64_t>(chunk, ((1ull << 31) - 1) / per_cand_lines);
+    S2H_TRY(dev_alloc(ctx, p->g1, chunk * b1));
+    S2H_TRY(dev_alloc(ctx, p->g2, chunk * b2));
+    S2H_TRY(dev_alloc(ctx, p->g3, chunk * b3));
+    S2H_TRY(dev_alloc(ctx, p->cube, chunk * b4));
+    const Fac fcx = to_fac(p->ax[0]), fcy = to_fac(p->ax[1]), fcz = to_fac(p->ax[2]);
+    hipStream_t s = ctx->stream;
+    const int ty = p->tile_y, tx = p->tile_x;
+    const int tiles_y = (H + ty - 1) / ty, tiles_x = (H + tx - 1) / tx;
+    const int ppb = z_lines_per_block<double>(Fz);
+    const size_t lds_z = lds_bytes<double>(Fz, ppb, p->sh_z), lds_y = lds_bytes<double>(Fy, ty, p->sh_y),
+                 lds_x = lds_bytes<double>(Fx, 2 * tx, p->sh_x);
+    S2H_TRY(set_lds((const void *)fftd_z_fwd<double>, lds_z));
+    S2H_TRY(set_lds((const void *)fft_y_fwd<double>, lds_y));
+    S2H_TRY(set_lds((const void *)fftd_x_cross<double>, lds_x));
+    S2H_TRY(set_lds((const void *)fft_y_inv<double>, lds_y));
+    S2H_TRY(set_lds((const void *)fft_z_inv<double>, lds_z));
+    const double scale = 1.0 / ((double)Fx * (double)Fy * (double)Fz);
+    const V *tw0 = (const V *)p->tw[0].ptr, *tw1 = (const V *)p->tw[1].ptr, *tw2 = (const V *)p->tw[2].ptr;
+    V *g1 = (V *)p->g1.ptr, *g2 = (V *)p->g2.ptr, *g3 = (V *)p->g3.ptr;
+    double *cube = (double *)p->cube.ptr;
+    for (uint64_t c0 = 0; c0 < ncand; c0 += chunk) {
+        const int nc = (int)std::min<uint64_t>(chunk, ncand - c0);
+        const FftdCand *cc = cands + c0;
+        const int zl = nc * Fx * Fy;
+        fftd_z_fwd<double><<<(zl + ppb - 1) / ppb, kFftThreads, lds_z, s>>>(
+            cc, zl, sd.words, sd.dims[0], sd.dims[1], sd.dims[2], (int)sd.row_words, ad, nwd, Rx, Ry, Rz, Fx, Fy, H, P,
+            fcz, Fz, tw2, (const int *)p->pos_z.ptr, ppb, g1, p->sh_z, fast_div((uint32_t)(Fx * Fy)),
+            fast_div((uint32_t)Fy), fast_div((uint32_t)H));
+        fft_y_fwd<double><<<nc * 2 * Fx * tiles_y, pass_threads(Fy, fcy, ty), lds_y, s>>>(Fy, Fy, H, P, tiles_y, ty,
+                                                                                          fcy, tw1, g1, p->sh_y);
+        fftd_x_cross<double><<<nc * Fy * tiles_x, pass_threads(Fx, fcx, tx), lds_x, s>>>(Fx, Fy, H, P, tiles_x, tx,
+                                                                                         fcx, tw0, Rx, g1, g2, p->sh_x);
+        fft_y_inv<double><<<nc * nxo * tiles_y, pass_threads(Fy, fcy, ty), lds_y, s>>>(Fy, H, P, tiles_y, ty, fcy, tw1,
+                                                                                       Ry, g2, g3, p->sh_y);
+        const int zi = nc * nxo * nyo;
+        fft_z_inv<double><<<((zi + 1) / 2 + ppb - 1) / ppb, kFftThreads, lds_z, s>>>(
+            zi, H, P, fcz, Fz, tw2, (const int *)p->rev_z.ptr, ppb, Rz, scale, g3, cube, p->sh_z,
+            fast_div((uint32_t)nzo));
+        fftd_gather<<<2048, 256, 0, s>>>(cc, (uint32_t)nc, sd.offs, sd.sg, sd.noffs, cube, Rx, Ry, Rz, delta);
+        S2H_HIP(hipGetLastError());
+        ctx->launches += 6;
+    }
     return S2H_OK;
 }
 

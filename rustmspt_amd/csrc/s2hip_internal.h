@@ -432,6 +432,7 @@ struct s2h_ctx {
         exact_plans;
     std::map<std::tuple<uint32_t, uint64_t>, std::unique_ptr<s2h::McPlan>> mc_plans;
     std::map<std::tuple<uint64_t, uint64_t, uint64_t>, s2h::FftPlan *> fft_plans;
+    std::map<std::tuple<uint64_t, uint64_t, uint64_t>, s2h::FftPlan *> fftd_plans;  // region-correlation frames
     s2h::DevBuf d_scalar;   // small scratch (popcount result etc.)
     bool profiling = false;
     bool autotune = true;   // time launch-shape candidates on a new exact plan
@@ -483,6 +484,37 @@ void exact_plan_release(s2h_ctx *ctx, ExactPlan *plan);
 void shell_devs_release(s2h_ctx *ctx);  // device copies of shared shell data
 // Launch the count kernels for vol into plan->d_table (zeroed first) on ctx->stream.
 s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan);
+// Delta counts of batch candidates by FP64 correlation over a frame around
+// each changed box (s2hip_fft.hip).  A candidate: its changed box [b0, b1)
+// in voxels, its (A, D) rows in the batch's change buffer (rows of the box,
+// x-major, nwd 32-bit word pairs each) and its delta table (canonical shell
+// order).
+struct FftdCand {
+    int32_t b0[3], b1[3];
+    uint64_t ad_off, delta_off;
+};
+// The seed side: packed volume, shell extent, shell-order offsets and their
+// canonical indices (-1: the zero offset), occupied voxels.
+struct FftdSeed {
+    const uint64_t *words = nullptr;
+    uint64_t row_words = 0;
+    int dims[3] = {0, 0, 0};
+    int R[3] = {0, 0, 0};
+    const int32_t *sg = nullptr;
+    const Offset *offs = nullptr;
+    uint64_t noffs = 0;
+    uint64_t occupied = 0;
+};
+// Frame lengths for changed boxes up to bmax (false: unsupported lengths).
+bool fft_delta_frame(const int bmax[3], const int R[3], int F[3]);
+// Worst-case |computed - exact| of a frame's delta values (exact counts below 1/2).
+double fft_delta_error_bound(const int F[3], uint64_t ones, uint64_t changed);
+// Modelled device time (ms) of ncand candidates on frames F.
+double fft_delta_model_ms(const int F[3], const int R[3], uint32_t ncand);
+// Fill the delta tables of ncand candidates (device array) on ctx->stream.
+s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &seed, const FftdCand *cands, uint32_t ncand, const int F[3],
+                         const uint2 *ad, int nwd, int32_t *delta);
+
 // The pruned correlation cube an FFT evaluation leaves on the device: values
 // of offsets |dx| <= R[0], |dy| <= R[1], |dz| <= R[2], x-major, each axis
 // holding 0 .. R then the negative offsets -R .. -1.  FP32 values are used as

@@ -436,6 +436,7 @@ void s2h_destroy(s2h_ctx *ctx) {
         dev_free(ctx, kv.second->d_hv);
     }
     for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
+    for (auto &kv : ctx->fftd_plans) fft_plan_release(ctx, kv.second);
     dev_free(ctx, ctx->fft_arena);
     dev_free(ctx, ctx->d_scalar);
     batch_ws_release(ctx);

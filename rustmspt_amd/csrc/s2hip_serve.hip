@@ -230,6 +230,8 @@ void exact_plans_clear(s2h_ctx *ctx) {
     shell_devs_release(ctx);
     for (auto &kv : ctx->fft_plans) fft_plan_release(ctx, kv.second);
     ctx->fft_plans.clear();
+    for (auto &kv : ctx->fftd_plans) fft_plan_release(ctx, kv.second);
+    ctx->fftd_plans.clear();
 }
 
 }  // namespace s2h

@@ -136,3 +136,51 @@ def test_particle_with_two_components(ctx):
     moves = [(0, 0, pair.vertices + np.array([1.25, -0.5, 0.75])), (0, 1, meshes[1].vertices - 1.0)]
     for (si, i, newv), got in zip(moves, batch.batch_eval(ctx, [sd], moves)):
         assert np.array_equal(got, full_eval(ctx, sd, i, newv))
+
+
+@pytest.mark.parametrize("mode", ["fft", "direct"])
+def test_region_correlation_and_direct_deltas_agree(ctx, monkeypatch, mode):
+    """Delta counts by FP64 correlation over frames around the changed boxes
+    (forced with S2HIP_BATCH_DELTA=fft) and by direct counting give the same
+    curves, equal to full evaluations: mixed grids, anisotropic grids,
+    particles crossing the grid faces, moves far out of the grid, overlaps."""
+    seeds, parts_all, rngs = [], [], []
+    for g, n, r in [(28, 6, 7), (44, 12, 11), (60, 16, 15)]:
+        sd, parts, rng = make_seed(ctx, g, n, 5 * g + 1, r, overlap=True)
+        seeds.append(sd)
+        parts_all.append(parts)
+        rngs.append(rng)
+    rng = synth.rng_for(12, 7)
+    aparts = synth.place_particles(7, 30, rng, synth.ellipsoid_maker(2.5, 4.5), overlap=True)
+    # an anisotropic grid with particles cut by its faces
+    aparts += [synth.Particle(np.array(c, float), np.array([4.0, 2.5, 3.0]), synth.random_rotation(rng))
+               for c in ([0.5, 10, 10], [29.6, 20, 5], [12, 0.2, 23.5])]
+    seeds.append(batch.Seed(ctx, [p.mesh() for p in aparts], s2.BoundingBox((0, 0, 0), (30, 22, 26)), 1.0, 8))
+    parts_all.append(aparts)
+    rngs.append(rng)
+    moves = []
+    for k in range(28):
+        si = k % 4
+        i = int(rngs[si].integers(0, len(parts_all[si])))
+        reach = 9.0 if k % 7 == 0 else 2.5
+        moves.append((si, i, random_move(rngs[si], parts_all[si][i], reach)))
+    moves.append((0, 1, parts_all[0][1].moved(np.array([80.0, 0, 0]), np.eye(3)).mesh().vertices))
+    monkeypatch.setenv("S2HIP_BATCH_DELTA", mode)
+    got = batch.batch_eval(ctx, seeds, moves)
+    monkeypatch.setenv("S2HIP_BATCH_DELTA", "direct" if mode == "fft" else "fft")
+    other = batch.batch_eval(ctx, seeds, moves)
+    for k, ((si, i, newv), a, b) in enumerate(zip(moves, got, other)):
+        assert np.array_equal(a, b), k
+        if k % 3 == 0:
+            assert np.array_equal(a, full_eval(ctx, seeds[si], i, newv)), k
+
+
+def test_accept_with_region_correlation_deltas(ctx, monkeypatch):
+    monkeypatch.setenv("S2HIP_BATCH_DELTA", "fft")
+    sd, parts, rng = make_seed(ctx, 48, 12, 29, 12, overlap=True)
+    for step in range(3):
+        i = int(rng.integers(0, len(parts)))
+        newv = random_move(rng, parts[i])
+        want = full_eval(ctx, sd, i, newv)
+        assert np.array_equal(sd.accept(i, newv), want), step
+        assert np.array_equal(sd.exact_s2(), want)
