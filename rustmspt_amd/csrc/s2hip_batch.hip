@@ -736,8 +736,12 @@ static constexpr double kDirectMsPerUnit = 2.0e-10;
 
 // Per seed, the delta counts of its candidates: direct counting, or region
 // correlation where its frames are supported, provably exact (error bound
-// below 1/4) and modelled faster.  S2HIP_BATCH_DELTA=direct or =fft forces
-// one (fft still only where supported and exact).
+// below 1/4) and modelled faster.  Each candidate's own frame is the least
+// supported length of box + 2R per axis; candidates are sorted by frame
+// volume and grouped, each group on the largest frame of its members, a new
+// group starting where that would exceed 1.2 x the volume of the group's
+// first frame.  S2HIP_BATCH_DELTA=direct or =fft forces one (fft still only
+// where supported and exact).
 static void choose_delta_modes(s2h_seed *const *seeds, uint32_t nseeds, HostBatch &hb) {
     const size_t ncand = hb.cd.size();
     hb.cfftd.assign(ncand, 0);
@@ -746,39 +750,73 @@ static void choose_delta_modes(s2h_seed *const *seeds, uint32_t nseeds, HostBatc
     const char *env = std::getenv("S2HIP_BATCH_DELTA");
     const int force = !env ? 0 : std::strcmp(env, "fft") == 0 ? 2 : std::strcmp(env, "direct") == 0 ? 1 : 0;
     if (force == 1) return;
-    std::vector<uint32_t> mine;
+    struct Own {
+        uint32_t c;
+        int F[3];
+        double vol;
+        uint64_t changed;
+    };
+    std::vector<Own> mine;
     for (uint32_t si = 0; si < nseeds; ++si) {
         const s2h_seed *sd = seeds[si];
-        mine.clear();
-        int bmax[3] = {1, 1, 1};
-        uint64_t changed = 1;
-        double direct_ms = 0.0;
-        for (uint32_t c = 0; c < ncand; ++c) {
-            const CandDev &d = hb.cd[c];
-            if (d.seed != si) continue;
-            mine.push_back(c);
-            const int b[3] = {d.bx1 - d.bx0, d.by1 - d.by0, d.bz1 - d.bz0};
-            for (int a = 0; a < 3; ++a) bmax[a] = std::max(bmax[a], b[a]);
-            changed = std::max<uint64_t>(changed, (uint64_t)b[0] * b[1] * b[2]);
-            direct_ms += kDirectMsPerUnit * (double)b[0] * b[1] * (double)sd->ncanon * (b[2] / 32.0 + 2.0);
-        }
-        if (mine.empty()) continue;
         const int *ext = sd->plan->shells->host->extent;
         const int R[3] = {ext[0], ext[1], ext[2]};
-        int F[3];
-        if (!fft_delta_frame(bmax, R, F)) continue;
-        if (fft_delta_error_bound(F, sd->occupied, changed) > 0.25) continue;
-        if (force != 2 && !(fft_delta_model_ms(F, R, (uint32_t)mine.size()) < direct_ms)) continue;
-        HostBatch::FftdGroup g;
-        g.seed = si;
-        g.start = (uint32_t)hb.fcands.size();
-        g.count = (uint32_t)mine.size();
-        for (int a = 0; a < 3; ++a) g.F[a] = F[a];
-        hb.groups.push_back(g);
-        for (uint32_t c : mine) {
+        mine.clear();
+        double direct_ms = 0.0;
+        bool ok = true;
+        for (uint32_t c = 0; c < ncand && ok; ++c) {
             const CandDev &d = hb.cd[c];
-            hb.cfftd[c] = 1;
-            hb.fcands.push_back(FftdCand{{d.bx0, d.by0, d.bz0}, {d.bx1, d.by1, d.bz1}, d.ad_off, d.delta_off});
+            if (d.seed != si) continue;
+            const int b[3] = {d.bx1 - d.bx0, d.by1 - d.by0, d.bz1 - d.bz0};
+            direct_ms += kDirectMsPerUnit * (double)b[0] * b[1] * (double)sd->ncanon * (b[2] / 32.0 + 2.0);
+            Own o;
+            o.c = c;
+            o.changed = std::max<uint64_t>(1, (uint64_t)b[0] * b[1] * b[2]);
+            ok = fft_delta_frame(b, R, o.F);
+            o.vol = (double)o.F[0] * o.F[1] * o.F[2];
+            mine.push_back(o);
+        }
+        if (!ok || mine.empty()) continue;
+        std::sort(mine.begin(), mine.end(), [](const Own &a, const Own &b) { return a.vol < b.vol; });
+        // groups over the sorted candidates
+        std::vector<HostBatch::FftdGroup> gs;
+        std::vector<uint64_t> gchanged;
+        double fft_ms = 0.0;
+        for (size_t k = 0; k < mine.size();) {
+            HostBatch::FftdGroup g;
+            g.seed = si;
+            g.start = (uint32_t)k;  // position in `mine` for now
+            for (int a = 0; a < 3; ++a) g.F[a] = mine[k].F[a];
+            const double first = mine[k].vol;
+            uint64_t changed = mine[k].changed;
+            size_t e = k + 1;
+            for (; e < mine.size(); ++e) {
+                const int F2[3] = {std::max(g.F[0], mine[e].F[0]), std::max(g.F[1], mine[e].F[1]),
+                                   std::max(g.F[2], mine[e].F[2])};
+                if ((double)F2[0] * F2[1] * F2[2] > 1.2 * first) break;
+                for (int a = 0; a < 3; ++a) g.F[a] = F2[a];
+                changed = std::max(changed, mine[e].changed);
+            }
+            g.count = (uint32_t)(e - k);
+            fft_ms += fft_delta_model_ms(g.F, R, g.count);
+            gs.push_back(g);
+            gchanged.push_back(changed);
+            k = e;
+        }
+        bool exact = true;
+        for (size_t i = 0; i < gs.size(); ++i)
+            exact = exact && fft_delta_error_bound(gs[i].F, sd->occupied, gchanged[i]) <= 0.25;
+        if (!exact) continue;
+        if (force != 2 && !(fft_ms < direct_ms)) continue;
+        for (HostBatch::FftdGroup &g : gs) {
+            const uint32_t first = g.start;
+            g.start = (uint32_t)hb.fcands.size();
+            for (uint32_t k = first; k < first + g.count; ++k) {
+                const CandDev &d = hb.cd[mine[k].c];
+                hb.cfftd[mine[k].c] = 1;
+                hb.fcands.push_back(FftdCand{{d.bx0, d.by0, d.bz0}, {d.bx1, d.by1, d.bz1}, d.ad_off, d.delta_off});
+            }
+            hb.groups.push_back(g);
         }
     }
 }
