@@ -154,6 +154,94 @@ def cpu_baseline(occ_words, dims, r_max, offs, pair_work_total, target_s):
     return out
 
 
+def unpack_occupancy(words, dims):
+    """uint8 grid from packed rows (row_words = nz / 64 + 1 words, bit z % 64 of word z / 64)."""
+    import numpy as np
+
+    nx, ny, nz = dims
+    rw = (nz + 63) // 64 + 1
+    bits = np.unpackbits(np.ascontiguousarray(words, "<u8").view(np.uint8), bitorder="little")
+    return np.ascontiguousarray(bits.reshape(nx, ny, rw * 64)[:, :, :nz])
+
+
+def cpu_legs(ctx, c2_words, c2_dims, c2_r, c2_hits, ct, target_s):
+    """The further CPU timings beside the GPU: (1) the f64 FFT kernel forced at
+    the single large measure, where the reference's planner would not take it
+    (its working set exceeds the 768 MiB budget) -- autocorrelation on the
+    reference's padding, counts at every shell offset, the curve; (2) the
+    plain measure (64 icospheres, 128^3, r_max 32) with the kernel the
+    reference's planner picks (the FFT there), all threads and 8; (3) the
+    CT-like volume (768^3, r_max 128): the direct kernel on every 997th
+    in-domain offset in shell order, extrapolated by pair work.  Each leg
+    checks its counts or curve against the GPU's."""
+    import numpy as np
+
+    from oracle import oracle as o
+    from rustmspt_amd import s2, synth
+
+    out = {}
+    threads = o.num_threads()
+    offs, rs = s2.shell_order_offsets(c2_dims, c2_r, 1.0)
+    occ = unpack_occupancy(c2_words, c2_dims)
+    padded = o.padded_fft_dims(c2_dims)
+    t0 = time.perf_counter()
+    corr = o.fft_autocorr(occ, padded)
+    hits = o.fft_hits(corr, offs)
+    del corr
+    vf = float(occ.sum()) / occ.size
+    o.curve_from_hits(c2_dims, c2_r, offs, rs, hits, vf)
+    dt = time.perf_counter() - t0
+    plan = o.plan_exact_cpu(c2_dims, c2_r, 1.0, threads)
+    out["single_large_fft_forced"] = {
+        "seconds_per_eval": dt, "threads": threads, "padded": padded, "counts_equal_gpu": bool(np.array_equal(hits, c2_hits)),
+        "planner_pick": plan["method"], "planner_fft_bytes": plan["fft_bytes"],
+        "note": "f64 FFT kernel of the oracle (mixed-radix 2,3,5, OpenMP over lines) on the reference's padding"}
+    progress(f"cpu: forced FFT at {c2_dims[0]}^3 {dt:.1f} s")
+    del occ, hits
+    # plain measure with the planner's kernel
+    mesh, bbox, _ = synth.config_plain(seed=1, n=64, grid=128)
+    occ1 = o.voxelize(mesh.vertices, mesh.faces, [0.0, 0.0, 0.0], 1.0, [128, 128, 128])
+    vol = s2.Volume.from_bytes(ctx, occ1)
+    gpu_curve = vol.exact_s2(32, 1.0)
+    vol.free()
+    offs1, rs1 = s2.shell_order_offsets([128] * 3, 32, 1.0)
+    leg = {}
+    for th in sorted({threads, 8}):
+        o.set_threads(th)
+        pick = o.plan_exact_cpu([128] * 3, 32, 1.0, th)
+        t0 = time.perf_counter()
+        if pick["method"] == "fft":
+            corr = o.fft_autocorr(occ1)
+            h1 = o.fft_hits(corr, offs1)
+            del corr
+            curve = o.curve_from_hits([128] * 3, 32, offs1, rs1, h1, float(occ1.mean()))
+        else:
+            curve = o.exact_s2(occ1, 32)
+        leg[f"seconds_per_eval_{th}_threads"] = time.perf_counter() - t0
+        leg["planner_pick"] = pick["method"]
+        leg["curve_equal_gpu"] = bool(np.array_equal(curve, gpu_curve))
+    o.set_threads(threads)
+    out["plain_measure_planner"] = leg
+    progress("cpu: plain measure done")
+    # CT-like volume: stride sample of the direct kernel
+    if ct is not None:
+        ct_words, ct_dims, ct_r, ct_hits_of = ct
+        offs5, _ = s2.shell_order_offsets(ct_dims, ct_r, 1.0)
+        valid = np.prod(np.array(ct_dims, dtype=np.float64)[None, :] - np.abs(offs5), axis=1)
+        idx = np.arange(0, len(offs5), 997)
+        t0 = time.perf_counter()
+        h5 = o.hits_list(ct_words, ct_dims, offs5[idx])
+        dt = time.perf_counter() - t0
+        W, w = float(valid.sum()), float(valid[idx].sum())
+        out["ct_volume_direct_sample"] = {
+            "seconds_per_eval": dt * W / w, "sample_seconds": dt, "threads": threads,
+            "sample": f"every 997th of {len(offs5)} in-domain offsets in shell order ({len(idx)} offsets, "
+                      f"{w:.3e} of {W:.3e} pair-tests), extrapolated by pair work",
+            "counts_equal_gpu": bool(np.array_equal(h5, ct_hits_of(idx)))}
+        progress(f"cpu: CT volume sample {dt:.1f} s")
+    return out
+
+
 def batched_leg(ctx, ncand: int, seed: int, plain_per_grid: int = 8):
     """256 one-move candidates over 4 resident seed packings (grids 96, 128, 192,
     256; 150-400 ellipsoids each; r_max = grid/4), evaluated in one batch, vs the
@@ -542,6 +630,8 @@ def main():
     ap.add_argument("--particles", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-legs", action="store_true",
+                    help="skip the forced-FFT, planner-pick and CT-volume CPU timings")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-candidates", type=int, default=256)
     ap.add_argument("--no-fft", action="store_true")
@@ -669,11 +759,15 @@ def main():
     if not args.no_mc:
         out["mesh_monte_carlo"] = mesh_mc_leg(ctx, mesh, bbox, r_max, args.mc_samples, seed=11)
         progress("mesh monte carlo done")
+    ct_cpu = None
     if not args.no_ingest:
         out["stl_ingest"] = stl_ingest_leg(ctx, mesh, bbox)
         ct_vol, out["ingest"] = ingest_leg(ctx, args.ingest_grid, seed=5, keep=args.ct_r_max > 0)
         if ct_vol is not None:
             out["ct_volume"] = ct_leg(ctx, ct_vol, args.ct_r_max, steps=2)
+            if not args.no_cpu:
+                ct_hits = ct_vol.exact_counts(args.ct_r_max, 1.0)
+                ct_cpu = (ct_vol.read_packed(), list(ct_vol.dims), args.ct_r_max, lambda idx: ct_hits[idx])
             ct_vol.free()
         progress("ingest and CT volume done")
     if not args.no_batch:
@@ -684,6 +778,10 @@ def main():
     if not args.no_cpu:
         words = vol.read_packed()
         out["cpu_baseline"] = cpu_baseline(words, [grid] * 3, r_max, offs, pair_work, args.cpu_seconds)
+        progress("cpu: direct sample done")
+        if not args.no_cpu_legs:
+            out["cpu_baseline"]["legs"] = cpu_legs(ctx, words, [grid] * 3, r_max, vol.exact_counts(r_max, 1.0),
+                                                   ct_cpu, args.cpu_seconds)
         out["cpu_baseline"]["gpu_speedup"] = out["value"] / out["cpu_baseline"]["value"]
         out["cpu_baseline"]["gpu_direct_speedup"] = direct["evals_per_s"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))

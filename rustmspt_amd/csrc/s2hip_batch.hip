@@ -1203,6 +1203,7 @@ static s2h_status download_batch(s2h_ctx *ctx, BatchWS &w, const HostBatch &hb, 
     uint8_t *sup = p + hb.curve_total * 8;
     int *occ = (int *)(p + (hb.curve_total * 9 + 15) / 16 * 16);
     uint32_t *nover = (uint32_t *)(p + (hb.curve_total * 9 + 15) / 16 * 16 + ((uint64_t)ncand * 4 + 15) / 16 * 16);
+    S2H_TRY(inject(kFailReadback));
     S2H_HIP(hipMemcpyAsync(vals, w.curves.ptr, hb.curve_total * 8, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipMemcpyAsync(sup, curve_support(w), hb.curve_total, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipMemcpyAsync(occ, w.occ.ptr, (uint64_t)ncand * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1437,6 +1438,7 @@ s2h_status s2h_batch_eval(s2h_ctx *ctx, s2h_seed *const *seeds, uint32_t nseeds,
                           uint32_t ncand, double *curves, uint64_t ncurve) {
     if (!ctx || (!seeds && nseeds) || (!cands && ncand) || (!curves && ncurve)) return fail(S2H_ERR_ARG, "null argument");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     batch::HostBatch hb;
     batch::BatchOut out;
     return batch::evaluate(ctx, seeds, nseeds, cands, ncand, curves, ncurve, hb, out);
@@ -1449,6 +1451,7 @@ s2h_status s2h_seed_accept(s2h_ctx *ctx, s2h_seed *sd, uint32_t particle, const 
     if (particle >= sd->nparticles) return fail(S2H_ERR_ARG, "particle index out of range");
     if (curve && n != sd->r_max + 1) return fail(S2H_ERR_SIZE, "curve buffer must hold r_max+1 values");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     std::vector<double> tmp(sd->r_max + 1);
     s2h_seed *const seeds[1] = {sd};
     const s2h_candidate cand{0, particle, new_xyz};

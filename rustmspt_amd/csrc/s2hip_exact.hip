@@ -1084,6 +1084,7 @@ s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *p, double vf, doubl
     const uint32_t n = p->r_max + 1;
     void *stage = nullptr;  // page-locked: the copy is one DMA the stream sync waits for
     S2H_TRY(pinned_staging(ctx, (uint64_t)n * 9, &stage));
+    S2H_TRY(inject(kFailReadback));
     S2H_HIP(hipMemcpyAsync(stage, p->d_curve.ptr, (size_t)n * 9, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
@@ -1107,6 +1108,7 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
                             uint64_t n_offsets) {
     if (!ctx || !vol || (!hits && n_offsets)) return fail(S2H_ERR_ARG, "null argument");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     ExactPlan *p = nullptr;
     S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pitch, &p));
     if (p->offs.size() != n_offsets)
@@ -1117,6 +1119,7 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
     FftCube cube;
     S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &cube));
     S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube));
+    S2H_TRY(inject(kFailReadback));
     if (n_offsets)
         S2H_HIP(hipMemcpyAsync(hits, p->d_hits.ptr, n_offsets * 8, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
@@ -1173,6 +1176,7 @@ s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     if (!ctx || !vol || !s2) return fail(S2H_ERR_ARG, "null argument");
     if (n != (uint64_t)r_max + 1) return fail(S2H_ERR_SIZE, "s2 buffer must hold r_max+1 values");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     uint64_t occupied = 0;
     S2H_TRY(vol_popcount(ctx, vol, &occupied));
     if (occupied == 0) {

@@ -1682,6 +1682,7 @@ s2h_status s2h_fft32_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
     if (!ctx || !vol || (!counts && n_offsets)) return fail(S2H_ERR_ARG, "null argument");
     if (!(pitch > 0.0) || !std::isfinite(pitch)) return fail(S2H_ERR_ARG, "pitch must be finite and > 0");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     ExactPlan *ep = nullptr;
     S2H_TRY(exact_plan_get(ctx, vol->dims, r_max, pitch, &ep));
     if (ep->offs.size() != n_offsets)
@@ -1691,6 +1692,7 @@ s2h_status s2h_fft32_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
     S2H_TRY(fft_counts_device<float>(ctx, vol, ep, &fp, true));
     prof_mark(ctx, 1);
     prof_mark(ctx, 2);
+    S2H_TRY(inject(kFailReadback));
     if (n_offsets)
         S2H_HIP(hipMemcpyAsync(counts, fp->fcounts.ptr, n_offsets * 4, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
@@ -1702,6 +1704,7 @@ s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     if (!ctx || !vol || !s2) return fail(S2H_ERR_ARG, "null argument");
     if (n != (uint64_t)r_max + 1) return fail(S2H_ERR_SIZE, "s2 buffer must hold r_max+1 values");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     uint64_t occupied = 0;
     S2H_TRY(vol_popcount(ctx, vol, &occupied));
     if (occupied == 0) {

@@ -18,6 +18,11 @@ namespace s2h {
 void set_error(const std::string &msg);
 s2h_status fail(s2h_status code, const std::string &msg);
 
+// Failure injection for error-path tests (S2HIP_FAIL_AT): S2H_ERR_HIP when
+// this pass through the site is the one the variable names, else S2H_OK.
+enum FailSite { kFailAlloc = 0, kFailLaunch = 1, kFailReadback = 2 };
+s2h_status inject(FailSite site);
+
 #define S2H_HIP(expr)                                                                   \
     do {                                                                                \
         hipError_t e_ = (expr);                                                         \

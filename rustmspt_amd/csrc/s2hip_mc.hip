@@ -113,6 +113,7 @@ static s2h_status mc_run(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitc
             (uint32_t)(seed >> 32), (uint32_t)bpr, (unsigned long long *)p->d_hv.ptr);
         S2H_HIP(hipGetLastError());
     }
+    S2H_TRY(inject(kFailReadback));
     S2H_HIP(hipMemcpyAsync(hv.data(), p->d_hv.ptr, hv.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
     return S2H_OK;
@@ -322,6 +323,7 @@ s2h_status s2h_mc_counts_noalloc(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, dou
                                         " table entries; the no-allocation scratch holds " +
                                         std::to_string(l.capacity));
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     if (ctx->device < 0 || ctx->device >= kNoallocMaxDevices || !g_noalloc_ptr[ctx->device])
         S2H_TRY(noalloc_scratch_init(ctx->device));
     unsigned char *base = (unsigned char *)g_noalloc_ptr[ctx->device];
@@ -350,6 +352,7 @@ s2h_status s2h_mc_counts_noalloc(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, dou
         S2H_HIP(hipGetLastError());
     }
     unsigned long long flags[2] = {0, 0};
+    S2H_TRY(inject(kFailReadback));
     S2H_HIP(hipMemcpyAsync(flags, hdr, sizeof(flags), hipMemcpyDeviceToHost, s));
     S2H_HIP(hipMemcpyAsync(hits_valid, hv, n * 8, hipMemcpyDeviceToHost, s));
     S2H_HIP(hipStreamSynchronize(s));
@@ -365,6 +368,7 @@ s2h_status s2h_mc_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitc
     if (n != 2 * ((uint64_t)r_max + 1)) return fail(S2H_ERR_SIZE, "hits_valid must hold 2*(r_max+1) entries");
     if (!(pitch > 0.0) || !std::isfinite(pitch)) return fail(S2H_ERR_ARG, "pitch must be finite and > 0");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     std::vector<uint64_t> hv;
     S2H_TRY(mc_run(ctx, vol, r_max, pitch, samples, seed, hv));
     std::memcpy(hits_valid, hv.data(), n * 8);
@@ -376,6 +380,7 @@ s2h_status s2h_mc_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch, u
     if (!ctx || !vol || !s2) return fail(S2H_ERR_ARG, "null argument");
     if (n != (uint64_t)r_max + 1) return fail(S2H_ERR_SIZE, "s2 buffer must hold r_max+1 values");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     uint64_t occupied = 0;
     S2H_TRY(vol_popcount(ctx, vol, &occupied));
     if (occupied == 0) {

@@ -527,11 +527,14 @@ s2h_status bin_and_run(s2h_ctx *ctx, Grid g, double cmax, double smax, double am
     }
     ctx->launches = 7 + 5 * passes;
     prof_mark(ctx, 2);
-    if (e == hipSuccess) e = hipMemcpyAsync(hits_valid, d_hv.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream);
+    const s2h_status injected = e == hipSuccess ? inject(kFailReadback) : S2H_OK;
+    if (e == hipSuccess && injected == S2H_OK)
+        e = hipMemcpyAsync(hits_valid, d_hv.ptr, n * 8, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     prof_collect(ctx);
     release_pass();
     release();
+    if (injected != S2H_OK) return injected;
     if (e != hipSuccess) return fail(S2H_ERR_HIP, std::string("mesh Monte Carlo: ") + hipGetErrorString(e));
     return S2H_OK;
 }
@@ -563,6 +566,7 @@ s2h_status s2h_mesh_mc_counts(s2h_ctx *ctx, const double *xyz, uint64_t nverts, 
         if (tri[f] >= nverts) return fail(S2H_ERR_ARG, "vertex index out of range");
     if (ntris > 0x7FFFFFFFull) return fail(S2H_ERR_UNSUPPORTED, "mesh too large");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     std::fill(hits_valid, hits_valid + n, 0);
     if (r_max == 0) return S2H_OK;
     // whole-mesh vertex bbox (mesh_bbox, src/geometry/bbox.rs:8-25) and amax
@@ -595,6 +599,7 @@ s2h_status s2h_mesh_mc_counts_resident(s2h_ctx *ctx, const s2h_mesh *m, const do
     if (!ctx || !m) return fail(S2H_ERR_ARG, "null argument");
     S2H_TRY(check_mc_args(bbox_min, bbox_max, r_max, hits_valid, n));
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     std::fill(hits_valid, hits_valid + n, 0);
     if (r_max == 0) return S2H_OK;
     Grid g{};

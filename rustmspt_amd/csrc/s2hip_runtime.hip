@@ -1,5 +1,7 @@
 // Context, device-memory accounting, packed volumes and the C-ABI glue shared by
 // every path of the S2 engine.
+#include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -16,6 +18,34 @@ s2h_status fail(s2h_status code, const std::string &msg) {
     g_last_error = msg;
     return code;
 }
+
+// Injected failures (S2HIP_FAIL_AT=alloc|launch|readback[:n]): the n-th (default
+// first) pass through that site after the variable last changed returns
+// S2H_ERR_HIP before any device work there, so callers' error paths can be
+// exercised without a faulting kernel.  The variable is read on each pass;
+// the counters are process-wide.
+s2h_status inject(FailSite site) {
+    static std::mutex mu;
+    static std::string cached;
+    static std::atomic<int> armed{-1};
+    static uint64_t at = 1, seen = 0;
+    const char *env = std::getenv("S2HIP_FAIL_AT");
+    if (!env && armed < 0) return S2H_OK;
+    std::lock_guard<std::mutex> lk(mu);
+    const std::string v = env ? env : "";
+    if (v != cached) {
+        cached = v;
+        const size_t colon = v.find(':');
+        const std::string name = v.substr(0, colon);
+        armed = name == "alloc" ? kFailAlloc : name == "launch" ? kFailLaunch : name == "readback" ? kFailReadback : -1;
+        at = colon == std::string::npos ? 1 : std::max<uint64_t>(1, std::strtoull(v.c_str() + colon + 1, nullptr, 10));
+        seen = 0;
+    }
+    if (armed.load() != (int)site || ++seen != at) return S2H_OK;
+    static const char *names[] = {"allocation", "launch", "read-back"};
+    return fail(S2H_ERR_HIP, std::string("injected ") + names[site] + " failure (S2HIP_FAIL_AT)");
+}
+
 
 // Device memory: blocks released by a context are kept in its cache and
 // reused for later requests of a similar size.  hipFree synchronizes the whole
@@ -41,6 +71,7 @@ static void cache_trim(s2h_ctx *ctx, uint64_t keep) {
 
 s2h_status dev_alloc(s2h_ctx *ctx, DevBuf &b, uint64_t bytes) {
     if (b.ptr && !b.borrowed && b.bytes >= bytes) return S2H_OK;
+    S2H_TRY(inject(kFailAlloc));
     dev_free(ctx, b);
     if (bytes == 0) bytes = 8;
     const uint64_t want = round_block(bytes);

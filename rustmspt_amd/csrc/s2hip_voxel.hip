@@ -144,6 +144,7 @@ extern "C" s2h_status s2h_voxelize(s2h_ctx *ctx, const double *xyz, uint64_t nve
         return fail(S2H_ERR_ARG, "null argument");
     if (!(pitch > 0.0) || !std::isfinite(pitch)) return fail(S2H_ERR_ARG, "pitch must be finite and > 0");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     for (uint64_t f = 0; f < ntris; ++f) {
         if (granule_of_tri[f] >= ngranules) return fail(S2H_ERR_ARG, "granule id out of range");
         for (int k = 0; k < 3; ++k)

@@ -20,6 +20,7 @@ import numpy as np
 from . import config as cfg
 from . import report
 from . import s2
+from ._lib import S2HipError
 from .devmesh import DeviceMesh
 from .mesh import BoundingBox, granules, load_stl_or_merge_folder, mesh_bbox
 
@@ -67,15 +68,23 @@ class MeasurePipeline:
         # With a HIP-capable mode the STL is welded and split into granules on the
         # device (the resident mesh is voxelized without host preparation); the
         # host copy read back serves the volume fraction and the mesh MC.
+        mesh = particle_count = None
         if requested != cfg.AccelerationMode.CPU and self.env.get("S2HIP_HOST_INGEST", "0") != "1":
-            budget = (p.acceleration.gpu_memory_limit_mb or 0) * 1024 * 1024
-            ctx = res["ctx"] = s2.Context(0, budget)
-            dmesh = res["dmesh"] = DeviceMesh.load_stl(ctx, p.stl_path)
-            mesh, _ = dmesh.read()
-            particle_count = dmesh.ngranules
-        else:
+            try:
+                budget = (p.acceleration.gpu_memory_limit_mb or 0) * 1024 * 1024
+                ctx = res["ctx"] = s2.Context(0, budget)
+                dmesh = res["dmesh"] = DeviceMesh.load_stl(ctx, p.stl_path)
+                mesh, _ = dmesh.read()
+                particle_count = dmesh.ngranules
+            except S2HipError as err:
+                # the host loader reads the same file into the same mesh
+                print(f"[Warning] device STL ingest: {err}; loading on the host", file=sys.stderr)
+                if res["dmesh"] is not None:
+                    res["dmesh"].free()
+                dmesh = res["dmesh"] = None
+                mesh = particle_count = None
+        if mesh is None:
             mesh = load_stl_or_merge_folder(p.stl_path)
-            particle_count = None
         timer.stage("load")
         bbox = None
         for box in (p.bounding_box, p.stl_bounding_box):
@@ -127,23 +136,33 @@ class MeasurePipeline:
                 raise CpuPathUnavailable(
                     f"measure {m}: the policy selected the CPU backend, which is the host crate's own "
                     f"implementation and not part of this engine")
-            if ctx is None:
-                budget = (p.acceleration.gpu_memory_limit_mb or 0) * 1024 * 1024
-                ctx = res["ctx"] = s2.Context(0, budget)
             seed = int(self.env.get("RUSTMSPT_S2_SEED", "0"))
-            if mesh_mc:
-                hv = s2.mesh_mc_counts(ctx, mesh, bbox, p.r_max, samples, seed).astype(np.float64)
-                curve = np.array([vf] + [hv[r, 0] / hv[r, 1] if hv[r, 1] else 0.0
-                                         for r in range(1, p.r_max + 1)])
-            else:
-                if grid is None:
-                    self.log("[Info] Preparing shared HIP S2 occupancy grid")
-                    if dmesh is not None:
-                        grid = s2.VoxelS2Hip.from_volume(dmesh.voxelize(bbox, max(pitch, 1e-9)), pitch)
-                    else:
-                        grid = s2.VoxelS2Hip(ctx, mesh, bbox, pitch)
-                    res["grid"] = grid
-                curve = grid.calculate_seeded(p.r_max, m, samples, seed)
+            try:
+                if ctx is None:
+                    budget = (p.acceleration.gpu_memory_limit_mb or 0) * 1024 * 1024
+                    ctx = res["ctx"] = s2.Context(0, budget)
+                if mesh_mc:
+                    hv = s2.mesh_mc_counts(ctx, mesh, bbox, p.r_max, samples, seed).astype(np.float64)
+                    curve = np.array([vf] + [hv[r, 0] / hv[r, 1] if hv[r, 1] else 0.0
+                                             for r in range(1, p.r_max + 1)])
+                else:
+                    if grid is None:
+                        self.log("[Info] Preparing shared HIP S2 occupancy grid")
+                        if dmesh is not None:
+                            grid = s2.VoxelS2Hip.from_volume(dmesh.voxelize(bbox, max(pitch, 1e-9)), pitch)
+                        else:
+                            grid = s2.VoxelS2Hip(ctx, mesh, bbox, pitch)
+                        res["grid"] = grid
+                    curve = grid.calculate_seeded(p.r_max, m, samples, seed)
+            except S2HipError as err:
+                # a failing device run: an error without cpu_fallback, else the
+                # reference's warning and its CPU path (src/pipeline/measure.rs:225-246)
+                if not p.acceleration.cpu_fallback:
+                    raise cfg.GpuError(f"measure {m}: {err}") from err
+                print(f"[Warning] measure {m}: {err}; falling back to CPU {m}", file=sys.stderr)
+                raise CpuPathUnavailable(
+                    f"measure {m}: the HIP run failed and the CPU fallback is the host crate's own "
+                    f"implementation, not part of this engine") from err
             backends.append(f"{m}=hip/{ctx.device_name}")
             self.log(f"[Info] Measure execution: method={m}, backend=hip, device={ctx.device_name}")
             curves.append(curve)
