@@ -27,16 +27,18 @@ sys.path.insert(0, str(ROOT))
 # that (the on-device probe measures it), so the roofline's peak is the probe's
 # figure and this one is reported beside it.
 VALU_4CYCLE_OPS = 256 * 64 * 2.4e9
-# HBM bytes of one evaluation's count phase at 256^3 r_max=64 (500-particle
-# packing, launch shape band 3 / 65536 waves), from rocprofv3 --pmc FETCH_SIZE
-# (x2.0) and WRITE_SIZE (x1.0) of the count dispatch; factors from a 512 MiB
-# device copy.  Command lines and figures: profiles/exact_count_kernel_256cube_r64.md.
-COUNT_PHASE_TRAFFIC = 14840 * 1024 * 2.0 + 862900 * 1024 * 1.0
-# The same for the FP64 FFT kernel at 256^3 r_max=64: FETCH_SIZE + WRITE_SIZE
-# summed over its six dispatches (z fwd, y fwd, x, y inv, z inv, gather), raw
-# KiB -- for these LDS-tiled passes the raw FETCH_SIZE already counts the bytes
-# read (the y forward pass fetches at least the 263,168 KiB it must read).
-FFT64_PHASE_TRAFFIC = (1519355 + 1363687) * 1024.0
+# HBM bytes per evaluation, from rocprofv3 --pmc on this workload (256^3,
+# r_max 64, the 500-particle packing): reads = 32 B x TCC_EA0_RDREQ_32B + 64 B x
+# TCC_EA0_RDREQ_64B + 128 B x TCC_EA0_RDREQ_128B, writes = WRITE_SIZE.  On this
+# part FETCH_SIZE counts a 128-byte read request as 64 bytes: a 512 MiB device
+# copy shows 4,194,304 128-byte requests (512 MiB) and FETCH_SIZE 262,156 KiB,
+# WRITE_SIZE 524,288 KiB; the request counts give the bytes read directly.
+# Direct counting (exact_count_kernel, launch shape band 3 / 65536 waves):
+COUNT_PHASE_TRAFFIC = 64 * 59351.5 + 128 * 177855 + 909816 * 1024.0
+# The FP64 FFT passes (z fwd, y fwd, x fwd + |F|^2 + inv, y inv, z inv):
+FFT64_PHASE_TRAFFIC = (3.3 + 169.9 + 176.4 + 212.3 + 220.4 + 107.0 + 111.2 + 43.1 + 44.8 + 17.2) * 1e6
+# The FP32 FFT passes, the same five:
+FFT32_PHASE_TRAFFIC = (3.3 + 86.0 + 92.4 + 107.5 + 115.4 + 54.2 + 58.3 + 21.8 + 23.5 + 8.6) * 1e6
 
 
 def fft64_bytes(dims, offs, r_max, complex_bytes=16.0):
@@ -317,12 +319,18 @@ def fft_leg(ctx, vol, r_max, hits, offs, rs, curve_exact, steps):
     import numpy as np
 
     vol.fft32_s2(r_max, 1.0)  # plan + warm-up
-    dev = []
+    ctx.profile(False)
     t0 = time.perf_counter()
     for _ in range(steps):
         curve = vol.fft32_s2(r_max, 1.0)
-        dev.append(ctx.profile_read()["device_ms"])
     el = time.perf_counter() - t0
+    ctx.profile(True)
+    dev, passes = [], []
+    for _ in range(3):
+        vol.fft32_s2(r_max, 1.0)
+        p = ctx.profile_read()
+        dev.append(p["device_ms"])
+        passes.append(p["count_ms"])
     fc = vol.fft32_counts(r_max, 1.0).astype(np.float64)
     err = np.abs(fc - hits.astype(np.float64))
     per_bin_abs = [0.0] + [float(err[rs[r]:rs[r + 1]].max()) for r in range(1, r_max + 1)]
@@ -333,11 +341,13 @@ def fft_leg(ctx, vol, r_max, hits, offs, rs, curve_exact, steps):
 
     f = s2.fft_padded_dims(vol.dims, r_max, 1.0)
     unpruned, comp = fft64_bytes(vol.dims, offs, r_max, 8.0)
-    dev_s = float(np.mean(dev)) * 1e-3
+    sec = float(np.mean(passes)) * 1e-3  # the FFT passes (z fwd .. z inv)
     return {"evals_per_s": steps / el, "device_ms_per_eval": float(np.mean(dev)),
-            "roofline": {"bound": "hbm", "achieved": comp / dev_s / 1e9, "peak": 8000.0, "unit": "GB/s",
-                         "frac": comp / dev_s / 8e12, "algorithmic_bytes_per_eval": comp,
-                         "unpruned_bytes_per_eval": unpruned, "frac_unpruned": unpruned / dev_s / 8e12,
+            "roofline": {"bound": "hbm", "achieved": comp / sec / 1e9, "peak": 8000.0, "unit": "GB/s",
+                         "frac": comp / sec / 8e12, "kernel": "FP32 FFT passes (z fwd, y fwd, x, y inv, z inv)",
+                         "kernel_ms": sec * 1e3, "algorithmic_bytes_per_eval": comp,
+                         "traffic": FFT32_PHASE_TRAFFIC if (list(vol.dims), r_max) == ([256] * 3, 64) else None,
+                         "unpruned_bytes_per_eval": unpruned, "frac_unpruned": unpruned / sec / 8e12,
                          "padded": f},
             "max_abs_count_error": float(err.max()), "max_rel_count_error": float(max(per_bin_rel)),
             "rounds_to_exact": bool(np.all(np.rint(fc) == hits)),

@@ -1275,7 +1275,7 @@ int pad_rule(int L, int lines) {
 // whose row segment fills a whole 128-byte line (else the widest up to 4), or,
 // on a context that autotunes, for each pass kind the fastest of those
 // candidates timed on the device (HIP events around each pass of one run per
-// candidate, after a warm-up).  A segment narrower than a line still fetches
+// candidate, after a warm-up) among the whole-line tiles.  A segment narrower than a line still fetches
 // the whole line, and the neighbouring tile's workgroup seldom finds it in L2.
 // The y passes favour wide tiles (longer contiguous kz runs per row), the x
 // pass the narrowest whole-line tile (more workgroups per CU).  Every run
@@ -1333,12 +1333,15 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
             if (st == S2H_OK && e != hipSuccess)
                 st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
             if (st != S2H_OK) break;
-            if (yf + yi < best_y) {
+            // only whole-line tiles are taken: a narrower one reads each
+            // 128-byte line once per workgroup that shares it
+            const bool whole = (size_t)t * sizeof(C2<T>) >= 128;
+            if (yf + yi < best_y && whole) {
                 best_y = yf + yi;
                 pick_y = t;
                 psh_y = sh;
             }
-            if (xm < best_x) {
+            if (xm < best_x && whole) {
                 best_x = xm;
                 pick_x = t;
                 psh_x = sh;
