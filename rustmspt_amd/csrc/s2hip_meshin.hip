@@ -679,6 +679,7 @@ s2h_status s2h_mesh_voxelize(s2h_ctx *ctx, const s2h_mesh *m, const double bbox_
     if (!ctx || !m || !bbox_min || !dims || !out) return fail(S2H_ERR_ARG, "null argument");
     if (!(pitch > 0.0) || !(pitch <= 1.7976931348623157e308)) return fail(S2H_ERR_ARG, "pitch must be finite and > 0");
     S2H_HIP(hipSetDevice(ctx->device));
+    S2H_TRY(inject(kFailLaunch));
     if ((double)dims[0] * (double)dims[1] * (double)dims[2] / 256.0 >= 2147483647.0)
         return fail(S2H_ERR_UNSUPPORTED, "grid too large for the resident-mesh voxelizer's tile count");
     s2h_vol *vol = nullptr;
