@@ -28,6 +28,16 @@
  * 1..r_max, each radius' offsets in shell_offset_iter order (dx, then dy, then dz
  * ascending; src/geometry/s2.rs:364-385) -- keeping only in-domain offsets
  * (|d_i| < n_i, src/geometry/s2.rs:1028-1030).
+ *
+ * Environment (read by the library):
+ *   S2HIP_AUTOTUNE=0            launch shapes and FFT tiles from the cost model only
+ *                               (no device timing on a plan's first run)
+ *   S2HIP_BATCH_DELTA=direct|fft  force the batched candidates' delta kernel
+ *                               (default: chosen per seed by a cost model)
+ *   S2HIP_FAIL_AT=alloc|launch|readback[:n]  the n-th (default first) pass through
+ *                               that site after the variable last changed returns
+ *                               S2H_ERR_HIP before any device work there, to exercise
+ *                               callers' error paths (no kernel is affected)
  */
 #ifndef S2HIP_H
 #define S2HIP_H
@@ -389,7 +399,11 @@ s2h_status s2h_seed_accept(s2h_ctx *ctx, s2h_seed *seed, uint32_t particle, cons
  * its own stream, run admitted requests concurrently; a waiting request is
  * admitted as soon as a slot is free and its footprint fits the budget, in
  * arrival order (a request that does not fit is passed over and counted as
- * deferred).  Volumes stay resident in HBM while their request runs. */
+ * deferred).  Volumes stay resident in HBM while their request runs.  With a
+ * budget, a quarter of it is left to the slots' cached shell plans, a quarter
+ * to per-slot FFT arenas reserved at create (an FFT plan whose spectra fit its
+ * slot's arena is carved from it, so no request allocates spectra), and
+ * requests are admitted against the other half. */
 typedef struct s2h_server s2h_server;
 typedef struct {
     const uint64_t *words;  /* packed volume (copied at submit) */

@@ -1392,11 +1392,17 @@ s2h_status fft_counts_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *ep, FftPlan 
 // |c^ - c| <= ||c^ - c||_2 <= (3 rho + 3u) K^{3/2}, rho = L * 8u.  L counts
 // each radix-8, -4 and -2 stage as the 3, 2 and 1 radix-2 levels its
 // butterfly is built of (its internal twiddles are +-1, +-i and (1 +- i)/sqrt 2),
-// a radix-3 stage as 3 levels and a radix-5 stage as 4 (their butterflies
-// satisfy ||y^ - y||_2 <= (sqrt 3 gamma_3 + mu) ||y||_2 < 3 * 8u and
-// (10 gamma_5 / sqrt 5 + mu) ||y||_2 < 4 * 8u: each output is at most three or
-// four roundings deep over terms of modulus at most 2|x_q|), over the three
-// axes, plus two for the real-to-complex packing and unpacking along z.  With
+// a radix-3 stage as 3 levels and a radix-5 stage as 4.  For those two, as
+// small_dft codes them: the input twiddle product errs by at most
+// tau |x_q|, tau = mu + sqrt 2 gamma_2 (1 + mu) <= 4.9u; each real output
+// component is a sum of terms +-k Re x_q, +-k Im x_q with constants k in
+// {1, 1/2, cos, sin} (rounded within u), at most d = 3 (radix 3) or 5
+// (radix 5) roundings deep, and sum |terms| <= sum_q |x_q| <= sqrt p ||x||_2
+// since each x_q enters with a (cos, sin) pair.  So the p outputs err by at
+// most sqrt(2p) gamma_{d+1} ||y||_2 (||y||_2 = sqrt p ||x||_2): eta_3 <=
+// 4.9u + sqrt 6 gamma_4 < 3 * 8u and eta_5 <= 4.9u + sqrt 10 gamma_6 < 4 * 8u.
+// Over the three axes, plus two for the real-to-complex packing and
+// unpacking along z.  With
 // the bound below 1/2 the rounded values are the exact integer counts; the
 // path requires 1/4.
 double fft64_error_bound(const uint64_t dims[3], const int R[3], uint64_t occupied) {
