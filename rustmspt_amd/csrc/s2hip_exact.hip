@@ -226,37 +226,36 @@ __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32
     terms[i] = h / (double)valid;
 }
 
-// One workgroup per radius folds the radius' terms in shell order: the
-// reference's sequential left fold, bit for bit.  The terms are cut into blocks
-// of 512 (64 lanes x kFoldK).  Inside one binade of the running sum S (grid
+// The ordered fold of segments of terms (the curve's radii, or the segments
+// of s2h_ordered_sum): each segment's sequential left fold, bit for bit.  The
+// terms are cut into blocks of 128 (64 lanes x kFoldK), numbered across all
+// segments.  Inside one binade of the running sum S (grid
 // u = 2^(ilogb(S) - 52)), a block's effect on S / u is the sum G of its terms'
 // rounded grid increments -- which depends on u and the terms only -- as long
 // as no term is a tie on the grid or too large and S stays in the binade
-// (fold_block's non-stopping case).  So:
-//   pass 0  all waves: each block's plain f64 sum (any order) and whether it
-//           holds a negative or non-finite term;
-//   pass 0b wave 0: a running sum of those from the exact S so far guesses
-//           the binade at each block's start;
-//   pass 1  all waves: G of each block on its guessed grid, and whether a tie
-//           or a large term (y >= 2^43, so every sum of 512 is exact) occurs;
-//   pass 2  wave 0, in order: a block whose guess matches the exact S's
-//           binade, with no flag and S / u + G < 2^53, adds G (the same
-//           integers fold_block would add); any other block runs fold_block
-//           itself, or one term at a time after a negative or non-finite term
-//           (as a one-wave fold would, block for block).
+// (fold_block's non-stopping case).  Three launches:
+//   fold_sums_kernel   one wave per block: the block's plain f64 sum (any
+//                      order) and whether it holds a negative or non-finite
+//                      term;
+//   fold_incr_kernel   one wave per block: the plain sums of the segment's
+//                      earlier blocks (any order) guess the binade at the
+//                      block's start; G on that grid, and whether a tie or a
+//                      large term (y >= 2^43, so every sum of 128 is exact)
+//                      occurs;
+//   fold_walk_kernel   one wave per segment, in order: a block whose guess
+//                      matches the exact S's binade, with no flag and
+//                      S / u + G < 2^53, adds G (the same integers fold_block
+//                      would add); any other block runs fold_block itself, or
+//                      one term at a time after a negative or non-finite term
+//                      (as a one-wave fold would, block for block).
 // The guesses only pick which blocks take the short path; every value comes
-// from exact arithmetic.
-static constexpr int kFoldK = 8;          // terms per lane per block (512 per block)
-static constexpr int kWideWaves = 16;     // waves per radius
-static constexpr int kWideBlocks = 1024;  // blocks per pass (a longer radius takes several passes)
-static constexpr int kWideAhead = 4;      // blocks a wave loads at once in passes 0 and 1
-static constexpr int kNoBinade = -100000; // guessed start S == 0
-
-struct WideLds {
-    double g[kWideBlocks];     // approximate block sums, then grid increments
-    int32_t m[kWideBlocks];    // guessed binade at the block's start
-    uint32_t flag[kWideBlocks];  // 2: negative or non-finite term; 1: tie or large term
-};
+// from exact arithmetic.  The first two launches spread over the whole GPU; the
+// walk's time is its blocks that are not short (binade crossings, the first
+// blocks of a segment, ties), which cost least at 128 terms per block.
+static constexpr int kFoldK = 2;                         // terms per lane per block
+static constexpr uint32_t kFoldB = 64 * kFoldK;          // terms per block
+static constexpr int kFoldWaves = 4;                     // waves per workgroup of the per-block launches
+static constexpr int kNoBinade = -100000;                // guessed start S == 0
 
 __device__ __forceinline__ double wave_sum_any(double x) {
 #pragma unroll
@@ -275,7 +274,7 @@ __device__ __forceinline__ void load_block(const double *__restrict__ t, uint64_
 }
 
 // A block's terms in any order (lane-interleaved, so each load instruction
-// reads 64 consecutive terms): for the passes whose results do not depend on
+// reads 64 consecutive terms): for the launches whose results do not depend on
 // the order -- the plain block sum that only guesses binades, the exact
 // integer increments and the flags.
 __device__ __forceinline__ void load_block_any(const double *__restrict__ t, uint64_t n, uint64_t start, uint32_t lane,
@@ -288,163 +287,249 @@ __device__ __forceinline__ void load_block_any(const double *__restrict__ t, uin
     }
 }
 
-// The ordered sum of t[0 .. n), valid in wave 0 of the workgroup (all threads call it).
-__device__ __forceinline__ double fold_wide(const double *__restrict__ t, uint64_t n, WideLds &L) {
-    constexpr double kTwo53 = 9007199254740992.0, kTwo43 = 8796093022208.0;
-    constexpr uint64_t B = 64 * kFoldK;
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Device tables and per-block scratch of one segment list: segment g owns
+// blocks bbase[g] .. bbase[g+1] - 1, block b belongs to segment bseg[b].
+struct FoldView {
+    const uint32_t *bbase;
+    const uint32_t *bseg;
+    double *sum;     // plain block sums
+    double *inc;     // grid increments G on the guessed grid
+    int32_t *m;      // guessed binade at the block's start
+    uint32_t *flag;  // 2: negative or non-finite term; 1: tie or large term
+    uint32_t nblk;
+};
+
+// The wave's block: its segment's first term s0, the segment length n and the
+// block's first term within the segment.
+__device__ __forceinline__ uint32_t fold_block_of(const uint64_t *__restrict__ seg_start, const FoldView &v,
+                                                  uint32_t w, uint64_t &s0, uint64_t &n, uint32_t &i) {
+    const uint32_t seg = v.bseg[w];
+    s0 = seg_start[seg];
+    n = seg_start[seg + 1] - s0;
+    i = w - v.bbase[seg];
+    return seg;
+}
+
+__global__ __launch_bounds__(64 * kFoldWaves) void fold_sums_kernel(const double *__restrict__ terms,
+                                                                     const uint64_t *__restrict__ seg_start,
+                                                                     FoldView v) {
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kFoldWaves + threadIdx.x / 64));
+    if (w >= v.nblk) return;
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t s0, n;
+    uint32_t i;
+    fold_block_of(seg_start, v, w, s0, n, i);
+    double x[kFoldK];
+    load_block_any(terms + s0, n, (uint64_t)i * kFoldB, lane, x);
+    double sum = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < kFoldK; ++k) {
+        sum += x[k];
+        bad = bad || !(x[k] >= 0.0 && x[k] <= 1.7976931348623157e308);
+    }
+    sum = wave_sum_any(sum);
+    const bool anybad = __ballot(bad) != 0;
+    if (lane == 0) {
+        v.sum[w] = sum;
+        v.flag[w] = anybad ? 2u : 0u;
+    }
+}
+
+__global__ __launch_bounds__(64 * kFoldWaves) void fold_incr_kernel(const double *__restrict__ terms,
+                                                                     const uint64_t *__restrict__ seg_start,
+                                                                     FoldView v) {
+    constexpr double kTwo43 = 8796093022208.0;
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kFoldWaves + threadIdx.x / 64));
+    if (w >= v.nblk) return;
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t s0, n;
+    uint32_t i;
+    fold_block_of(seg_start, v, w, s0, n, i);
+    const uint32_t b0 = w - i;
+    double pre = 0.0;  // the earlier blocks' plain sums, any order: a guess of S at the block's start
+    for (uint32_t j = lane; j < i; j += 64) pre += v.sum[b0 + j];
+    pre = wave_sum_any(pre);
+    const int m = (pre > 0.0 && pre <= 1.7976931348623157e308) ? ilogb(pre) : kNoBinade;
+    uint32_t flag = v.flag[w];
+    double T = 0.0;
+    if (flag == 0 && m != kNoBinade) {
+        double x[kFoldK];
+        load_block_any(terms + s0, n, (uint64_t)i * kFoldB, lane, x);
+        const double inv_u = ldexp(1.0, 52 - m);
+        bool fl = false;
+#pragma unroll
+        for (int k = 0; k < kFoldK; ++k) {
+            const double y = x[k] * inv_u;  // exact: scaling by a power of two (0 past the end)
+            const bool big = !(y < kTwo43);
+            const double nfl = big ? 0.0 : floor(y);
+            const double f = y - nfl;
+            T += nfl + (f > 0.5 ? 1.0 : 0.0);
+            fl = fl || big || f == 0.5;
+        }
+        T = wave_sum_any(T);  // integers below 2^53: exact in any order
+        if (__ballot(fl) != 0) flag = 1u;
+    }
+    if (lane == 0) {
+        v.inc[w] = T;
+        v.m[w] = m;
+        v.flag[w] = flag;
+    }
+}
+
+// The walk of one segment on one wave (all 64 lanes; the result is
+// wave-uniform).
+__device__ __forceinline__ double fold_walk(const double *__restrict__ t, uint64_t n, const FoldView &v, uint32_t b0,
+                                            uint32_t nb, uint32_t lane) {
+    constexpr double kTwo53 = 9007199254740992.0;
     FoldState st;
     bool seq = false;
-    for (uint64_t c0 = 0; c0 < n; c0 += (uint64_t)kWideBlocks * B) {
-        const uint32_t nb = (uint32_t)min<uint64_t>((uint64_t)kWideBlocks, (n - c0 + B - 1) / B);
-        // kWideAhead blocks per wave at a time: their loads are all in flight together
-        for (uint32_t b0 = wave * kWideAhead; b0 < nb; b0 += kWideWaves * kWideAhead) {
-            double v[kWideAhead][kFoldK];
-#pragma unroll
-            for (int q = 0; q < kWideAhead; ++q)
-                load_block_any(t, n, c0 + (uint64_t)min(b0 + q, nb - 1) * B, lane, v[q]);
-#pragma unroll
-            for (int q = 0; q < kWideAhead; ++q) {
-                double sum = 0.0;
-                bool bad = false;
-#pragma unroll
-                for (int i = 0; i < kFoldK; ++i) {
-                    sum += v[q][i];
-                    bad = bad || !(v[q][i] >= 0.0 && v[q][i] <= 1.7976931348623157e308);
-                }
-                sum = wave_sum_any(sum);
-                const bool anybad = __ballot(bad) != 0;
-                if (lane == 0 && b0 + q < nb) {
-                    L.g[b0 + q] = sum;
-                    L.flag[b0 + q] = anybad ? 2u : 0u;
-                }
-            }
+    for (uint32_t g0 = 0; g0 < nb; g0 += 64) {
+        const uint32_t bl = g0 + lane;
+        double gl = 0.0;
+        int ml = kNoBinade;
+        uint32_t fl = 1;
+        if (bl < nb) {
+            gl = v.inc[b0 + bl];
+            ml = v.m[b0 + bl];
+            fl = v.flag[b0 + bl];
         }
-        __syncthreads();
-        if (wave == 0) {
-            const uint32_t per = (nb + 63) / 64;
-            const uint32_t b0 = min(nb, lane * per), b1 = min(nb, b0 + per);
-            double loc = 0.0;
-            for (uint32_t b = b0; b < b1; ++b) loc += L.g[b];
-            double pre = st.S + wave_exclusive_sum_f64(loc);
-            for (uint32_t b = b0; b < b1; ++b) {
-                L.m[b] = pre > 0.0 ? ilogb(pre) : kNoBinade;
-                pre += L.g[b];
+        const uint32_t cntb = min(64u, nb - g0);
+        for (uint32_t k = 0; k < cntb; ++k) {
+            // The blocks from k on that take the short path, all at once:
+            // lane j's block is short when its guess names the exact binade
+            // (unchanged while the blocks before it are short), it has no
+            // flag, and S / u plus the increments of blocks k .. j stays below
+            // 2^53.  Their increments are added in one step, up to the first
+            // block that is not.
+            const bool live = lane >= k && lane < cntb;
+            const double inc = wave_inclusive_sum_f64(live ? gl : 0.0);
+            const bool shortp = live && !seq && fl == 0 && ml != kNoBinade && st.S > 0.0 &&
+                                ldexp(1.0, ml - 52) == st.u && st.a + inc < kTwo53;
+            const uint64_t stops = __ballot(!shortp) & (~0ull << k);
+            const uint32_t j = stops ? (uint32_t)(__ffsll((long long)stops) - 1) : 64u;
+            if (j > k) {
+                st.a += read_lane_f64(inc, (int)(j - 1));
+                st.S = st.a * st.u;
             }
-        }
-        __syncthreads();
-        for (uint32_t b0 = wave * kWideAhead; b0 < nb; b0 += kWideWaves * kWideAhead) {
-            double v[kWideAhead][kFoldK];
+            if (j >= cntb) break;
+            k = j;  // block j runs the exact path
+            const uint64_t start = (uint64_t)(g0 + k) * kFoldB;
+            const uint32_t cnt = (uint32_t)min<uint64_t>(kFoldB, n - start);
+            double x[kFoldK];
+            load_block(t, n, start, lane, x);
+            bool bad = false;
 #pragma unroll
-            for (int q = 0; q < kWideAhead; ++q)
-                load_block_any(t, n, c0 + (uint64_t)min(b0 + q, nb - 1) * B, lane, v[q]);
-#pragma unroll
-            for (int q = 0; q < kWideAhead; ++q) {
-                const uint32_t b = min(b0 + q, nb - 1);
-                const int m = L.m[b];
-                // blocks with a flag or no binade run the exact path (uniform per wave)
-                if (b0 + q >= nb || m == kNoBinade || L.flag[b] != 0) continue;
-                const double inv_u = ldexp(1.0, 52 - m);
-                double T = 0.0;
-                bool fl = false;
-#pragma unroll
-                for (int i = 0; i < kFoldK; ++i) {
-                    const double y = v[q][i] * inv_u;  // exact: scaling by a power of two (0 past the end)
-                    const bool big = !(y < kTwo43);
-                    const double nfl = big ? 0.0 : floor(y);
-                    const double f = y - nfl;
-                    T += nfl + (f > 0.5 ? 1.0 : 0.0);
-                    fl = fl || big || f == 0.5;
-                }
-                T = wave_sum_any(T);  // integers below 2^53: exact in any order
-                const bool anyfl = __ballot(fl) != 0;
-                if (lane == 0) {
-                    L.g[b] = T;
-                    L.flag[b] = anyfl ? 1u : 0u;
-                }
-            }
-        }
-        __syncthreads();
-        if (wave == 0) {
-            for (uint32_t g0 = 0; g0 < nb; g0 += 64) {
-                const uint32_t bl = g0 + lane;
-                double gl = 0.0;
-                int ml = kNoBinade;
-                uint32_t fl = 1;
-                if (bl < nb) {
-                    gl = L.g[bl];
-                    ml = L.m[bl];
-                    fl = L.flag[bl];
-                }
-                const uint32_t cntb = min(64u, nb - g0);
-                for (uint32_t k = 0; k < cntb; ++k) {
-                    // The blocks from k on that take the short path, all at
-                    // once: lane j's block is short when its guess names the
-                    // exact binade (unchanged while the blocks before it are
-                    // short), it has no flag, and S / u plus the increments of
-                    // blocks k .. j stays below 2^53.  Their increments are
-                    // added in one step, up to the first block that is not.
-                    const bool live = lane >= k && lane < cntb;
-                    const double inc = wave_inclusive_sum_f64(live ? gl : 0.0);
-                    const bool shortp = live && !seq && fl == 0 && ml != kNoBinade && st.S > 0.0 &&
-                                        ldexp(1.0, ml - 52) == st.u && st.a + inc < kTwo53;
-                    const uint64_t stops = __ballot(!shortp) & (~0ull << k);
-                    const uint32_t j = stops ? (uint32_t)(__ffsll((long long)stops) - 1) : 64u;
-                    if (j > k) {
-                        st.a += read_lane_f64(inc, (int)(j - 1));
-                        st.S = st.a * st.u;
-                    }
-                    if (j >= cntb) break;
-                    k = j;  // block j runs the exact path
-                    const uint64_t start = c0 + (uint64_t)(g0 + k) * B;
-                    const uint32_t cnt = (uint32_t)min<uint64_t>(B, n - start);
-                    double v[kFoldK];
-                    load_block(t, n, start, lane, v);
-                    bool bad = false;
+            for (int i = 0; i < kFoldK; ++i)
+                bad = bad || (lane * kFoldK + i < cnt && !(x[i] >= 0.0 && x[i] <= 1.7976931348623157e308));
+            if (!seq && __ballot(bad)) seq = true;
+            if (seq) {
+                for (uint32_t l = 0; l * kFoldK < cnt; ++l)
 #pragma unroll
                     for (int i = 0; i < kFoldK; ++i)
-                        bad = bad || (lane * kFoldK + i < cnt && !(v[i] >= 0.0 && v[i] <= 1.7976931348623157e308));
-                    if (!seq && __ballot(bad)) seq = true;
-                    if (seq) {
-                        for (uint32_t l = 0; l * kFoldK < cnt; ++l)
-#pragma unroll
-                            for (int i = 0; i < kFoldK; ++i)
-                                if (l * kFoldK + i < cnt) st.S = st.S + read_lane_f64(v[i], (int)l);
-                    } else {
-                        fold_block<kFoldK>(st, v, cnt, lane);
-                    }
-                }
+                        if (l * kFoldK + i < cnt) st.S = st.S + read_lane_f64(x[i], (int)l);
+            } else {
+                fold_block<kFoldK>(st, x, cnt, lane);
             }
         }
-        __syncthreads();
     }
     return st.S;
 }
 
-__global__ __launch_bounds__(64 * kWideWaves) void fold_kernel(const double *__restrict__ terms,
-                                                               const uint64_t *__restrict__ rstart, uint32_t r_max,
-                                                               double *__restrict__ values,
-                                                               uint8_t *__restrict__ support) {
-    __shared__ WideLds L;
-    const uint32_t r = blockIdx.x + 1;
-    if (r > r_max) return;
-    const uint64_t s = rstart[r], n = rstart[r + 1] - s;
-    const double sum = fold_wide(terms + s, n, L);
-    if (threadIdx.x == 0) {
-        values[r] = n > 0 ? sum / (double)n : 0.0;
-        support[r] = n > 0 ? 1 : 0;
+// One wave per segment.  With values set, segment g is radius g + 1 of the
+// curve: values[g + 1] = sum / n and support[g + 1] = (n > 0); else sums[g].
+__global__ __launch_bounds__(64) void fold_walk_kernel(const double *__restrict__ terms,
+                                                       const uint64_t *__restrict__ seg_start, uint32_t nseg,
+                                                       FoldView v, double *__restrict__ values,
+                                                       uint8_t *__restrict__ support, double *__restrict__ sums) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (uint32_t g = blockIdx.x; g < nseg; g += gridDim.x) {
+        const uint64_t s0 = seg_start[g], n = seg_start[g + 1] - s0;
+        const uint32_t b0 = v.bbase[g], nb = v.bbase[g + 1] - b0;
+        const double sum = fold_walk(terms + s0, n, v, b0, nb, lane);
+        if (lane == 0) {
+            if (values) {
+                values[g + 1] = n > 0 ? sum / (double)n : 0.0;
+                support[g + 1] = n > 0 ? 1 : 0;
+            } else {
+                sums[g] = sum;
+            }
+        }
     }
 }
 
-__global__ __launch_bounds__(64 * kWideWaves) void segment_sum_kernel(const double *__restrict__ terms,
-                                                                      const uint64_t *__restrict__ starts,
-                                                                      uint32_t nseg, double *__restrict__ sums) {
-    __shared__ WideLds L;
-    for (uint32_t g = blockIdx.x; g < nseg; g += gridDim.x) {
-        const uint64_t s = starts[g];
-        const double sum = fold_wide(terms + s, starts[g + 1] - s, L);
-        if (threadIdx.x == 0) sums[g] = sum;
+// Host side of the fold: block tables of a segment list (segment g holds
+// terms starts[g] .. starts[g + 1] - 1), laid out with the scratch in one
+// device buffer, and the three launches.
+struct FoldLayout {
+    uint64_t off_bseg, off_sum, off_inc, off_m, off_flag, bytes;
+};
+
+static FoldLayout fold_layout(uint32_t nseg, uint32_t nblk) {
+    auto up8 = [](uint64_t x) { return (x + 7) / 8 * 8; };
+    FoldLayout L;
+    L.off_bseg = up8(((uint64_t)nseg + 1) * 4);
+    L.off_sum = L.off_bseg + up8((uint64_t)nblk * 4);
+    L.off_inc = L.off_sum + (uint64_t)nblk * 8;
+    L.off_m = L.off_inc + (uint64_t)nblk * 8;
+    L.off_flag = L.off_m + (uint64_t)nblk * 4;
+    L.bytes = std::max<uint64_t>(L.off_flag + (uint64_t)nblk * 4, 8);
+    return L;
+}
+
+// Blocks of a segment list, or an error past 2^31 blocks.
+static s2h_status fold_block_count(const uint64_t *starts, uint32_t nseg, uint32_t *nblk) {
+    uint64_t nb = 0;
+    for (uint32_t g = 0; g < nseg; ++g) nb += (starts[g + 1] - starts[g] + kFoldB - 1) / kFoldB;
+    if (nb >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "ordered fold: too many terms");
+    *nblk = (uint32_t)nb;
+    return S2H_OK;
+}
+
+// Allocates buf and uploads the block tables of the segment list.
+static s2h_status fold_prepare(s2h_ctx *ctx, DevBuf &buf, const uint64_t *starts, uint32_t nseg, uint32_t *nblk) {
+    S2H_TRY(fold_block_count(starts, nseg, nblk));
+    const FoldLayout L = fold_layout(nseg, *nblk);
+    std::vector<uint8_t> host(L.off_sum, 0);
+    uint32_t *bbase = (uint32_t *)host.data();
+    uint32_t *bseg = (uint32_t *)(host.data() + L.off_bseg);
+    uint32_t b = 0;
+    for (uint32_t g = 0; g < nseg; ++g) {
+        bbase[g] = b;
+        const uint32_t nb = (uint32_t)((starts[g + 1] - starts[g] + kFoldB - 1) / kFoldB);
+        for (uint32_t k = 0; k < nb; ++k) bseg[b + k] = g;
+        b += nb;
     }
+    bbase[nseg] = b;
+    S2H_TRY(dev_alloc(ctx, buf, L.bytes));
+    S2H_HIP(hipMemcpyAsync(buf.ptr, host.data(), L.off_sum, hipMemcpyHostToDevice, ctx->stream));
+    S2H_HIP(hipStreamSynchronize(ctx->stream));  // the host tables go out of scope here
+    return S2H_OK;
+}
+
+static FoldView fold_view(const DevBuf &buf, uint32_t nseg, uint32_t nblk) {
+    const FoldLayout L = fold_layout(nseg, nblk);
+    uint8_t *base = (uint8_t *)buf.ptr;
+    return FoldView{(const uint32_t *)base, (const uint32_t *)(base + L.off_bseg), (double *)(base + L.off_sum),
+                    (double *)(base + L.off_inc), (int32_t *)(base + L.off_m), (uint32_t *)(base + L.off_flag), nblk};
+}
+
+static s2h_status fold_launch(s2h_ctx *ctx, const double *terms, const uint64_t *seg_start, uint32_t nseg,
+                              const FoldView &v, double *values, uint8_t *support, double *sums) {
+    if (nseg == 0) return S2H_OK;
+    if (v.nblk) {
+        const unsigned grid = (v.nblk + kFoldWaves - 1) / kFoldWaves;
+        fold_sums_kernel<<<grid, 64 * kFoldWaves, 0, ctx->stream>>>(terms, seg_start, v);
+        S2H_HIP(hipGetLastError());
+        fold_incr_kernel<<<grid, 64 * kFoldWaves, 0, ctx->stream>>>(terms, seg_start, v);
+        S2H_HIP(hipGetLastError());
+        ctx->launches += 2;
+    }
+    fold_walk_kernel<<<std::min<uint32_t>(nseg, 65536), 64, 0, ctx->stream>>>(terms, seg_start, nseg, v, values,
+                                                                             support, sums);
+    S2H_HIP(hipGetLastError());
+    ++ctx->launches;
+    return S2H_OK;
 }
 
 // ---------------------------------------------------------------- plan (host)
@@ -602,6 +687,7 @@ static s2h_status plan_buffers(s2h_ctx *ctx, ExactPlan *p) {
     S2H_TRY(dev_alloc(ctx, p->d_terms, std::max<uint64_t>(K, 1) * 8));
     S2H_TRY(dev_alloc(ctx, p->d_hits, std::max<uint64_t>(K, 1) * 8));
     S2H_TRY(dev_alloc(ctx, p->d_curve, (p->r_max + 1) * 9 + 16));
+    S2H_TRY(fold_prepare(ctx, p->d_fold, p->radius_start.data() + 1, p->r_max, &p->fold_nblk));
     return S2H_OK;
 }
 
@@ -848,7 +934,7 @@ static bool autotune_enabled(const s2h_ctx *ctx) {
 }
 
 void exact_plan_release(s2h_ctx *ctx, ExactPlan *p) {
-    DevBuf *bufs[] = {&p->d_cols, &p->d_items, &p->d_table, &p->d_terms, &p->d_hits, &p->d_curve};
+    DevBuf *bufs[] = {&p->d_cols, &p->d_items, &p->d_table, &p->d_terms, &p->d_hits, &p->d_curve, &p->d_fold};
     for (DevBuf *b : bufs) dev_free(ctx, *b);
 }
 
@@ -1014,12 +1100,9 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, cons
     }
     double *values = (double *)p->d_curve.ptr;
     uint8_t *support = (uint8_t *)(values + p->r_max + 1);
-    if (p->r_max > 0) {
-        fold_kernel<<<p->r_max, 64 * kWideWaves, 0, ctx->stream>>>((const double *)p->d_terms.ptr,
-                                                       (const uint64_t *)p->d_rstart.ptr, p->r_max,
-                                                       values, support);
-        S2H_HIP(hipGetLastError());
-    }
+    // segments: radii 1 .. r_max (radius_start[r] .. radius_start[r + 1])
+    S2H_TRY(fold_launch(ctx, (const double *)p->d_terms.ptr, (const uint64_t *)p->d_rstart.ptr + 1, p->r_max,
+                        fold_view(p->d_fold, p->r_max, p->fold_nblk), values, support, nullptr));
     prof_mark(ctx, 2);
     return S2H_OK;
 }
@@ -1202,18 +1285,21 @@ extern "C" s2h_status s2h_ordered_sum(s2h_ctx *ctx, const double *terms, uint64_
         if (starts[g] > starts[g + 1] || starts[g + 1] > n) return fail(S2H_ERR_ARG, "segment bounds out of order");
     if (nseg == 0) return S2H_OK;
     S2H_HIP(hipSetDevice(ctx->device));
-    DevBuf d_t, d_s, d_o;
+    DevBuf d_t, d_s, d_o, d_f;
+    uint32_t nblk = 0;
     s2h_status st = upload(ctx, d_t, terms, std::max<uint64_t>(n, 1) * 8);
     if (st == S2H_OK) st = upload(ctx, d_s, starts, ((uint64_t)nseg + 1) * 8);
     if (st == S2H_OK) st = dev_alloc(ctx, d_o, (uint64_t)nseg * 8);
+    if (st == S2H_OK) st = fold_prepare(ctx, d_f, starts, nseg, &nblk);
+    if (st == S2H_OK)
+        st = fold_launch(ctx, (const double *)d_t.ptr, (const uint64_t *)d_s.ptr, nseg, fold_view(d_f, nseg, nblk),
+                         nullptr, nullptr, (double *)d_o.ptr);
     hipError_t e = hipSuccess;
     if (st == S2H_OK) {
-        segment_sum_kernel<<<std::min<uint32_t>(nseg, 4096), 64 * kWideWaves, 0, ctx->stream>>>(
-            (const double *)d_t.ptr, (const uint64_t *)d_s.ptr, nseg, (double *)d_o.ptr);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(sums, d_o.ptr, (uint64_t)nseg * 8, hipMemcpyDeviceToHost, ctx->stream);
+        e = hipMemcpyAsync(sums, d_o.ptr, (uint64_t)nseg * 8, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     }
+    dev_free(ctx, d_f);
     dev_free(ctx, d_t);
     dev_free(ctx, d_s);
     dev_free(ctx, d_o);

@@ -396,6 +396,8 @@ struct ExactPlan {
     DevBuf d_terms;    // double[K]
     DevBuf d_hits;     // uint64[K]
     DevBuf d_curve;    // double[r_max+1] values, then uint8 support
+    DevBuf d_fold;     // the ordered fold's block tables and per-block scratch
+    uint32_t fold_nblk = 0;
 };
 
 struct McPlan {
