@@ -190,42 +190,6 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void exact_count_kernel(
 
 // ---------------------------------------------------------------- gather / fold
 
-// term[i] = hits / valid in f64 (division correctly rounded, as Rust's `/`).
-// With an FFT cube the hits are read at the offset's cube position: FP32
-// values as they are, FP64 values rounded to the nearest integer, at least 0
-// (`.re.round().max(0.0)` of the reference's FFT kernel, src/geometry/s2.rs:1245-1248).
-__global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32_t *__restrict__ gather,
-                                    uint64_t n, const unsigned long long *__restrict__ table,
-                                    const float *__restrict__ fcube, const double *__restrict__ dcube, int Rx,
-                                    int Ry, int Rz, uint64_t occupied, uint64_t nx, uint64_t ny, uint64_t nz,
-                                    unsigned long long *__restrict__ hits, double *__restrict__ terms) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const Offset o = offs[i];
-    const uint64_t valid = (nx - (uint64_t)abs(o.dx)) * (ny - (uint64_t)abs(o.dy)) * (nz - (uint64_t)abs(o.dz));
-    uint64_t ci = 0;
-    if (fcube || dcube) {
-        const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
-        const int ix = o.dx >= 0 ? o.dx : 2 * Rx + 1 + o.dx;
-        const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
-        const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
-        ci = ((uint64_t)ix * nyo + iy) * nzo + iz;
-    }
-    double h;
-    if (fcube) {
-        h = (double)fcube[ci];
-    } else if (dcube) {
-        h = fmax(round(dcube[ci]), 0.0);
-        hits[i] = (unsigned long long)h;
-    } else {
-        const int32_t g = gather[i];
-        const uint64_t hv = g >= 0 ? table[g] : occupied;
-        hits[i] = hv;
-        h = (double)hv;
-    }
-    terms[i] = h / (double)valid;
-}
-
 // The ordered fold of segments of terms (the curve's radii, or the segments
 // of s2h_ordered_sum): each segment's sequential left fold, bit for bit.  The
 // terms are cut into blocks of 128 (64 lanes x kFoldK), numbered across all
@@ -239,13 +203,16 @@ __global__ void gather_terms_kernel(const Offset *__restrict__ offs, const int32
 //                      term;
 //   fold_incr_kernel   one wave per block: the plain sums of the segment's
 //                      earlier blocks (any order) guess the binade at the
-//                      block's start; G on that grid, and whether a tie or a
-//                      large term (y >= 2^43, so every sum of 128 is exact)
-//                      occurs;
+//                      block's start; G on that grid (ties counted at their
+//                      floor), whether a large term (y >= 2^43, so every sum
+//                      of 128 is exact) occurs, and up to kFoldTies ties with
+//                      the increments before each;
 //   fold_walk_kernel   one wave per segment, in order: a block whose guess
 //                      matches the exact S's binade, with no flag and
 //                      S / u + G < 2^53, adds G (the same integers fold_block
-//                      would add); any other block runs fold_block itself, or
+//                      would add); a block with ties adds G plus, tie by tie,
+//                      1 where S / u before the tie plus its floor is odd
+//                      (round half to even); any other block runs fold_block itself, or
 //                      one term at a time after a negative or non-finite term
 //                      (as a one-wave fold would, block for block).
 // The guesses only pick which blocks take the short path; every value comes
@@ -256,6 +223,8 @@ static constexpr int kFoldK = 2;                         // terms per lane per b
 static constexpr uint32_t kFoldB = 64 * kFoldK;          // terms per block
 static constexpr int kFoldWaves = 4;                     // waves per workgroup of the per-block launches
 static constexpr int kNoBinade = -100000;                // guessed start S == 0
+static constexpr int kFoldTies = 4;                      // ties a block may hold and still skip fold_block
+static constexpr uint32_t kFoldTieFlag = 4;              // flag 4 + t: t ties, recorded in FoldView::ties
 
 __device__ __forceinline__ double wave_sum_any(double x) {
 #pragma unroll
@@ -295,7 +264,8 @@ struct FoldView {
     double *sum;     // plain block sums
     double *inc;     // grid increments G on the guessed grid
     int32_t *m;      // guessed binade at the block's start
-    uint32_t *flag;  // 2: negative or non-finite term; 1: tie or large term
+    uint32_t *flag;  // 2: negative or non-finite term; 1: large term or too many ties; 4 + t: t ties
+    double *ties;    // per block, kFoldTies pairs (increments before the tie, the tie's floor)
     uint32_t nblk;
 };
 
@@ -336,6 +306,68 @@ __global__ __launch_bounds__(64 * kFoldWaves) void fold_sums_kernel(const double
     }
 }
 
+// term = hits / valid in f64 (division correctly rounded, as Rust's `/`) of
+// every offset, one wave per fold block, with the block's plain sum and flag
+// (fold_sums_kernel's outputs).  With an FFT cube the hits are read at the
+// offset's cube position: FP32 values as they are, FP64 values rounded to the
+// nearest integer, at least 0 (`.re.round().max(0.0)` of the reference's FFT
+// kernel, src/geometry/s2.rs:1245-1248); else from the count table through
+// the offset's gather index (the zero offset: the occupied count).  hits, when
+// set, receives the integer counts (not for an FP32 cube).
+__global__ __launch_bounds__(64 * kFoldWaves) void terms_kernel(
+    const Offset *__restrict__ offs, const int32_t *__restrict__ gather, const unsigned long long *__restrict__ table,
+    const float *__restrict__ fcube, const double *__restrict__ dcube, int Rx, int Ry, int Rz, uint64_t occupied,
+    uint64_t nx, uint64_t ny, uint64_t nz, unsigned long long *__restrict__ hits, double *__restrict__ terms,
+    const uint64_t *__restrict__ seg_start, FoldView v) {
+    const uint32_t w = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * kFoldWaves + threadIdx.x / 64));
+    if (w >= v.nblk) return;
+    const uint32_t lane = threadIdx.x & 63;
+    uint64_t s0, n;
+    uint32_t i;
+    fold_block_of(seg_start, v, w, s0, n, i);
+    const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
+    double sum = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < kFoldK; ++k) {
+        const uint64_t j = (uint64_t)i * kFoldB + (uint64_t)k * 64 + lane;
+        if (j >= n) continue;
+        const uint64_t gi = s0 + j;
+        const Offset o = offs[gi];
+        const uint64_t valid =
+            (nx - (uint64_t)abs(o.dx)) * (ny - (uint64_t)abs(o.dy)) * (nz - (uint64_t)abs(o.dz));
+        uint64_t ci = 0;
+        if (fcube || dcube) {
+            const int ix = o.dx >= 0 ? o.dx : 2 * Rx + 1 + o.dx;
+            const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
+            const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
+            ci = ((uint64_t)ix * nyo + iy) * nzo + iz;
+        }
+        double h;
+        if (fcube) {
+            h = (double)fcube[ci];
+        } else if (dcube) {
+            h = fmax(round(dcube[ci]), 0.0);
+            if (hits) hits[gi] = (unsigned long long)h;
+        } else {
+            const int32_t g = gather[gi];
+            const uint64_t hv = g >= 0 ? table[g] : occupied;
+            if (hits) hits[gi] = hv;
+            h = (double)hv;
+        }
+        const double t = h / (double)valid;
+        terms[gi] = t;
+        sum += t;
+        bad = bad || !(t >= 0.0 && t <= 1.7976931348623157e308);
+    }
+    sum = wave_sum_any(sum);
+    const bool anybad = __ballot(bad) != 0;
+    if (lane == 0) {
+        v.sum[w] = sum;
+        v.flag[w] = anybad ? 2u : 0u;
+    }
+}
+
 __global__ __launch_bounds__(64 * kFoldWaves) void fold_incr_kernel(const double *__restrict__ terms,
                                                                      const uint64_t *__restrict__ seg_start,
                                                                      FoldView v) {
@@ -354,21 +386,75 @@ __global__ __launch_bounds__(64 * kFoldWaves) void fold_incr_kernel(const double
     uint32_t flag = v.flag[w];
     double T = 0.0;
     if (flag == 0 && m != kNoBinade) {
+        // shell order: lane l holds terms l K .. l K + K - 1
         double x[kFoldK];
-        load_block_any(terms + s0, n, (uint64_t)i * kFoldB, lane, x);
+        load_block(terms + s0, n, (uint64_t)i * kFoldB, lane, x);
         const double inv_u = ldexp(1.0, 52 - m);
-        bool fl = false;
+        bool big = false;
+        bool tie[kFoldK];
+        double q[kFoldK], nfl[kFoldK];
 #pragma unroll
         for (int k = 0; k < kFoldK; ++k) {
             const double y = x[k] * inv_u;  // exact: scaling by a power of two (0 past the end)
-            const bool big = !(y < kTwo43);
-            const double nfl = big ? 0.0 : floor(y);
-            const double f = y - nfl;
-            T += nfl + (f > 0.5 ? 1.0 : 0.0);
-            fl = fl || big || f == 0.5;
+            const bool b = !(y < kTwo43);
+            nfl[k] = b ? 0.0 : floor(y);
+            const double f = y - nfl[k];
+            q[k] = nfl[k] + (f > 0.5 ? 1.0 : 0.0);
+            tie[k] = !b && f == 0.5;
+            big = big || b;
+            T += q[k];
         }
-        T = wave_sum_any(T);  // integers below 2^53: exact in any order
-        if (__ballot(fl) != 0) flag = 1u;
+        // increments of the lanes before (integers below 2^53: exact)
+        const double before = wave_exclusive_sum_f64(T);
+        T = wave_sum_any(T);
+        uint32_t nt = 0;
+        uint64_t tm[kFoldK];
+#pragma unroll
+        for (int k = 0; k < kFoldK; ++k) {
+            tm[k] = __ballot(tie[k]);
+            nt += (uint32_t)__popcll(tm[k]);
+        }
+        if (__ballot(big) != 0 || nt > (uint32_t)kFoldTies) {
+            flag = 1u;
+        } else if (nt > 0) {
+            // the ties in shell order: (increments of the block's terms before
+            // the tie, the tie's floor) -- the walk rounds each to even
+            flag = kFoldTieFlag + nt;
+            double pre_k = before;
+            double pk[kFoldK];
+#pragma unroll
+            for (int k = 0; k < kFoldK; ++k) {
+                pk[k] = pre_k;
+                pre_k += q[k];
+            }
+            double *tz = v.ties + (uint64_t)w * (2 * kFoldTies);
+            for (uint32_t e = 0; e < nt; ++e) {
+                // the earliest tie left: lowest lane, then lowest k within it
+                int bl = 64, bk = 0;
+#pragma unroll
+                for (int k = 0; k < kFoldK; ++k) {
+                    if (tm[k]) {
+                        const int l = __ffsll((long long)tm[k]) - 1;
+                        if (l < bl) {
+                            bl = l;
+                            bk = k;
+                        }
+                    }
+                }
+                double pv = 0.0, nv = 0.0;
+#pragma unroll
+                for (int k = 0; k < kFoldK; ++k)
+                    if (k == bk) {
+                        pv = read_lane_f64(pk[k], bl);
+                        nv = read_lane_f64(nfl[k], bl);
+                        tm[k] &= tm[k] - 1;
+                    }
+                if (lane == 0) {
+                    tz[2 * e] = pv;
+                    tz[2 * e + 1] = nv;
+                }
+            }
+        }
     }
     if (lane == 0) {
         v.inc[w] = T;
@@ -414,6 +500,25 @@ __device__ __forceinline__ double fold_walk(const double *__restrict__ t, uint64
             }
             if (j >= cntb) break;
             k = j;  // block j runs the exact path
+            // a block with a few ties, on the exact binade: its increments,
+            // and each tie rounded to the even neighbour of S / u before it
+            const uint32_t fj = (uint32_t)__builtin_amdgcn_readlane((int)fl, (int)j);
+            if (fj > kFoldTieFlag && !seq && st.S > 0.0 &&
+                ldexp(1.0, __builtin_amdgcn_readlane(ml, (int)j) - 52) == st.u) {
+                const uint32_t nt = fj - kFoldTieFlag;
+                const double gj = read_lane_f64(gl, (int)j);
+                if (st.a + (gj + (double)nt) < kTwo53) {
+                    const double *tz = v.ties + (uint64_t)(b0 + g0 + j) * (2 * kFoldTies);
+                    double extra = 0.0;
+                    for (uint32_t e = 0; e < nt; ++e) {
+                        const double x = st.a + tz[2 * e] + extra + tz[2 * e + 1];  // an integer below 2^53
+                        extra += (double)((uint64_t)x & 1ull);
+                    }
+                    st.a += gj + extra;
+                    st.S = st.a * st.u;
+                    continue;
+                }
+            }
             const uint64_t start = (uint64_t)(g0 + k) * kFoldB;
             const uint32_t cnt = (uint32_t)min<uint64_t>(kFoldB, n - start);
             double x[kFoldK];
@@ -462,7 +567,7 @@ __global__ __launch_bounds__(64) void fold_walk_kernel(const double *__restrict_
 // terms starts[g] .. starts[g + 1] - 1), laid out with the scratch in one
 // device buffer, and the three launches.
 struct FoldLayout {
-    uint64_t off_bseg, off_sum, off_inc, off_m, off_flag, bytes;
+    uint64_t off_bseg, off_sum, off_inc, off_ties, off_m, off_flag, bytes;
 };
 
 static FoldLayout fold_layout(uint32_t nseg, uint32_t nblk) {
@@ -471,7 +576,8 @@ static FoldLayout fold_layout(uint32_t nseg, uint32_t nblk) {
     L.off_bseg = up8(((uint64_t)nseg + 1) * 4);
     L.off_sum = L.off_bseg + up8((uint64_t)nblk * 4);
     L.off_inc = L.off_sum + (uint64_t)nblk * 8;
-    L.off_m = L.off_inc + (uint64_t)nblk * 8;
+    L.off_ties = L.off_inc + (uint64_t)nblk * 8;
+    L.off_m = L.off_ties + (uint64_t)nblk * 16 * kFoldTies;
     L.off_flag = L.off_m + (uint64_t)nblk * 4;
     L.bytes = std::max<uint64_t>(L.off_flag + (uint64_t)nblk * 4, 8);
     return L;
@@ -510,20 +616,26 @@ static s2h_status fold_prepare(s2h_ctx *ctx, DevBuf &buf, const uint64_t *starts
 static FoldView fold_view(const DevBuf &buf, uint32_t nseg, uint32_t nblk) {
     const FoldLayout L = fold_layout(nseg, nblk);
     uint8_t *base = (uint8_t *)buf.ptr;
-    return FoldView{(const uint32_t *)base, (const uint32_t *)(base + L.off_bseg), (double *)(base + L.off_sum),
-                    (double *)(base + L.off_inc), (int32_t *)(base + L.off_m), (uint32_t *)(base + L.off_flag), nblk};
+    return FoldView{(const uint32_t *)base,         (const uint32_t *)(base + L.off_bseg),
+                    (double *)(base + L.off_sum),   (double *)(base + L.off_inc),
+                    (int32_t *)(base + L.off_m),    (uint32_t *)(base + L.off_flag),
+                    (double *)(base + L.off_ties),  nblk};
 }
 
+// sums_ready: the block sums and flags are already in v (terms_kernel).
 static s2h_status fold_launch(s2h_ctx *ctx, const double *terms, const uint64_t *seg_start, uint32_t nseg,
-                              const FoldView &v, double *values, uint8_t *support, double *sums) {
+                              const FoldView &v, double *values, uint8_t *support, double *sums, bool sums_ready) {
     if (nseg == 0) return S2H_OK;
     if (v.nblk) {
         const unsigned grid = (v.nblk + kFoldWaves - 1) / kFoldWaves;
-        fold_sums_kernel<<<grid, 64 * kFoldWaves, 0, ctx->stream>>>(terms, seg_start, v);
-        S2H_HIP(hipGetLastError());
+        if (!sums_ready) {
+            fold_sums_kernel<<<grid, 64 * kFoldWaves, 0, ctx->stream>>>(terms, seg_start, v);
+            S2H_HIP(hipGetLastError());
+            ++ctx->launches;
+        }
         fold_incr_kernel<<<grid, 64 * kFoldWaves, 0, ctx->stream>>>(terms, seg_start, v);
         S2H_HIP(hipGetLastError());
-        ctx->launches += 2;
+        ++ctx->launches;
     }
     fold_walk_kernel<<<std::min<uint32_t>(nseg, 65536), 64, 0, ctx->stream>>>(terms, seg_start, nseg, v, values,
                                                                              support, sums);
@@ -1086,23 +1198,24 @@ s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
     return launch_counts(ctx, vol, p);
 }
 
-s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const FftCube *cube) {
-    const uint64_t K = p->offs.size();
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const FftCube *cube, bool want_hits) {
     const FftCube none;
     const FftCube &c = cube ? *cube : none;
     prof_mark(ctx, 1);
-    if (K) {
-        gather_terms_kernel<<<(unsigned)((K + 255) / 256), 256, 0, ctx->stream>>>(
-            (const Offset *)p->d_offs.ptr, (const int32_t *)p->d_gather.ptr, K,
-            (const unsigned long long *)p->d_table.ptr, c.f32, c.f64, c.R[0], c.R[1], c.R[2], occupied, p->dims[0],
-            p->dims[1], p->dims[2], (unsigned long long *)p->d_hits.ptr, (double *)p->d_terms.ptr);
+    const FoldView v = fold_view(p->d_fold, p->r_max, p->fold_nblk);
+    // segments: radii 1 .. r_max (radius_start[r] .. radius_start[r + 1])
+    const uint64_t *seg_start = (const uint64_t *)p->d_rstart.ptr + 1;
+    if (v.nblk) {
+        terms_kernel<<<(v.nblk + kFoldWaves - 1) / kFoldWaves, 64 * kFoldWaves, 0, ctx->stream>>>(
+            (const Offset *)p->d_offs.ptr, (const int32_t *)p->d_gather.ptr, (const unsigned long long *)p->d_table.ptr,
+            c.f32, c.f64, c.R[0], c.R[1], c.R[2], occupied, p->dims[0], p->dims[1], p->dims[2],
+            want_hits ? (unsigned long long *)p->d_hits.ptr : nullptr, (double *)p->d_terms.ptr, seg_start, v);
         S2H_HIP(hipGetLastError());
+        ++ctx->launches;
     }
     double *values = (double *)p->d_curve.ptr;
     uint8_t *support = (uint8_t *)(values + p->r_max + 1);
-    // segments: radii 1 .. r_max (radius_start[r] .. radius_start[r + 1])
-    S2H_TRY(fold_launch(ctx, (const double *)p->d_terms.ptr, (const uint64_t *)p->d_rstart.ptr + 1, p->r_max,
-                        fold_view(p->d_fold, p->r_max, p->fold_nblk), values, support, nullptr));
+    S2H_TRY(fold_launch(ctx, (const double *)p->d_terms.ptr, seg_start, p->r_max, v, values, support, nullptr, true));
     prof_mark(ctx, 2);
     return S2H_OK;
 }
@@ -1201,7 +1314,7 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
     S2H_TRY(vol_popcount(ctx, vol, &occupied));
     FftCube cube;
     S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &cube));
-    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube));
+    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube, true));
     S2H_TRY(inject(kFailReadback));
     if (n_offsets)
         S2H_HIP(hipMemcpyAsync(hits, p->d_hits.ptr, n_offsets * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1272,7 +1385,7 @@ s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
     FftCube cube;
     S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &cube));
-    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube));
+    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube, false));
     return finish_curve_from_device(ctx, p, vf, s2);
 }
 
@@ -1293,7 +1406,7 @@ extern "C" s2h_status s2h_ordered_sum(s2h_ctx *ctx, const double *terms, uint64_
     if (st == S2H_OK) st = fold_prepare(ctx, d_f, starts, nseg, &nblk);
     if (st == S2H_OK)
         st = fold_launch(ctx, (const double *)d_t.ptr, (const uint64_t *)d_s.ptr, nseg, fold_view(d_f, nseg, nblk),
-                         nullptr, nullptr, (double *)d_o.ptr);
+                         nullptr, nullptr, (double *)d_o.ptr, false);
     hipError_t e = hipSuccess;
     if (st == S2H_OK) {
         e = hipMemcpyAsync(sums, d_o.ptr, (uint64_t)nseg * 8, hipMemcpyDeviceToHost, ctx->stream);

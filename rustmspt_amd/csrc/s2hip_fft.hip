@@ -1728,7 +1728,7 @@ s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     FftCube cube;
     cube.f32 = (const float *)fp->cube.ptr;
     for (int a = 0; a < 3; ++a) cube.R[a] = fp->R[a];
-    S2H_TRY(exact_gather_fold(ctx, ep, occupied, &cube));
+    S2H_TRY(exact_gather_fold(ctx, ep, occupied, &cube, false));
     const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
     return finish_curve_from_device(ctx, ep, vf, s2);
 }

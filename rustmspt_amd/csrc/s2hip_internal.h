@@ -531,10 +531,10 @@ struct FftCube {
     const double *f64 = nullptr;
     int R[3] = {0, 0, 0};
 };
-// Gather per-offset hits and f64 terms -- from the direct count table, or from
-// an FFT cube (cube != nullptr and one of its pointers set) -- and fold per
-// radius into d_curve.
-s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied, const FftCube *cube);
+// Gather per-offset f64 terms -- from the direct count table, or from an FFT
+// cube (cube != nullptr and one of its pointers set) -- and fold per radius
+// into d_curve; with want_hits the integer counts also go to d_hits.
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied, const FftCube *cube, bool want_hits);
 // FP64 FFT exact kernel (s2hip_fft.hip): worst-case |computed - exact| of the
 // correlation values for a volume with `occupied` voxels (rounding is exact
 // below 1/2), whether the shape's lengths are supported, the algorithmic bytes

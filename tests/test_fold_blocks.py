@@ -44,8 +44,41 @@ def block_increment(ts, m):
     return g, flagged
 
 
-def wide_fold(ts):
-    """The device's per-radius fold: guessed binades, short path or term by term."""
+MAX_TIES = 4
+
+
+def block_ties(ts, m):
+    """(G, ties, big) of a block on the grid of binade m, as the device's
+    increment pass computes them: G sums every term's increment with ties
+    counted at n (their floor), ties lists (P, n) in shell order -- the
+    increments of the block's terms before the tie, and the tie's floor."""
+    inv_u = math.ldexp(1.0, 52 - m)
+    g, ties, big = 0.0, [], False
+    for t in ts:
+        y = t * inv_u
+        b = not (y < TWO43)
+        big = big or b
+        nfl = 0.0 if b else math.floor(y)
+        f = y - nfl
+        if not b and f == 0.5:
+            ties.append((g, nfl))
+        g += nfl + (1.0 if f > 0.5 else 0.0)
+    return g, ties, big
+
+
+def tie_block(a, g, ties):
+    """S / u after a block with ties, from S / u = a before it: each tie rounds
+    to the even neighbour of (a + increments before it + n)."""
+    extra = 0.0
+    for p, n in ties:
+        x = a + p + extra + n
+        extra += float(int(x) & 1)
+    return a + g + extra
+
+
+def wide_fold(ts, use_ties=True):
+    """The device's per-radius fold: guessed binades, short path (blocks
+    without ties, or with at most MAX_TIES of them) or term by term."""
     blocks = [ts[i:i + BLOCK] for i in range(0, len(ts), BLOCK)]
     bad = [any(not (0.0 <= t <= 1.7976931348623157e308) for t in b) for b in blocks]
     approx = [sum(b) for b in blocks]  # any order
@@ -62,6 +95,11 @@ def wide_fold(ts):
             a = s / u
             if not flagged and a + g < TWO53:
                 s = (a + g) * u
+                short += 1
+                continue
+            g, ties, big = block_ties(b, m)
+            if use_ties and not big and 0 < len(ties) <= MAX_TIES and a + (g + len(ties)) < TWO53:
+                s = tie_block(a, g, ties) * u
                 short += 1
                 continue
         if bd:
@@ -90,6 +128,8 @@ def segments(seed, count):
             ts[rng.randrange(n)] = -rng.random()
         elif kind == 5:  # terms that push S across many binades
             ts = [rng.random() * 2.0 ** (i // 300) for i in range(n)]
+        elif kind == 6 and trial % 2:  # hits / valid with power-of-two valid counts: ties
+            ts = [rng.randint(0, 2 ** 12) / 2.0 ** rng.randint(8, 14) for _ in range(n)]
         else:
             ts = [rng.random() * 1e-3 for _ in range(n)]
         out.append(ts)
@@ -120,6 +160,44 @@ def test_wide_fold_equals_sequential_fold():
         assert math.copysign(1.0, got) == math.copysign(1.0, want) and got == want
         short += k
     assert short > 10  # the short path is taken, not only the term-by-term one
+
+
+def test_tie_rule_is_exact_whatever_the_running_sum():
+    # blocks of dyadic terms that land exactly halfway on the grid, from any S
+    rng = random.Random(17)
+    checked = 0
+    for _ in range(600):
+        m = rng.randint(-10, 10)
+        s = math.ldexp(1.0 + rng.random(), m)
+        u = math.ldexp(1.0, m - 52)
+        ts = []
+        for _ in range(rng.randint(1, BLOCK)):
+            if rng.random() < 0.03:
+                ts.append((rng.randint(0, 2 ** 20) + 0.5) * u)      # a tie
+            else:
+                ts.append(rng.random() * math.ldexp(1.0, m - rng.randint(9, 30)))
+        g, ties, big = block_ties(ts, m)
+        a = s / u
+        if big or not ties or len(ties) > MAX_TIES or a + g + len(ties) >= TWO53:
+            continue
+        assert seq_sum(ts, s) == tie_block(a, g, ties) * u
+        checked += 1
+    assert checked > 150
+
+
+def test_dropping_the_parity_breaks_it():
+    # the tie rule is not the floor: a planted mistake (ties never round up) fails
+    rng = random.Random(5)
+    wrong = 0
+    for _ in range(300):
+        m = 0
+        u = math.ldexp(1.0, m - 52)
+        s = math.ldexp(1.0 + rng.random(), m)
+        ts = [(rng.randint(0, 2 ** 20) + 0.5) * u for _ in range(3)]
+        g, ties, big = block_ties(ts, m)
+        if (s / u + g) * u != seq_sum(ts, s):
+            wrong += 1
+    assert wrong > 50
 
 
 def test_long_shell_takes_the_short_path():
