@@ -485,22 +485,19 @@ __device__ __forceinline__ void load_first_inv(V *d, int L, int S, int sh, int t
     }
 
 // Stage 0 of the z forward transform (sub-length L, R = f_0, mp = L / R)
-// from the packed rows: line pl holds rows y (real part) and y + 1 (imaginary
-// part) of column x; z >= nz and a missing row y + 1 are zero.
+// from the packed rows staged in LDS: line pl holds rows y (real part) and
+// y + 1 (imaginary part) of column x, at sw[(2 pl + i) row_words] (a missing
+// row y + 1 staged as zeros); z >= nz is zero.
 template <int R, class V>
-__device__ __forceinline__ void z_first_stage(V *d, int S, int sh, int np, int p0, const FastDiv dyp, int ny, int nz,
-                                              int row_words, const uint64_t *__restrict__ words, const Fac &fc,
-                                              const V *__restrict__ tw) {
+__device__ __forceinline__ void z_first_stage(V *d, int S, int sh, int np, int nz, int row_words,
+                                              const uint64_t *sw, const Fac &fc, const V *__restrict__ tw) {
     using T = decltype(V::x);
     const FastDiv dmp = fc.mp[0];
     const int mp = (int)dmp.n;
     for (int t = threadIdx.x; t < mp * np; t += blockDim.x) {
-        int pl, b, x, yp;
+        int pl, b;
         divmod(t, dmp, pl, b);
-        divmod(p0 + pl, dyp, x, yp);
-        const int y = 2 * yp;
-        const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
-        const bool two = y + 1 < ny;
+        const uint64_t *ra = sw + 2 * pl * row_words;
         V a[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -508,7 +505,7 @@ __device__ __forceinline__ void z_first_stage(V *d, int S, int sh, int np, int p
             T re = 0, im = 0;
             if (z < nz) {
                 re = (T)((ra[z >> 6] >> (z & 63)) & 1ull);
-                if (two) im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
+                im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
             }
             a[r] = mk<V>(re, im);
         }
@@ -522,6 +519,20 @@ __device__ __forceinline__ void z_first_stage(V *d, int S, int sh, int np, int p
         for (int r = 0; r < R; ++r) d[pl * S + pidx(b + mp * r, sh)] = a[r];
     }
     __syncthreads();
+}
+
+// LDS of a z inverse workgroup: its lines, then the natural frequency of
+// every DIF position.
+template <class T>
+size_t z_inv_lds_bytes(int L, int pairs, int sh) {
+    return (size_t)pairs * lstride(L, sh) * sizeof(C2<T>) + (size_t)L * 4;
+}
+
+// LDS of a z forward workgroup: its lines, then the packed rows of its pairs
+// and the DIF output positions.
+template <class T>
+size_t z_fwd_lds_bytes(int L, int pairs, int sh, int row_words) {
+    return (size_t)pairs * lstride(L, sh) * sizeof(C2<T>) + (size_t)pairs * 2 * row_words * 8 + (size_t)L * 4;
 }
 
 // ---- z forward: pairs of real rows (x, y), (x, y+1) -> half spectra in G1
@@ -539,22 +550,33 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
     const int npairs = nx * ((ny + 1) / 2);
     const int p0 = blockIdx.x * pairs_per_block;
     const int np = min(pairs_per_block, npairs - p0);
+    // the pairs' packed rows and the DIF output positions, staged in LDS
+    uint64_t *sw = reinterpret_cast<uint64_t *>(sm + (size_t)pairs_per_block * S);
+    int *spos = reinterpret_cast<int *>(sw + (size_t)pairs_per_block * 2 * row_words);
+    for (int t = threadIdx.x; t < np * 2 * row_words; t += blockDim.x) {
+        const int pl = t / (2 * row_words), rem = t - pl * 2 * row_words;
+        const int i = rem / row_words, wd = rem - i * row_words;
+        int x, yp;
+        divmod(p0 + pl, dyp, x, yp);
+        const int y = 2 * yp + i;
+        sw[t] = y < ny ? words[((uint64_t)x * ny + y) * row_words + wd] : 0ull;
+    }
+    for (int t = threadIdx.x; t < L; t += blockDim.x) spos[t] = pos[t];
+    __syncthreads();
     if (fc.n >= 1) {
-        // the first DIF stage straight from the packed bits: thread (line, b)
-        // reads z = b + mp r of the two rows (lanes share each 64-bit word)
-        S2H_RADIX_SWITCH(fc.f[0], (z_first_stage<RR>(d, S, sh, np, p0, dyp, ny, nz, row_words, words, fc, tw)));
+        // the first DIF stage from the staged bits: thread (line, b) takes
+        // z = b + mp r of the two rows
+        S2H_RADIX_SWITCH(fc.f[0], (z_first_stage<RR>(d, S, sh, np, nz, row_words, sw, fc, tw)));
         for (int s = 1; s < fc.n; ++s) fft_stage_any<false>(d, S, sh, np, fc, s, tw);
     } else {
         for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
-            int pl, z, x, yp;
+            int pl, z;
             divmod(t, fc.len, pl, z);
-            divmod(p0 + pl, dyp, x, yp);
-            const int y = 2 * yp;
             T re = 0, im = 0;
             if (z < nz) {
-                const uint64_t *ra = words + ((uint64_t)x * ny + y) * row_words;
+                const uint64_t *ra = sw + 2 * pl * row_words;
                 re = (T)((ra[z >> 6] >> (z & 63)) & 1ull);
-                if (y + 1 < ny) im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
+                im = (T)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
             }
             d[pl * S + pidx(z, sh)] = mk<V>(re, im);
         }
@@ -566,8 +588,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
         divmod(t, dh, pl, k);
         divmod(p0 + pl, dyp, x, yp);
         const int y = 2 * yp;
-        const V zk = d[pl * S + pidx(pos[k], sh)];
-        const V zn = d[pl * S + pidx(pos[k == 0 ? 0 : L - k], sh)];
+        const V zk = d[pl * S + pidx(spos[k], sh)];
+        const V zn = d[pl * S + pidx(spos[k == 0 ? 0 : L - k], sh)];
         // A = (Z[k] + conj Z[-k]) / 2,  B = (Z[k] - conj Z[-k]) / (2i)
         const T half = (T)0.5;
         const V A = mk<V>(half * (zk.x + zn.x), half * (zk.y - zn.y));
@@ -715,11 +737,14 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_inv(int nlines, int H, int 
     const int p0 = blockIdx.x * pairs_per_block;
     const int npairs = (nlines + 1) / 2;
     const int np = min(pairs_per_block, npairs - p0);
+    int *srev = reinterpret_cast<int *>(sm + (size_t)pairs_per_block * S);  // natural frequency per position
+    for (int t = threadIdx.x; t < L; t += blockDim.x) srev[t] = rev[t];
+    __syncthreads();
     for (int t = threadIdx.x; t < np * L; t += blockDim.x) {
         int pl, pos;
         divmod(t, fc.len, pl, pos);
         const int la = 2 * (p0 + pl), lb = la + 1;
-        const int k = rev[pos];
+        const int k = srev[pos];
         const bool upper = k >= H;
         const int kk = upper ? L - k : k;
         V A = g3[(uint64_t)la * P + kk];
@@ -1203,7 +1228,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     // z forward
     {
         const int ppb = z_lines_per_block<T>(fz);
-        const size_t lds = lds_bytes<T>(fz, ppb, p->sh_z);
+        const size_t lds = z_fwd_lds_bytes<T>(fz, ppb, p->sh_z, (int)vol->row_words);
         S2H_TRY(set_lds((const void *)fft_z_fwd<T>, lds));
         const int npairs = nx * ((ny + 1) / 2);
         fft_z_fwd<T><<<(npairs + ppb - 1) / ppb, kFftThreads, lds, s>>>(
@@ -1246,7 +1271,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     if (ev) S2H_HIP(hipEventRecord(ev[3], s));
     {
         const int ppb = z_lines_per_block<T>(fz);
-        const size_t lds = lds_bytes<T>(fz, ppb, p->sh_z);
+        const size_t lds = z_inv_lds_bytes<T>(fz, ppb, p->sh_z);
         S2H_TRY(set_lds((const void *)fft_z_inv<T>, lds));
         const int nlines = nxo * nyo;
         const int npairs = (nlines + 1) / 2;
@@ -1590,7 +1615,8 @@ s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands
     S2H_TRY(set_lds((const void *)fft_y_fwd<double>, lds_y));
     S2H_TRY(set_lds((const void *)fftd_x_cross<double>, lds_x));
     S2H_TRY(set_lds((const void *)fft_y_inv<double>, lds_y));
-    S2H_TRY(set_lds((const void *)fft_z_inv<double>, lds_z));
+    const size_t lds_zi = z_inv_lds_bytes<double>(Fz, ppb, p->sh_z);
+    S2H_TRY(set_lds((const void *)fft_z_inv<double>, lds_zi));
     const double scale = 1.0 / ((double)Fx * (double)Fy * (double)Fz);
     const V *tw0 = (const V *)p->tw[0].ptr, *tw1 = (const V *)p->tw[1].ptr, *tw2 = (const V *)p->tw[2].ptr;
     V *g1 = (V *)p->g1.ptr, *g2 = (V *)p->g2.ptr, *g3 = (V *)p->g3.ptr;
@@ -1610,7 +1636,7 @@ s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands
         fft_y_inv<double><<<nc * nxo * tiles_y, pass_threads(Fy, fcy, ty), lds_y, s>>>(Fy, H, P, tiles_y, ty, fcy, tw1,
                                                                                        Ry, g2, g3, p->sh_y);
         const int zi = nc * nxo * nyo;
-        fft_z_inv<double><<<((zi + 1) / 2 + ppb - 1) / ppb, kFftThreads, lds_z, s>>>(
+        fft_z_inv<double><<<((zi + 1) / 2 + ppb - 1) / ppb, kFftThreads, lds_zi, s>>>(
             zi, H, P, fcz, Fz, tw2, (const int *)p->rev_z.ptr, ppb, Rz, scale, g3, cube, p->sh_z,
             fast_div((uint32_t)nzo));
         fftd_gather<<<2048, 256, 0, s>>>(cc, (uint32_t)nc, sd.offs, sd.sg, sd.noffs, cube, Rx, Ry, Rz, delta);
