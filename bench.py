@@ -51,8 +51,8 @@ def fft64_bytes(dims, offs, r_max, complex_bytes=16.0):
     passes must move -- each pass reads its input and writes its output once
     (z forward: packed rows in, n_x n_y rows of half spectra out; y forward:
     n_x planes in, f_y rows each out; x forward + |F|^2 + inverse keeping the
-    2R_x+1 needed planes; y inverse keeping 2R_y+1 rows; z inverse keeping
-    2R_z+1 real values)."""
+    R_x+1 planes dx >= 0 -- the correlation is even, C(d) = C(-d); y inverse
+    keeping 2R_y+1 rows; z inverse keeping 2R_z+1 real values)."""
     import numpy as np
 
     from rustmspt_amd import s2
@@ -65,9 +65,9 @@ def fft64_bytes(dims, offs, r_max, complex_bytes=16.0):
     row_words = (nz + 63) // 64 + 1
     z_fwd = nx * ny * row_words * 8 + nx * ny * h * c
     y_fwd = nx * ny * h * c + nx * fy * h * c
-    x_pow = nx * fy * h * c + (2 * rx + 1) * fy * h * c
-    y_inv = (2 * rx + 1) * fy * h * c + (2 * rx + 1) * (2 * ry + 1) * h * c
-    z_inv = (2 * rx + 1) * (2 * ry + 1) * h * c + (2 * rx + 1) * (2 * ry + 1) * (2 * rz + 1) * c / 2.0
+    x_pow = nx * fy * h * c + (rx + 1) * fy * h * c
+    y_inv = (rx + 1) * fy * h * c + (rx + 1) * (2 * ry + 1) * h * c
+    z_inv = (rx + 1) * (2 * ry + 1) * h * c + (rx + 1) * (2 * ry + 1) * (2 * rz + 1) * c / 2.0
     return 12.0 * fx * fy * h * c, float(z_fwd + y_fwd + x_pow + y_inv + z_inv)
 
 

@@ -308,8 +308,8 @@ __global__ __launch_bounds__(64 * kFoldWaves) void fold_sums_kernel(const double
 
 // term = hits / valid in f64 (division correctly rounded, as Rust's `/`) of
 // every offset, one wave per fold block, with the block's plain sum and flag
-// (fold_sums_kernel's outputs).  With an FFT cube the hits are read at the
-// offset's cube position: FP32 values as they are, FP64 values rounded to the
+// (fold_sums_kernel's outputs).  With an FFT cube (dx >= 0 only; dx < 0 reads
+// the mirror) the hits are read at the offset's cube position: FP32 values as they are, FP64 values rounded to the
 // nearest integer, at least 0 (`.re.round().max(0.0)` of the reference's FFT
 // kernel, src/geometry/s2.rs:1245-1248); else from the count table through
 // the offset's gather index (the zero offset: the occupied count).  hits, when
@@ -325,7 +325,6 @@ __global__ __launch_bounds__(64 * kFoldWaves) void terms_kernel(
     uint64_t s0, n;
     uint32_t i;
     fold_block_of(seg_start, v, w, s0, n, i);
-    const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
     double sum = 0.0;
     bool bad = false;
 #pragma unroll
@@ -336,13 +335,7 @@ __global__ __launch_bounds__(64 * kFoldWaves) void terms_kernel(
         const Offset o = offs[gi];
         const uint64_t valid =
             (nx - (uint64_t)abs(o.dx)) * (ny - (uint64_t)abs(o.dy)) * (nz - (uint64_t)abs(o.dz));
-        uint64_t ci = 0;
-        if (fcube || dcube) {
-            const int ix = o.dx >= 0 ? o.dx : 2 * Rx + 1 + o.dx;
-            const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
-            const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
-            ci = ((uint64_t)ix * nyo + iy) * nzo + iz;
-        }
+        const uint64_t ci = (fcube || dcube) ? half_cube_index(o, Ry, Rz) : 0;
         double h;
         if (fcube) {
             h = (double)fcube[ci];

@@ -382,7 +382,7 @@ __device__ void lds_power(V *d, int S, int sh, int lines, const Fac &fc, const V
 //     runs the butterfly and its twiddles, stores to LDS.
 //   store_last_inv: stage 0 of the inverse (DIT) transform: butterfly from
 //     LDS, conjugate twiddles, inverse DFT; writes only the kept j
-//     (j <= Rk -> row j, j >= L - Rk -> row j - L + 2Rk + 1).
+//     (j <= Rk -> row j, and unless half j >= L - Rk -> row j - L + 2Rk + 1).
 //   store_last_fwd: the forward transform's last stage (sub-length R, mp = 1,
 //     no twiddles) from LDS, every output j = R b + r written to g.
 //   load_first_inv: the inverse's first stage (sub-length R, mp = 1) from g.
@@ -416,7 +416,8 @@ __device__ __forceinline__ void load_first_fwd(V *d, int L, int S, int sh, int t
 
 template <int R, class V>
 __device__ __forceinline__ void store_last_inv(const V *d, int L, int S, int sh, int tile, int lgt, int nk,
-                                               V *__restrict__ g, uint64_t rs, int Rk, const V *__restrict__ tw) {
+                                               V *__restrict__ g, uint64_t rs, int Rk, bool half,
+                                               const V *__restrict__ tw) {
     const int mp = L / R, nko = 2 * Rk + 1;
     for (int t = threadIdx.x; t < mp * tile; t += blockDim.x) {
         int b, c;
@@ -434,7 +435,7 @@ __device__ __forceinline__ void store_last_inv(const V *d, int L, int S, int sh,
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = b + mp * r;
-            const int o = j <= Rk ? j : (j >= L - Rk ? j - L + nko : -1);
+            const int o = j <= Rk ? j : (!half && j >= L - Rk ? j - L + nko : -1);
             if (o >= 0) g[(uint64_t)o * rs + c] = a[r];
         }
     }
@@ -662,7 +663,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_x_power(int nx, int fy, in
                                                        nx, tw + fc.tw[0])));
         lds_power_inner(d, S, sh, nk, fc, tw);
         S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk, g2 + (uint64_t)ky * P + kz0, rs,
-                                                       Rx, tw + fc.tw[0])));
+                                                       Rx, true, tw + fc.tw[0])));
         return;
     }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
@@ -674,13 +675,11 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_x_power(int nx, int fy, in
     }
     __syncthreads();
     lds_power(d, S, sh, nk, fc, tw);
-    const int nxo = 2 * Rx + 1;
-    for (int t = threadIdx.x; t < nxo * tile; t += blockDim.x) {
+    for (int t = threadIdx.x; t < (Rx + 1) * tile; t += blockDim.x) {
         int ix, c;
         divmod(t, tile, lgt, ix, c);
         if (c >= nk) continue;
-        const int x = ix <= Rx ? ix : L - nxo + ix;
-        g2[((uint64_t)ix * fy + ky) * P + kz0 + c] = d[c * S + pidx(x, sh)];
+        g2[((uint64_t)ix * fy + ky) * P + kz0 + c] = d[c * S + pidx(ix, sh)];
     }
 }
 
@@ -702,7 +701,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_y_inv(int fy, int H, int P
         for (int s = fc.n - 2; s >= 1; --s) fft_stage_any<true>(d, S, sh, nk, fc, s, tw);
         S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk,
                                                        g3 + (uint64_t)ix * (2 * Ry + 1) * P + kz0, (uint64_t)P, Ry,
-                                                       tw + fc.tw[0])));
+                                                       false, tw + fc.tw[0])));
         return;
     }
     for (int t = threadIdx.x; t < L * tile; t += blockDim.x) {
@@ -775,11 +774,7 @@ __global__ void fft_gather(const Offset *__restrict__ offs, uint64_t n, const T 
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Offset o = offs[i];
-    const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
-    const int ix = o.dx >= 0 ? o.dx : 2 * Rx + 1 + o.dx;
-    const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
-    const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
-    out[i] = cube[((uint64_t)ix * nyo + iy) * nzo + iz];
+    out[i] = cube[half_cube_index(o, Ry, Rz)];
 }
 
 // ---------------------------------------------------------------- delta counts by region correlation
@@ -939,10 +934,10 @@ __global__ __launch_bounds__(kFftMaxThreads) void fftd_x_cross(int Fx, int Fy, i
         default: cross_stage<5>(d, S, sh, tile, fc.nb[last]); break;
     }
     for (int s = last - 1; s >= 1; --s) fft_stage_any<true>(d, S, sh, tile, fc, s, tw);
-    const int nxo = 2 * Rx + 1;
+    const int nxo = Rx + 1;  // dx >= 0 only: the delta correlation is even
     S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk,
                                                    g2 + (uint64_t)c * nxo * rs + (uint64_t)ky * P + kz0, rs, Rx,
-                                                   tw + fc.tw[0])));
+                                                   true, tw + fc.tw[0])));
 }
 
 // Delta counts of the candidates: the value at each shell offset rounded to
@@ -952,17 +947,14 @@ __global__ void fftd_gather(const FftdCand *__restrict__ cands, uint32_t ncand, 
                             const int32_t *__restrict__ sg, uint64_t noffs, const double *__restrict__ cube, int Rx,
                             int Ry, int Rz, int32_t *__restrict__ delta) {
     const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
-    const uint64_t per = (uint64_t)(2 * Rx + 1) * nyo * nzo, total = (uint64_t)ncand * noffs;
+    const uint64_t per = (uint64_t)(Rx + 1) * nyo * nzo, total = (uint64_t)ncand * noffs;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t c = t / noffs, i = t - c * noffs;
         const int32_t k = sg[i];
         if (k < 0) continue;
         const Offset o = offs[i];
-        const int ix = o.dx >= 0 ? o.dx : 2 * Rx + 1 + o.dx;
-        const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
-        const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
-        const double v = cube[c * per + ((uint64_t)ix * nyo + iy) * nzo + iz];
+        const double v = cube[c * per + half_cube_index(o, Ry, Rz)];
         delta[cands[c].delta_off + k] = (int32_t)rint(v);
     }
 }
@@ -1112,7 +1104,7 @@ s2h_status fft_plan_tables(s2h_ctx *ctx, FftPlan *p) {
 void plan_buffer_sizes(const uint64_t dims[3], const int R[3], int real_bytes, uint64_t sizes[4]) {
     const uint64_t fy = fft_length(dims[1], (uint64_t)R[1]);
     const uint64_t H = spectrum_pitch(fft_length(dims[2], (uint64_t)R[2]), real_bytes);
-    const uint64_t nxo = 2 * (uint64_t)R[0] + 1, nyo = 2 * (uint64_t)R[1] + 1, nzo = 2 * (uint64_t)R[2] + 1;
+    const uint64_t nxo = (uint64_t)R[0] + 1, nyo = 2 * (uint64_t)R[1] + 1, nzo = 2 * (uint64_t)R[2] + 1;
     const uint64_t cb = 2 * (uint64_t)real_bytes;  // bytes per complex value
     const uint64_t raw[4] = {dims[0] * fy * H * cb, nxo * fy * H * cb, nxo * nyo * H * cb,
                              nxo * nyo * nzo * (uint64_t)real_bytes};
@@ -1220,7 +1212,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     const int fx = (int)p->f[0], fy = (int)p->f[1], fz = (int)p->f[2];
     const int H = fz / 2 + 1, P = (int)spectrum_pitch((uint64_t)fz, (int)sizeof(T));
     const int Rx = p->R[0], Ry = p->R[1], Rz = p->R[2];
-    const int nxo = 2 * Rx + 1, nyo = 2 * Ry + 1;
+    const int nxo = Rx + 1, nyo = 2 * Ry + 1;  // x planes dx = 0 .. R_x: the correlation is even
     const Fac fcx = to_fac(p->ax[0]), fcy = to_fac(p->ax[1]), fcz = to_fac(p->ax[2]);
     hipStream_t s = ctx->stream;
     const int ty = p->tile_y, tx = p->tile_x;
@@ -1593,7 +1585,7 @@ s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands
     const int Fx = F[0], Fy = F[1], Fz = F[2];
     const int H = Fz / 2 + 1, P = (int)spectrum_pitch((uint64_t)Fz, 8);
     const int Rx = sd.R[0], Ry = sd.R[1], Rz = sd.R[2];
-    const int nxo = 2 * Rx + 1, nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
+    const int nxo = Rx + 1, nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;  // x planes dx = 0 .. R_x
     const uint64_t b1 = 2ull * Fx * Fy * P * 16, b2 = (uint64_t)nxo * Fy * P * 16, b3 = (uint64_t)nxo * nyo * P * 16,
                    b4 = (uint64_t)nxo * nyo * nzo * 8;
     uint64_t chunk = std::max<uint64_t>(1, std::min<uint64_t>(ncand, kFftdChunkBytes / (b1 + b2 + b3 + b4)));
@@ -1652,7 +1644,7 @@ double fft64_model_bytes(const uint64_t dims[3], const int R[3]) {
     const double fy = (double)fft_length(dims[1], (uint64_t)R[1]);
     const double h = (double)(fft_length(dims[2], (uint64_t)R[2]) / 2 + 1);
     const double nx = (double)dims[0], ny = (double)dims[1], nz = (double)dims[2];
-    const double xo = 2.0 * R[0] + 1, yo = 2.0 * R[1] + 1, zo = 2.0 * R[2] + 1;
+    const double xo = R[0] + 1.0, yo = 2.0 * R[1] + 1, zo = 2.0 * R[2] + 1;
     const double c = 16.0;
     return nx * ny * (std::ceil(nz / 64.0) + 1) * 8.0 + nx * ny * h * c + (nx * ny * h + nx * fy * h) * c +
            (nx * fy * h + xo * fy * h) * c + (xo * fy * h + xo * yo * h) * c + xo * yo * h * c + xo * yo * zo * 8.0;

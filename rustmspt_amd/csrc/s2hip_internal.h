@@ -290,6 +290,18 @@ struct Offset {
     int32_t dx, dy, dz;
 };
 
+// Index in the half cube [R_x + 1][2R_y + 1][2R_z + 1] of offset o: the
+// correlation is even (C(d) = C(-d), it counts the same pairs), so the
+// inverse passes keep dx >= 0 only and an offset with dx < 0 reads its
+// mirror -o.
+__device__ __forceinline__ uint64_t half_cube_index(Offset o, int Ry, int Rz) {
+    if (o.dx < 0) o = Offset{-o.dx, -o.dy, -o.dz};
+    const int nyo = 2 * Ry + 1, nzo = 2 * Rz + 1;
+    const int iy = o.dy >= 0 ? o.dy : nyo + o.dy;
+    const int iz = o.dz >= 0 ? o.dz : nzo + o.dz;
+    return ((uint64_t)o.dx * nyo + iy) * nzo + iz;
+}
+
 // In-domain offsets of radii 1..r_max in the reference's shell order, with
 // radius_start[r_max+2].
 void shell_order_offsets(const uint64_t dims[3], uint32_t r_max, double pitch,
