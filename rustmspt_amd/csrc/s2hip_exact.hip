@@ -1191,7 +1191,8 @@ s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
     return launch_counts(ctx, vol, p);
 }
 
-s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const FftCube *cube, bool want_hits) {
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, const FftCube *cube, bool want_hits,
+                             void *curve_dst) {
     const FftCube none;
     const FftCube &c = cube ? *cube : none;
     prof_mark(ctx, 1);
@@ -1206,7 +1207,7 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, cons
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
-    double *values = (double *)p->d_curve.ptr;
+    double *values = (double *)(curve_dst ? curve_dst : p->d_curve.ptr);
     uint8_t *support = (uint8_t *)(values + p->r_max + 1);
     S2H_TRY(fold_launch(ctx, (const double *)p->d_terms.ptr, seg_start, p->r_max, v, values, support, nullptr, true));
     prof_mark(ctx, 2);
@@ -1269,12 +1270,17 @@ static s2h_status exact_counts_pick(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p, ui
     return exact_run_counts(ctx, vol, p);
 }
 
-s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *p, double vf, double *s2) {
+s2h_status curve_staging(s2h_ctx *ctx, ExactPlan *p, void **stage) {
+    return pinned_staging(ctx, ((uint64_t)p->r_max + 1) * 9, stage);
+}
+
+s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *p, double vf, double *s2, void *staged) {
     const uint32_t n = p->r_max + 1;
-    void *stage = nullptr;  // page-locked: the copy is one DMA the stream sync waits for
-    S2H_TRY(pinned_staging(ctx, (uint64_t)n * 9, &stage));
+    void *stage = staged;  // page-locked
+    if (!stage) S2H_TRY(pinned_staging(ctx, (uint64_t)n * 9, &stage));
     S2H_TRY(inject(kFailReadback));
-    S2H_HIP(hipMemcpyAsync(stage, p->d_curve.ptr, (size_t)n * 9, hipMemcpyDeviceToHost, ctx->stream));
+    // without a staged curve: one DMA of the device curve the stream sync waits for
+    if (!staged) S2H_HIP(hipMemcpyAsync(stage, p->d_curve.ptr, (size_t)n * 9, hipMemcpyDeviceToHost, ctx->stream));
     S2H_HIP(hipStreamSynchronize(ctx->stream));
     prof_collect(ctx);
     std::vector<uint8_t> sup(n, 0);
@@ -1307,7 +1313,7 @@ s2h_status s2h_exact_counts(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double p
     S2H_TRY(vol_popcount(ctx, vol, &occupied));
     FftCube cube;
     S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &cube));
-    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube, true));
+    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube, true, nullptr));
     S2H_TRY(inject(kFailReadback));
     if (n_offsets)
         S2H_HIP(hipMemcpyAsync(hits, p->d_hits.ptr, n_offsets * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1378,8 +1384,11 @@ s2h_status s2h_exact_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
     FftCube cube;
     S2H_TRY(exact_counts_pick(ctx, vol, p, occupied, &cube));
-    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube, false));
-    return finish_curve_from_device(ctx, p, vf, s2);
+    // the fold writes the curve straight into page-locked host memory
+    void *stage = nullptr;
+    S2H_TRY(curve_staging(ctx, p, &stage));
+    S2H_TRY(exact_gather_fold(ctx, p, occupied, &cube, false, stage));
+    return finish_curve_from_device(ctx, p, vf, s2, stage);
 }
 
 }  // extern "C"

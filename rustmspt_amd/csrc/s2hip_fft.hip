@@ -397,10 +397,13 @@ __device__ __forceinline__ void load_first_fwd(V *d, int L, int S, int sh, int t
         divmod(t, tile, lgt, b, c);
         if (c >= nk) continue;
         V a[R];
+        const V *gp = g + (uint64_t)b * rs + c;
+        const uint64_t step = (uint64_t)mp * rs;  // rows j = b + mp r: pointer steps, no per-row multiply
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = b + mp * r;
-            a[r] = j < nvalid ? g[(uint64_t)j * rs + c] : mk<V>(0, 0);
+            a[r] = j < nvalid ? *gp : mk<V>(0, 0);
+            gp += step;
         }
         small_dft<-1>(a, R);
         if (mp > 1) {
@@ -432,11 +435,12 @@ __device__ __forceinline__ void store_last_inv(const V *d, int L, int S, int sh,
             for (int q = 1; q < R; ++q) a[q] = cmulc(a[q], w[q - 1]);
         }
         small_dft<1>(a, R);
+        V *gc = g + c;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = b + mp * r;
             const int o = j <= Rk ? j : (!half && j >= L - Rk ? j - L + nko : -1);
-            if (o >= 0) g[(uint64_t)o * rs + c] = a[r];
+            if (o >= 0) gc[(uint64_t)(uint32_t)o * rs] = a[r];
         }
     }
 }
@@ -453,8 +457,12 @@ __device__ __forceinline__ void store_last_fwd(const V *d, int L, int S, int sh,
 #pragma unroll
         for (int r = 0; r < R; ++r) a[r] = d[c * S + pidx(b * R + r, sh)];
         small_dft<-1>(a, R);
+        V *gp = g + (uint64_t)(b * R) * rs + c;
 #pragma unroll
-        for (int r = 0; r < R; ++r) g[(uint64_t)(b * R + r) * rs + c] = a[r];
+        for (int r = 0; r < R; ++r) {
+            *gp = a[r];
+            gp += rs;
+        }
     }
 }
 
@@ -467,8 +475,12 @@ __device__ __forceinline__ void load_first_inv(V *d, int L, int S, int sh, int t
         divmod(t, tile, lgt, b, c);
         if (c >= nk) continue;
         V a[R];
+        const V *gp = g + (uint64_t)(b * R) * rs + c;
 #pragma unroll
-        for (int r = 0; r < R; ++r) a[r] = g[(uint64_t)(b * R + r) * rs + c];
+        for (int r = 0; r < R; ++r) {
+            a[r] = *gp;
+            gp += rs;
+        }
         small_dft<1>(a, R);
 #pragma unroll
         for (int r = 0; r < R; ++r) d[c * S + pidx(b * R + r, sh)] = a[r];
@@ -1746,9 +1758,11 @@ s2h_status s2h_fft32_s2(s2h_ctx *ctx, s2h_vol *vol, uint32_t r_max, double pitch
     FftCube cube;
     cube.f32 = (const float *)fp->cube.ptr;
     for (int a = 0; a < 3; ++a) cube.R[a] = fp->R[a];
-    S2H_TRY(exact_gather_fold(ctx, ep, occupied, &cube, false));
+    void *stage = nullptr;
+    S2H_TRY(curve_staging(ctx, ep, &stage));
+    S2H_TRY(exact_gather_fold(ctx, ep, occupied, &cube, false, stage));
     const double vf = (double)occupied / (double)(vol->dims[0] * vol->dims[1] * vol->dims[2]);
-    return finish_curve_from_device(ctx, ep, vf, s2);
+    return finish_curve_from_device(ctx, ep, vf, s2, stage);
 }
 
 }  // extern "C"

@@ -545,8 +545,10 @@ struct FftCube {
 };
 // Gather per-offset f64 terms -- from the direct count table, or from an FFT
 // cube (cube != nullptr and one of its pointers set) -- and fold per radius
-// into d_curve; with want_hits the integer counts also go to d_hits.
-s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied, const FftCube *cube, bool want_hits);
+// into d_curve, or into curve_dst (page-locked host memory of (r_max + 1) * 9
+// bytes) when set; with want_hits the integer counts also go to d_hits.
+s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *plan, uint64_t occupied, const FftCube *cube, bool want_hits,
+                             void *curve_dst);
 // FP64 FFT exact kernel (s2hip_fft.hip): worst-case |computed - exact| of the
 // correlation values for a volume with `occupied` voxels (rounding is exact
 // below 1/2), whether the shape's lengths are supported, the algorithmic bytes
@@ -568,8 +570,12 @@ bool fft64_plan_cached(s2h_ctx *ctx, const uint64_t dims[3], const int R[3]);
 // Release every FFT plan of the context (their blocks go to the block cache).
 void fft_plans_release(s2h_ctx *ctx);
 s2h_status fft64_exact_device(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *plan, FftCube *cube);
-// Copy folded curve values and supported flags back and finish the curve.
-s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *plan, double vf, double *s2);
+// Copy folded curve values and supported flags back (unless the fold wrote
+// them to `staged`) and finish the curve.
+s2h_status finish_curve_from_device(s2h_ctx *ctx, ExactPlan *plan, double vf, double *s2, void *staged);
+// The context's page-locked buffer for a curve of the plan (values, then
+// support bytes), which the fold can write directly.
+s2h_status curve_staging(s2h_ctx *ctx, ExactPlan *plan, void **stage);
 // Per-device address of the no-allocation entry point's scratch array
 // (s2hip_mc.hip); the caller has made `device` current.
 s2h_status noalloc_scratch_init(int device);
