@@ -378,7 +378,7 @@ __device__ void lds_power(V *d, int S, int sh, int lines, const Fac &fc, const V
 // segment).  Each saves the LDS round trip and barrier of a plain load or
 // store loop.
 //   load_first_fwd: stage 0 of the forward (DIF) transform (sub-length L,
-//     mp = L / R): thread (b, c) loads j = b + mp r (j >= nvalid are zero),
+//     mp = L / R): thread (b, c) loads j = b + mp r (j < jlo, j >= nvalid are zero),
 //     runs the butterfly and its twiddles, stores to LDS.
 //   store_last_inv: stage 0 of the inverse (DIT) transform: butterfly from
 //     LDS, conjugate twiddles, inverse DFT; writes only the kept j
@@ -389,7 +389,7 @@ __device__ void lds_power(V *d, int S, int sh, int lines, const Fac &fc, const V
 // tw0 is stage 0's twiddle rows (row b: w^(b q), q = 1 .. R - 1).
 template <int R, class V>
 __device__ __forceinline__ void load_first_fwd(V *d, int L, int S, int sh, int tile, int lgt, int nk,
-                                               const V *__restrict__ g, uint64_t rs, int nvalid,
+                                               const V *__restrict__ g, uint64_t rs, int jlo, int nvalid,
                                                const V *__restrict__ tw) {
     const int mp = L / R;
     for (int t = threadIdx.x; t < mp * tile; t += blockDim.x) {
@@ -402,7 +402,7 @@ __device__ __forceinline__ void load_first_fwd(V *d, int L, int S, int sh, int t
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int j = b + mp * r;
-            a[r] = j < nvalid ? *gp : mk<V>(0, 0);
+            a[r] = (j >= jlo && j < nvalid) ? *gp : mk<V>(0, 0);
             gp += step;
         }
         small_dft<-1>(a, R);
@@ -615,17 +615,32 @@ __global__ __launch_bounds__(kFftThreads) void fft_z_fwd(const uint64_t *__restr
 // ---- y forward: per (x, kz tile), rows y >= ny are zero; in place in G1
 template <class T>
 __global__ __launch_bounds__(kFftMaxThreads) void fft_y_fwd(int ny, int fy, int H, int P, int tiles_kz, int tile, Fac fc,
-                                                         const C2<T> *__restrict__ tw, C2<T> *__restrict__ g1, int sh) {
+                                                         const C2<T> *__restrict__ tw, C2<T> *__restrict__ g1, int sh,
+                                                         const FftdCand *__restrict__ dcands, int Fx, int Rx,
+                                                         int Ry) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
     using V = C2<T>;
     V *sm = reinterpret_cast<V *>(sm_raw);
     const int L = fy, S = lstride(L, sh), lgt = log2_if_pow2(tile);
     V *d = sm;
     const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * tile;
+    int jlo = 0, jhi = ny;  // rows that may be nonzero
+    if (dcands) {
+        // frames: plane x is field (x / F_x) % 2 of candidate x / (2 F_x); a
+        // delta plane outside the box's frame rows R_x .. R_x + |B_x| - 1 is
+        // zero and is not read again (the x pass loads only the box rows),
+        // and inside them delta is zero outside rows R_y .. R_y + |B_y| - 1
+        const int c = x / (2 * Fx), field = (x / Fx) & 1, i = x % Fx;
+        if (field == 1) {
+            if (i < Rx || i >= Rx + (dcands[c].b1[0] - dcands[c].b0[0])) return;
+            jlo = Ry;
+            jhi = Ry + (dcands[c].b1[1] - dcands[c].b0[1]);
+        }
+    }
     const int nk = min(tile, H - kz0);
     V *base = g1 + (uint64_t)x * fy * P + kz0;
     if (fc.n >= 2) {
-        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P, ny,
+        S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P, jlo, jhi,
                                                        tw + fc.tw[0])));
         for (int s = 1; s + 1 < fc.n; ++s) fft_stage_any<false>(d, S, sh, nk, fc, s, tw);
         S2H_RADIX_SWITCH(fc.f[fc.n - 1], (store_last_fwd<RR>(d, L, S, sh, tile, lgt, nk, base, (uint64_t)P)));
@@ -635,7 +650,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_y_fwd(int ny, int fy, int 
         int y, c;
         divmod(t, tile, lgt, y, c);
         V v = mk<V>(0, 0);
-        if (y < ny && c < nk) v = base[(uint64_t)y * P + c];
+        if (y >= jlo && y < jhi && c < nk) v = base[(uint64_t)y * P + c];
         d[c * S + pidx(y, sh)] = v;
     }
     __syncthreads();
@@ -672,7 +687,7 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_x_power(int nx, int fy, in
     if (fc.n >= 2) {
         const uint64_t rs = (uint64_t)fy * P;
         S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, g1 + (uint64_t)ky * P + kz0, rs,
-                                                       nx, tw + fc.tw[0])));
+                                                       0, nx, tw + fc.tw[0])));
         lds_power_inner(d, S, sh, nk, fc, tw);
         S2H_RADIX_SWITCH(fc.f[0], (store_last_inv<RR>(d, L, S, sh, tile, lgt, nk, g2 + (uint64_t)ky * P + kz0, rs,
                                                        Rx, true, tw + fc.tw[0])));
@@ -919,8 +934,9 @@ __device__ __forceinline__ void cross_stage(V *d, int S, int sh, int lines, cons
 // lines forward, the cross spectrum, the inverse keeping the 2R_x + 1 needed
 // x (g2 planes c (2R_x + 1) + ix).
 template <class T>
-__global__ __launch_bounds__(kFftMaxThreads) void fftd_x_cross(int Fx, int Fy, int H, int P, int tiles_kz, int tile,
-                                                            Fac fc, const C2<T> *__restrict__ tw, int Rx,
+__global__ __launch_bounds__(kFftMaxThreads) void fftd_x_cross(const FftdCand *__restrict__ cands, int Fx, int Fy,
+                                                            int H, int P, int tiles_kz, int tile, Fac fc,
+                                                            const C2<T> *__restrict__ tw, int Rx,
                                                             const C2<T> *__restrict__ g1, C2<T> *__restrict__ g2,
                                                             int sh) {
     extern __shared__ __align__(16) unsigned char sm_raw[];
@@ -934,8 +950,11 @@ __global__ __launch_bounds__(kFftMaxThreads) void fftd_x_cross(int Fx, int Fy, i
     const uint64_t rs = (uint64_t)Fy * P, field = (uint64_t)Fx * Fy * P;
     const V *go = g1 + (uint64_t)(2 * c) * field + (uint64_t)ky * P + kz0;
     const int last = fc.n - 1;
-    S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, go, rs, L, tw + fc.tw[0])));
-    S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d + tile * S, L, S, sh, tile, lgt, nk, go + field, rs, L,
+    // delta is zero outside the box rows R_x .. R_x + |B_x| - 1 of the frame
+    // (those planes are not transformed along y either)
+    const int dlo = Rx, dhi = Rx + (cands[c].b1[0] - cands[c].b0[0]);
+    S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d, L, S, sh, tile, lgt, nk, go, rs, 0, L, tw + fc.tw[0])));
+    S2H_RADIX_SWITCH(fc.f[0], (load_first_fwd<RR>(d + tile * S, L, S, sh, tile, lgt, nk, go + field, rs, dlo, dhi,
                                                    tw + fc.tw[0])));
     for (int s = 1; s < last; ++s) fft_stage_any<false>(d, S, sh, 2 * tile, fc, s, tw);
     switch (fc.f[last]) {
@@ -1248,7 +1267,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         S2H_TRY(set_lds((const void *)fft_y_fwd<T>, lds));
         fft_y_fwd<T><<<nx * tiles_y, pass_threads(fy, fcy, ty), lds, s>>>(ny, fy, H, P, tiles_y, ty, fcy,
                                                                             (const V *)p->tw[1].ptr,
-                                                            (V *)p->g1.ptr, p->sh_y);
+                                                            (V *)p->g1.ptr, p->sh_y, nullptr, 0, 0, 0);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
@@ -1634,9 +1653,10 @@ s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands
             fcz, Fz, tw2, (const int *)p->pos_z.ptr, ppb, g1, p->sh_z, fast_div((uint32_t)(Fx * Fy)),
             fast_div((uint32_t)Fy), fast_div((uint32_t)H));
         fft_y_fwd<double><<<nc * 2 * Fx * tiles_y, pass_threads(Fy, fcy, ty), lds_y, s>>>(Fy, Fy, H, P, tiles_y, ty,
-                                                                                          fcy, tw1, g1, p->sh_y);
-        fftd_x_cross<double><<<nc * Fy * tiles_x, pass_threads(Fx, fcx, tx), lds_x, s>>>(Fx, Fy, H, P, tiles_x, tx,
-                                                                                         fcx, tw0, Rx, g1, g2, p->sh_x);
+                                                                                          fcy, tw1, g1, p->sh_y, cc,
+                                                                                          Fx, Rx, Ry);
+        fftd_x_cross<double><<<nc * Fy * tiles_x, pass_threads(Fx, fcx, tx), lds_x, s>>>(
+            cc, Fx, Fy, H, P, tiles_x, tx, fcx, tw0, Rx, g1, g2, p->sh_x);
         fft_y_inv<double><<<nc * nxo * tiles_y, pass_threads(Fy, fcy, ty), lds_y, s>>>(Fy, H, P, tiles_y, ty, fcy, tw1,
                                                                                        Ry, g2, g3, p->sh_y);
         const int zi = nc * nxo * nyo;
