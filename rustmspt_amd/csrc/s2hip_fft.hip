@@ -820,40 +820,36 @@ __global__ void fft_gather(const Offset *__restrict__ offs, uint64_t n, const T 
 // z forward of frame rows: line (c, i, j) is the complex row o + i delta of
 // candidate c at frame x = i, y = j; its transform splits into the half
 // spectra of o (field 0) and delta (field 1), written to g1 planes
-// (2c + field) F_x + i.
+// (2c + field) F_x + i.  The workgroup's seed rows (zero outside the volume)
+// and (A, D) rows (zero outside the box) are staged in LDS first.
+struct FftdLine {
+    int32_t z0;       // volume z of the frame's first z
+    int32_t drow;     // 1: the line lies in the box's (x, y) range
+    uint64_t row;     // g1 row of the o field
+};
+
 template <int R, class V>
-__device__ __forceinline__ void fftd_first_stage(V *d, int S, int sh, int np, int p0, const FftdCand *__restrict__ cands,
-                                                 const uint64_t *__restrict__ words, int nx, int ny, int nz,
-                                                 int row_words, const uint2 *__restrict__ ad, int nwd, int Rx, int Ry,
-                                                 int Rz, FastDiv dplane, FastDiv dFy, const Fac &fc,
-                                                 const V *__restrict__ tw) {
+__device__ __forceinline__ void fftd_first_stage(V *d, int S, int sh, int np, int nz, int row_words, int nwd,
+                                                 const uint64_t *so, const uint2 *sd, const FftdLine *sl,
+                                                 const Fac &fc, const V *__restrict__ tw) {
     using T = decltype(V::x);
     const FastDiv dmp = fc.mp[0];
     const int mp = (int)dmp.n;
     for (int t = threadIdx.x; t < mp * np; t += blockDim.x) {
-        int pl, b, c, rem, i, j;
+        int pl, b;
         divmod(t, dmp, pl, b);
-        divmod(p0 + pl, dplane, c, rem);
-        divmod(rem, dFy, i, j);
-        const FftdCand cd = cands[c];
-        const int x = cd.b0[0] - Rx + i, y = cd.b0[1] - Ry + j, z0 = cd.b0[2] - Rz;
-        const bool orow = x >= 0 && x < nx && y >= 0 && y < ny;
-        const bool drow = x >= cd.b0[0] && x < cd.b1[0] && y >= cd.b0[1] && y < cd.b1[1];
-        const uint64_t *ro = words + (orow ? ((uint64_t)x * ny + y) * row_words : 0);
-        const uint2 *rd = ad + (drow ? cd.ad_off + ((uint64_t)(x - cd.b0[0]) * (cd.b1[1] - cd.b0[1]) +
-                                                    (y - cd.b0[1])) * nwd
-                                     : 0);
+        const uint64_t *ro = so + (size_t)pl * row_words;
+        const uint2 *rd = sd + (size_t)pl * nwd;
+        const int z0 = sl[pl].z0;
         V a[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int z = z0 + b + mp * r;
             T re = 0, im = 0;
             if (z >= 0 && z < nz) {
-                if (orow) re = (T)((ro[z >> 6] >> (z & 63)) & 1ull);
-                if (drow) {
-                    const uint2 w = rd[z >> 5];
-                    im = (T)(int)(((w.x >> (z & 31)) & 1u)) - (T)(int)(((w.y >> (z & 31)) & 1u));
-                }
+                re = (T)((ro[z >> 6] >> (z & 63)) & 1ull);
+                const uint2 w = rd[z >> 5];
+                im = (T)(int)(((w.x >> (z & 31)) & 1u)) - (T)(int)(((w.y >> (z & 31)) & 1u));
             }
             a[r] = mk<V>(re, im);
         }
@@ -867,6 +863,14 @@ __device__ __forceinline__ void fftd_first_stage(V *d, int S, int sh, int np, in
         for (int r = 0; r < R; ++r) d[pl * S + pidx(b + mp * r, sh)] = a[r];
     }
     __syncthreads();
+}
+
+// LDS of a frame z forward workgroup: its lines, the staged seed and (A, D)
+// rows, the line records and the DIF output positions.
+template <class T>
+size_t fftd_z_lds_bytes(int L, int lines, int sh, int row_words, int nwd) {
+    return (size_t)lines * lstride(L, sh) * sizeof(C2<T>) + (size_t)lines * ((size_t)row_words * 8 + (size_t)nwd * 8) +
+           (size_t)lines * sizeof(FftdLine) + (size_t)L * 4;
 }
 
 template <class T>
@@ -883,22 +887,61 @@ __global__ __launch_bounds__(kFftThreads) void fftd_z_fwd(const FftdCand *__rest
     const int S = lstride(L, sh);
     const int p0 = blockIdx.x * lines_per_block;
     const int np = min(lines_per_block, nlines - p0);
-    S2H_RADIX_SWITCH(fc.f[0], (fftd_first_stage<RR>(d, S, sh, np, p0, cands, words, nx, ny, nz, row_words, ad, nwd,
-                                                     Rx, Ry, Rz, dplane, dFy, fc, tw)));
-    for (int s = 1; s < fc.n; ++s) fft_stage_any<false>(d, S, sh, np, fc, s, tw);
-    for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
-        int pl, k, c, rem, i, j;
-        divmod(t, dh, pl, k);
+    uint64_t *so = reinterpret_cast<uint64_t *>(d + (size_t)lines_per_block * S);
+    uint2 *sdw = reinterpret_cast<uint2 *>(so + (size_t)lines_per_block * row_words);
+    FftdLine *sl = reinterpret_cast<FftdLine *>(sdw + (size_t)lines_per_block * nwd);
+    int *spos = reinterpret_cast<int *>(sl + lines_per_block);
+    for (int pl = threadIdx.x; pl < np; pl += blockDim.x) {
+        int c, rem, i, j;
         divmod(p0 + pl, dplane, c, rem);
         divmod(rem, dFy, i, j);
-        const V zk = d[pl * S + pidx(pos[k], sh)];
-        const V zn = d[pl * S + pidx(pos[k == 0 ? 0 : L - k], sh)];
+        const FftdCand cd = cands[c];
+        const int x = cd.b0[0] - Rx + i, y = cd.b0[1] - Ry + j;
+        FftdLine ln;
+        ln.z0 = cd.b0[2] - Rz;
+        ln.drow = (x >= cd.b0[0] && x < cd.b1[0] && y >= cd.b0[1] && y < cd.b1[1]) ? 1 : 0;
+        ln.row = ((uint64_t)(2 * c) * Fx + i) * Fy + j;
+        sl[pl] = ln;
+    }
+    for (int t = threadIdx.x; t < np * row_words; t += blockDim.x) {
+        const int pl = t / row_words, wd = t - pl * row_words;
+        int c, rem, i, j;
+        divmod(p0 + pl, dplane, c, rem);
+        divmod(rem, dFy, i, j);
+        const FftdCand cd = cands[c];
+        const int x = cd.b0[0] - Rx + i, y = cd.b0[1] - Ry + j;
+        const bool orow = x >= 0 && x < nx && y >= 0 && y < ny;
+        so[t] = orow ? words[((uint64_t)x * ny + y) * row_words + wd] : 0ull;
+    }
+    for (int t = threadIdx.x; t < np * nwd; t += blockDim.x) {
+        const int pl = t / nwd, wd = t - pl * nwd;
+        int c, rem, i, j;
+        divmod(p0 + pl, dplane, c, rem);
+        divmod(rem, dFy, i, j);
+        const FftdCand cd = cands[c];
+        const int x = cd.b0[0] - Rx + i, y = cd.b0[1] - Ry + j;
+        const bool drow = x >= cd.b0[0] && x < cd.b1[0] && y >= cd.b0[1] && y < cd.b1[1];
+        sdw[t] = drow ? ad[cd.ad_off + ((uint64_t)(x - cd.b0[0]) * (cd.b1[1] - cd.b0[1]) + (y - cd.b0[1])) * nwd + wd]
+                      : make_uint2(0u, 0u);
+    }
+    for (int t = threadIdx.x; t < L; t += blockDim.x) spos[t] = pos[t];
+    __syncthreads();
+    S2H_RADIX_SWITCH(fc.f[0], (fftd_first_stage<RR>(d, S, sh, np, nz, row_words, nwd, so, sdw, sl, fc, tw)));
+    for (int s = 1; s < fc.n; ++s) fft_stage_any<false>(d, S, sh, np, fc, s, tw);
+    for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
+        int pl, k;
+        divmod(t, dh, pl, k);
+        const V zk = d[pl * S + pidx(spos[k], sh)];
+        const V zn = d[pl * S + pidx(spos[k == 0 ? 0 : L - k], sh)];
         const T half = (T)0.5;
         const V A = mk<V>(half * (zk.x + zn.x), half * (zk.y - zn.y));   // spectrum of o
-        const V B = mk<V>(half * (zk.y + zn.y), -half * (zk.x - zn.x));  // spectrum of delta
-        const uint64_t row = ((uint64_t)(2 * c) * Fx + i) * Fy + j;
+        const uint64_t row = sl[pl].row;
         g1[row * P + k] = A;
-        g1[(row + (uint64_t)Fx * Fy) * P + k] = B;
+        // delta rows outside the box are zero: the y and x passes never read them
+        if (sl[pl].drow) {
+            const V B = mk<V>(half * (zk.y + zn.y), -half * (zk.x - zn.x));  // spectrum of delta
+            g1[(row + (uint64_t)Fx * Fy) * P + k] = B;
+        }
     }
 }
 
@@ -1632,8 +1675,8 @@ s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands
     const int ty = p->tile_y, tx = p->tile_x;
     const int tiles_y = (H + ty - 1) / ty, tiles_x = (H + tx - 1) / tx;
     const int ppb = z_lines_per_block<double>(Fz);
-    const size_t lds_z = lds_bytes<double>(Fz, ppb, p->sh_z), lds_y = lds_bytes<double>(Fy, ty, p->sh_y),
-                 lds_x = lds_bytes<double>(Fx, 2 * tx, p->sh_x);
+    const size_t lds_z = fftd_z_lds_bytes<double>(Fz, ppb, p->sh_z, (int)sd.row_words, nwd),
+                 lds_y = lds_bytes<double>(Fy, ty, p->sh_y), lds_x = lds_bytes<double>(Fx, 2 * tx, p->sh_x);
     S2H_TRY(set_lds((const void *)fftd_z_fwd<double>, lds_z));
     S2H_TRY(set_lds((const void *)fft_y_fwd<double>, lds_y));
     S2H_TRY(set_lds((const void *)fftd_x_cross<double>, lds_x));
