@@ -9,11 +9,14 @@
 //     gathered values for 32-bit ones) gives the modal boundary value, smallest
 //     on ties, over exactly the voxels detect_background_mode visits
 //     (for_each_boundary_value, src/pipeline/crop.rs:312-330);
-//   - pack_kernel: one wave per (64 x values, y, z word); lane x reads its 64
-//     samples along z (each read is 64 consecutive x: coalesced), decodes the
-//     file's width, sign and byte order (decode_raw_slice,
-//     src/io/volume.rs:287-332), classifies, and writes the row word.  Bits at
-//     z >= depth and the pad word are written as zero.
+//   - pack_tile_kernel (rows whose byte length is a multiple of 8): a
+//     workgroup classifies a tile of x values by 4 rows for all z words,
+//     lanes reading 8 bytes at consecutive x per slice (whole 128-byte lines),
+//     decodes the file's width, sign and byte order (decode_raw_slice,
+//     src/io/volume.rs:287-332), classifies, and writes the tile's row words
+//     from LDS as contiguous runs; pack_kernel (any width): one wave per
+//     (64 x values, y, z word), one sample per lane.  Bits at z >= depth and
+//     the pad word are written as zero.
 // File axes map one to one: S2 (x, y, z) = file (x, y, slice).
 #include <algorithm>
 #include <cstring>
@@ -72,37 +75,146 @@ __global__ __launch_bounds__(64) void pack_kernel(const uint8_t *__restrict__ s,
     }
 }
 
-// Rows of 8-byte multiples: each lane loads 8 bytes (8 / B samples at
-// consecutive x) per slice, so a wave reads 512 contiguous bytes per load.
-// One word per lane and x: the 8-byte row words land 104+ bytes apart, which
-// the memory system writes as partial sectors (about 3.7x the packed bytes at
-// 768^3); the kernel is still far below the host read and PCIe copy it follows.
-template <int B>
-__global__ __launch_bounds__(64) void pack_wide_kernel(const uint8_t *__restrict__ s, uint64_t W, uint64_t H,
-                                                       uint64_t D, int sgn, int big, Rule rule, uint64_t row_words,
-                                                       uint64_t nblocks, unsigned long long *__restrict__ words) {
+// Rule for samples of at most 16 bits, in "biased" unsigned form: the sample
+// bits u (sign bit flipped for signed samples, so u orders like the value) are
+// tested as u - lo <= span in uint32 (range rule: lo <= u <= lo + span) or
+// u != bg (foreground).  Bounds outside the sample range are clamped to
+// -1 / 2^bits, which keeps every comparison's outcome; an empty range becomes
+// lo = 2^bits, span 0, which no sample matches.
+struct Rule16 {
+    uint32_t lo, span, bg;
+    uint32_t flip;  // XOR mask applied to a whole 32-bit load (sign bits)
+    uint32_t perm;  // v_perm selector: byte swap per 16-bit sample (big-endian) or identity
+};
+
+inline Rule16 rule16(const Rule &r, int bits, bool sgn, bool big) {
+    const int64_t bias = sgn ? (int64_t(1) << (bits - 1)) : 0, top = int64_t(1) << bits;
+    auto clamp = [&](int64_t v) {
+        v = std::min<int64_t>(top, std::max<int64_t>(-top, v));  // no overflow in v + bias
+        return std::min<int64_t>(top, std::max<int64_t>(-1, v + bias));
+    };
+    int64_t lo = clamp(r.lo), hi = clamp(r.hi);
+    if (lo > hi) lo = hi = top;
+    Rule16 q;
+    q.lo = (uint32_t)lo;
+    q.span = (uint32_t)(hi - lo);
+    q.bg = (uint32_t)clamp(r.bg);
+    q.flip = !sgn ? 0u : bits == 8 ? 0x80808080u : 0x80008000u;
+    q.perm = bits == 16 && big ? 0x02030001u : 0x03020100u;
+    return q;
+}
+
+// Occupancy of the S = 8 / B samples of one 8-byte load as S flags.
+template <int B, bool RANGE>
+__device__ __forceinline__ void classify8(uint2 v, bool *occ, int sgn, int big, const Rule &rule, const Rule16 &r16) {
     constexpr int S = 8 / B;
-    const uint64_t plane = W * H;
-    for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        const uint64_t w = b % row_words, t = b / row_words;
-        const uint64_t y = t % H, xs = ((t / H) * 64 + threadIdx.x) * S;
-        if (xs >= W) continue;
-        const uint64_t z0 = w * 64;
-        const uint32_t nz = z0 < D ? (uint32_t)std::min<uint64_t>(64, D - z0) : 0;
-        uint64_t bits[S];
-#pragma unroll
-        for (int j = 0; j < S; ++j) bits[j] = 0;
-        const uint64_t base = z0 * plane + y * W + xs;  // in samples; W * B % 8 == 0 keeps this 8-byte aligned
-        for (uint32_t k = 0; k < nz; ++k) {
-            const uint2 v = *(const uint2 *)(s + (base + k * plane) * B);
-            const uint8_t *vb = (const uint8_t *)&v;
-#pragma unroll
-            for (int j = 0; j < S; ++j)
-                if (occupied(sample<B>(vb, j, sgn, big), rule)) bits[j] |= 1ull << k;
+    if (B <= 2) {
+        uint32_t w0 = v.x, w1 = v.y;
+        if (B == 2) {
+            w0 = __builtin_amdgcn_perm(w0, w0, r16.perm);
+            w1 = __builtin_amdgcn_perm(w1, w1, r16.perm);
         }
+        w0 ^= r16.flip;
+        w1 ^= r16.flip;
 #pragma unroll
-        for (int j = 0; j < S; ++j)
-            if (xs + j < W) words[((xs + j) * H + y) * row_words + w] = bits[j];
+        for (int j = 0; j < S; ++j) {
+            const uint32_t u = ((j < S / 2 ? w0 : w1) >> (8 * B * (j % (S / 2)))) & (B == 1 ? 0xffu : 0xffffu);
+            occ[j] = RANGE ? (u - r16.lo <= r16.span) : (u != r16.bg);
+        }
+    } else {
+        const uint8_t *vb = (const uint8_t *)&v;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int64_t x = sample<B>(vb, j, sgn, big);
+            occ[j] = RANGE ? ((rule.lo <= x) & (x <= rule.hi)) : (x != rule.bg);
+        }
+    }
+}
+
+// One row word per sample column: bit k of bits[j] is slice z0 + k of column
+// j.  p points at slice z0 of the lane's 8 bytes; step is one slice in bytes.
+template <int B, bool RANGE>
+__device__ __forceinline__ void pack_word(const uint8_t *p, uint64_t step, uint32_t nz, uint64_t *bits, int sgn,
+                                          int big, const Rule &rule, const Rule16 &r16) {
+    constexpr int S = 8 / B;
+    bool occ[S];
+    if (nz == 64) {
+        // 16 loads in flight per lane, then their 16 x S samples
+#pragma unroll
+        for (int kb = 0; kb < 64; kb += 16) {
+            uint2 v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                v[i] = *(const uint2 *)p;
+                p += step;
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                classify8<B, RANGE>(v[i], occ, sgn, big, rule, r16);
+#pragma unroll
+                for (int j = 0; j < S; ++j) bits[j] |= (uint64_t)occ[j] << (kb + i);
+            }
+        }
+    } else {
+        for (uint32_t k = 0; k < nz; ++k, p += step) {
+            classify8<B, RANGE>(*(const uint2 *)p, occ, sgn, big, rule, r16);
+#pragma unroll
+            for (int j = 0; j < S; ++j) bits[j] |= (uint64_t)occ[j] << k;
+        }
+    }
+}
+
+// Rows whose byte length is a multiple of 8.  A workgroup packs a tile of
+// XT = 16 x S x values by YT = 4 rows y, for all z words of those rows.  Lane
+// (g, r) of a wave loads 8 bytes (S = 8 / B samples at consecutive x) of row
+// y0 + r per slice, so the 16 lanes of a row read one whole 128-byte line per
+// slice; wave v packs the row words v, v + 4, ... of a chunk of wc words into
+// LDS.  The chunk then leaves LDS in volume order: for one x the YT rows are
+// adjacent rows of the volume, so each x's YT x wc words are one contiguous
+// run when the chunk holds the whole row (wc == row_words), and the stores are
+// whole runs instead of single 8-byte words of rows 100+ bytes apart.  The
+// last word of every row is the zero pad and is written as zero.
+template <int B>
+__global__ __launch_bounds__(256, 3) void pack_tile_kernel(const uint8_t *__restrict__ s, uint64_t W, uint64_t H,
+                                                        uint64_t D, int sgn, int big, Rule rule, Rule16 r16,
+                                                        uint64_t row_words, uint32_t wc, uint64_t tiles_x,
+                                                        uint64_t ntiles, unsigned long long *__restrict__ words) {
+    constexpr int S = 8 / B, XT = 16 * S, YT = 4;
+    extern __shared__ unsigned long long tile[];  // [XT][YT * wc + 1]
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t g = lane & 15, r = lane >> 4;
+    const uint32_t pitch = YT * wc + 1;
+    const uint64_t plane = W * H, step = plane * B, nw = row_words - 1;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t x0 = (t % tiles_x) * XT, y0 = (t / tiles_x) * YT;
+        const uint64_t xs = x0 + g * S, y = y0 + r;
+        const bool live = xs < W && y < H;  // W % S == 0: a live lane's S samples are all inside the row
+        for (uint64_t c0 = 0; c0 < row_words; c0 += wc) {
+            const uint32_t cw = (uint32_t)std::min<uint64_t>(wc, row_words - c0);
+            for (uint32_t wi = wave; wi < cw; wi += 4) {
+                const uint64_t w = c0 + wi;
+                uint64_t bits[S];
+#pragma unroll
+                for (int j = 0; j < S; ++j) bits[j] = 0;
+                if (live && w < nw) {
+                    const uint64_t z0 = w * 64;
+                    const uint32_t nz = (uint32_t)std::min<uint64_t>(64, D - z0);
+                    const uint8_t *p = s + (z0 * plane + y * W + xs) * B;  // 8-byte aligned: W * B % 8 == 0
+                    if (rule.range) pack_word<B, true>(p, step, nz, bits, sgn, big, rule, r16);
+                    else pack_word<B, false>(p, step, nz, bits, sgn, big, rule, r16);
+                }
+#pragma unroll
+                for (int j = 0; j < S; ++j) tile[(g * S + j) * pitch + r * wc + wi] = bits[j];
+            }
+            __syncthreads();
+            const uint32_t run = YT * cw, total = XT * run;
+            for (uint32_t l = threadIdx.x; l < total; l += 256) {
+                const uint32_t xi = l / run, q = l - xi * run, yi = q / cw, wi = q - yi * cw;
+                const uint64_t x = x0 + xi, yy = y0 + yi;
+                if (x < W && yy < H) words[(x * H + yy) * row_words + c0 + wi] = tile[xi * pitch + yi * wc + wi];
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -272,21 +384,26 @@ s2h_status pack_samples(s2h_ctx *ctx, const uint8_t *d_samples, const s2h_volume
     prof_mark(ctx, 0);
     const uint32_t bytes = l->bits / 8;
     if ((l->width * bytes) % 8 == 0 && ((uintptr_t)d_samples & 7) == 0) {
-        const uint64_t per_block = 64 * (8 / bytes);
-        const uint64_t nblocks = (l->width + per_block - 1) / per_block * l->height * vol->row_words;
-        const unsigned grid = (unsigned)std::min<uint64_t>(nblocks, 1u << 22);
+        // Tile of 16 x (8 / bytes) x values by 4 rows; chunks of wc row words keep
+        // the tile's LDS within 64 KiB.
+        const uint64_t xt = 16 * (8 / bytes), yt = 4;
+        const uint32_t wc = (uint32_t)std::min<uint64_t>(vol->row_words, (65536 / (xt * 8) - 1) / yt);
+        const uint64_t tiles_x = (l->width + xt - 1) / xt, ntiles = tiles_x * ((l->height + yt - 1) / yt);
+        const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 1u << 22);
+        const size_t lds = (size_t)xt * (yt * wc + 1) * 8;
+        const volin::Rule16 r16 = volin::rule16(rule, (int)l->bits, l->is_signed, l->big_endian);
         if (bytes == 1)
-            volin::pack_wide_kernel<1><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
-                                                                     l->is_signed, l->big_endian, rule,
-                                                                     vol->row_words, nblocks, words);
+            volin::pack_tile_kernel<1><<<grid, 256, lds, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
+                                                                        l->is_signed, l->big_endian, rule, r16,
+                                                                        vol->row_words, wc, tiles_x, ntiles, words);
         else if (bytes == 2)
-            volin::pack_wide_kernel<2><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
-                                                                     l->is_signed, l->big_endian, rule,
-                                                                     vol->row_words, nblocks, words);
+            volin::pack_tile_kernel<2><<<grid, 256, lds, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
+                                                                        l->is_signed, l->big_endian, rule, r16,
+                                                                        vol->row_words, wc, tiles_x, ntiles, words);
         else
-            volin::pack_wide_kernel<4><<<grid, 64, 0, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
-                                                                     l->is_signed, l->big_endian, rule,
-                                                                     vol->row_words, nblocks, words);
+            volin::pack_tile_kernel<4><<<grid, 256, lds, ctx->stream>>>(d_samples, l->width, l->height, l->depth,
+                                                                        l->is_signed, l->big_endian, rule, r16,
+                                                                        vol->row_words, wc, tiles_x, ntiles, words);
     } else {
         const uint64_t nblocks = (l->width + 63) / 64 * l->height * vol->row_words;
         const unsigned grid = (unsigned)std::min<uint64_t>(nblocks, 1u << 22);

@@ -42,6 +42,43 @@ def test_pack_matches_oracle_all_sample_types(ctx, dt, W):
     assert lv.background == bg and lv.occupied == n and np.array_equal(packed(lv), want)
 
 
+@pytest.mark.parametrize("dt", ["u1", "i1", "<u2", ">i2", ">u4", "<i4"])
+def test_pack_rule_edges(ctx, dt):
+    """Thresholds beyond the sample range, empty ranges, int64 extremes and
+    backgrounds no sample can take: the wide kernel clamps 8/16-bit rules to
+    32-bit comparisons, which must keep every outcome."""
+    D, H, W = 70, 6, 48
+    info = np.iinfo(np.dtype(dt))
+    rng = np.random.default_rng(W + sum(map(ord, dt)))
+    vol = rng.integers(info.min, info.max, (D, H, W), endpoint=True).astype(dt)
+    vol[0, 0, :4] = [info.min, info.max, 0, -1 if info.min < 0 else 1]
+    i64 = np.iinfo(np.int64)
+    ranges = [(5, 3), (i64.min, i64.max), (i64.min, int(info.min) - 1), (int(info.max) + 1, i64.max),
+              (int(info.min), int(info.min)), (int(info.max), int(info.max)), (-1, 0), (-200, -100),
+              (70000, 80000), (int(info.min) - 5, int(info.min) + 3), (int(info.max) - 3, int(info.max) + 9)]
+    for lo, hi in ranges:
+        lv = V.from_samples(ctx, vol, V.Phase.threshold(lo, hi))
+        want, n = o.classify_pack(vol, "range", lo, hi)
+        assert lv.occupied == n and np.array_equal(packed(lv), want), (lo, hi)
+    for bg in [int(info.min) - 1, int(info.max) + 1, int(info.min), int(info.max), 0, i64.min, i64.max]:
+        lv = V.from_samples(ctx, vol, V.Phase.foreground(bg))
+        want, n = o.classify_pack(vol, "foreground", background=bg)
+        assert lv.occupied == n and np.array_equal(packed(lv), want), bg
+
+
+@pytest.mark.parametrize("dt", ["u1", "<u2", "<i4"])
+def test_pack_deep_stacks(ctx, dt):
+    """2,200 slices: 36 row words, more than one LDS chunk of the wide kernel
+    for 8-bit (15 words a chunk) and 16-bit (31) samples; rows and columns not
+    multiples of the kernel's tile."""
+    D, H, W = 2200, 7, 136
+    rng = np.random.default_rng(D + H + W)
+    vol = (rng.random((D, H, W)) < 0.4).astype(dt) * 7
+    lv = V.from_samples(ctx, vol, V.Phase.foreground(0))
+    want, n = o.classify_pack(vol, "foreground", background=0)
+    assert lv.volume.dims == [W, H, D] and lv.occupied == n and np.array_equal(packed(lv), want)
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 1), (1, 5, 7), (2, 3, 4), (7, 1, 9), (6, 5, 1), (65, 9, 3), (9, 8, 200)])
 @pytest.mark.parametrize("dt", ["u1", "i2", "u4"])
 def test_background_mode_matches_oracle(ctx, shape, dt):
