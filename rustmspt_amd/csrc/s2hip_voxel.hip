@@ -14,10 +14,12 @@
 // first keeps the granule's triangles whose projection on the plane normal to
 // the ray direction can reach the tile (a conservative 2-D bound whose margin
 // dominates every rounding error of the predicate -- culled triangles provably
-// miss, so the hit multiset is unchanged), then each thread runs the predicate
-// for its voxel centre over that short list, reading triangle data through
-// uniform (scalar) loads.  Rays with more than kMaxHits hits go to an overflow
-// list that a second kernel resolves exactly with a repeated-minimum scan.
+// miss, so the hit multiset is unchanged).  Lanes then take (triangle, x slice)
+// items and run the predicate only at the slice's centres whose projection can
+// fall in the triangle's bound, appending hits to per-centre lists in LDS; each
+// thread finally sorts and counts its centre's hits.  Rays with more than
+// kMaxHits hits go to an overflow list that a second kernel resolves exactly
+// with a repeated-minimum scan.
 #include <algorithm>
 #include <cmath>
 #include <cstring>

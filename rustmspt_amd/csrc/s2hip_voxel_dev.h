@@ -13,7 +13,7 @@ namespace vox {
 constexpr double kRayX = 0.9428090415820634, kRayY = 0.2705980500730985, kRayZ = 0.19611613513818402;
 constexpr int kTileX = 4, kTileY = 8, kTileZ = 8;  // 256 voxel centres per workgroup
 constexpr int kMaxHits = 16;
-constexpr int kListCap = 2048;  // culled triangle indices per pass
+constexpr int kListCap = 1024;  // culled triangle indices per pass
 
 struct TriGeo {
     double ax, ay, az;     // vertex a
@@ -142,15 +142,73 @@ __device__ __forceinline__ void set_voxel(unsigned long long *words, uint64_t ny
 }
 
 
+// Local index window [lo, hi] (clamped to [0, n - 1]) of the lattice points
+// q(i) = o + (base + i + 0.5) * pitch whose coordinate can lie in [a, b]: the
+// real-valued indices of a and b widened by `slack` (far above the rounding
+// error of every coordinate and projection here).  Infinite or NaN bounds give
+// the whole range.
+__device__ __forceinline__ void index_window(double a, double b, double o, double ipitch, uint32_t base, int n,
+                                             double slack, int &lo, int &hi) {
+    const double ia = (a - o) * ipitch - 0.5 - (double)base, ib = (b - o) * ipitch - 0.5 - (double)base;
+    const double l = floor(fmin(ia, ib) - slack), h = ceil(fmax(ia, ib) + slack);
+    lo = (int)fmax(0.0, fmin((double)(n - 1), l));   // fmin / fmax drop a NaN operand
+    hi = (int)fmax(0.0, fmin((double)(n - 1), h));
+    if (!(l <= h)) lo = 0, hi = n - 1;               // NaN: everything
+}
+
+// Sorted hits of one centre (N >= nh slots, unused ones +inf) with the
+// anchored 1e-8 dedup (src/geometry/mesh_query.rs:105-114); returns the count.
+template <int N>
+__device__ __forceinline__ int unique_hits(const double *slot, uint32_t stride, uint32_t nh) {
+    double h[N];
+#pragma unroll
+    for (int s = 0; s < N; ++s) h[s] = s < (int)nh ? slot[s * stride] : INFINITY;
+#pragma unroll
+    for (int i = 1; i < N; ++i)
+#pragma unroll
+        for (int j = i; j > 0; --j) {
+            const double lo = h[j - 1] < h[j] ? h[j - 1] : h[j];
+            const double hi = h[j - 1] < h[j] ? h[j] : h[j - 1];
+            h[j - 1] = lo;
+            h[j] = hi;
+        }
+    int unique = 0;
+    double last = -INFINITY;
+#pragma unroll
+    for (int s = 0; s < N; ++s) {
+        if (s < (int)nh && fabs(h[s] - last) > 1e-8) {
+            ++unique;
+            last = h[s];
+        }
+    }
+    return unique;
+}
+
 // One workgroup (256 threads) evaluates the parity predicate at the centres of
 // one tile; out(gi, x, y, z) receives every centre found inside.  Rays with more
 // than kMaxHits hits are appended (tile_id, thread) to `overflow`.
+//
+// The tile's triangles are first culled to those whose projected bound (on the
+// plane normal to the ray) meets the tile's centres.  Work items are then
+// (triangle, x slice of the tile): a lane enumerates only the slice's centres
+// whose projection can fall inside the triangle's bound (a y window from the u
+// bound, then a z window from the v bound), applies the per-centre bound test
+// and the Moller-Trumbore predicate to those, and appends each hit t to the
+// centre's list in LDS.  A centre outside the windows fails the bound test,
+// and the bound test is conservative (every hit passes it), so each centre
+// collects the same hit multiset as testing every triangle; a triangle
+// typically touches a few centres of a slice, so no lane runs the predicate
+// for a centre it cannot hit.  Finally each thread sorts its centre's hits and
+// counts them with the anchored rule.
 template <class Out>
 __device__ void voxelize_tile(const TriGeo *__restrict__ tris, const GranuleInfo *__restrict__ gran, const Tile tl,
                               uint32_t tile_id, double bx, double by, double bz, double pitch,
                               uint32_t *__restrict__ overflow, uint32_t overflow_cap, Out out) {
+    constexpr int kCentres = kTileX * kTileY * kTileZ;
     __shared__ uint32_t list[kListCap];
     __shared__ uint32_t list_n;
+    __shared__ double hit[kMaxHits * kCentres];  // [slot][centre]
+    __shared__ uint32_t nhit[kCentres];
     const GranuleInfo gi = gran[tl.granule];
     const int tx = threadIdx.x / (kTileY * kTileZ), ty = (threadIdx.x / kTileZ) % kTileY, tz = threadIdx.x % kTileZ;
     const uint32_t x = tl.x0 + tx, y = tl.y0 + ty, z = tl.z0 + tz;
@@ -187,9 +245,11 @@ __device__ void voxelize_tile(const TriGeo *__restrict__ tris, const GranuleInfo
     tv1 = fmax(fmax(wb[0][3], wb[1][3]), fmax(wb[2][3], wb[3][3]));
     if (!(tu0 <= tu1)) return;  // no live centre in this tile (uniform)
 
-    double hits[kMaxHits];
-    int nh = 0;
-    bool over = false;
+    nhit[threadIdx.x] = 0;
+    const double ipitch = 1.0 / pitch;
+    // index slack: 1e-9 of the largest coordinate magnitude in voxels, against
+    // rounding errors of a few ulps of it
+    const double slack = 1e-6 + 1e-9 * (fabs(bx) + fabs(by) + fabs(bz) + 3.0 * 65536.0 * pitch) * ipitch;
     for (uint32_t c0 = gi.tri_start; c0 < gi.tri_end; c0 += kListCap) {
         const uint32_t c1 = min(gi.tri_end, c0 + (uint32_t)kListCap);
         if (threadIdx.x == 0) list_n = 0;
@@ -202,36 +262,45 @@ __device__ void voxelize_tile(const TriGeo *__restrict__ tris, const GranuleInfo
             }
         }
         __syncthreads();
-        const uint32_t n = list_n;
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t i = __builtin_amdgcn_readfirstlane(list[k]);
-            const TriGeo g = tris[i];
-            if (!live || pu < g.umin || pu > g.umax || pv < g.vmin || pv > g.vmax) continue;
-            const double t = mt_hit(g, px, py, pz);
-            if (t < 0.0) continue;
-            if (nh >= kMaxHits) {
-                over = true;
-                continue;
-            }
-            // sorted insert with static register indices
-            double cur = t;
-#pragma unroll
-            for (int s = 0; s < kMaxHits; ++s) {
-                if (s < nh) {
-                    const double lo = hits[s] < cur ? hits[s] : cur;
-                    const double hi = hits[s] < cur ? cur : hits[s];
-                    hits[s] = lo;
-                    cur = hi;
-                } else if (s == nh) {
-                    hits[s] = cur;
+        const uint32_t items = list_n * kTileX;
+        for (uint32_t it = threadIdx.x; it < items; it += blockDim.x) {
+            const int sx = (int)(it % kTileX);
+            const uint32_t qxi = tl.x0 + sx;
+            if (qxi >= tl.x1) continue;
+            const TriGeo g = tris[list[it / kTileX]];
+            const double qx = bx + ((double)qxi + 0.5) * pitch;
+            // u = kUx x + kUy y (kUz = 0): the y window of u in [umin, umax]
+            int ya, yb;
+            index_window((g.umin - kUx * qx) / kUy, (g.umax - kUx * qx) / kUy, by, ipitch, tl.y0, kTileY, slack, ya,
+                         yb);
+            for (int ly = ya; ly <= yb; ++ly) {
+                const uint32_t qyi = tl.y0 + ly;
+                if (qyi >= tl.y1) break;
+                const double qy = by + ((double)qyi + 0.5) * pitch;
+                int za, zb;
+                const double rest = kVx * qx + kVy * qy;
+                index_window((g.vmin - rest) / kVz, (g.vmax - rest) / kVz, bz, ipitch, tl.z0, kTileZ, slack, za, zb);
+                for (int lz = za; lz <= zb; ++lz) {
+                    const uint32_t qzi = tl.z0 + lz;
+                    if (qzi >= tl.z1) break;
+                    const double qz = bz + ((double)qzi + 0.5) * pitch;
+                    if (outside_bbox(gi, qx, qy, qz)) continue;
+                    const double qu = kUx * qx + kUy * qy + kUz * qz;
+                    const double qv = kVx * qx + kVy * qy + kVz * qz;
+                    if (qu < g.umin || qu > g.umax || qv < g.vmin || qv > g.vmax) continue;
+                    const double t = mt_hit(g, qx, qy, qz);
+                    if (t < 0.0) continue;
+                    const int c = (sx * kTileY + ly) * kTileZ + lz;
+                    const uint32_t slot = atomicAdd(&nhit[c], 1u);
+                    if (slot < (uint32_t)kMaxHits) hit[slot * kCentres + c] = t;
                 }
             }
-            ++nh;
         }
         __syncthreads();
     }
     if (!live) return;
-    if (over) {
+    const uint32_t nh = nhit[threadIdx.x];
+    if (nh > (uint32_t)kMaxHits) {
         const uint32_t k = atomicAdd(&overflow[0], 1u);
         if (k < overflow_cap) {
             overflow[1 + 2 * k] = tile_id;
@@ -239,16 +308,9 @@ __device__ void voxelize_tile(const TriGeo *__restrict__ tris, const GranuleInfo
         }
         return;
     }
-    // anchored dedup, src/geometry/mesh_query.rs:105-114
-    int unique = 0;
-    double last = -INFINITY;
-#pragma unroll
-    for (int s = 0; s < kMaxHits; ++s) {
-        if (s < nh && fabs(hits[s] - last) > 1e-8) {
-            ++unique;
-            last = hits[s];
-        }
-    }
+    const double *slot = hit + threadIdx.x;
+    const int unique = nh <= 4 ? unique_hits<4>(slot, kCentres, nh)
+                               : (nh <= 8 ? unique_hits<8>(slot, kCentres, nh) : unique_hits<kMaxHits>(slot, kCentres, nh));
     if (unique & 1) out(gi, x, y, z);
 }
 
