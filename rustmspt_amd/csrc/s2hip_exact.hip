@@ -785,12 +785,12 @@ void shell_devs_release(s2h_ctx *ctx) {
     ctx->shell_devs.clear();
 }
 
-// Per-plan device buffers.
+// Per-plan device buffers.  The direct kernel's count table and the per-offset
+// hits are allocated on first use (exact_run_counts, exact_gather_fold with
+// hits): a plan the FFT kernel serves never holds them.
 static s2h_status plan_buffers(s2h_ctx *ctx, ExactPlan *p) {
     const uint64_t K = p->offs.size();
-    S2H_TRY(dev_alloc(ctx, p->d_table, std::max<uint64_t>(p->table_size, 1) * 8));
     S2H_TRY(dev_alloc(ctx, p->d_terms, std::max<uint64_t>(K, 1) * 8));
-    S2H_TRY(dev_alloc(ctx, p->d_hits, std::max<uint64_t>(K, 1) * 8));
     S2H_TRY(dev_alloc(ctx, p->d_curve, (p->r_max + 1) * 9 + 16));
     S2H_TRY(fold_prepare(ctx, p->d_fold, p->radius_start.data() + 1, p->r_max, &p->fold_nblk));
     return S2H_OK;
@@ -971,6 +971,7 @@ static s2h_status apply_shape(s2h_ctx *ctx, ExactPlan *p, const Grouping &g, int
     for (size_t k = 0; k < io.size(); ++k) sorted[k] = p->items[io[k]];
     p->items.swap(sorted);
     p->model_ms = model_ms(g, waves, &p->model_ops);
+    p->shaped = true;
     dev_free(ctx, p->d_cols);
     dev_free(ctx, p->d_items);
     S2H_TRY(upload(ctx, p->d_cols, g.lanes.data(), g.lanes.size() * sizeof(ColInfo)));
@@ -982,6 +983,22 @@ static s2h_status apply_tune(s2h_ctx *ctx, ExactPlan *p, ExactTune t) {
     Grouping g;
     group_columns(p, t.band, g);
     return apply_shape(ctx, p, g, t.waves);
+}
+
+// A plan's shape without its work items: the model's figures for the kernel
+// choice.  The items (and their upload) wait for the first direct-count run,
+// so a plan served by the FFT kernel never builds them.
+static void model_tune(ExactPlan *p, ExactTune t) {
+    Grouping g;
+    group_columns(p, t.band, g);
+    p->tune = t;
+    p->ngroups = g.ngroups;
+    p->model_ms = model_ms(g, t.waves, &p->model_ops);
+    p->shaped = false;
+}
+
+static s2h_status plan_shape(s2h_ctx *ctx, ExactPlan *p) {
+    return p->shaped ? S2H_OK : apply_tune(ctx, p, p->tune);
 }
 
 struct Ranked {
@@ -1064,15 +1081,15 @@ s2h_status exact_plan_get(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max, 
     if (st == S2H_OK) {
         TuneEntry e;
         if (tune_lookup(p.get(), e)) {
-            st = apply_tune(ctx, p.get(), e.tune);
+            model_tune(p.get(), e.tune);
             // a model pick may still be timed by a context that tunes
             p->tuned = (e.tuned == 1 || !autotune_enabled(ctx)) ? e.tuned : 0;
             p->timed_candidates = e.timed;
         } else {
             const std::vector<Ranked> ranked = rank_tunes(p.get());
-            st = apply_tune(ctx, p.get(), ranked.front().tune);
+            model_tune(p.get(), ranked.front().tune);
             p->tuned = autotune_enabled(ctx) ? 0 : -1;
-            if (st == S2H_OK) tune_store(p.get(), TuneEntry{ranked.front().tune, -1, 0});
+            tune_store(p.get(), TuneEntry{ranked.front().tune, -1, 0});
         }
     }
     if (st == S2H_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = fail(S2H_ERR_HIP, "plan upload failed");
@@ -1182,6 +1199,8 @@ s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
         return fail(S2H_ERR_UNSUPPORTED, "packed volume above 2 GiB");
     if (supported_nw((int)((vol->dims[2] + 31) / 32)) < 0)
         return fail(S2H_ERR_UNSUPPORTED, "nz=" + std::to_string(vol->dims[2]) + " exceeds 1024");
+    S2H_TRY(plan_shape(ctx, p));
+    if (!p->d_table.ptr) S2H_TRY(dev_alloc(ctx, p->d_table, std::max<uint64_t>(p->table_size, 1) * 8));
     if (p->tuned == 0) {
         if (autotune_enabled(ctx)) S2H_TRY(autotune(ctx, vol, p));
         if (p->tuned == 0) p->tuned = -1;  // the model's choice is kept; not retried
@@ -1199,6 +1218,8 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, cons
     const FoldView v = fold_view(p->d_fold, p->r_max, p->fold_nblk);
     // segments: radii 1 .. r_max (radius_start[r] .. radius_start[r + 1])
     const uint64_t *seg_start = (const uint64_t *)p->d_rstart.ptr + 1;
+    if (want_hits && !p->d_hits.ptr) S2H_TRY(dev_alloc(ctx, p->d_hits, std::max<uint64_t>(p->offs.size(), 1) * 8));
+    if (!c.f32 && !c.f64 && !p->d_table.ptr) return fail(S2H_ERR_ARG, "no counts to gather: neither cube nor table");
     if (v.nblk) {
         terms_kernel<<<(v.nblk + kFoldWaves - 1) / kFoldWaves, 64 * kFoldWaves, 0, ctx->stream>>>(
             (const Offset *)p->d_offs.ptr, (const int32_t *)p->d_gather.ptr, (const unsigned long long *)p->d_table.ptr,
@@ -1329,6 +1350,8 @@ s2h_status s2h_exact_tune_info(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_
     S2H_HIP(hipSetDevice(ctx->device));
     ExactPlan *p = nullptr;
     S2H_TRY(exact_plan_get(ctx, dims, r_max, pitch < 1e-9 ? 1e-9 : pitch, &p));
+    S2H_TRY(plan_shape(ctx, p));
+    S2H_HIP(hipStreamSynchronize(ctx->stream));
     info[0] = p->tune.band;
     info[1] = p->tune.waves;
     info[2] = p->tuned;

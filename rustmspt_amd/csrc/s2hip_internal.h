@@ -391,8 +391,10 @@ struct ExactPlan {
     const std::vector<ColInfo> &colset;
     const uint64_t table_size;
     const DevBuf &d_offs, &d_gather, &d_rstart;
-    // tuned part: groups, blocks and items for `tune`
+    // tuned part: groups, blocks and items for `tune`, built on the first
+    // direct-count run (`shaped`); until then only the model's figures are set
     ExactTune tune;
+    bool shaped = false;
     int tuned = 0;                        // 0 not yet, 1 timed on the device, -1 model pick
     int timed_candidates = 0;
     double model_ops = 0;                 // modelled executed VALU lane-ops per run
