@@ -763,7 +763,6 @@ static s2h_status shell_dev(s2h_ctx *ctx, const uint64_t dims[3], uint32_t r_max
     if (b.table_size >= (1ull << 31)) return fail(S2H_ERR_UNSUPPORTED, "offset table too large");
     const uint64_t K = b.offs.size();
     s2h_status st = upload(ctx, d->d_offs, b.offs.data(), std::max<uint64_t>(K, 1) * sizeof(Offset));
-    if (st == S2H_OK) st = upload(ctx, d->d_gather, b.gather.data(), std::max<uint64_t>(K, 1) * sizeof(int32_t));
     if (st == S2H_OK) st = upload(ctx, d->d_rstart, b.radius_start.data(), b.radius_start.size() * 8);
     if (st != S2H_OK) {
         dev_free(ctx, d->d_offs);
@@ -785,12 +784,11 @@ void shell_devs_release(s2h_ctx *ctx) {
     ctx->shell_devs.clear();
 }
 
-// Per-plan device buffers.  The direct kernel's count table and the per-offset
-// hits are allocated on first use (exact_run_counts, exact_gather_fold with
-// hits): a plan the FFT kernel serves never holds them.
+// Per-plan device buffers.  The direct kernel's count table, the shells'
+// gather indices and the per-offset hits are allocated on first use
+// (exact_run_counts, exact_gather_fold with hits): a plan the FFT kernel
+// serves never holds them.  The terms live in the context's scratch.
 static s2h_status plan_buffers(s2h_ctx *ctx, ExactPlan *p) {
-    const uint64_t K = p->offs.size();
-    S2H_TRY(dev_alloc(ctx, p->d_terms, std::max<uint64_t>(K, 1) * 8));
     S2H_TRY(dev_alloc(ctx, p->d_curve, (p->r_max + 1) * 9 + 16));
     S2H_TRY(fold_prepare(ctx, p->d_fold, p->radius_start.data() + 1, p->r_max, &p->fold_nblk));
     return S2H_OK;
@@ -1056,7 +1054,7 @@ static bool autotune_enabled(const s2h_ctx *ctx) {
 }
 
 void exact_plan_release(s2h_ctx *ctx, ExactPlan *p) {
-    DevBuf *bufs[] = {&p->d_cols, &p->d_items, &p->d_table, &p->d_terms, &p->d_hits, &p->d_curve, &p->d_fold};
+    DevBuf *bufs[] = {&p->d_cols, &p->d_items, &p->d_table, &p->d_hits, &p->d_curve, &p->d_fold};
     for (DevBuf *b : bufs) dev_free(ctx, *b);
 }
 
@@ -1201,6 +1199,10 @@ s2h_status exact_run_counts(s2h_ctx *ctx, s2h_vol *vol, ExactPlan *p) {
         return fail(S2H_ERR_UNSUPPORTED, "nz=" + std::to_string(vol->dims[2]) + " exceeds 1024");
     S2H_TRY(plan_shape(ctx, p));
     if (!p->d_table.ptr) S2H_TRY(dev_alloc(ctx, p->d_table, std::max<uint64_t>(p->table_size, 1) * 8));
+    if (!p->shells->d_gather.ptr) {
+        const std::vector<int32_t> &g = p->shells->host->gather;
+        S2H_TRY(upload(ctx, p->shells->d_gather, g.data(), std::max<uint64_t>(g.size(), 1) * sizeof(int32_t)));
+    }
     if (p->tuned == 0) {
         if (autotune_enabled(ctx)) S2H_TRY(autotune(ctx, vol, p));
         if (p->tuned == 0) p->tuned = -1;  // the model's choice is kept; not retried
@@ -1219,18 +1221,20 @@ s2h_status exact_gather_fold(s2h_ctx *ctx, ExactPlan *p, uint64_t occupied, cons
     // segments: radii 1 .. r_max (radius_start[r] .. radius_start[r + 1])
     const uint64_t *seg_start = (const uint64_t *)p->d_rstart.ptr + 1;
     if (want_hits && !p->d_hits.ptr) S2H_TRY(dev_alloc(ctx, p->d_hits, std::max<uint64_t>(p->offs.size(), 1) * 8));
-    if (!c.f32 && !c.f64 && !p->d_table.ptr) return fail(S2H_ERR_ARG, "no counts to gather: neither cube nor table");
+    if (!c.f32 && !c.f64 && (!p->d_table.ptr || !p->d_gather.ptr))
+        return fail(S2H_ERR_ARG, "no counts to gather: neither cube nor table");
+    S2H_TRY(dev_alloc(ctx, ctx->terms, std::max<uint64_t>(p->offs.size(), 1) * 8));
     if (v.nblk) {
         terms_kernel<<<(v.nblk + kFoldWaves - 1) / kFoldWaves, 64 * kFoldWaves, 0, ctx->stream>>>(
             (const Offset *)p->d_offs.ptr, (const int32_t *)p->d_gather.ptr, (const unsigned long long *)p->d_table.ptr,
             c.f32, c.f64, c.R[0], c.R[1], c.R[2], occupied, p->dims[0], p->dims[1], p->dims[2],
-            want_hits ? (unsigned long long *)p->d_hits.ptr : nullptr, (double *)p->d_terms.ptr, seg_start, v);
+            want_hits ? (unsigned long long *)p->d_hits.ptr : nullptr, (double *)ctx->terms.ptr, seg_start, v);
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
     double *values = (double *)(curve_dst ? curve_dst : p->d_curve.ptr);
     uint8_t *support = (uint8_t *)(values + p->r_max + 1);
-    S2H_TRY(fold_launch(ctx, (const double *)p->d_terms.ptr, seg_start, p->r_max, v, values, support, nullptr, true));
+    S2H_TRY(fold_launch(ctx, (const double *)ctx->terms.ptr, seg_start, p->r_max, v, values, support, nullptr, true));
     prof_mark(ctx, 2);
     return S2H_OK;
 }

@@ -373,7 +373,7 @@ struct ShellBase {
 struct ShellDev {
     std::shared_ptr<const ShellBase> host;
     DevBuf d_offs;     // Offset[K]
-    DevBuf d_gather;   // int32[K]
+    DevBuf d_gather;   // int32[K], uploaded for the first direct-count run
     DevBuf d_rstart;   // uint64[r_max+2]
 };
 
@@ -407,7 +407,6 @@ struct ExactPlan {
     DevBuf d_cols;     // ColInfo[ngroups*64]
     DevBuf d_items;    // WorkItem[items]
     DevBuf d_table;    // uint64[table_size]
-    DevBuf d_terms;    // double[K]
     DevBuf d_hits;     // uint64[K]
     DevBuf d_curve;    // double[r_max+1] values, then uint8 support
     DevBuf d_fold;     // the ordered fold's block tables and per-block scratch
@@ -455,6 +454,7 @@ struct s2h_ctx {
     std::map<std::tuple<uint64_t, uint64_t, uint64_t>, s2h::FftPlan *> fft_plans;
     std::map<std::tuple<uint64_t, uint64_t, uint64_t>, s2h::FftPlan *> fftd_plans;  // region-correlation frames
     s2h::DevBuf d_scalar;   // small scratch (popcount result etc.)
+    s2h::DevBuf terms;      // the exact path's per-offset terms, shared by the context's plans (grow-only)
     bool profiling = false;
     bool autotune = true;   // time launch-shape candidates on a new exact plan
     int exact_method = 0;   // exact kernel: 0 by the cost model, 1 direct counting, 2 FP64 FFT rounded

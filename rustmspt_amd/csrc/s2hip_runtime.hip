@@ -470,6 +470,7 @@ void s2h_destroy(s2h_ctx *ctx) {
     for (auto &kv : ctx->fftd_plans) fft_plan_release(ctx, kv.second);
     dev_free(ctx, ctx->fft_arena);
     dev_free(ctx, ctx->d_scalar);
+    dev_free(ctx, ctx->terms);
     batch_ws_release(ctx);
     dev_cache_release(ctx);
     for (hipEvent_t &e : ctx->ev)
@@ -518,7 +519,21 @@ s2h_status s2h_vol_from_packed(s2h_ctx *ctx, const uint64_t *words, uint64_t nwo
         return fail(S2H_ERR_SIZE, "packed volume needs " + std::to_string(v->nwords) + " words, got " +
                                       std::to_string(nwords));
     }
-    hipError_t e = hipMemcpyAsync(v->d_words, words, nwords * 8, hipMemcpyHostToDevice, ctx->stream);
+    // Volumes of 256 KiB and more go through the context's page-locked staging
+    // buffer: one host copy, then a DMA at full link speed (a copy from
+    // pageable memory is staged by the runtime in small synchronous pieces).
+    const uint64_t bytes = nwords * 8;
+    void *stage = nullptr;
+    if (bytes >= (256u << 10)) {
+        const s2h_status st = pinned_staging(ctx, bytes, &stage);
+        if (st != S2H_OK) {
+            s2h_vol_free(ctx, v);
+            return st;
+        }
+        std::memcpy(stage, words, bytes);
+    }
+    hipError_t e = hipMemcpyAsync(v->d_words, stage ? stage : (const void *)words, bytes, hipMemcpyHostToDevice,
+                                  ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         s2h_vol_free(ctx, v);
