@@ -68,12 +68,18 @@ struct FftPlan {
     int tile_y = 0;  // kz columns per workgroup in the y passes (0: not chosen yet)
     int tile_x = 0;  // kz columns per workgroup in the x pass
     int sh_z = 3, sh_y = 3, sh_x = 3;  // LDS padding shift per pass kind (3: a slot every 8; 30: none)
+    // x pass in two register stages (fft_x_power2) where f_x = A B is one of
+    // its lengths: A, B (0: the LDS-staged pass), its block padding in slots,
+    // and the stage-A twiddle rows w^(b q) (b < B, q = 1 .. A - 1)
+    int x2_a = 0, x2_b = 0, x2_pad = 0;
+    bool x2_on = false;
+    DevBuf tw_x2;
 };
 
 void fft_plan_release(s2h_ctx *ctx, FftPlan *p) {
     if (!p) return;
     for (auto &b : p->tw) dev_free(ctx, b);
-    DevBuf *bufs[] = {&p->rev_z, &p->pos_z, &p->g1, &p->g2, &p->g3, &p->cube, &p->fcounts};
+    DevBuf *bufs[] = {&p->rev_z, &p->pos_z, &p->g1, &p->g2, &p->g3, &p->cube, &p->fcounts, &p->tw_x2};
     for (DevBuf *b : bufs) dev_free(ctx, *b);
     delete p;
 }
@@ -255,6 +261,75 @@ __device__ __forceinline__ void small_dft(V *a, int R) {
         a[2] = cadd(m2, n2);
         a[3] = csub(m2, n2);
     }
+}
+
+// cos and sin of 2 pi k / N, correctly rounded, for the lengths of the
+// composite in-register transforms below (k < N).
+__host__ __device__ constexpr double unit_root_cos(int N, int k) {
+    constexpr double c16[16] = {1.0, 0.923879532511286756128, 0.707106781186547524401, 0.382683432365089771728,
+                                0.0, -0.382683432365089771728, -0.707106781186547524401, -0.923879532511286756128,
+                                -1.0, -0.923879532511286756128, -0.707106781186547524401, -0.382683432365089771728,
+                                0.0, 0.382683432365089771728, 0.707106781186547524401, 0.923879532511286756128};
+    constexpr double c20[20] = {1.0, 0.951056516295153572116, 0.809016994374947424102, 0.587785252292473129169,
+                                0.309016994374947424102, 0.0, -0.309016994374947424102, -0.587785252292473129169,
+                                -0.809016994374947424102, -0.951056516295153572116, -1.0, -0.951056516295153572116,
+                                -0.809016994374947424102, -0.587785252292473129169, -0.309016994374947424102, 0.0,
+                                0.309016994374947424102, 0.587785252292473129169, 0.809016994374947424102,
+                                0.951056516295153572116};
+    return N == 16 ? c16[k % 16] : c20[k % 20];
+}
+__host__ __device__ constexpr double unit_root_sin(int N, int k) {
+    return unit_root_cos(N, (k + 3 * N / 4) % N);  // sin t = cos(t - pi/2)
+}
+
+// x * exp(SIGN 2 pi i e / N) with e a compile-time constant after unrolling;
+// the quarter turns are exact swaps and sign flips.
+template <int N, int SIGN, class V>
+__device__ __forceinline__ V mul_root(V x, int e) {
+    using T = decltype(V::x);
+    e %= N;
+    if (e == 0) return x;
+    if (4 * e == N) return SIGN > 0 ? mk<V>(-x.y, x.x) : mk<V>(x.y, -x.x);
+    if (2 * e == N) return mk<V>(-x.x, -x.y);
+    if (4 * e == 3 * N) return SIGN > 0 ? mk<V>(x.y, -x.x) : mk<V>(-x.y, x.x);
+    const T c = (T)unit_root_cos(N, e), s = (T)(SIGN * unit_root_sin(N, e));
+    return mk<V>(x.x * c - x.y * s, x.x * s + x.y * c);
+}
+
+// N = N1 N2 point DFT in registers, natural order in and out (sign as
+// small_dft): input n = N2 n1 + n2, output k = k1 + N1 k2;
+// X[k1 + N1 k2] = sum_n2 w_N2^(n2 k2) w_N^(n2 k1) sum_n1 x[N2 n1 + n2] w_N1^(n1 k1).
+// Its round-off is that of its two factor stages and the twiddle product
+// between them, as for the same factors run as LDS stages.
+template <int N1, int N2, int SIGN, class V>
+__device__ __forceinline__ void dft_comp(V *a) {
+    constexpr int N = N1 * N2;
+    V u[N2][N1];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) u[n2][n1] = a[N2 * n1 + n2];
+        small_dft<SIGN>(u[n2], N1);
+#pragma unroll
+        for (int k1 = 1; k1 < N1; ++k1) u[n2][k1] = mul_root<N, SIGN>(u[n2][k1], n2 * k1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) {
+        V v[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = u[n2][k1];
+        small_dft<SIGN>(v, N2);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) a[k1 + N1 * k2] = v[k2];
+    }
+}
+
+// The in-register DFTs of the two-stage x pass: 16 = 4 x 4, 20 = 4 x 5.
+template <int N, int SIGN, class V>
+__device__ __forceinline__ void dft_reg(V *a) {
+    static_assert(N == 16 || N == 20, "in-register DFT length");
+    if constexpr (N == 16) dft_comp<4, 4, SIGN>(a);
+    else dft_comp<4, 5, SIGN>(a);
 }
 
 // One radix-R stage s over all lines: in-place butterflies on sub-length
@@ -710,6 +785,91 @@ __global__ __launch_bounds__(kFftMaxThreads) void fft_x_power(int nx, int fy, in
     }
 }
 
+// ---- x in two register stages, for L = A B (A = 16, B = 16 or 20): one block
+// per (ky, tile of TILE kz columns, 128 bytes of each row).
+//   stage A: thread (b, c), b < B: the forward DIF butterfly of radix A on rows
+//     x = b + B r (x >= nx zero) straight from G1, its twiddles w_L^(b q),
+//     into LDS;
+//   stage B: thread (q, c), q < A: the B-point block q (its sub-length after
+//     stage A) forward, |F|^2 and back, all in registers;
+//   stage A inverse: thread (b, c): conjugate twiddles, inverse DFT over q,
+//     rows dx = b + B r <= Rx to G2.
+// Every value makes two LDS round trips (the LDS-staged pass makes four) and
+// the block has two barriers.  LDS: element j = B q + k of column c at
+// q (B TILE + pad) + k TILE + c, i.e. column-interleaved so that the lanes of
+// a wave (c fastest) touch consecutive 128-byte rows of slots, with an
+// optional pad-slot gap after each B-element block (pad = 128 bytes puts
+// stage B's neighbouring blocks on the other half of the banks).
+template <class T>
+__host__ __device__ constexpr int x2_tile() { return (int)(128 / (2 * sizeof(T))); }
+template <class T>
+size_t x2_lds_bytes(int A, int B, int pad) {
+    return ((size_t)A * B * x2_tile<T>() + (size_t)(A - 1) * pad) * sizeof(C2<T>);
+}
+template <class T, int A, int B>
+__global__ __launch_bounds__(512) void fft_x_power2(int nx, int fy, int H, int P, int tiles_kz, int Rx, int pad,
+                                                    const C2<T> *__restrict__ twA, const C2<T> *__restrict__ g1,
+                                                    C2<T> *__restrict__ g2) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    constexpr int TILE = x2_tile<T>();
+    V *d = reinterpret_cast<V *>(sm_raw);
+    const int ky = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * TILE;
+    const int nk = min(TILE, H - kz0);
+    const uint64_t rs = (uint64_t)fy * P;
+    const int t = threadIdx.x, c = t % TILE, grp = t / TILE;
+    if (grp < B && c < nk) {
+        const int b = grp;
+        V a[A];
+        const V *gp = g1 + (uint64_t)ky * P + kz0 + c + (uint64_t)b * rs;
+        const uint64_t step = (uint64_t)B * rs;
+#pragma unroll
+        for (int r = 0; r < A; ++r) {
+            a[r] = b + B * r < nx ? *gp : mk<V>(0, 0);
+            gp += step;
+        }
+        dft_reg<A, -1>(a);
+        const V *w = twA + b * (A - 1);
+#pragma unroll
+        for (int q = 1; q < A; ++q) a[q] = cmul(a[q], w[q - 1]);
+        V *dp = d + b * TILE + c;
+#pragma unroll
+        for (int q = 0; q < A; ++q) dp[q * (B * TILE + pad)] = a[q];
+    }
+    __syncthreads();
+    if (grp < A && c < nk) {
+        V a[B];
+        V *dp = d + grp * (B * TILE + pad) + c;
+#pragma unroll
+        for (int k = 0; k < B; ++k) a[k] = dp[k * TILE];
+        dft_reg<B, -1>(a);
+#pragma unroll
+        for (int k = 0; k < B; ++k) a[k] = mk<V>(a[k].x * a[k].x + a[k].y * a[k].y, 0);
+        dft_reg<B, 1>(a);
+#pragma unroll
+        for (int k = 0; k < B; ++k) dp[k * TILE] = a[k];
+    }
+    __syncthreads();
+    if (grp < B && c < nk) {
+        const int b = grp;
+        V a[A];
+        const V *dp = d + b * TILE + c;
+#pragma unroll
+        for (int q = 0; q < A; ++q) a[q] = dp[q * (B * TILE + pad)];
+        const V *w = twA + b * (A - 1);
+#pragma unroll
+        for (int q = 1; q < A; ++q) a[q] = cmulc(a[q], w[q - 1]);
+        dft_reg<A, 1>(a);
+        V *gp = g2 + (uint64_t)ky * P + kz0 + c + (uint64_t)b * rs;
+        const uint64_t step = (uint64_t)B * rs;
+#pragma unroll
+        for (int r = 0; r < A; ++r) {
+            if (b + B * r <= Rx) *gp = a[r];
+            gp += step;
+        }
+    }
+}
+
 // ---- y inverse: per (ix, kz tile), keep the needed y
 template <class T>
 __global__ __launch_bounds__(kFftMaxThreads) void fft_y_inv(int fy, int H, int P, int tiles_kz, int tile, Fac fc,
@@ -1135,6 +1295,22 @@ std::tuple<uint64_t, uint64_t, uint64_t> fft_plan_key(const uint64_t dims[3], co
                            dims[1] | ((uint64_t)R[1] << 32), dims[2] | ((uint64_t)R[2] << 32));
 }
 
+// The lengths the two-stage x pass runs: L = A B with its in-register DFTs.
+bool x2_split(int L, int *A, int *B) {
+    if (L == 320) {
+        *A = 16;
+        *B = 20;
+        return true;
+    }
+    if (L == 256) {
+        *A = 16;
+        *B = 16;
+        return true;
+    }
+    *A = *B = 0;
+    return false;
+}
+
 // The per-length tables of a plan whose f[] and ax[] are set: each axis'
 // stage twiddle rows in the plan's precision, and the z axis' DIF output order.
 s2h_status fft_plan_tables(s2h_ctx *ctx, FftPlan *p) {
@@ -1152,6 +1328,19 @@ s2h_status fft_plan_tables(s2h_ctx *ctx, FftPlan *p) {
         const std::vector<float2> rows32 = stage_twiddles(p->ax[a], tw32);
         S2H_TRY(real_bytes == 8 ? upload(ctx, p->tw[a], rows.data(), rows.size() * sizeof(double2))
                                 : upload(ctx, p->tw[a], rows32.data(), rows32.size() * sizeof(float2)));
+    }
+    if (x2_split((int)p->f[0], &p->x2_a, &p->x2_b)) {
+        const int L = (int)p->f[0], A = p->x2_a, B = p->x2_b;
+        std::vector<double2> rows;
+        std::vector<float2> rows32;
+        for (int b = 0; b < B; ++b)
+            for (int q = 1; q < A; ++q) {
+                const double ang = -2.0 * M_PI * (double)((b * q) % L) / (double)L;
+                rows.push_back(make_double2(std::cos(ang), std::sin(ang)));
+                rows32.push_back(make_float2((float)rows.back().x, (float)rows.back().y));
+            }
+        S2H_TRY(real_bytes == 8 ? upload(ctx, p->tw_x2, rows.data(), rows.size() * sizeof(double2))
+                                : upload(ctx, p->tw_x2, rows32.data(), rows32.size() * sizeof(float2)));
     }
     const AxisPlan &az = p->ax[2];  // natural frequency held at each DIF output position (z axis)
     std::vector<int> rev(az.L);
@@ -1279,6 +1468,31 @@ static int pass_threads(int L, const Fac &fc, int tile) {
     return std::min(kFftMaxThreads, std::max(kFftThreads, t));
 }
 
+// The two-stage x pass of a plan with x2_a, x2_b set (f_x = A B).
+template <class T>
+s2h_status fft_x2_launch(FftPlan *p, int nx, int fy, int H, int P, int Rx, hipStream_t s) {
+    using V = C2<T>;
+    const int A = p->x2_a, B = p->x2_b, tile = x2_tile<T>();
+    if (A * B != (int)p->f[0] || nx > A * B || Rx >= A * B) return fail(S2H_ERR_UNSUPPORTED, "two-stage x pass shape");
+    const int tiles = (H + tile - 1) / tile;
+    const int threads = (std::max(A, B) * tile + 63) / 64 * 64;
+    const size_t lds = x2_lds_bytes<T>(A, B, p->x2_pad);
+    const V *tw = (const V *)p->tw_x2.ptr;
+    const V *g1 = (const V *)p->g1.ptr;
+    V *g2 = (V *)p->g2.ptr;
+    if (A == 16 && B == 20) {
+        S2H_TRY(set_lds((const void *)fft_x_power2<T, 16, 20>, lds));
+        fft_x_power2<T, 16, 20><<<fy * tiles, threads, lds, s>>>(nx, fy, H, P, tiles, Rx, p->x2_pad, tw, g1, g2);
+    } else if (A == 16 && B == 16) {
+        S2H_TRY(set_lds((const void *)fft_x_power2<T, 16, 16>, lds));
+        fft_x_power2<T, 16, 16><<<fy * tiles, threads, lds, s>>>(nx, fy, H, P, tiles, Rx, p->x2_pad, tw, g1, g2);
+    } else {
+        return fail(S2H_ERR_UNSUPPORTED, "two-stage x pass length");
+    }
+    S2H_HIP(hipGetLastError());
+    return S2H_OK;
+}
+
 template <class T>
 s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = nullptr) {
     using V = C2<T>;
@@ -1315,7 +1529,10 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[1], s));
-    {
+    if (p->x2_on) {
+        S2H_TRY(fft_x2_launch<T>(p, nx, fy, H, P, Rx, s));
+        ++ctx->launches;
+    } else {
         const size_t lds = lds_bytes<T>(fx, tx, p->sh_x);
         S2H_TRY(set_lds((const void *)fft_x_power<T>, lds));
         fft_x_power<T><<<fy * tiles_x, pass_threads(fx, fcx, tx), lds, s>>>(nx, fy, H, P, tiles_x, tx, fcx,
@@ -1395,6 +1612,9 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     p->sh_z = pad_rule<T>(fz, z_lines_per_block<T>(fz));
     p->sh_y = lds_bytes<T>((int)p->f[1], p->tile_y, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[1], p->tile_y) : kNoPad;
     p->sh_x = lds_bytes<T>((int)p->f[0], p->tile_x, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[0], p->tile_x) : kNoPad;
+    // the two-stage x pass where its length allows, blocks padded
+    p->x2_on = p->x2_a > 0;
+    p->x2_pad = (int)(128 / sizeof(C2<T>));
     const char *env = std::getenv("S2HIP_AUTOTUNE");
     const double cells = (double)p->f[0] * (double)p->f[1] * (double)p->f[2];
     // tune only where a run is short (up to a 1536^3 padded grid: about 50 ms
@@ -1407,6 +1627,9 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
         if (st == S2H_OK && hipEventCreate(&e) != hipSuccess) st = fail(S2H_ERR_HIP, "hipEventCreate failed");
     float best_y = 1e30f, best_x = 1e30f;
     int pick_y = p->tile_y, pick_x = p->tile_x, psh_y = p->sh_y, psh_x = p->sh_x;
+    const bool x2_ok = p->x2_on;
+    int pick_x2 = -1;  // the two-stage x pass' padding if it is fastest, else -1
+    p->x2_on = false;
     for (int t : cands) {
         for (int sh : {kPad, kNoPad}) {
             if (st != S2H_OK) break;
@@ -1439,12 +1662,34 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
             }
         }
     }
+    // the two-stage x pass, unpadded and padded, against the best LDS-staged one
+    p->tile_y = pick_y;
+    p->sh_y = psh_y;
+    for (int pad : {0, (int)(128 / sizeof(C2<T>))}) {
+        if (st != S2H_OK || !x2_ok) break;
+        p->x2_on = true;
+        p->x2_pad = pad;
+        st = fft_run<T>(ctx, vol, p);  // warm-up
+        if (st == S2H_OK) st = fft_run<T>(ctx, vol, p, ev);
+        hipError_t e = hipSuccess;
+        float xm = 0;
+        if (st == S2H_OK) e = hipEventSynchronize(ev[3]);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&xm, ev[1], ev[2]);
+        if (st == S2H_OK && e != hipSuccess)
+            st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
+        if (st == S2H_OK && xm < best_x) {
+            best_x = xm;
+            pick_x2 = pad;
+        }
+    }
     for (hipEvent_t &e : ev)
         if (e) (void)hipEventDestroy(e);
     p->tile_y = pick_y;
     p->tile_x = pick_x;
     p->sh_y = psh_y;
     p->sh_x = psh_x;
+    p->x2_on = pick_x2 >= 0;
+    p->x2_pad = pick_x2 >= 0 ? pick_x2 : 0;
     return st;
 }
 
