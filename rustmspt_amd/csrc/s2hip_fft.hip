@@ -74,12 +74,18 @@ struct FftPlan {
     int x2_a = 0, x2_b = 0, x2_pad = 0;
     bool x2_on = false;
     DevBuf tw_x2;
+    // the same for the y passes (forward and inverse together: the two-stage
+    // forward leaves natural frequency order, the LDS-staged one DIF order)
+    // and for the z forward pass
+    int y2_a = 0, y2_b = 0, y2_pad = 0, z2_a = 0, z2_b = 0;
+    bool y2_on = false, z2_on = false;
+    DevBuf tw_y2, tw_z2;
 };
 
 void fft_plan_release(s2h_ctx *ctx, FftPlan *p) {
     if (!p) return;
     for (auto &b : p->tw) dev_free(ctx, b);
-    DevBuf *bufs[] = {&p->rev_z, &p->pos_z, &p->g1, &p->g2, &p->g3, &p->cube, &p->fcounts, &p->tw_x2};
+    DevBuf *bufs[] = {&p->rev_z, &p->pos_z, &p->g1, &p->g2, &p->g3, &p->cube, &p->fcounts, &p->tw_x2, &p->tw_y2, &p->tw_z2};
     for (DevBuf *b : bufs) dev_free(ctx, *b);
     delete p;
 }
@@ -870,6 +876,188 @@ __global__ __launch_bounds__(512) void fft_x_power2(int nx, int fy, int H, int P
     }
 }
 
+// ---- y in two register stages (f_y = A B, as the x pass above): the
+// forward pass writes its spectrum in natural frequency order, which the
+// inverse reads back; the x pass in between works per ky line, in any order.
+//   forward, per (x, tile): stage A from G1 rows y = b + B r (y >= ny zero)
+//     into LDS; stage B: block q's B-point DFT, X[q + A k] to row q + A k.
+//   inverse, per (ix, tile): the B-point inverse DFT of rows q + A k, k < B,
+//     into block q of LDS; inverse stage A; kept rows |dy| <= Ry to G3.
+template <class T, int A, int B>
+__global__ __launch_bounds__(512) void fft_y_fwd2(int ny, int fy, int P, int H, int tiles_kz, int pad,
+                                                  const C2<T> *__restrict__ twA, C2<T> *__restrict__ g1) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    constexpr int TILE = x2_tile<T>();
+    V *d = reinterpret_cast<V *>(sm_raw);
+    const int x = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * TILE;
+    const int nk = min(TILE, H - kz0);
+    V *base = g1 + (uint64_t)x * fy * P + kz0;
+    const int t = threadIdx.x, c = t % TILE, grp = t / TILE;
+    if (grp < B && c < nk) {
+        const int b = grp;
+        V a[A];
+        const V *gp = base + c + (uint64_t)b * P;
+#pragma unroll
+        for (int r = 0; r < A; ++r) {
+            a[r] = b + B * r < ny ? *gp : mk<V>(0, 0);
+            gp += (uint64_t)B * P;
+        }
+        dft_reg<A, -1>(a);
+        const V *w = twA + b * (A - 1);
+#pragma unroll
+        for (int q = 1; q < A; ++q) a[q] = cmul(a[q], w[q - 1]);
+        V *dp = d + b * TILE + c;
+#pragma unroll
+        for (int q = 0; q < A; ++q) dp[q * (B * TILE + pad)] = a[q];
+    }
+    __syncthreads();
+    if (grp < A && c < nk) {
+        const int q = grp;
+        V a[B];
+        const V *dp = d + q * (B * TILE + pad) + c;
+#pragma unroll
+        for (int k = 0; k < B; ++k) a[k] = dp[k * TILE];
+        dft_reg<B, -1>(a);
+        V *gp = base + c + (uint64_t)q * P;
+#pragma unroll
+        for (int k = 0; k < B; ++k) gp[(uint64_t)(A * k) * P] = a[k];
+    }
+}
+
+template <class T, int A, int B>
+__global__ __launch_bounds__(512) void fft_y_inv2(int fy, int P, int H, int tiles_kz, int Ry, int pad,
+                                                  const C2<T> *__restrict__ twA, const C2<T> *__restrict__ g2,
+                                                  C2<T> *__restrict__ g3) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    constexpr int TILE = x2_tile<T>(), L = A * B;
+    V *d = reinterpret_cast<V *>(sm_raw);
+    const int ix = blockIdx.x / tiles_kz, kz0 = (blockIdx.x % tiles_kz) * TILE;
+    const int nk = min(TILE, H - kz0);
+    const int t = threadIdx.x, c = t % TILE, grp = t / TILE;
+    if (grp < A && c < nk) {
+        const int q = grp;
+        V a[B];
+        const V *gp = g2 + (uint64_t)ix * fy * P + kz0 + c + (uint64_t)q * P;
+#pragma unroll
+        for (int k = 0; k < B; ++k) a[k] = gp[(uint64_t)(A * k) * P];
+        dft_reg<B, 1>(a);
+        V *dp = d + q * (B * TILE + pad) + c;
+#pragma unroll
+        for (int k = 0; k < B; ++k) dp[k * TILE] = a[k];
+    }
+    __syncthreads();
+    if (grp < B && c < nk) {
+        const int b = grp;
+        V a[A];
+        const V *dp = d + b * TILE + c;
+#pragma unroll
+        for (int q = 0; q < A; ++q) a[q] = dp[q * (B * TILE + pad)];
+        const V *w = twA + b * (A - 1);
+#pragma unroll
+        for (int q = 1; q < A; ++q) a[q] = cmulc(a[q], w[q - 1]);
+        dft_reg<A, 1>(a);
+        const int nyo = 2 * Ry + 1;
+        V *gp = g3 + (uint64_t)ix * nyo * P + kz0 + c;
+#pragma unroll
+        for (int r = 0; r < A; ++r) {
+            const int y = b + B * r;
+            const int o = y <= Ry ? y : (y >= L - Ry ? y - L + nyo : -1);
+            if (o >= 0) gp[(uint64_t)(uint32_t)o * P] = a[r];
+        }
+    }
+}
+
+// ---- z forward in two register stages (f_z = A B): lines (two packed rows
+// per complex line, as fft_z_fwd) of A blocks of B + 1 slots in LDS.
+//   stage A: thread (line, b): the radix-A butterfly of z = b + B r straight
+//     from the staged bits, twiddles, into block q at slot b;
+//   stage B: thread (line, q): block q's B-point DFT in place, X[q + A k] at
+//     slot k of block q;
+//   split: thread (line, k), k < H: the two half spectra from X[k] and
+//     X[L - k], as fft_z_fwd.
+// The odd block stride puts the lanes of stage B and the split on distinct
+// 16-byte bank groups.
+template <class T>
+size_t z2_lds_bytes(int A, int B, int lines, int row_words) {
+    return (size_t)lines * A * (B + 1) * sizeof(C2<T>) + (size_t)lines * 2 * row_words * 8;
+}
+template <class T, int A, int B>
+__global__ __launch_bounds__(256) void fft_z_fwd2(const uint64_t *__restrict__ words, int nx, int ny, int nz,
+                                                  int row_words, int fy, int H, int P, int lines_per_block,
+                                                  const C2<T> *__restrict__ twA, C2<T> *__restrict__ g1,
+                                                  FastDiv dyp, FastDiv dh) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    constexpr int L = A * B, BS = B + 1, LS = A * BS;
+    V *d = reinterpret_cast<V *>(sm_raw);
+    const int npairs = nx * ((ny + 1) / 2);
+    const int p0 = blockIdx.x * lines_per_block;
+    const int np = min(lines_per_block, npairs - p0);
+    uint64_t *sw = reinterpret_cast<uint64_t *>(d + (size_t)lines_per_block * LS);
+    for (int t = threadIdx.x; t < np * 2 * row_words; t += blockDim.x) {
+        const int pl = t / (2 * row_words), rem = t - pl * 2 * row_words;
+        const int i = rem / row_words, wd = rem - i * row_words;
+        int x, yp;
+        divmod(p0 + pl, dyp, x, yp);
+        const int y = 2 * yp + i;
+        sw[t] = y < ny ? words[((uint64_t)x * ny + y) * row_words + wd] : 0ull;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < np * B; t += blockDim.x) {
+        const int pl = t / B, b = t - pl * B;
+        const uint64_t *ra = sw + 2 * pl * row_words;
+        using R = decltype(V::x);
+        V a[A];
+#pragma unroll
+        for (int r = 0; r < A; ++r) {
+            const int z = b + B * r;
+            R re = 0, im = 0;
+            if (z < nz) {
+                re = (R)((ra[z >> 6] >> (z & 63)) & 1ull);
+                im = (R)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
+            }
+            a[r] = mk<V>(re, im);
+        }
+        dft_reg<A, -1>(a);
+        const V *w = twA + b * (A - 1);
+#pragma unroll
+        for (int q = 1; q < A; ++q) a[q] = cmul(a[q], w[q - 1]);
+        V *dp = d + pl * LS + b;
+#pragma unroll
+        for (int q = 0; q < A; ++q) dp[q * BS] = a[q];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < np * A; t += blockDim.x) {
+        const int pl = t / A, q = t - pl * A;
+        V *dp = d + pl * LS + q * BS;
+        V a[B];
+#pragma unroll
+        for (int k = 0; k < B; ++k) a[k] = dp[k];
+        dft_reg<B, -1>(a);
+#pragma unroll
+        for (int k = 0; k < B; ++k) dp[k] = a[k];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < np * H; t += blockDim.x) {
+        int pl, k, x, yp;
+        divmod(t, dh, pl, k);
+        divmod(p0 + pl, dyp, x, yp);
+        const int y = 2 * yp;
+        const int kn = k == 0 ? 0 : L - k;
+        const V zk = d[pl * LS + (k % A) * BS + k / A];
+        const V zn = d[pl * LS + (kn % A) * BS + kn / A];
+        // A = (Z[k] + conj Z[-k]) / 2,  B = (Z[k] - conj Z[-k]) / (2i)
+        using R = decltype(V::x);
+        const R half = (R)0.5;
+        const V va = mk<V>(half * (zk.x + zn.x), half * (zk.y - zn.y));
+        const V vb = mk<V>(half * (zk.y + zn.y), -half * (zk.x - zn.x));
+        g1[((uint64_t)x * fy + y) * P + k] = va;
+        if (y + 1 < ny) g1[((uint64_t)x * fy + y + 1) * P + k] = vb;
+    }
+}
+
 // ---- y inverse: per (ix, kz tile), keep the needed y
 template <class T>
 __global__ __launch_bounds__(kFftMaxThreads) void fft_y_inv(int fy, int H, int P, int tiles_kz, int tile, Fac fc,
@@ -1329,19 +1517,23 @@ s2h_status fft_plan_tables(s2h_ctx *ctx, FftPlan *p) {
         S2H_TRY(real_bytes == 8 ? upload(ctx, p->tw[a], rows.data(), rows.size() * sizeof(double2))
                                 : upload(ctx, p->tw[a], rows32.data(), rows32.size() * sizeof(float2)));
     }
-    if (x2_split((int)p->f[0], &p->x2_a, &p->x2_b)) {
-        const int L = (int)p->f[0], A = p->x2_a, B = p->x2_b;
+    // stage-A twiddle rows of the two-stage passes: w^(b q), b < B, q = 1 .. A - 1
+    auto two_stage = [&](int L, int *A, int *B, DevBuf &buf) -> s2h_status {
+        if (!x2_split(L, A, B)) return S2H_OK;
         std::vector<double2> rows;
         std::vector<float2> rows32;
-        for (int b = 0; b < B; ++b)
-            for (int q = 1; q < A; ++q) {
+        for (int b = 0; b < *B; ++b)
+            for (int q = 1; q < *A; ++q) {
                 const double ang = -2.0 * M_PI * (double)((b * q) % L) / (double)L;
                 rows.push_back(make_double2(std::cos(ang), std::sin(ang)));
                 rows32.push_back(make_float2((float)rows.back().x, (float)rows.back().y));
             }
-        S2H_TRY(real_bytes == 8 ? upload(ctx, p->tw_x2, rows.data(), rows.size() * sizeof(double2))
-                                : upload(ctx, p->tw_x2, rows32.data(), rows32.size() * sizeof(float2)));
-    }
+        return real_bytes == 8 ? upload(ctx, buf, rows.data(), rows.size() * sizeof(double2))
+                               : upload(ctx, buf, rows32.data(), rows32.size() * sizeof(float2));
+    };
+    S2H_TRY(two_stage((int)p->f[0], &p->x2_a, &p->x2_b, p->tw_x2));
+    S2H_TRY(two_stage((int)p->f[1], &p->y2_a, &p->y2_b, p->tw_y2));
+    S2H_TRY(two_stage((int)p->f[2], &p->z2_a, &p->z2_b, p->tw_z2));
     const AxisPlan &az = p->ax[2];  // natural frequency held at each DIF output position (z axis)
     std::vector<int> rev(az.L);
     for (int pos = 0; pos < az.L; ++pos) {
@@ -1493,6 +1685,69 @@ s2h_status fft_x2_launch(FftPlan *p, int nx, int fy, int H, int P, int Rx, hipSt
     return S2H_OK;
 }
 
+// The two-stage y passes (forward: inverse == false) of a plan with y2_a,
+// y2_b set (f_y = A B).
+template <class T>
+s2h_status fft_y2_launch(FftPlan *p, bool inverse, int nplanes, int ny, int H, int P, int Ry, hipStream_t s) {
+    using V = C2<T>;
+    const int A = p->y2_a, B = p->y2_b, tile = x2_tile<T>(), fy = (int)p->f[1];
+    if (A * B != fy || ny > fy || Ry >= fy) return fail(S2H_ERR_UNSUPPORTED, "two-stage y pass shape");
+    const int tiles = (H + tile - 1) / tile;
+    const int threads = (std::max(A, B) * tile + 63) / 64 * 64;
+    const size_t lds = x2_lds_bytes<T>(A, B, p->y2_pad);
+    const V *tw = (const V *)p->tw_y2.ptr;
+    V *g1 = (V *)p->g1.ptr, *g2 = (V *)p->g2.ptr, *g3 = (V *)p->g3.ptr;
+#define S2H_Y2(AA, BB)                                                                                            \
+    if (A == AA && B == BB) {                                                                                     \
+        if (!inverse) {                                                                                           \
+            S2H_TRY(set_lds((const void *)fft_y_fwd2<T, AA, BB>, lds));                                          \
+            fft_y_fwd2<T, AA, BB><<<nplanes * tiles, threads, lds, s>>>(ny, fy, P, H, tiles, p->y2_pad, tw, g1);  \
+        } else {                                                                                                  \
+            S2H_TRY(set_lds((const void *)fft_y_inv2<T, AA, BB>, lds));                                          \
+            fft_y_inv2<T, AA, BB><<<nplanes * tiles, threads, lds, s>>>(fy, P, H, tiles, Ry, p->y2_pad, tw, g2,  \
+                                                                         g3);                                    \
+        }                                                                                                         \
+        S2H_HIP(hipGetLastError());                                                                               \
+        return S2H_OK;                                                                                            \
+    }
+    S2H_Y2(16, 20)
+    S2H_Y2(16, 16)
+#undef S2H_Y2
+    return fail(S2H_ERR_UNSUPPORTED, "two-stage y pass length");
+}
+
+// Lines (row pairs) per workgroup of the two-stage z forward pass.
+static constexpr int kZ2Lines = 6;
+
+// The two-stage z forward pass of a plan with z2_a, z2_b set (f_z = A B).
+template <class T>
+s2h_status fft_z2_launch(FftPlan *p, s2h_vol *vol, int H, int P, hipStream_t s) {
+    using V = C2<T>;
+    const int A = p->z2_a, B = p->z2_b, fz = (int)p->f[2], fy = (int)p->f[1];
+    const int nx = (int)vol->dims[0], ny = (int)vol->dims[1], nz = (int)vol->dims[2];
+    if (A * B != fz || nz > fz) return fail(S2H_ERR_UNSUPPORTED, "two-stage z pass shape");
+    const int lines = kZ2Lines, threads = (lines * std::max(A, B) + 63) / 64 * 64;
+    const size_t lds = z2_lds_bytes<T>(A, B, lines, (int)vol->row_words);
+    const int npairs = nx * ((ny + 1) / 2);
+    const unsigned grid = (unsigned)((npairs + lines - 1) / lines);
+    const V *tw = (const V *)p->tw_z2.ptr;
+    V *g1 = (V *)p->g1.ptr;
+    const FastDiv dyp = fast_div((uint32_t)((ny + 1) / 2)), dh = fast_div((uint32_t)H);
+    if (A == 16 && B == 20) {
+        S2H_TRY(set_lds((const void *)fft_z_fwd2<T, 16, 20>, lds));
+        fft_z_fwd2<T, 16, 20><<<grid, threads, lds, s>>>(vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, P,
+                                                          lines, tw, g1, dyp, dh);
+    } else if (A == 16 && B == 16) {
+        S2H_TRY(set_lds((const void *)fft_z_fwd2<T, 16, 16>, lds));
+        fft_z_fwd2<T, 16, 16><<<grid, threads, lds, s>>>(vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, P,
+                                                          lines, tw, g1, dyp, dh);
+    } else {
+        return fail(S2H_ERR_UNSUPPORTED, "two-stage z pass length");
+    }
+    S2H_HIP(hipGetLastError());
+    return S2H_OK;
+}
+
 template <class T>
 s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = nullptr) {
     using V = C2<T>;
@@ -1506,7 +1761,10 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     const int ty = p->tile_y, tx = p->tile_x;
     const int tiles_y = (H + ty - 1) / ty, tiles_x = (H + tx - 1) / tx;
     // z forward
-    {
+    if (p->z2_on) {
+        S2H_TRY(fft_z2_launch<T>(p, vol, H, P, s));
+        ++ctx->launches;
+    } else {
         const int ppb = z_lines_per_block<T>(fz);
         const size_t lds = z_fwd_lds_bytes<T>(fz, ppb, p->sh_z, (int)vol->row_words);
         S2H_TRY(set_lds((const void *)fft_z_fwd<T>, lds));
@@ -1519,7 +1777,10 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[0], s));
-    {
+    if (p->y2_on) {
+        S2H_TRY(fft_y2_launch<T>(p, false, nx, ny, H, P, Ry, s));
+        ++ctx->launches;
+    } else {
         const size_t lds = lds_bytes<T>(fy, ty, p->sh_y);
         S2H_TRY(set_lds((const void *)fft_y_fwd<T>, lds));
         fft_y_fwd<T><<<nx * tiles_y, pass_threads(fy, fcy, ty), lds, s>>>(ny, fy, H, P, tiles_y, ty, fcy,
@@ -1542,7 +1803,10 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[2], s));
-    {
+    if (p->y2_on) {
+        S2H_TRY(fft_y2_launch<T>(p, true, nxo, ny, H, P, Ry, s));
+        ++ctx->launches;
+    } else {
         const size_t lds = lds_bytes<T>(fy, ty, p->sh_y);
         S2H_TRY(set_lds((const void *)fft_y_inv<T>, lds));
         fft_y_inv<T><<<nxo * tiles_y, pass_threads(fy, fcy, ty), lds, s>>>(fy, H, P, tiles_y, ty, fcy,
@@ -1612,9 +1876,12 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     p->sh_z = pad_rule<T>(fz, z_lines_per_block<T>(fz));
     p->sh_y = lds_bytes<T>((int)p->f[1], p->tile_y, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[1], p->tile_y) : kNoPad;
     p->sh_x = lds_bytes<T>((int)p->f[0], p->tile_x, kPad) <= 150 * 1024 ? pad_rule<T>((int)p->f[0], p->tile_x) : kNoPad;
-    // the two-stage x pass where its length allows, blocks padded
+    // the two-stage passes where their lengths allow, blocks padded
     p->x2_on = p->x2_a > 0;
     p->x2_pad = (int)(128 / sizeof(C2<T>));
+    p->y2_on = p->y2_a > 0;
+    p->y2_pad = (int)(128 / sizeof(C2<T>));
+    p->z2_on = p->z2_a > 0;
     const char *env = std::getenv("S2HIP_AUTOTUNE");
     const double cells = (double)p->f[0] * (double)p->f[1] * (double)p->f[2];
     // tune only where a run is short (up to a 1536^3 padded grid: about 50 ms
@@ -1627,9 +1894,11 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
         if (st == S2H_OK && hipEventCreate(&e) != hipSuccess) st = fail(S2H_ERR_HIP, "hipEventCreate failed");
     float best_y = 1e30f, best_x = 1e30f;
     int pick_y = p->tile_y, pick_x = p->tile_x, psh_y = p->sh_y, psh_x = p->sh_x;
-    const bool x2_ok = p->x2_on;
+    const bool x2_ok = p->x2_on, y2_ok = p->y2_on;
     int pick_x2 = -1;  // the two-stage x pass' padding if it is fastest, else -1
+    int pick_y2 = -1;  // the same for the two-stage y passes
     p->x2_on = false;
+    p->y2_on = false;
     for (int t : cands) {
         for (int sh : {kPad, kNoPad}) {
             if (st != S2H_OK) break;
@@ -1662,24 +1931,33 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
             }
         }
     }
-    // the two-stage x pass, unpadded and padded, against the best LDS-staged one
+    // the two-stage x and y passes, unpadded and padded, against the best
+    // LDS-staged ones
     p->tile_y = pick_y;
     p->sh_y = psh_y;
     for (int pad : {0, (int)(128 / sizeof(C2<T>))}) {
-        if (st != S2H_OK || !x2_ok) break;
-        p->x2_on = true;
+        if (st != S2H_OK || !(x2_ok || y2_ok)) break;
+        p->x2_on = x2_ok;
         p->x2_pad = pad;
+        p->y2_on = y2_ok;
+        p->y2_pad = pad;
         st = fft_run<T>(ctx, vol, p);  // warm-up
         if (st == S2H_OK) st = fft_run<T>(ctx, vol, p, ev);
         hipError_t e = hipSuccess;
-        float xm = 0;
+        float yf = 0, xm = 0, yi = 0;
         if (st == S2H_OK) e = hipEventSynchronize(ev[3]);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yf, ev[0], ev[1]);
         if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&xm, ev[1], ev[2]);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yi, ev[2], ev[3]);
         if (st == S2H_OK && e != hipSuccess)
             st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
-        if (st == S2H_OK && xm < best_x) {
+        if (st == S2H_OK && x2_ok && xm < best_x) {
             best_x = xm;
             pick_x2 = pad;
+        }
+        if (st == S2H_OK && y2_ok && yf + yi < best_y) {
+            best_y = yf + yi;
+            pick_y2 = pad;
         }
     }
     for (hipEvent_t &e : ev)
@@ -1690,6 +1968,8 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     p->sh_x = psh_x;
     p->x2_on = pick_x2 >= 0;
     p->x2_pad = pick_x2 >= 0 ? pick_x2 : 0;
+    p->y2_on = pick_y2 >= 0;
+    p->y2_pad = pick_y2 >= 0 ? pick_y2 : 0;
     return st;
 }
 
