@@ -35,11 +35,12 @@ VALU_4CYCLE_OPS = 256 * 64 * 2.4e9
 # WRITE_SIZE 524,288 KiB; the request counts give the bytes read directly.
 # Direct counting (exact_count_kernel, launch shape band 3 / 65536 waves):
 COUNT_PHASE_TRAFFIC = 64 * 59351.5 + 128 * 177855 + 909816 * 1024.0
-# The FP64 FFT passes (z fwd, y fwd, x fwd + |F|^2 + inv, y inv, z inv), steady-state
-# dispatches (the tile choice's trial runs excluded), read then write MB per pass:
-FFT64_PHASE_TRAFFIC = (3.3 + 169.9 + 176.3 + 212.3 + 220.4 + 53.9 + 56.1 + 21.7 + 22.6 + 8.7) * 1e6
+# The FP64 FFT passes (z fwd, y fwd, x fwd + |F|^2 + inv, y inv, z inv; the two-stage
+# kernels at 320 = 16 x 20), steady-state dispatches (the tile choice's trial runs
+# excluded), read then write MB per pass:
+FFT64_PHASE_TRAFFIC = (3.3 + 169.9 + 176.3 + 212.3 + 220.3 + 53.9 + 56.0 + 21.7 + 22.7 + 8.7) * 1e6
 # The FP32 FFT passes, the same five:
-FFT32_PHASE_TRAFFIC = (3.3 + 86.0 + 92.4 + 107.5 + 115.6 + 27.3 + 29.4 + 11.0 + 11.9 + 4.3) * 1e6
+FFT32_PHASE_TRAFFIC = (3.2 + 86.0 + 92.3 + 107.5 + 115.4 + 27.3 + 29.4 + 11.0 + 11.9 + 4.4) * 1e6
 
 
 def fft64_bytes(dims, offs, r_max, complex_bytes=16.0):

@@ -1058,6 +1058,72 @@ __global__ __launch_bounds__(256) void fft_z_fwd2(const uint64_t *__restrict__ w
     }
 }
 
+// ---- z inverse in two register stages (f_z = A B): two Hermitian half
+// spectra per complex inverse, as fft_z_inv, read in natural order.
+//   stage B: thread (pair, q): the values k = q + A j, j < B (k >= H from the
+//     conjugate of L - k), the B-point inverse DFT into block q of LDS;
+//   stage A: thread (pair, b): conjugate twiddles, inverse DFT over q, the
+//     kept z = b + B r (|z| <= Rz) scaled to the cube.
+template <class T, int A, int B>
+__global__ __launch_bounds__(256) void fft_z_inv2(int nlines, int H, int P, int pairs_per_block, int Rz, T scale,
+                                                  const C2<T> *__restrict__ twA, const C2<T> *__restrict__ g3,
+                                                  T *__restrict__ cube) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    constexpr int L = A * B, BS = B + 1, LS = A * BS;
+    V *d = reinterpret_cast<V *>(sm_raw);
+    const int p0 = blockIdx.x * pairs_per_block;
+    const int npairs = (nlines + 1) / 2;
+    const int np = min(pairs_per_block, npairs - p0);
+    for (int t = threadIdx.x; t < np * A; t += blockDim.x) {
+        const int pl = t / A, q = t - pl * A;
+        const int la = 2 * (p0 + pl), lb = la + 1;
+        const V *ga = g3 + (uint64_t)la * P;
+        const V *gb = g3 + (uint64_t)lb * P;
+        V a[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) {
+            const int k = q + A * j;
+            const bool upper = k >= H;
+            const int kk = upper ? L - k : k;
+            V va = ga[kk];
+            V vb = lb < nlines ? gb[kk] : mk<V>(0, 0);
+            if (upper) {
+                va.y = -va.y;
+                vb.y = -vb.y;
+            }
+            a[j] = mk<V>(va.x - vb.y, va.y + vb.x);  // A + iB
+        }
+        dft_reg<B, 1>(a);
+        V *dp = d + pl * LS + q * BS;
+#pragma unroll
+        for (int j = 0; j < B; ++j) dp[j] = a[j];
+    }
+    __syncthreads();
+    const int nzo = 2 * Rz + 1;
+    for (int t = threadIdx.x; t < np * B; t += blockDim.x) {
+        const int pl = t / B, b = t - pl * B;
+        const int la = 2 * (p0 + pl), lb = la + 1;
+        const V *dp = d + pl * LS + b;
+        V a[A];
+#pragma unroll
+        for (int q = 0; q < A; ++q) a[q] = dp[q * BS];
+        const V *w = twA + b * (A - 1);
+#pragma unroll
+        for (int q = 1; q < A; ++q) a[q] = cmulc(a[q], w[q - 1]);
+        dft_reg<A, 1>(a);
+#pragma unroll
+        for (int r = 0; r < A; ++r) {
+            const int z = b + B * r;
+            const int o = z <= Rz ? z : (z >= L - Rz ? z - L + nzo : -1);
+            if (o >= 0) {
+                cube[(uint64_t)la * nzo + o] = a[r].x * scale;
+                if (lb < nlines) cube[(uint64_t)lb * nzo + o] = a[r].y * scale;
+            }
+        }
+    }
+}
+
 // ---- y inverse: per (ix, kz tile), keep the needed y
 template <class T>
 __global__ __launch_bounds__(kFftMaxThreads) void fft_y_inv(int fy, int H, int P, int tiles_kz, int tile, Fac fc,
@@ -1716,8 +1782,37 @@ s2h_status fft_y2_launch(FftPlan *p, bool inverse, int nplanes, int ny, int H, i
     return fail(S2H_ERR_UNSUPPORTED, "two-stage y pass length");
 }
 
-// Lines (row pairs) per workgroup of the two-stage z forward pass.
+// Lines (row pairs) per workgroup of the two-stage z forward pass, and line
+// pairs per workgroup of the two-stage z inverse (a smaller pass: more
+// workgroups to spread over the CUs).
 static constexpr int kZ2Lines = 6;
+static constexpr int kZ2InvPairs = 3;
+
+// The two-stage z inverse pass of a plan with z2_a, z2_b set.
+template <class T>
+s2h_status fft_z2inv_launch(FftPlan *p, int nlines, int H, int P, int Rz, T scale, hipStream_t s) {
+    using V = C2<T>;
+    const int A = p->z2_a, B = p->z2_b, fz = (int)p->f[2];
+    if (A * B != fz || Rz >= fz) return fail(S2H_ERR_UNSUPPORTED, "two-stage z inverse shape");
+    const int pairs = kZ2InvPairs, threads = (pairs * std::max(A, B) + 63) / 64 * 64;
+    const size_t lds = (size_t)pairs * A * (B + 1) * sizeof(V);
+    const int npairs = (nlines + 1) / 2;
+    const unsigned grid = (unsigned)((npairs + pairs - 1) / pairs);
+    const V *tw = (const V *)p->tw_z2.ptr;
+    if (A == 16 && B == 20) {
+        S2H_TRY(set_lds((const void *)fft_z_inv2<T, 16, 20>, lds));
+        fft_z_inv2<T, 16, 20><<<grid, threads, lds, s>>>(nlines, H, P, pairs, Rz, scale, tw, (const V *)p->g3.ptr,
+                                                          (T *)p->cube.ptr);
+    } else if (A == 16 && B == 16) {
+        S2H_TRY(set_lds((const void *)fft_z_inv2<T, 16, 16>, lds));
+        fft_z_inv2<T, 16, 16><<<grid, threads, lds, s>>>(nlines, H, P, pairs, Rz, scale, tw, (const V *)p->g3.ptr,
+                                                          (T *)p->cube.ptr);
+    } else {
+        return fail(S2H_ERR_UNSUPPORTED, "two-stage z inverse length");
+    }
+    S2H_HIP(hipGetLastError());
+    return S2H_OK;
+}
 
 // The two-stage z forward pass of a plan with z2_a, z2_b set (f_z = A B).
 template <class T>
@@ -1816,7 +1911,11 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         ++ctx->launches;
     }
     if (ev) S2H_HIP(hipEventRecord(ev[3], s));
-    {
+    if (p->z2_on) {
+        const T scale = (T)(1.0 / ((double)fx * (double)fy * (double)fz));
+        S2H_TRY(fft_z2inv_launch<T>(p, nxo * nyo, H, P, Rz, scale, s));
+        ++ctx->launches;
+    } else {
         const int ppb = z_lines_per_block<T>(fz);
         const size_t lds = z_inv_lds_bytes<T>(fz, ppb, p->sh_z);
         S2H_TRY(set_lds((const void *)fft_z_inv<T>, lds));
