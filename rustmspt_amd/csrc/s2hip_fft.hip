@@ -282,10 +282,25 @@ __host__ __device__ constexpr double unit_root_cos(int N, int k) {
                                 -0.809016994374947424102, -0.587785252292473129169, -0.309016994374947424102, 0.0,
                                 0.309016994374947424102, 0.587785252292473129169, 0.809016994374947424102,
                                 0.951056516295153572116};
-    return N == 16 ? c16[k % 16] : c20[k % 20];
+    constexpr double c10[10] = {1.0, 0.809016994374947424102, 0.309016994374947424102, -0.309016994374947424102, -0.809016994374947424102, -1.0, -0.809016994374947424102, -0.309016994374947424102, 0.309016994374947424102, 0.809016994374947424102};
+    constexpr double c12[12] = {1.0, 0.866025403784438646764, 0.500000000000000000000, 0.0, -0.500000000000000000000, -0.866025403784438646764, -1.0, -0.866025403784438646764, -0.500000000000000000000, 0.0, 0.500000000000000000000, 0.866025403784438646764};
+    constexpr double c15[15] = {1.0, 0.913545457642600895502, 0.669130606358858213826, 0.309016994374947424102, -0.104528463267653471400, -0.500000000000000000000, -0.809016994374947424102, -0.978147600733805637929, -0.978147600733805637929, -0.809016994374947424102, -0.500000000000000000000, -0.104528463267653471400, 0.309016994374947424102, 0.669130606358858213826, 0.913545457642600895502};
+    constexpr double c24[24] = {1.0, 0.965925826289068286750, 0.866025403784438646764, 0.707106781186547524401, 0.500000000000000000000, 0.258819045102520762349, 0.0, -0.258819045102520762349, -0.500000000000000000000, -0.707106781186547524401, -0.866025403784438646764, -0.965925826289068286750, -1.0, -0.965925826289068286750, -0.866025403784438646764, -0.707106781186547524401, -0.500000000000000000000, -0.258819045102520762349, 0.0, 0.258819045102520762349, 0.500000000000000000000, 0.707106781186547524401, 0.866025403784438646764, 0.965925826289068286750};
+    switch (N) {
+        case 10: return c10[k % 10];
+        case 12: return c12[k % 12];
+        case 15: return c15[k % 15];
+        case 16: return c16[k % 16];
+        case 20: return c20[k % 20];
+        default: return c24[k % 24];
+    }
 }
 __host__ __device__ constexpr double unit_root_sin(int N, int k) {
-    return unit_root_cos(N, (k + 3 * N / 4) % N);  // sin t = cos(t - pi/2)
+    constexpr double s10[10] = {0.0, 0.587785252292473129169, 0.951056516295153572116, 0.951056516295153572116, 0.587785252292473129169, 0.0, -0.587785252292473129169, -0.951056516295153572116, -0.951056516295153572116, -0.587785252292473129169};
+    constexpr double s15[15] = {0.0, 0.406736643075800207754, 0.743144825477394235015, 0.951056516295153572116, 0.994521895368273336923, 0.866025403784438646764, 0.587785252292473129169, 0.207911690817759337102, -0.207911690817759337102, -0.587785252292473129169, -0.866025403784438646764, -0.994521895368273336923, -0.951056516295153572116, -0.743144825477394235015, -0.406736643075800207754};
+    if (N == 10) return s10[k % 10];
+    if (N == 15) return s15[k % 15];
+    return unit_root_cos(N, (k + 3 * N / 4) % N);  // sin t = cos(t - pi/2) where 4 | N
 }
 
 // x * exp(SIGN 2 pi i e / N) with e a compile-time constant after unrolling;
@@ -330,12 +345,19 @@ __device__ __forceinline__ void dft_comp(V *a) {
     }
 }
 
-// The in-register DFTs of the two-stage x pass: 16 = 4 x 4, 20 = 4 x 5.
+// The in-register DFTs of the two-stage passes: 8 as small_dft, 10 = 2 x 5,
+// 12 = 4 x 3, 15 = 3 x 5, 16 = 4 x 4, 20 = 4 x 5, 24 = 8 x 3.
 template <int N, int SIGN, class V>
 __device__ __forceinline__ void dft_reg(V *a) {
-    static_assert(N == 16 || N == 20, "in-register DFT length");
-    if constexpr (N == 16) dft_comp<4, 4, SIGN>(a);
-    else dft_comp<4, 5, SIGN>(a);
+    static_assert(N == 8 || N == 10 || N == 12 || N == 15 || N == 16 || N == 20 || N == 24,
+                  "in-register DFT length");
+    if constexpr (N == 8) small_dft<SIGN>(a, 8);
+    else if constexpr (N == 10) dft_comp<2, 5, SIGN>(a);
+    else if constexpr (N == 12) dft_comp<4, 3, SIGN>(a);
+    else if constexpr (N == 15) dft_comp<3, 5, SIGN>(a);
+    else if constexpr (N == 16) dft_comp<4, 4, SIGN>(a);
+    else if constexpr (N == 20) dft_comp<4, 5, SIGN>(a);
+    else dft_comp<8, 3, SIGN>(a);
 }
 
 // One radix-R stage s over all lines: in-place butterflies on sub-length
@@ -1549,18 +1571,17 @@ std::tuple<uint64_t, uint64_t, uint64_t> fft_plan_key(const uint64_t dims[3], co
                            dims[1] | ((uint64_t)R[1] << 32), dims[2] | ((uint64_t)R[2] << 32));
 }
 
-// The lengths the two-stage x pass runs: L = A B with its in-register DFTs.
+// The lengths the two-stage passes run: L = A B with their in-register DFTs
+// (the instantiations listed in S2H_TWO_STAGE_LENGTHS).
 bool x2_split(int L, int *A, int *B) {
-    if (L == 320) {
-        *A = 16;
-        *B = 20;
-        return true;
-    }
-    if (L == 256) {
-        *A = 16;
-        *B = 16;
-        return true;
-    }
+    static constexpr int kSplits[][3] = {{96, 8, 12},   {120, 8, 15},  {128, 8, 16},  {160, 16, 10},
+                                         {192, 16, 12}, {256, 16, 16}, {320, 16, 20}, {384, 16, 24}};
+    for (const auto &e : kSplits)
+        if (e[0] == L) {
+            *A = e[1];
+            *B = e[2];
+            return true;
+        }
     *A = *B = 0;
     return false;
 }
@@ -1706,9 +1727,10 @@ s2h_status set_lds(const void *fn, size_t bytes) {
     return S2H_OK;
 }
 
-// Run every pass; leaves the correlation cube in p->cube.  With ev, records
-// ev[0] before the y forward pass and ev[1..3] after it, the x pass and the y
-// inverse pass.
+// Run every pass; leaves the correlation cube in p->cube.  With ev (six
+// events), records ev[4] before the z forward pass, ev[0] before the y forward
+// pass, ev[1..3] after it, the x pass and the y inverse pass, and ev[5] after
+// the z inverse pass.
 // Line pairs per workgroup of the z passes: one radix-8 butterfly per thread
 // per stage (2048 points over 256 threads), capped by 48 KB of LDS.
 template <class T>
@@ -1726,6 +1748,9 @@ static int pass_threads(int L, const Fac &fc, int tile) {
     return std::min(kFftMaxThreads, std::max(kFftThreads, t));
 }
 
+// The (A, B) instantiations of the two-stage kernels (x2_split's table).
+#define S2H_TWO_STAGE_LENGTHS(M) M(8, 12) M(8, 15) M(8, 16) M(16, 10) M(16, 12) M(16, 16) M(16, 20) M(16, 24)
+
 // The two-stage x pass of a plan with x2_a, x2_b set (f_x = A B).
 template <class T>
 s2h_status fft_x2_launch(FftPlan *p, int nx, int fy, int H, int P, int Rx, hipStream_t s) {
@@ -1738,17 +1763,16 @@ s2h_status fft_x2_launch(FftPlan *p, int nx, int fy, int H, int P, int Rx, hipSt
     const V *tw = (const V *)p->tw_x2.ptr;
     const V *g1 = (const V *)p->g1.ptr;
     V *g2 = (V *)p->g2.ptr;
-    if (A == 16 && B == 20) {
-        S2H_TRY(set_lds((const void *)fft_x_power2<T, 16, 20>, lds));
-        fft_x_power2<T, 16, 20><<<fy * tiles, threads, lds, s>>>(nx, fy, H, P, tiles, Rx, p->x2_pad, tw, g1, g2);
-    } else if (A == 16 && B == 16) {
-        S2H_TRY(set_lds((const void *)fft_x_power2<T, 16, 16>, lds));
-        fft_x_power2<T, 16, 16><<<fy * tiles, threads, lds, s>>>(nx, fy, H, P, tiles, Rx, p->x2_pad, tw, g1, g2);
-    } else {
-        return fail(S2H_ERR_UNSUPPORTED, "two-stage x pass length");
+#define S2H_X2(AA, BB)                                                                                          \
+    if (A == AA && B == BB) {                                                                                   \
+        S2H_TRY(set_lds((const void *)fft_x_power2<T, AA, BB>, lds));                                          \
+        fft_x_power2<T, AA, BB><<<fy * tiles, threads, lds, s>>>(nx, fy, H, P, tiles, Rx, p->x2_pad, tw, g1, g2); \
+        S2H_HIP(hipGetLastError());                                                                             \
+        return S2H_OK;                                                                                          \
     }
-    S2H_HIP(hipGetLastError());
-    return S2H_OK;
+    S2H_TWO_STAGE_LENGTHS(S2H_X2)
+#undef S2H_X2
+    return fail(S2H_ERR_UNSUPPORTED, "two-stage x pass length");
 }
 
 // The two-stage y passes (forward: inverse == false) of a plan with y2_a,
@@ -1776,16 +1800,16 @@ s2h_status fft_y2_launch(FftPlan *p, bool inverse, int nplanes, int ny, int H, i
         S2H_HIP(hipGetLastError());                                                                               \
         return S2H_OK;                                                                                            \
     }
-    S2H_Y2(16, 20)
-    S2H_Y2(16, 16)
+    S2H_TWO_STAGE_LENGTHS(S2H_Y2)
 #undef S2H_Y2
     return fail(S2H_ERR_UNSUPPORTED, "two-stage y pass length");
 }
 
 // Lines (row pairs) per workgroup of the two-stage z forward pass, and line
-// pairs per workgroup of the two-stage z inverse (a smaller pass: more
-// workgroups to spread over the CUs).
-static constexpr int kZ2Lines = 6;
+// pairs per workgroup of the two-stage z inverse, each the fastest of 1-12
+// timed at 320^3 (z forward: 4 lines 53 us, 6 lines 58 us; z inverse: 3 pairs
+// 11.0 us, 1 pair 16.6 us).
+static constexpr int kZ2Lines = 4;
 static constexpr int kZ2InvPairs = 3;
 
 // The two-stage z inverse pass of a plan with z2_a, z2_b set.
@@ -1799,19 +1823,17 @@ s2h_status fft_z2inv_launch(FftPlan *p, int nlines, int H, int P, int Rz, T scal
     const int npairs = (nlines + 1) / 2;
     const unsigned grid = (unsigned)((npairs + pairs - 1) / pairs);
     const V *tw = (const V *)p->tw_z2.ptr;
-    if (A == 16 && B == 20) {
-        S2H_TRY(set_lds((const void *)fft_z_inv2<T, 16, 20>, lds));
-        fft_z_inv2<T, 16, 20><<<grid, threads, lds, s>>>(nlines, H, P, pairs, Rz, scale, tw, (const V *)p->g3.ptr,
-                                                          (T *)p->cube.ptr);
-    } else if (A == 16 && B == 16) {
-        S2H_TRY(set_lds((const void *)fft_z_inv2<T, 16, 16>, lds));
-        fft_z_inv2<T, 16, 16><<<grid, threads, lds, s>>>(nlines, H, P, pairs, Rz, scale, tw, (const V *)p->g3.ptr,
-                                                          (T *)p->cube.ptr);
-    } else {
-        return fail(S2H_ERR_UNSUPPORTED, "two-stage z inverse length");
+#define S2H_Z2I(AA, BB)                                                                                        \
+    if (A == AA && B == BB) {                                                                                  \
+        S2H_TRY(set_lds((const void *)fft_z_inv2<T, AA, BB>, lds));                                           \
+        fft_z_inv2<T, AA, BB><<<grid, threads, lds, s>>>(nlines, H, P, pairs, Rz, scale, tw,                   \
+                                                          (const V *)p->g3.ptr, (T *)p->cube.ptr);             \
+        S2H_HIP(hipGetLastError());                                                                            \
+        return S2H_OK;                                                                                         \
     }
-    S2H_HIP(hipGetLastError());
-    return S2H_OK;
+    S2H_TWO_STAGE_LENGTHS(S2H_Z2I)
+#undef S2H_Z2I
+    return fail(S2H_ERR_UNSUPPORTED, "two-stage z inverse length");
 }
 
 // The two-stage z forward pass of a plan with z2_a, z2_b set (f_z = A B).
@@ -1828,19 +1850,17 @@ s2h_status fft_z2_launch(FftPlan *p, s2h_vol *vol, int H, int P, hipStream_t s) 
     const V *tw = (const V *)p->tw_z2.ptr;
     V *g1 = (V *)p->g1.ptr;
     const FastDiv dyp = fast_div((uint32_t)((ny + 1) / 2)), dh = fast_div((uint32_t)H);
-    if (A == 16 && B == 20) {
-        S2H_TRY(set_lds((const void *)fft_z_fwd2<T, 16, 20>, lds));
-        fft_z_fwd2<T, 16, 20><<<grid, threads, lds, s>>>(vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, P,
-                                                          lines, tw, g1, dyp, dh);
-    } else if (A == 16 && B == 16) {
-        S2H_TRY(set_lds((const void *)fft_z_fwd2<T, 16, 16>, lds));
-        fft_z_fwd2<T, 16, 16><<<grid, threads, lds, s>>>(vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, P,
-                                                          lines, tw, g1, dyp, dh);
-    } else {
-        return fail(S2H_ERR_UNSUPPORTED, "two-stage z pass length");
+#define S2H_Z2(AA, BB)                                                                                        \
+    if (A == AA && B == BB) {                                                                                 \
+        S2H_TRY(set_lds((const void *)fft_z_fwd2<T, AA, BB>, lds));                                          \
+        fft_z_fwd2<T, AA, BB><<<grid, threads, lds, s>>>(vol->d_words, nx, ny, nz, (int)vol->row_words, fy, H, \
+                                                          P, lines, tw, g1, dyp, dh);                         \
+        S2H_HIP(hipGetLastError());                                                                           \
+        return S2H_OK;                                                                                        \
     }
-    S2H_HIP(hipGetLastError());
-    return S2H_OK;
+    S2H_TWO_STAGE_LENGTHS(S2H_Z2)
+#undef S2H_Z2
+    return fail(S2H_ERR_UNSUPPORTED, "two-stage z pass length");
 }
 
 template <class T>
@@ -1855,6 +1875,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
     hipStream_t s = ctx->stream;
     const int ty = p->tile_y, tx = p->tile_x;
     const int tiles_y = (H + ty - 1) / ty, tiles_x = (H + tx - 1) / tx;
+    if (ev) S2H_HIP(hipEventRecord(ev[4], s));
     // z forward
     if (p->z2_on) {
         S2H_TRY(fft_z2_launch<T>(p, vol, H, P, s));
@@ -1928,6 +1949,7 @@ s2h_status fft_run(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p, hipEvent_t *ev = null
         S2H_HIP(hipGetLastError());
         ++ctx->launches;
     }
+    if (ev) S2H_HIP(hipEventRecord(ev[5], s));
     return S2H_OK;
 }
 
@@ -1987,17 +2009,20 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     // per FP64 run there, 12 runs once per plan)
     if (!ctx->autotune || (env && env[0] == '0') || cands.size() < 2 || cells > 3.7e9) return S2H_OK;
     if (cands.size() > 3) cands.resize(3);
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     s2h_status st = S2H_OK;
     for (hipEvent_t &e : ev)
         if (st == S2H_OK && hipEventCreate(&e) != hipSuccess) st = fail(S2H_ERR_HIP, "hipEventCreate failed");
     float best_y = 1e30f, best_x = 1e30f;
     int pick_y = p->tile_y, pick_x = p->tile_x, psh_y = p->sh_y, psh_x = p->sh_x;
-    const bool x2_ok = p->x2_on, y2_ok = p->y2_on;
+    const bool x2_ok = p->x2_on, y2_ok = p->y2_on, z2_ok = p->z2_on;
     int pick_x2 = -1;  // the two-stage x pass' padding if it is fastest, else -1
     int pick_y2 = -1;  // the same for the two-stage y passes
+    bool pick_z2 = false;
+    float best_z = 1e30f;  // z forward + z inverse of the LDS-staged passes
     p->x2_on = false;
     p->y2_on = false;
+    p->z2_on = false;
     for (int t : cands) {
         for (int sh : {kPad, kNoPad}) {
             if (st != S2H_OK) break;
@@ -2007,14 +2032,17 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
             st = fft_run<T>(ctx, vol, p);  // warm-up
             if (st == S2H_OK) st = fft_run<T>(ctx, vol, p, ev);
             hipError_t e = hipSuccess;
-            if (st == S2H_OK) e = hipEventSynchronize(ev[3]);
-            float yf = 0, xm = 0, yi = 0;
+            if (st == S2H_OK) e = hipEventSynchronize(ev[5]);
+            float yf = 0, xm = 0, yi = 0, zf = 0, zi = 0;
             if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yf, ev[0], ev[1]);
             if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&xm, ev[1], ev[2]);
             if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yi, ev[2], ev[3]);
+            if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&zf, ev[4], ev[0]);
+            if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&zi, ev[3], ev[5]);
             if (st == S2H_OK && e != hipSuccess)
                 st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
             if (st != S2H_OK) break;
+            best_z = std::min(best_z, zf + zi);
             // only whole-line tiles are taken: a narrower one reads each
             // 128-byte line once per workgroup that shares it
             const bool whole = (size_t)t * sizeof(C2<T>) >= 128;
@@ -2030,26 +2058,33 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
             }
         }
     }
-    // the two-stage x and y passes, unpadded and padded, against the best
-    // LDS-staged ones
+    // the two-stage x and y passes, unpadded and padded, and the two-stage z
+    // passes, against the best LDS-staged ones
     p->tile_y = pick_y;
     p->sh_y = psh_y;
     for (int pad : {0, (int)(128 / sizeof(C2<T>))}) {
-        if (st != S2H_OK || !(x2_ok || y2_ok)) break;
+        if (st != S2H_OK || !(x2_ok || y2_ok || z2_ok)) break;
         p->x2_on = x2_ok;
         p->x2_pad = pad;
         p->y2_on = y2_ok;
         p->y2_pad = pad;
+        p->z2_on = z2_ok;
         st = fft_run<T>(ctx, vol, p);  // warm-up
         if (st == S2H_OK) st = fft_run<T>(ctx, vol, p, ev);
         hipError_t e = hipSuccess;
-        float yf = 0, xm = 0, yi = 0;
-        if (st == S2H_OK) e = hipEventSynchronize(ev[3]);
+        float yf = 0, xm = 0, yi = 0, zf = 0, zi = 0;
+        if (st == S2H_OK) e = hipEventSynchronize(ev[5]);
         if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yf, ev[0], ev[1]);
         if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&xm, ev[1], ev[2]);
         if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&yi, ev[2], ev[3]);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&zf, ev[4], ev[0]);
+        if (st == S2H_OK && e == hipSuccess) e = hipEventElapsedTime(&zi, ev[3], ev[5]);
         if (st == S2H_OK && e != hipSuccess)
             st = fail(S2H_ERR_HIP, std::string("FFT tile tuning: ") + hipGetErrorString(e));
+        if (st == S2H_OK && z2_ok && zf + zi < best_z) {
+            best_z = zf + zi;
+            pick_z2 = true;
+        }
         if (st == S2H_OK && x2_ok && xm < best_x) {
             best_x = xm;
             pick_x2 = pad;
@@ -2069,6 +2104,7 @@ s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     p->x2_pad = pick_x2 >= 0 ? pick_x2 : 0;
     p->y2_on = pick_y2 >= 0;
     p->y2_pad = pick_y2 >= 0 ? pick_y2 : 0;
+    p->z2_on = pick_z2;
     return st;
 }
 

@@ -121,3 +121,34 @@ def test_budgeted_context_counts_directly(ctx, method):
         assert small.exact_tune_info(occ.shape, 40, 1.0)["kernel"] == "direct"
         assert small.bytes_in_use <= 256 << 20
         v2.free()
+
+
+# Every length the two-stage register kernels take (96 = 8 x 12 ... 384 = 16 x 24), on each
+# axis in turn: axis 0 runs the x pass, axis 1 the y passes, axis 2 the z passes.  Untimed
+# plans (autotune off) take the two-stage kernels wherever the length allows.
+TWO_STAGE_CASES = [((60, 80, 103), 24, (96, 120, 128)), ((100, 140, 200), 32, (160, 192, 256)),
+                   ((200, 316, 250), 64, (320, 384, 320))]
+
+
+@pytest.mark.parametrize("case", range(len(TWO_STAGE_CASES)))
+@pytest.mark.parametrize("rot", range(3))
+def test_two_stage_lengths_equal_direct(ctx, method, case, rot):
+    shape, r_max, lengths = TWO_STAGE_CASES[case]
+    shape = tuple(np.roll(shape, rot))
+    assert tuple(s2.fft_padded_dims(shape, r_max)) == tuple(np.roll(lengths, rot))
+    occ = synth.grf_volume(max(shape), 4.0, 0.3, seed=40 + case)[:shape[0], :shape[1], :shape[2]].copy()
+    ctx.set_autotune(False)
+    try:
+        out = both(ctx, method, occ, r_max)
+        vol = s2.Volume.from_bytes(ctx, occ)
+        c32 = vol.fft32_counts(r_max, 1.0).astype(np.float64)
+        vol.free()
+    finally:
+        ctx.set_autotune(True)
+    cd, cf = out["direct"][0], out["fft64"][0]
+    assert out["fft64"][2] == "fft64"
+    assert np.array_equal(cd, cf)
+    assert out["direct"][1].tobytes() == out["fft64"][1].tobytes()
+    # FP32 on the same kernels: within the stated per-bin bound (tests/test_gpu_fft.py)
+    tol = 2.5e-8 * float(cd.max()) * np.log2(float(np.prod(s2.fft_padded_dims(shape, r_max)))) + 0.05
+    assert np.max(np.abs(c32 - cd.astype(np.float64))) <= tol
