@@ -1031,15 +1031,30 @@ __global__ __launch_bounds__(256) void fft_z_fwd2(const uint64_t *__restrict__ w
         const int pl = t / B, b = t - pl * B;
         const uint64_t *ra = sw + 2 * pl * row_words;
         using R = decltype(V::x);
+        // the two rows shifted right by b once, so that the bit of z = b + B r
+        // sits at the compile-time position B r of the shifted words
+        constexpr int NW = (A * B + 63) / 64;
+        uint64_t sa[NW], sb[NW];
+        {
+            uint64_t wa[NW + 1], wb[NW + 1];
+#pragma unroll
+            for (int i = 0; i <= NW; ++i) {
+                wa[i] = i < row_words ? ra[i] : 0ull;
+                wb[i] = i < row_words ? ra[row_words + i] : 0ull;
+            }
+#pragma unroll
+            for (int i = 0; i < NW; ++i) {
+                sa[i] = b ? (wa[i] >> b) | (wa[i + 1] << (64 - b)) : wa[i];
+                sb[i] = b ? (wb[i] >> b) | (wb[i + 1] << (64 - b)) : wb[i];
+            }
+        }
         V a[A];
 #pragma unroll
         for (int r = 0; r < A; ++r) {
-            const int z = b + B * r;
-            R re = 0, im = 0;
-            if (z < nz) {
-                re = (R)((ra[z >> 6] >> (z & 63)) & 1ull);
-                im = (R)((ra[row_words + (z >> 6)] >> (z & 63)) & 1ull);
-            }
+            const int pos = B * r;
+            const bool in = b + pos < nz;
+            const R re = in ? (R)((sa[pos >> 6] >> (pos & 63)) & 1ull) : (R)0;
+            const R im = in ? (R)((sb[pos >> 6] >> (pos & 63)) & 1ull) : (R)0;
             a[r] = mk<V>(re, im);
         }
         dft_reg<A, -1>(a);
