@@ -286,7 +286,9 @@ __host__ __device__ constexpr double unit_root_cos(int N, int k) {
     constexpr double c12[12] = {1.0, 0.866025403784438646764, 0.500000000000000000000, 0.0, -0.500000000000000000000, -0.866025403784438646764, -1.0, -0.866025403784438646764, -0.500000000000000000000, 0.0, 0.500000000000000000000, 0.866025403784438646764};
     constexpr double c15[15] = {1.0, 0.913545457642600895502, 0.669130606358858213826, 0.309016994374947424102, -0.104528463267653471400, -0.500000000000000000000, -0.809016994374947424102, -0.978147600733805637929, -0.978147600733805637929, -0.809016994374947424102, -0.500000000000000000000, -0.104528463267653471400, 0.309016994374947424102, 0.669130606358858213826, 0.913545457642600895502};
     constexpr double c24[24] = {1.0, 0.965925826289068286750, 0.866025403784438646764, 0.707106781186547524401, 0.500000000000000000000, 0.258819045102520762349, 0.0, -0.258819045102520762349, -0.500000000000000000000, -0.707106781186547524401, -0.866025403784438646764, -0.965925826289068286750, -1.0, -0.965925826289068286750, -0.866025403784438646764, -0.707106781186547524401, -0.500000000000000000000, -0.258819045102520762349, 0.0, 0.258819045102520762349, 0.500000000000000000000, 0.707106781186547524401, 0.866025403784438646764, 0.965925826289068286750};
+    constexpr double c32[32] = {1.0, 0.980785280403230449126, 0.923879532511286756128, 0.831469612302545237079, 0.707106781186547524401, 0.555570233019602224743, 0.382683432365089771728, 0.195090322016128267848, 0.0, -0.195090322016128267848, -0.382683432365089771728, -0.555570233019602224743, -0.707106781186547524401, -0.831469612302545237079, -0.923879532511286756128, -0.980785280403230449126, -1.0, -0.980785280403230449126, -0.923879532511286756128, -0.831469612302545237079, -0.707106781186547524401, -0.555570233019602224743, -0.382683432365089771728, -0.195090322016128267848, 0.0, 0.195090322016128267848, 0.382683432365089771728, 0.555570233019602224743, 0.707106781186547524401, 0.831469612302545237079, 0.923879532511286756128, 0.980785280403230449126};
     switch (N) {
+        case 32: return c32[k % 32];
         case 10: return c10[k % 10];
         case 12: return c12[k % 12];
         case 15: return c15[k % 15];
@@ -346,10 +348,10 @@ __device__ __forceinline__ void dft_comp(V *a) {
 }
 
 // The in-register DFTs of the two-stage passes: 8 as small_dft, 10 = 2 x 5,
-// 12 = 4 x 3, 15 = 3 x 5, 16 = 4 x 4, 20 = 4 x 5, 24 = 8 x 3.
+// 12 = 4 x 3, 15 = 3 x 5, 16 = 4 x 4, 20 = 4 x 5, 24 = 8 x 3, 32 = 4 x 8.
 template <int N, int SIGN, class V>
 __device__ __forceinline__ void dft_reg(V *a) {
-    static_assert(N == 8 || N == 10 || N == 12 || N == 15 || N == 16 || N == 20 || N == 24,
+    static_assert(N == 8 || N == 10 || N == 12 || N == 15 || N == 16 || N == 20 || N == 24 || N == 32,
                   "in-register DFT length");
     if constexpr (N == 8) small_dft<SIGN>(a, 8);
     else if constexpr (N == 10) dft_comp<2, 5, SIGN>(a);
@@ -357,7 +359,8 @@ __device__ __forceinline__ void dft_reg(V *a) {
     else if constexpr (N == 15) dft_comp<3, 5, SIGN>(a);
     else if constexpr (N == 16) dft_comp<4, 4, SIGN>(a);
     else if constexpr (N == 20) dft_comp<4, 5, SIGN>(a);
-    else dft_comp<8, 3, SIGN>(a);
+    else if constexpr (N == 24) dft_comp<8, 3, SIGN>(a);
+    else dft_comp<4, 8, SIGN>(a);
 }
 
 // One radix-R stage s over all lines: in-place butterflies on sub-length
@@ -1590,7 +1593,8 @@ std::tuple<uint64_t, uint64_t, uint64_t> fft_plan_key(const uint64_t dims[3], co
 // (the instantiations listed in S2H_TWO_STAGE_LENGTHS).
 bool x2_split(int L, int *A, int *B) {
     static constexpr int kSplits[][3] = {{96, 8, 12},   {120, 8, 15},  {128, 8, 16},  {160, 16, 10},
-                                         {192, 16, 12}, {256, 16, 16}, {320, 16, 20}, {384, 16, 24}};
+                                         {192, 16, 12}, {256, 16, 16}, {320, 16, 20}, {360, 15, 24},
+                                         {384, 16, 24}, {400, 20, 20}, {512, 16, 32}};
     for (const auto &e : kSplits)
         if (e[0] == L) {
             *A = e[1];
@@ -1764,7 +1768,8 @@ static int pass_threads(int L, const Fac &fc, int tile) {
 }
 
 // The (A, B) instantiations of the two-stage kernels (x2_split's table).
-#define S2H_TWO_STAGE_LENGTHS(M) M(8, 12) M(8, 15) M(8, 16) M(16, 10) M(16, 12) M(16, 16) M(16, 20) M(16, 24)
+#define S2H_TWO_STAGE_LENGTHS(M) \
+    M(8, 12) M(8, 15) M(8, 16) M(16, 10) M(16, 12) M(16, 16) M(16, 20) M(15, 24) M(16, 24) M(20, 20) M(16, 32)
 
 // The two-stage x pass of a plan with x2_a, x2_b set (f_x = A B).
 template <class T>

@@ -123,11 +123,11 @@ def test_budgeted_context_counts_directly(ctx, method):
         v2.free()
 
 
-# Every length the two-stage register kernels take (96 = 8 x 12 ... 384 = 16 x 24), on each
+# Every length the two-stage register kernels take (96 = 8 x 12 ... 512 = 16 x 32), on each
 # axis in turn: axis 0 runs the x pass, axis 1 the y passes, axis 2 the z passes.  Untimed
 # plans (autotune off) take the two-stage kernels wherever the length allows.
 TWO_STAGE_CASES = [((60, 80, 103), 24, (96, 120, 128)), ((100, 140, 200), 32, (160, 192, 256)),
-                   ((200, 316, 250), 64, (320, 384, 320))]
+                   ((200, 316, 250), 64, (320, 384, 320)), ((280, 340, 360), 48, (360, 400, 512))]
 
 
 @pytest.mark.parametrize("case", range(len(TWO_STAGE_CASES)))
