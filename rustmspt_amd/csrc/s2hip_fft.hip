@@ -26,6 +26,16 @@
 // digit-reversed order) and inverses the matching decimation-in-time (input in
 // that order, output natural), so no reordering pass is needed; the pointwise
 // |F|^2 is order-independent.
+//
+// Where an axis length is L = A B with in-register DFTs of A and B points
+// (x2_split: 96 ... 512), each pass has a two-stage form (fft_z_fwd2,
+// fft_y_fwd2, fft_x_power2, fft_y_inv2, fft_z_inv2): stage A, an A-point DFT
+// of x[b + B r] straight from global memory (or the staged bits) times
+// w^(b q), one LDS exchange, stage B, the B-point DFT of each block, giving
+// X[q + A k] in registers.  The two-stage y forward pass leaves natural
+// frequency order and the two-stage y inverse reads it, so the y pair is
+// switched together; the x pass (per ky line) and the z passes (natural kz at
+// both ends) mix freely with either form.  The tile tuner times both forms.
 #include <algorithm>
 #include <cstdlib>
 #include <cmath>
@@ -1994,7 +2004,10 @@ int pad_rule(int L, int lines) {
 // pass the narrowest whole-line tile (more workgroups per CU).  Every run
 // rewrites all of the plan's buffers, so the trial runs leave nothing behind.
 // The LDS padding of the y and x passes is timed with the tiles (each tile
-// padded and unpadded).
+// padded and unpadded).  Where the lengths allow, the two-stage passes (block
+// gap 0 or 128 bytes) are timed last: the x pass, the y pair and the z pair
+// each take the two-stage form if it beats the fastest LDS-staged one; untimed
+// plans take it wherever the length allows.
 template <class T>
 s2h_status fft_pick_tile(s2h_ctx *ctx, s2h_vol *vol, FftPlan *p) {
     const int lmax = (int)std::max(p->f[0], p->f[1]);
