@@ -1478,6 +1478,89 @@ __global__ __launch_bounds__(kFftMaxThreads) void fftd_x_cross(const FftdCand *_
                                                    true, tw + fc.tw[0])));
 }
 
+// The frames' x pass in two register stages (F_x = A B, as fft_x_power2): stage A
+// of both fields (delta only on its box rows), stage B's blocks of both fields
+// forward, the cross spectrum 2 Re(conj(O) X) + |X|^2 and its inverse in
+// registers, then the inverse stage A of the result keeping dx = 0 .. R_x.
+// LDS holds the two fields one after the other, each laid out as in
+// fft_x_power2.
+template <class T>
+size_t xc2_lds_bytes(int A, int B, int pad) { return 2 * x2_lds_bytes<T>(A, B, pad); }
+template <class T, int A, int B>
+__global__ __launch_bounds__(512) void fftd_x_cross2(const FftdCand *__restrict__ cands, int Fy, int H, int P,
+                                                     int tiles_kz, int Rx, int pad, const C2<T> *__restrict__ twA,
+                                                     const C2<T> *__restrict__ g1, C2<T> *__restrict__ g2) {
+    extern __shared__ __align__(16) unsigned char sm_raw[];
+    using V = C2<T>;
+    constexpr int TILE = x2_tile<T>(), L = A * B;
+    V *d = reinterpret_cast<V *>(sm_raw);
+    const int fsz = L * TILE + (A - 1) * pad;  // slots of one field
+    const int per = Fy * tiles_kz;
+    const int c = blockIdx.x / per, rem = blockIdx.x % per;
+    const int ky = rem / tiles_kz, kz0 = (rem % tiles_kz) * TILE;
+    const int nk = min(TILE, H - kz0);
+    const uint64_t rs = (uint64_t)Fy * P, field = (uint64_t)L * Fy * P;
+    const V *go = g1 + (uint64_t)(2 * c) * field + (uint64_t)ky * P + kz0;
+    // delta is zero outside the box rows R_x .. R_x + |B_x| - 1 of the frame
+    const int dlo = Rx, dhi = Rx + (cands[c].b1[0] - cands[c].b0[0]);
+    const int t = threadIdx.x, col = t % TILE, grp = t / TILE;
+    if (grp < 2 * B && col < nk) {
+        const int f = grp / B, b = grp - f * B;
+        const int lo = f ? dlo : 0, hi = f ? dhi : L;
+        V a[A];
+        const V *gp = go + (f ? field : 0) + col + (uint64_t)b * rs;
+#pragma unroll
+        for (int r = 0; r < A; ++r) {
+            const int x = b + B * r;
+            a[r] = (x >= lo && x < hi) ? *gp : mk<V>(0, 0);
+            gp += (uint64_t)B * rs;
+        }
+        dft_reg<A, -1>(a);
+        const V *w = twA + b * (A - 1);
+#pragma unroll
+        for (int q = 1; q < A; ++q) a[q] = cmul(a[q], w[q - 1]);
+        V *dp = d + f * fsz + b * TILE + col;
+#pragma unroll
+        for (int q = 0; q < A; ++q) dp[q * (B * TILE + pad)] = a[q];
+    }
+    __syncthreads();
+    if (grp < A && col < nk) {
+        V o[B], x[B];
+        V *dp = d + grp * (B * TILE + pad) + col;
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            o[k] = dp[k * TILE];
+            x[k] = dp[fsz + k * TILE];
+        }
+        dft_reg<B, -1>(o);
+        dft_reg<B, -1>(x);
+#pragma unroll
+        for (int k = 0; k < B; ++k)
+            o[k] = mk<V>(2 * (o[k].x * x[k].x + o[k].y * x[k].y) + (x[k].x * x[k].x + x[k].y * x[k].y), 0);
+        dft_reg<B, 1>(o);
+#pragma unroll
+        for (int k = 0; k < B; ++k) dp[k * TILE] = o[k];
+    }
+    __syncthreads();
+    if (grp < B && col < nk) {
+        const int b = grp;
+        V a[A];
+        const V *dp = d + b * TILE + col;
+#pragma unroll
+        for (int q = 0; q < A; ++q) a[q] = dp[q * (B * TILE + pad)];
+        const V *w = twA + b * (A - 1);
+#pragma unroll
+        for (int q = 1; q < A; ++q) a[q] = cmulc(a[q], w[q - 1]);
+        dft_reg<A, 1>(a);
+        V *gp = g2 + (uint64_t)c * (Rx + 1) * rs + (uint64_t)ky * P + kz0 + col + (uint64_t)b * rs;
+#pragma unroll
+        for (int r = 0; r < A; ++r) {
+            if (b + B * r <= Rx) *gp = a[r];
+            gp += (uint64_t)B * rs;
+        }
+    }
+}
+
 // Delta counts of the candidates: the value at each shell offset rounded to
 // the nearest integer, stored at the offset's canonical index (d and -d share
 // one; the zero offset has none).
@@ -2343,6 +2426,27 @@ double fft_delta_model_ms(const int F[3], const int R[3], uint32_t ncand) {
     return ncand * bytes / 4.0e12 * 1e3 + 0.04;
 }
 
+// The frames' two-stage x pass (fftd_x_cross2) for nc candidates.
+static s2h_status fftd_xc2_launch(FftPlan *p, const FftdCand *cc, int nc, int Fy, int H, int P, int Rx,
+                                  const double2 *g1, double2 *g2, hipStream_t s) {
+    const int A = p->x2_a, B = p->x2_b, tile = x2_tile<double>(), pad = 128 / (int)sizeof(double2);
+    const int tiles = (H + tile - 1) / tile;
+    const int threads = (std::max(2 * B, A) * tile + 63) / 64 * 64;
+    const size_t lds = xc2_lds_bytes<double>(A, B, pad);
+    const double2 *tw = (const double2 *)p->tw_x2.ptr;
+    const unsigned grid = (unsigned)((uint64_t)nc * Fy * tiles);
+#define S2H_XC2(AA, BB)                                                                                           \
+    if (A == AA && B == BB) {                                                                                     \
+        S2H_TRY(set_lds((const void *)fftd_x_cross2<double, AA, BB>, lds));                                      \
+        fftd_x_cross2<double, AA, BB><<<grid, threads, lds, s>>>(cc, Fy, H, P, tiles, Rx, pad, tw, g1, g2);       \
+        S2H_HIP(hipGetLastError());                                                                               \
+        return S2H_OK;                                                                                            \
+    }
+    S2H_XC2(8, 12) S2H_XC2(8, 15) S2H_XC2(8, 16) S2H_XC2(16, 10) S2H_XC2(16, 12) S2H_XC2(16, 16) S2H_XC2(16, 20)
+#undef S2H_XC2
+    return fail(S2H_ERR_UNSUPPORTED, "two-stage frame x pass length");
+}
+
 s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands, uint32_t ncand, const int F[3],
                          const uint2 *ad, int nwd, int32_t *delta) {
     using V = double2;
@@ -2377,6 +2481,9 @@ s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands
     const size_t lds_zi = z_inv_lds_bytes<double>(Fz, ppb, p->sh_z);
     S2H_TRY(set_lds((const void *)fft_z_inv<double>, lds_zi));
     const double scale = 1.0 / ((double)Fx * (double)Fy * (double)Fz);
+    // the x pass in two register stages where F_x allows it (stage B holds
+    // both fields' blocks: B <= 20)
+    const bool xc2 = p->x2_a > 0 && p->x2_b <= 20;
     const V *tw0 = (const V *)p->tw[0].ptr, *tw1 = (const V *)p->tw[1].ptr, *tw2 = (const V *)p->tw[2].ptr;
     V *g1 = (V *)p->g1.ptr, *g2 = (V *)p->g2.ptr, *g3 = (V *)p->g3.ptr;
     double *cube = (double *)p->cube.ptr;
@@ -2391,8 +2498,12 @@ s2h_status fft_delta_run(s2h_ctx *ctx, const FftdSeed &sd, const FftdCand *cands
         fft_y_fwd<double><<<nc * 2 * Fx * tiles_y, pass_threads(Fy, fcy, ty), lds_y, s>>>(Fy, Fy, H, P, tiles_y, ty,
                                                                                           fcy, tw1, g1, p->sh_y, cc,
                                                                                           Fx, Rx, Ry);
-        fftd_x_cross<double><<<nc * Fy * tiles_x, pass_threads(Fx, fcx, tx), lds_x, s>>>(
-            cc, Fx, Fy, H, P, tiles_x, tx, fcx, tw0, Rx, g1, g2, p->sh_x);
+        if (xc2) {
+            S2H_TRY(fftd_xc2_launch(p, cc, nc, Fy, H, P, Rx, g1, g2, s));
+        } else {
+            fftd_x_cross<double><<<nc * Fy * tiles_x, pass_threads(Fx, fcx, tx), lds_x, s>>>(
+                cc, Fx, Fy, H, P, tiles_x, tx, fcx, tw0, Rx, g1, g2, p->sh_x);
+        }
         fft_y_inv<double><<<nc * nxo * tiles_y, pass_threads(Fy, fcy, ty), lds_y, s>>>(Fy, H, P, tiles_y, ty, fcy, tw1,
                                                                                        Ry, g2, g3, p->sh_y);
         const int zi = nc * nxo * nyo;
